@@ -1,0 +1,814 @@
+// C ABI of libfribo_gpu (include/fribo_gpu.h): set-up, state transfer and the
+// per-step orchestration that replaces fluid_tsch.F90:25-166 on one GPU.
+//
+// One HIP stream per handle carries every kernel of a step; the only host
+// synchronisation per step is the final read-back of dt / cflm / the error
+// flag (the values the Fortran driver needs before rigid() and ttime()).
+#include "fribo_internal.h"
+#include "../../include/fribo_gpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+static thread_local std::string g_err = "";
+
+static int set_err(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return set_err(FRIBO_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static const char* kProfNames[FRIBO_NPROF] = {"cfl", "bc", "rhs", "mls_setup", "mls_force", "jsolve_helm",
+                                              "ksolve", "div", "gemm", "jsolve_poisson", "upd", "mean"};
+enum { PC_CFL, PC_BC, PC_RHS, PC_MLS_SETUP, PC_MLS_FORCE, PC_JH, PC_K, PC_DIV, PC_GEMM, PC_JP, PC_UPD, PC_MEAN };
+
+struct fribo_ctx {
+    fribo_params p;
+    FluidDev F;
+    hipStream_t st = nullptr;
+    std::vector<void*> allocs;
+    // host copies of the 1-D metric arrays (1-based, size n+2)
+    std::vector<double> h1[32];
+    std::vector<double> lmcc;   // host copy for the domain check
+    int nlm_alloc = 0;
+    std::vector<void*> mk_allocs;
+    DevScalars hs;
+    // profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> evpool;
+    size_t evused = 0;
+    struct Rec { int cls; hipEvent_t a, b; };
+    std::vector<Rec> pending;
+    double prof_ms[FRIBO_NPROF] = {0};
+    int prof_n[FRIBO_NPROF] = {0};
+    size_t lds_limit = 65536;
+    int zero_int = 0;
+};
+
+template <class T>
+static int dalloc(fribo_ctx* c, T** p, size_t n, std::vector<void*>* list = nullptr)
+{
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    HIPCHK(hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    (list ? list : &c->allocs)->push_back(q);
+    *p = (T*)q;
+    return FRIBO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grid metrics, restated from the reference set-up (host, once).
+namespace {
+enum { A_RC, A_RM, A_ZZ, A_ZM, A_G2RC, A_G2RM, A_G3RC, A_G3RM, A_UDX2C, A_UDX2M, A_UDX3C, A_UDX3M,
+       A_AM2J, A_AC2J, A_AP2J, A_AM3J, A_AC3J, A_AP3J, A_AM3CK, A_AC3CK, A_AP3CK, A_AM3SK, A_AC3SK, A_AP3SK,
+       A_AMPHJ, A_ACPHJ, A_APPHJ, A_AMPHK, A_ACPHK, A_APPHK, A_WR, A_COUNT };
+const char* kArrNames[A_COUNT] = {"rc", "rm", "zz", "zm", "g2rc", "g2rm", "g3rc", "g3rm", "udx2c", "udx2m", "udx3c",
+                                  "udx3m", "am2j", "ac2j", "ap2j", "am3j", "ac3j", "ap3j", "am3ck", "ac3ck", "ap3ck",
+                                  "am3sk", "ac3sk", "ap3sk", "amphj", "acphj", "apphj", "amphk", "acphk", "apphk", "wr"};
+bool is_zdir(int a)
+{
+    return a == A_ZZ || a == A_ZM || a == A_G3RC || a == A_G3RM || a == A_UDX3C || a == A_UDX3M || a == A_AM3CK ||
+           a == A_AC3CK || a == A_AP3CK || a == A_AM3SK || a == A_AC3SK || a == A_AP3SK || a == A_AMPHK ||
+           a == A_ACPHK || a == A_APPHK || a == A_WR;
+}
+}  // namespace
+
+static void host_metrics(fribo_ctx* c, const double* rc_in, const double* zz_in, std::vector<int>& jmv,
+                         std::vector<int>& jpv, std::vector<int>& kmv, std::vector<int>& kpv)
+{
+    const int n2 = c->p.n2, n3 = c->p.n3, n2m = n2 - 1, n3m = n3 - 1;
+    auto& H = c->h1;
+    for (int a = 0; a < A_COUNT; ++a) H[a].assign((is_zdir(a) ? n3 : n2) + 3, 0.0);
+    const double dx2 = c->F.dx2, dx3 = c->F.dx3, dx2q = c->F.dx2q, dx3q = c->F.dx3q;
+    auto& rc = H[A_RC]; auto& rm = H[A_RM]; auto& zz = H[A_ZZ]; auto& zm = H[A_ZM];
+    auto& g2rc = H[A_G2RC]; auto& g2rm = H[A_G2RM]; auto& g3rc = H[A_G3RC]; auto& g3rm = H[A_G3RM];
+    for (int j = 1; j <= n2; ++j) rc[j] = rc_in[j - 1];
+    for (int k = 1; k <= n3; ++k) zz[k] = zz_in[k - 1];
+    // indic.F90:99-125
+    jmv.assign(n2 + 3, 0); jpv.assign(n2 + 3, 0); kmv.assign(n3 + 3, 0); kpv.assign(n3 + 3, 0);
+    std::vector<int> jmc(n2 + 3, 0), jpc(n2 + 3, 0), kmc(n3 + 3, 0), kpc(n3 + 3, 0);
+    for (int kc = 1; kc <= n3m; ++kc) {
+        kmv[kc] = kc - 1; kpv[kc] = kc + 1;
+        if (kc == 1) kmv[kc] = kc;
+        if (kc == n3m) kpv[kc] = kc;
+    }
+    for (int kc = 1; kc <= n3m; ++kc) { kpc[kc] = kpv[kc] - kc; kmc[kc] = kc - kmv[kc]; }
+    for (int jc = 1; jc <= n2; ++jc) {
+        jmv[jc] = jc - 1; jpv[jc] = jc + 1;
+        if (jc == 1) jmv[jc] = jc;
+        if (jc == n2m) jpv[jc] = jc;
+        if (jc == n2) jpv[jc] = jc;
+    }
+    for (int jc = 1; jc <= n2m; ++jc) { jpc[jc] = jpv[jc] - jc; jmc[jc] = jc - jmv[jc]; }
+    // cordin.F90:106-123, 197-215
+    for (int j = 1; j <= n2m; ++j) { rm[j] = (rc[j] + rc[j + 1]) * 0.5; g2rm[j] = (rc[j + 1] - rc[j]) * dx2; }
+    for (int j = 2; j <= n2m; ++j) g2rc[j] = (rc[j + 1] - rc[j - 1]) * dx2 * 0.5;
+    g2rc[1] = (rc[2] - rc[1]) * dx2;
+    g2rc[n2] = (rc[n2] - rc[n2m]) * dx2;
+    rm[n2] = rc[n2];
+    for (int j = 1; j <= n2m; ++j) { H[A_UDX2M][j] = dx2 / g2rm[j]; H[A_UDX2C][j] = dx2 / g2rc[j]; }
+    H[A_UDX2C][n2] = dx2 / g2rc[n2];
+    for (int k = 1; k <= n3m; ++k) { zm[k] = (zz[k] + zz[k + 1]) * 0.5; g3rm[k] = (zz[k + 1] - zz[k]) * dx3; }
+    for (int k = 2; k <= n3m; ++k) g3rc[k] = (zz[k + 1] - zz[k - 1]) * dx3 * 0.5;
+    g3rc[1] = (zz[2] - zz[1]) * dx3;
+    g3rc[n3] = (zz[n3] - zz[n3m]) * dx3;
+    zm[n3] = zz[n3];
+    for (int k = 1; k <= n3m; ++k) { H[A_UDX3M][k] = dx3 / g3rm[k]; H[A_UDX3C][k] = dx3 / g3rc[k]; }
+    H[A_UDX3C][n3] = dx3 / g3rc[n3];
+    // MLS_SD_ind reads one node past its grid copy at the last cell
+    // (MLSint.F90:89,93); the nearest defined value is used.
+    rm[n2 + 1] = rc[n2];
+    zm[n3 + 1] = zz[n3];
+    // coetar.F90:18-141
+    const int inwn = c->p.inslwn ? 1 : 0, inws = c->p.inslws ? 1 : 0;
+    auto& am2j = H[A_AM2J]; auto& ac2j = H[A_AC2J]; auto& ap2j = H[A_AP2J];
+    auto& am3j = H[A_AM3J]; auto& ac3j = H[A_AC3J]; auto& ap3j = H[A_AP3J];
+    am2j[1] = 0.0; ap2j[1] = 0.0; ac2j[1] = 1.0; am2j[n2] = 0.0; ap2j[n2] = 0.0; ac2j[n2] = 1.0;
+    for (int jc = 2; jc <= n2m; ++jc) {
+        int jm = jc - 1;
+        double a22 = dx2q / g2rc[jc], a22p = 1.0 / g2rm[jc], a22m = 1.0 / g2rm[jm];
+        ap2j[jc] = a22 * a22p; am2j[jc] = a22 * a22m; ac2j[jc] = -(a22 * a22p + a22 * a22m);
+    }
+    for (int jc = 2; jc <= n2m - 1; ++jc) {
+        int jp = jpv[jc];
+        double a22 = dx2q / g2rm[jc], a22p = a22 / g2rc[jp], a22m = a22 / g2rc[jc];
+        ap3j[jc] = a22p; am3j[jc] = a22m; ac3j[jc] = -(a22p + a22m);
+    }
+    {
+        int jc = n2m, jp = jc + 1;
+        double a22 = dx2q / g2rm[jc], a22p = +a22 / g2rc[jp], a22m = +a22 / g2rc[jc];
+        am3j[jc] = a22m; ap3j[jc] = 2.0 * inwn * a22p; ac3j[jc] = -(a22m + inwn * a22p * 2.0);
+        jc = 1; jp = jc + 1;
+        a22 = dx2q / g2rm[jc]; a22p = +a22 / g2rc[jp]; a22m = +a22 / g2rc[jc];
+        ap3j[jc] = a22p; am3j[jc] = 2.0 * a22m * inws; ac3j[jc] = -(a22p + 2.0 * inws * a22m);
+    }
+    auto& am3ck = H[A_AM3CK]; auto& ac3ck = H[A_AC3CK]; auto& ap3ck = H[A_AP3CK];
+    auto& am3sk = H[A_AM3SK]; auto& ac3sk = H[A_AC3SK]; auto& ap3sk = H[A_AP3SK];
+    am3ck[1] = 0.0; ap3ck[1] = 0.0; ac3ck[1] = 1.0; am3ck[n3] = 0.0; ap3ck[n3] = 0.0; ac3ck[n3] = 1.0;
+    for (int kc = 2; kc <= n3m; ++kc) {
+        int km = kc - 1;
+        double a33 = dx3q / g3rc[kc], a33p = 1.0 / g3rm[kc], a33m = 1.0 / g3rm[km];
+        ap3ck[kc] = a33 * a33p; am3ck[kc] = a33 * a33m; ac3ck[kc] = -(ap3ck[kc] + am3ck[kc]);
+    }
+    for (int kc = 2; kc <= n3m - 1; ++kc) {
+        int kp = kc + 1;
+        double a33 = dx3q / g3rm[kc], a33p = +a33 / g3rc[kp], a33m = +a33 / g3rc[kc];
+        ap3sk[kc] = a33p; am3sk[kc] = a33m; ac3sk[kc] = -(ap3sk[kc] + am3sk[kc]);
+    }
+    {
+        int kc = 1, kp = kc + 1;
+        double a33 = dx3q / g3rm[kc], a33p = +a33 / g3rc[kp], a33m = +a33 / g3rc[kc];
+        ap3sk[kc] = a33p; am3sk[kc] = 0.0; ac3sk[kc] = -(a33p + a33m * 2.0);
+        kc = n3m; kp = kc + 1;
+        a33 = dx3q / g3rm[kc]; a33p = +a33 / g3rc[kp]; a33m = +a33 / g3rc[kc];
+        am3sk[kc] = a33m; ap3sk[kc] = 0.0; ac3sk[kc] = -(a33m + a33p * 2.0);
+    }
+    // phini.F90:121-142 (tridiag_matrices)
+    for (int jc = 1; jc <= n2m; ++jc) {
+        int jp = jpv[jc];
+        double a22icc = jmc[jc] * dx2q / g2rc[jc], a22icp = jpc[jc] * dx2q / g2rc[jp], ug = 1.0 / g2rm[jc];
+        H[A_AMPHJ][jc] = a22icc * ug; H[A_APPHJ][jc] = a22icp * ug;
+        H[A_ACPHJ][jc] = -(H[A_AMPHJ][jc] + H[A_APPHJ][jc]);
+    }
+    for (int kc = 1; kc <= n3m; ++kc) {
+        int kp = kpv[kc];
+        double a33icc = kmc[kc] * dx3q / g3rc[kc], a33icp = kpc[kc] * dx3q / g3rc[kp], ug = 1.0 / g3rm[kc];
+        H[A_AMPHK][kc] = a33icc * ug; H[A_APPHK][kc] = a33icp * ug;
+        H[A_ACPHK][kc] = -(H[A_AMPHK][kc] + H[A_APPHK][kc]);
+    }
+}
+
+static int setup_ksys(fribo_ctx* c, KSys& K, int n, int kofs)
+{
+    K.n = n;
+    K.kofs = kofs;
+    int nch = n / 2;
+    if (nch > FRIBO_KMAXCH) nch = FRIBO_KMAXCH;
+    if (nch < 1) nch = 1;
+    K.nch = nch;
+    std::vector<int> cs(nch + 1);
+    int maxm = 0;
+    for (int p = 0; p <= nch; ++p) cs[p] = (int)(((long)p * n) / nch);
+    for (int p = 0; p < nch; ++p) maxm = std::max(maxm, cs[p + 1] - cs[p]);
+    K.maxm = maxm;
+    if (maxm - 1 > 128) return set_err(FRIBO_ERR_LIMIT, "k-lines longer than 8256 rows are not supported");
+    int rc;
+    if ((rc = dalloc(c, &K.cs, nch + 1))) return rc;
+    HIPCHK(hipMemcpy(K.cs, cs.data(), (nch + 1) * sizeof(int), hipMemcpyHostToDevice));
+    double** arrs[] = {&K.a, &K.c, &K.S, &K.cp, &K.inv, &K.vp, &K.g};
+    for (auto a : arrs)
+        if ((rc = dalloc(c, a, n))) return rc;
+    if ((rc = dalloc(c, &K.wl, 64))) return rc;
+    if ((rc = dalloc(c, &K.k1, 6 * 64))) return rc;
+    if ((rc = dalloc(c, &K.k2, 6 * 64))) return rc;
+    if ((rc = dalloc(c, &K.ib, 64))) return rc;
+    if ((rc = dalloc(c, &K.bc, 2))) return rc;
+    return FRIBO_OK;
+}
+
+static int upload1d(fribo_ctx* c, double** dst, const std::vector<double>& h)
+{
+    int rc = dalloc(c, dst, h.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(*dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return FRIBO_OK;
+}
+
+static int upload1i(fribo_ctx* c, int** dst, const std::vector<int>& h)
+{
+    int rc = dalloc(c, dst, h.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(*dst, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_init(const fribo_params* p, const double* rc, const double* zz, const double* zmm,
+                          const double* zmmt, int ldz, const double* wr, fribo_handle* out)
+{
+    if (!p || !rc || !zz || !zmm || !zmmt || !wr || !out) return set_err(FRIBO_ERR_ARG, "null argument");
+    if (p->n2 < 5 || p->n3 < 5) return set_err(FRIBO_ERR_ARG, "grid too small (n2, n3 >= 5)");
+    if (p->nsst != 1 && p->nsst != 3) return set_err(FRIBO_ERR_ARG, "nsst must be 1 or 3");
+    if (ldz < p->n3 - 1) return set_err(FRIBO_ERR_ARG, "ldz < n3m");
+    if (p->wfun < 1 || p->wfun > 4) return set_err(FRIBO_ERR_ARG, "wfun must be 1..4");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    fribo_ctx* c = new fribo_ctx();
+    c->p = *p;
+    c->lds_limit = prop.sharedMemPerBlock;
+    FluidDev& F = c->F;
+    memset(&F, 0, sizeof(F));
+    F.n2 = p->n2; F.n3 = p->n3; F.n2m = p->n2 - 1; F.n3m = p->n3 - 1;
+    F.P = (F.n2 + 15) / 16 * 16;
+    F.nk = F.n3m + 6;
+    F.nsst = p->nsst; F.idtv = p->idtv; F.ibact = p->ibact; F.wfun = p->wfun;
+    F.inslws = p->inslws; F.inslwn = p->inslwn;
+    F.alx3 = p->alx3; F.rext2 = p->rext2; F.cou = p->cou; F.q3uw = p->q3uw; F.q3lw = p->q3lw;
+    F.dtmax = p->dtmax; F.cflmax = p->cflmax; F.cfllim = p->cfllim; F.radiusSD = p->radiusSD;
+    // fluid_pre.F90:33-70
+    F.dx2 = 1.0 / (p->rext2 / (double)F.n2m);
+    F.dx3 = 1.0 / (p->alx3 / (double)F.n3m);
+    F.dx2q = F.dx2 * F.dx2;
+    F.dx3q = F.dx3 * F.dx3;
+    F.ren = 1.0 / p->visc;
+    if (p->nsst > 1) {
+        F.gam[1] = 8.0 / 15.0; F.gam[2] = 5.0 / 12.0; F.gam[3] = 3.0 / 4.0;
+        F.rom[1] = 0.0; F.rom[2] = -17.0 / 60.0; F.rom[3] = -5.0 / 12.0;
+    } else {
+        F.gam[1] = 1.5; F.rom[1] = -0.50;
+    }
+    for (int n = 1; n <= p->nsst; ++n) F.alm[n] = F.gam[n] + F.rom[n];
+    if (jsolve_lds_bytes(F.n2) > c->lds_limit) {
+        delete c;
+        return set_err(FRIBO_ERR_LIMIT, "n2 too large for one j-line in LDS");
+    }
+
+    std::vector<int> jmv, jpv, kmv, kpv;
+    host_metrics(c, rc, zz, jmv, jpv, kmv, kpv);
+    for (int k = 1; k <= F.n3m; ++k) c->h1[A_WR][k] = wr[k - 1];
+    // null mode of the Neumann z operator
+    {
+        double wmax = 0.0, wmin = 1e300;
+        int imin = 0;
+        for (int k = 1; k <= F.n3m; ++k) {
+            double a = fabs(wr[k - 1]);
+            wmax = std::max(wmax, a);
+            if (a < wmin) { wmin = a; imin = k; }
+        }
+        F.null_mode = (wmin <= 1e-9 * wmax) ? imin : 0;
+    }
+    int e = 0;
+    double** d1[A_COUNT] = {&F.rc, &F.rm, &F.zz, &F.zm, &F.g2rc, &F.g2rm, &F.g3rc, &F.g3rm, &F.udx2c, &F.udx2m,
+                            &F.udx3c, &F.udx3m, &F.am2j, &F.ac2j, &F.ap2j, &F.am3j, &F.ac3j, &F.ap3j, &F.am3ck,
+                            &F.ac3ck, &F.ap3ck, &F.am3sk, &F.ac3sk, &F.ap3sk, &F.amphj, &F.acphj, &F.apphj,
+                            &F.amphk, &F.acphk, &F.apphk, &F.wr};
+    for (int a = 0; a < A_COUNT && !e; ++a) e = upload1d(c, d1[a], c->h1[a]);
+    if (!e) e = upload1i(c, &F.jmv, jmv);
+    if (!e) e = upload1i(c, &F.jpv, jpv);
+    if (!e) e = upload1i(c, &F.kmv, kmv);
+    if (!e) e = upload1i(c, &F.kpv, kpv);
+    const size_t fsz = (size_t)F.P * F.nk;
+    double** flds[] = {&F.q2, &F.q3, &F.pr, &F.dph, &F.ru2, &F.ru3, &F.rhs2, &F.rhs3, &F.sA, &F.sB, &F.hmn};
+    for (auto f : flds)
+        if (!e) e = dalloc(c, f, fsz);
+    double** bcs[] = {&F.qb2s, &F.qb3s, &F.qb2n, &F.qb3n, &F.dqb2s, &F.dqb3s, &F.dqb2n, &F.dqb3n, &F.dq2x2o, &F.dq3x2o};
+    for (auto b : bcs)
+        if (!e) e = dalloc(c, b, F.n2 + 3);
+    // Poisson matrices, packed with a 16-aligned leading dimension
+    F.ldz = (F.n3m + 15) / 16 * 16;
+    if (!e) e = dalloc(c, &F.zmm, (size_t)F.ldz * F.n3m);
+    if (!e) e = dalloc(c, &F.zmmt, (size_t)F.ldz * F.n3m);
+    if (!e) {
+        HIPCHK(hipMemcpy2D(F.zmm, F.ldz * sizeof(double), zmm, ldz * sizeof(double), F.n3m * sizeof(double), F.n3m,
+                           hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy2D(F.zmmt, F.ldz * sizeof(double), zmmt, ldz * sizeof(double), F.n3m * sizeof(double),
+                           F.n3m, hipMemcpyHostToDevice));
+    }
+    if (!e) e = dalloc(c, &F.cfl_part, 2 * FRIBO_NPART);
+    F.npr_part = ((F.n2m + 63) / 64) * ((F.n3m + 3) / 4);
+    if (!e) e = dalloc(c, &F.pr_part, F.npr_part);
+    if (!e) e = setup_ksys(c, F.ks[0], F.n3m, 1);
+    if (!e) e = setup_ksys(c, F.ks[1], F.n3m - 1, 2);
+    if (!e) e = dalloc(c, &F.kYs, (size_t)2 * FRIBO_KMAXCH * F.n2m);
+    if (!e) e = dalloc(c, &F.kYe, (size_t)2 * FRIBO_KMAXCH * F.n2m);
+    if (!e) e = dalloc(c, &F.kDe, (size_t)2 * FRIBO_KMAXCH * F.n2m);
+    if (!e) e = dalloc(c, &F.kX, (size_t)2 * FRIBO_KMAXCH * F.n2m);
+    if (!e) e = dalloc(c, &F.S, 1);
+    if (!e) {
+        hipError_t he = hipStreamCreate(&c->st);
+        if (he != hipSuccess) e = set_err(FRIBO_ERR_HIP, hipGetErrorString(he));
+    }
+    if (e) {
+        for (void* q : c->allocs) hipFree(q);
+        delete c;
+        return e;
+    }
+    memset(&c->hs, 0, sizeof(c->hs));
+    *out = c;
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_finalize(fribo_handle h)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    hipStreamSynchronize(h->st);
+    for (auto ev : h->evpool) hipEventDestroy(ev);
+    for (void* q : h->mk_allocs) hipFree(q);
+    for (void* q : h->allocs) hipFree(q);
+    hipStreamDestroy(h->st);
+    delete h;
+    return FRIBO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// State transfer (pitched copies between the reference layout and the device)
+static int put_field(fribo_ctx* c, double* dev, const double* host, int kfirst, int nplanes)
+{
+    const FluidDev& F = c->F;
+    HIPCHK(hipMemcpy2DAsync(dev + fidx(F, 1, kfirst), F.P * sizeof(double), host, F.n2 * sizeof(double),
+                            F.n2 * sizeof(double), nplanes, hipMemcpyHostToDevice, c->st));
+    return FRIBO_OK;
+}
+
+static int get_field(fribo_ctx* c, double* host, const double* dev, int kfirst, int nplanes)
+{
+    const FluidDev& F = c->F;
+    HIPCHK(hipMemcpy2DAsync(host, F.n2 * sizeof(double), dev + fidx(F, 1, kfirst), F.P * sizeof(double),
+                            F.n2 * sizeof(double), nplanes, hipMemcpyDeviceToHost, c->st));
+    return FRIBO_OK;
+}
+
+static int read_scalars(fribo_ctx* c)
+{
+    HIPCHK(hipMemcpyAsync(&c->hs, c->F.S, sizeof(DevScalars), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_set_state(fribo_handle h, const double* q2, const double* q3, const double* pr,
+                               const double* ru2, const double* ru3, const double* qb2s, const double* qb3s,
+                               const double* qb2n, const double* qb3n, const double* dq2x2o, const double* dq3x2o)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    FluidDev& F = h->F;
+    int e = 0;
+    if (q2 && !e) e = put_field(h, F.q2, q2, -2, F.nk);
+    if (q3 && !e) e = put_field(h, F.q3, q3, -2, F.nk);
+    if (pr && !e) e = put_field(h, F.pr, pr, -2, F.nk);
+    if (ru2 && !e) e = put_field(h, F.ru2, ru2, 1, F.n3m);
+    if (ru3 && !e) e = put_field(h, F.ru3, ru3, 1, F.n3m);
+    const double* b[] = {qb2s, qb3s, qb2n, qb3n, dq2x2o, dq3x2o};
+    double* d[] = {F.qb2s, F.qb3s, F.qb2n, F.qb3n, F.dq2x2o, F.dq3x2o};
+    for (int i = 0; i < 6 && !e; ++i)
+        if (b[i]) {
+            hipError_t he = hipMemcpyAsync(d[i] + 1, b[i], F.n2 * sizeof(double), hipMemcpyHostToDevice, h->st);
+            if (he != hipSuccess) e = set_err(FRIBO_ERR_HIP, hipGetErrorString(he));
+        }
+    if (e) return e;
+    if (pr) {
+        // the uploaded pressure is already mean-free: nothing pending
+        if ((e = read_scalars(h))) return e;
+        h->hs.pend = 0.0;
+        HIPCHK(hipMemcpyAsync(&F.S->pend, &h->hs.pend, sizeof(double), hipMemcpyHostToDevice, h->st));
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_get_state(fribo_handle h, double* q2, double* q3, double* pr, double* ru2, double* ru3,
+                               double* qb2s, double* qb3s, double* qb2n, double* qb3n, double* dq2x2o,
+                               double* dq3x2o)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    FluidDev& F = h->F;
+    int e = 0;
+    if (q2 && !e) e = get_field(h, q2, F.q2, -2, F.nk);
+    if (q3 && !e) e = get_field(h, q3, F.q3, -2, F.nk);
+    if (pr && !e) e = get_field(h, pr, F.pr, -2, F.nk);
+    if (ru2 && !e) e = get_field(h, ru2, F.ru2, 1, F.n3m);
+    if (ru3 && !e) e = get_field(h, ru3, F.ru3, 1, F.n3m);
+    double* b[] = {qb2s, qb3s, qb2n, qb3n, dq2x2o, dq3x2o};
+    double* d[] = {F.qb2s, F.qb3s, F.qb2n, F.qb3n, F.dq2x2o, F.dq3x2o};
+    for (int i = 0; i < 6 && !e; ++i)
+        if (b[i]) {
+            hipError_t he = hipMemcpyAsync(b[i], d[i] + 1, F.n2 * sizeof(double), hipMemcpyDeviceToHost, h->st);
+            if (he != hipSuccess) e = set_err(FRIBO_ERR_HIP, hipGetErrorString(he));
+        }
+    if (e) return e;
+    if ((e = read_scalars(h))) return e;
+    if (pr) {
+        // apply the pending mean to the interior (updvp.F90:141)
+        const double pend = h->hs.pend;
+        for (int k = 1; k <= F.n3m; ++k)
+            for (int j = 1; j <= F.n2m; ++j) {
+                double& v = pr[(size_t)(j - 1) + (size_t)(k + 2) * F.n2];
+                v = v - pend;
+            }
+    }
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc, const double* lmvel,
+                                 const double* lmA)
+{
+    if (!h || nlm < 0 || (nlm > 0 && (!lmcc || !lmvel || !lmA))) return set_err(FRIBO_ERR_ARG, "bad marker args");
+    FluidDev& F = h->F;
+    if (nlm != h->nlm_alloc) {
+        HIPCHK(hipStreamSynchronize(h->st));
+        for (void* q : h->mk_allocs) hipFree(q);
+        h->mk_allocs.clear();
+        int maxkeys = 1024;
+        while (maxkeys < nlm * FRIBO_DIMSD) maxkeys <<= 1;
+        if ((size_t)maxkeys * sizeof(unsigned long long) > h->lds_limit || nlm >= (1 << 20) / FRIBO_DIMSD)
+            return set_err(FRIBO_ERR_LIMIT, "too many markers for the in-LDS contribution sort");
+        F.maxkeys = maxkeys;
+        auto* L = &h->mk_allocs;
+        int e = 0;
+        if (!e) e = dalloc(h, &F.lmcc, 2 * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.lmvel, 2 * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.lmA, (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.lmind, 4 * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.eulins, 4 * FRIBO_DIMSD * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.phi, 2 * FRIBO_DIMSD * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.phipred, 2 * FRIBO_DIMSD * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.cl, 2 * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.active, (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.contrib, 2 * FRIBO_DIMSD * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.keys, 2 * (size_t)maxkeys, L);
+        if (!e) e = dalloc(h, &F.segstart, 2 * ((size_t)maxkeys + 1), L);
+        if (!e) e = dalloc(h, &F.segcell, 2 * (size_t)maxkeys, L);
+        if (!e) e = dalloc(h, &F.segval, 2 * (size_t)maxkeys, L);
+        if (e) return e;
+        h->nlm_alloc = nlm;
+        F.nlm = nlm;
+    }
+    if (nlm > 0) {
+        HIPCHK(hipMemcpyAsync(F.lmcc, lmcc, 2 * nlm * sizeof(double), hipMemcpyHostToDevice, h->st));
+        HIPCHK(hipMemcpyAsync(F.lmvel, lmvel, 2 * nlm * sizeof(double), hipMemcpyHostToDevice, h->st));
+        HIPCHK(hipMemcpyAsync(F.lmA, lmA, nlm * sizeof(double), hipMemcpyHostToDevice, h->st));
+        h->lmcc.assign(lmcc, lmcc + 2 * (size_t)nlm);
+        HIPCHK(hipStreamSynchronize(h->st));
+    }
+    return FRIBO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Step orchestration
+static hipEvent_t next_event(fribo_ctx* c)
+{
+    if (c->evused == c->evpool.size()) {
+        hipEvent_t ev;
+        hipEventCreate(&ev);
+        c->evpool.push_back(ev);
+    }
+    return c->evpool[c->evused++];
+}
+
+template <class Fn>
+static void run(fribo_ctx* c, int cls, Fn fn)
+{
+    if (!c->profiling) {
+        fn();
+        return;
+    }
+    hipEvent_t a = next_event(c), b = next_event(c);
+    hipEventRecord(a, c->st);
+    fn();
+    hipEventRecord(b, c->st);
+    c->pending.push_back({cls, a, b});
+}
+
+static void collect_profile(fribo_ctx* c)
+{
+    for (auto& r : c->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            c->prof_ms[r.cls] += ms;
+            c->prof_n[r.cls] += 1;
+        }
+    }
+    c->pending.clear();
+    c->evused = 0;
+}
+
+static int markers_inside(fribo_ctx* c)
+{
+    // lgrdistribution.F90:27-36
+    const int nlm = c->F.nlm;
+    if (!c->p.ibact || nlm <= 0) return FRIBO_OK;
+    double ymax = -1e300, zmax = -1e300, ymin = 1e300, zmin = 1e300;
+    for (int i = 0; i < nlm; ++i) {
+        ymax = std::max(ymax, c->lmcc[i]);
+        ymin = std::min(ymin, c->lmcc[i]);
+        zmax = std::max(zmax, c->lmcc[nlm + i]);
+        zmin = std::min(zmin, c->lmcc[nlm + i]);
+    }
+    if (ymax > c->p.rext2 || ymin < 0.0 || zmax > c->p.alx3 || zmin < 0.0)
+        return set_err(FRIBO_ERR_MARKER_OUTSIDE, "a Lagrangian marker lies outside the domain");
+    return FRIBO_OK;
+}
+
+static void enqueue_cfl(fribo_ctx* c, int ns, double dt, double dt_o)
+{
+    FluidDev& F = c->F;
+    run(c, PC_CFL, [&] { launch_cfl(F, c->st); });
+    (void)ns; (void)dt; (void)dt_o;
+}
+
+static void enqueue_substep(fribo_ctx* c, int n, int ns, double dt_in, double dt_o)
+{
+    FluidDev& F = c->F;
+    hipStream_t st = c->st;
+    run(c, PC_BC, [&] { launch_bc(F, n, ns, dt_in, dt_o, st); });
+    run(c, PC_RHS, [&] { launch_rhs(F, st); });
+    if (F.ibact && F.nlm > 0) {
+        if (n == 1) run(c, PC_MLS_SETUP, [&] { launch_mls_setup(F, st); });
+        run(c, PC_MLS_FORCE, [&] { launch_mls_force(F, st); });
+    }
+    run(c, PC_JH, [&] { launch_jsolve_helm(F, st); });
+    run(c, PC_K, [&] { launch_ksolve(F, st); });
+    run(c, PC_DIV, [&] { launch_div(F, st); });
+    // phcalc_outflow: hmn = dph * zmmt^T ; per-mode j solves ; dph = V * zmm^T
+    run(c, PC_GEMM, [&] {
+        launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.dph + fidx(F, 1, 1), F.P, F.zmmt, F.ldz, F.hmn, F.P, F.S, st);
+    });
+    run(c, PC_JP, [&] { launch_jsolve_poisson(F, st); });
+    run(c, PC_GEMM, [&] {
+        launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.hmn, F.P, F.zmm, F.ldz, F.dph + fidx(F, 1, 1), F.P, F.S, st);
+    });
+    run(c, PC_UPD, [&] { launch_upd(F, st); });
+    run(c, PC_MEAN, [&] { launch_mean(F, st); });
+}
+
+static int reset_err(fribo_ctx* c)
+{
+    c->zero_int = 0;
+    HIPCHK(hipMemcpyAsync(&c->F.S->err, &c->zero_int, sizeof(int), hipMemcpyHostToDevice, c->st));
+    return FRIBO_OK;
+}
+
+static int finish(fribo_ctx* c, double* dt, double* cflm)
+{
+    int e = read_scalars(c);
+    if (e) return e;
+    HIPCHK(hipGetLastError());
+    if (c->profiling) collect_profile(c);
+    if (dt) *dt = c->hs.dt;
+    if (cflm) *cflm = c->hs.cflm;
+    switch (c->hs.err) {
+    case 0: return FRIBO_OK;
+    case FRIBO_E_DT_TOO_SMALL: return set_err(FRIBO_ERR_DT_TOO_SMALL, "Too small time step size");
+    case FRIBO_E_CFL_TOO_LARGE: return set_err(FRIBO_ERR_CFL_TOO_LARGE, "Too large peak CFL for stability");
+    case FRIBO_E_PARTITION_UNITY: return set_err(FRIBO_ERR_PARTITION_UNITY, "partition of unity is not satisfied");
+    default: return set_err(FRIBO_ERR_HIP, "device error flag " + std::to_string(c->hs.err));
+    }
+}
+
+extern "C" int fribo_fluid_tsch(fribo_handle h, int32_t ns, double* dt, double dt_o, double* cflm)
+{
+    if (!h || !dt) return set_err(FRIBO_ERR_ARG, "null argument");
+    int e = markers_inside(h);
+    if (e) return e;
+    if ((e = reset_err(h))) return e;
+    const double dt_in = *dt;
+    enqueue_cfl(h, ns, dt_in, dt_o);
+    for (int n = 1; n <= h->F.nsst; ++n) enqueue_substep(h, n, ns, dt_in, dt_o);
+    return finish(h, dt, cflm);
+}
+
+// The split form keeps the values fluid_tsch carries between its parts.
+static thread_local double g_split_dt = 0.0, g_split_dto = 0.0;
+
+extern "C" int fribo_cfl_dt(fribo_handle h, int32_t ns, double* dt, double dt_o, double* cflm)
+{
+    if (!h || !dt) return set_err(FRIBO_ERR_ARG, "null argument");
+    int e = reset_err(h);
+    if (e) return e;
+    enqueue_cfl(h, ns, *dt, dt_o);
+    g_split_dt = *dt;
+    g_split_dto = dt_o;
+    // finalize the CFL / dt control without running a substep: launch the
+    // boundary kernel for substep 1 (it also prepares substep 1's factors)
+    run(h, PC_BC, [&] { launch_bc(h->F, 1, ns, *dt, dt_o, h->st); });
+    return finish(h, dt, cflm);
+}
+
+extern "C" int fribo_substep(fribo_handle h, int32_t n, int32_t ns)
+{
+    if (!h || n < 1 || n > h->F.nsst) return set_err(FRIBO_ERR_ARG, "bad substep");
+    int e = 0;
+    if (n == 1 && (e = markers_inside(h))) return e;
+    FluidDev& F = h->F;
+    hipStream_t st = h->st;
+    // substep 1's boundary kernel already ran in fribo_cfl_dt; for n == 1 run
+    // the body with the stored dt (launch_bc with n > 1 semantics would
+    // redo boucdq), so the body starts after k_bc.
+    if (n == 1) {
+        run(h, PC_RHS, [&] { launch_rhs(F, st); });
+        if (F.ibact && F.nlm > 0) {
+            run(h, PC_MLS_SETUP, [&] { launch_mls_setup(F, st); });
+            run(h, PC_MLS_FORCE, [&] { launch_mls_force(F, st); });
+        }
+        run(h, PC_JH, [&] { launch_jsolve_helm(F, st); });
+        run(h, PC_K, [&] { launch_ksolve(F, st); });
+        run(h, PC_DIV, [&] { launch_div(F, st); });
+        run(h, PC_GEMM, [&] {
+            launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.dph + fidx(F, 1, 1), F.P, F.zmmt, F.ldz, F.hmn, F.P, F.S, st);
+        });
+        run(h, PC_JP, [&] { launch_jsolve_poisson(F, st); });
+        run(h, PC_GEMM, [&] {
+            launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.hmn, F.P, F.zmm, F.ldz, F.dph + fidx(F, 1, 1), F.P, F.S, st);
+        });
+        run(h, PC_UPD, [&] { launch_upd(F, st); });
+        run(h, PC_MEAN, [&] { launch_mean(F, st); });
+    } else {
+        enqueue_substep(h, n, ns, g_split_dt, g_split_dto);
+    }
+    return finish(h, nullptr, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Named array access
+extern "C" long fribo_array_size(fribo_handle h, const char* name)
+{
+    if (!h || !name) return 0;
+    const FluidDev& F = h->F;
+    std::string s(name);
+    const long g = (long)F.n2 * (F.n3m + 6), r = (long)F.n2 * F.n3m;
+    if (s == "q2" || s == "q3" || s == "pr" || s == "dph") return g;
+    if (s == "ru2" || s == "ru3" || s == "rhs2" || s == "rhs3") return r;
+    if (s == "fey" || s == "fez") return (long)F.n2 * (F.n3m + 5);
+    if (s == "qb2s" || s == "qb3s" || s == "qb2n" || s == "qb3n" || s == "dqb2s" || s == "dqb3s" || s == "dqb2n" ||
+        s == "dqb3n" || s == "dq2x2o" || s == "dq3x2o")
+        return F.n2;
+    for (int a = 0; a < A_COUNT; ++a)
+        if (s == kArrNames[a]) return is_zdir(a) ? (a == A_WR ? F.n3m : F.n3) : F.n2;
+    if (s == "lmind") return 4L * F.nlm;
+    if (s == "eulins") return 4L * FRIBO_DIMSD * F.nlm;
+    if (s == "phi" || s == "phipred") return 2L * FRIBO_DIMSD * F.nlm;
+    if (s == "cl") return 2L * F.nlm;
+    if (s == "scalars") return 12;
+    return 0;
+}
+
+extern "C" int fribo_get_array(fribo_handle h, const char* name, void* out, long n)
+{
+    if (!h || !name || !out) return set_err(FRIBO_ERR_ARG, "null argument");
+    long need = fribo_array_size(h, name);
+    if (need == 0 || n < need) return set_err(FRIBO_ERR_ARG, std::string("unknown array or short buffer: ") + name);
+    FluidDev& F = h->F;
+    std::string s(name);
+    int e = 0;
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (s == "scalars") {
+        if ((e = read_scalars(h))) return e;
+        double* o = (double*)out;
+        const DevScalars& S = h->hs;
+        double v[12] = {S.dt, S.beta, S.al, S.ga, S.ro, S.cflm, S.q3avg, S.qinf, S.qout, F.dx2, F.dx3, F.ren};
+        memcpy(o, v, sizeof(v));
+        return FRIBO_OK;
+    }
+    if (s == "q2" || s == "q3" || s == "dph") {
+        double* src = s == "q2" ? F.q2 : (s == "q3" ? F.q3 : F.dph);
+        e = get_field(h, (double*)out, src, -2, F.nk);
+    } else if (s == "pr") {
+        return fribo_get_state(h, nullptr, nullptr, (double*)out, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, nullptr, nullptr);
+    } else if (s == "ru2" || s == "ru3" || s == "rhs2" || s == "rhs3") {
+        double* src = s == "ru2" ? F.ru2 : (s == "ru3" ? F.ru3 : (s == "rhs2" ? F.rhs2 : F.rhs3));
+        e = get_field(h, (double*)out, src, 1, F.n3m);
+    } else if (s == "fey" || s == "fez") {
+        const int d = (s == "fey") ? 0 : 1;
+        double* o = (double*)out;
+        std::fill(o, o + need, 0.0);
+        if ((e = read_scalars(h))) return e;
+        const int nseg = h->hs.nseg[d];
+        if (F.nlm > 0 && F.ibact && nseg > 0) {
+            std::vector<int> cell(nseg);
+            std::vector<double> val(nseg);
+            HIPCHK(hipMemcpy(cell.data(), F.segcell + (size_t)d * F.maxkeys, nseg * sizeof(int), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(val.data(), F.segval + (size_t)d * F.maxkeys, nseg * sizeof(double), hipMemcpyDeviceToHost));
+            for (int i = 0; i < nseg; ++i) {
+                int j = cell[i] % F.n2 + 1, k = cell[i] / F.n2 + 1;
+                o[(size_t)(j - 1) + (size_t)(k + 2) * F.n2] = val[i];
+            }
+        }
+        return FRIBO_OK;
+    } else if (s == "lmind" || s == "eulins") {
+        int* src = s == "lmind" ? F.lmind : F.eulins;
+        HIPCHK(hipMemcpy(out, src, need * sizeof(int), hipMemcpyDeviceToHost));
+    } else if (s == "phi" || s == "phipred" || s == "cl") {
+        double* src = s == "phi" ? F.phi : (s == "phipred" ? F.phipred : F.cl);
+        HIPCHK(hipMemcpy(out, src, need * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        double* bsrc = nullptr;
+        const char* bn[] = {"qb2s", "qb3s", "qb2n", "qb3n", "dqb2s", "dqb3s", "dqb2n", "dqb3n", "dq2x2o", "dq3x2o"};
+        double* bd[] = {F.qb2s, F.qb3s, F.qb2n, F.qb3n, F.dqb2s, F.dqb3s, F.dqb2n, F.dqb3n, F.dq2x2o, F.dq3x2o};
+        for (int i = 0; i < 10; ++i)
+            if (s == bn[i]) bsrc = bd[i];
+        if (bsrc) {
+            HIPCHK(hipMemcpy(out, bsrc + 1, F.n2 * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            for (int a = 0; a < A_COUNT; ++a)
+                if (s == kArrNames[a]) memcpy(out, h->h1[a].data() + 1, need * sizeof(double));
+        }
+    }
+    if (e) return e;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_sync(fribo_handle h)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->st));
+    return FRIBO_OK;
+}
+
+extern "C" const char* fribo_last_error(void) { return g_err.c_str(); }
+
+extern "C" int fribo_test_gemm(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
+                               int32_t ldb, double* C, int32_t ldc)
+{
+    if (M <= 0 || N <= 0 || K <= 0 || lda < M || ldb < N || ldc < M || !A || !B || !C)
+        return set_err(FRIBO_ERR_ARG, "bad gemm arguments");
+    double *dA, *dB, *dC;
+    HIPCHK(hipMalloc(&dA, (size_t)lda * K * sizeof(double)));
+    HIPCHK(hipMalloc(&dB, (size_t)ldb * K * sizeof(double)));
+    HIPCHK(hipMalloc(&dC, (size_t)ldc * N * sizeof(double)));
+    HIPCHK(hipMemcpy(dA, A, (size_t)lda * K * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dB, B, (size_t)ldb * K * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dC, C, (size_t)ldc * N * sizeof(double), hipMemcpyHostToDevice));
+    launch_gemm_nt(M, N, K, dA, lda, dB, ldb, dC, ldc, nullptr, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(C, dC, (size_t)ldc * N * sizeof(double), hipMemcpyDeviceToHost));
+    hipFree(dA);
+    hipFree(dB);
+    hipFree(dC);
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_set_profiling(fribo_handle h, int32_t on)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    h->profiling = on != 0;
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_get_profile(fribo_handle h, double* ms, int32_t* launches, int32_t nclass)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    for (int i = 0; i < nclass && i < FRIBO_NPROF; ++i) {
+        if (ms) ms[i] = h->prof_ms[i];
+        if (launches) launches[i] = h->prof_n[i];
+    }
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_reset_profile(fribo_handle h)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    for (int i = 0; i < FRIBO_NPROF; ++i) {
+        h->prof_ms[i] = 0.0;
+        h->prof_n[i] = 0;
+    }
+    return FRIBO_OK;
+}
+
+extern "C" const char* fribo_profile_name(int32_t cls)
+{
+    return (cls >= 0 && cls < FRIBO_NPROF) ? kProfNames[cls] : "";
+}

@@ -1,0 +1,129 @@
+// Internal definitions shared by the gfx950 kernels of libfribo_gpu.
+//
+// Device layout: every 2-D field uses the reference's logical extents
+// (1:n2, -2:n3m+3) with j fastest and a padded row pitch P (multiple of 16
+// doubles, 128-byte rows), so element (j,k) lives at (j-1) + (k+2)*P.
+// 1-D grid arrays are 1-based (index 0 unused) exactly like the Fortran
+// module arrays they mirror.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FRIBO_DIMSD 9
+#define FRIBO_KMAXCH 64      // max chunks of a partitioned k-line (one wave)
+#define FRIBO_NPART 512      // partial sums of the CFL reduction
+
+struct DevScalars {
+    double dt, dt_o, beta, al, ga, ro, alre, betadx;
+    double cflm, q3avg;
+    double pend;             // pressure mean not yet subtracted from the interior
+    double qinf, qout, area;
+    double ugkks, ugkkn;
+    int err;                 // first error code raised on the device
+    int nseg[2];             // MLS segments per direction
+    int nent[2];             // valid sorted MLS entries per direction
+    int pad;
+};
+
+// Precomputed, line-invariant factors of one partitioned k-direction system
+// (rows r = 0..n-1, normalised diagonal 1).  Chunk p covers rows
+// [cs[p], cs[p+1]); its last row is the separator X_p.
+struct KSys {
+    int n;                   // rows of the system
+    int nch;                 // chunks (<= FRIBO_KMAXCH)
+    int kofs;                // k index of row 0
+    int maxm;                // largest chunk length
+    int* cs;                 // [nch+1] chunk starts
+    double* a;               // [n] sub-diagonal (0 at row 0)
+    double* c;               // [n] super-diagonal (0 at row n-1)
+    double* S;               // [n] rhs scaling 1/(1 - betadx*ac)
+    double* cp;              // [n] forward multipliers c'
+    double* inv;             // [n] 1/pivot
+    double* vp;              // [n] forward left-coupling vector v'
+    double* g;               // [n] weights giving y_s from y' (sum g_i y'_i)
+    double* wl;              // [nch] w' at the last interior row
+    double* k1;              // [6*64] PCR multipliers (lower)
+    double* k2;              // [6*64] PCR multipliers (upper)
+    double* ib;              // [64] 1/beta after PCR
+    double* bc;              // [2] couplings of rows 0 / n-1 to the Dirichlet ends
+};
+
+struct FluidDev {
+    int n2, n3, n2m, n3m;
+    int P;                   // row pitch (doubles)
+    int nk;                  // planes (n3m+6)
+    int nsst, idtv, ibact, wfun, inslws, inslwn;
+    int nlm;
+    int null_mode;           // 1-based index of the singular Poisson mode, 0 if none
+    double alx3, rext2, ren, cou, q3uw, q3lw, dtmax, cflmax, cfllim, radiusSD;
+    double dx2, dx3, dx2q, dx3q;
+    double gam[4], rom[4], alm[4];
+    // fields (ghosted layout)
+    double *q2, *q3, *pr, *dph, *ru2, *ru3, *rhs2, *rhs3, *sA, *sB;
+    // Poisson transform
+    double *hmn;             // n2m x n3m, leading dimension P
+    double *zmm, *zmmt;      // n3m x n3m, leading dimension ldz
+    int ldz;
+    // outflow arrays (1-based, n2+2)
+    double *qb2s, *qb3s, *qb2n, *qb3n, *dqb2s, *dqb3s, *dqb2n, *dqb3n, *dq2x2o, *dq3x2o;
+    // 1-D grid (1-based)
+    double *rc, *rm, *zz, *zm, *g2rc, *g2rm, *g3rc, *g3rm, *udx2c, *udx2m, *udx3c, *udx3m;
+    double *am2j, *ac2j, *ap2j, *am3j, *ac3j, *ap3j;
+    double *am3ck, *ac3ck, *ap3ck, *am3sk, *ac3sk, *ap3sk;
+    double *amphj, *acphj, *apphj, *amphk, *acphk, *apphk, *wr;
+    int *jmv, *jpv, *kmv, *kpv;
+    // reductions
+    double *cfl_part;        // [2*FRIBO_NPART]
+    double *pr_part;         // [npr_part]
+    int npr_part;
+    // k-direction systems: [0] q2, [1] q3
+    KSys ks[2];
+    double *kYs, *kYe, *kDe, *kX;  // [2][64][n2m] reduced-system data
+    // markers
+    double *lmcc, *lmvel, *lmA;    // (nlm,2) column-major
+    int *lmind;                    // (nlm,2,2)
+    int *eulins;                   // (nlm,2,2,9)
+    double *phi, *phipred, *cl;    // (nlm,2,9), (nlm,2)
+    int *active;                   // [nlm]
+    double *contrib;               // [2][nlm*9]
+    unsigned long long *keys;      // [2][maxkeys]
+    int *segstart;                 // [2][maxkeys+1]
+    int *segcell;                  // [2][maxkeys]
+    double *segval;                // [2][maxkeys]
+    int maxkeys;
+    DevScalars* S;
+};
+
+__host__ __device__ inline size_t fidx(const FluidDev& F, int j, int k)
+{
+    return (size_t)(j - 1) + (size_t)(k + 2) * (size_t)F.P;
+}
+
+// Error codes (match include/fribo_gpu.h)
+enum {
+    FRIBO_E_OK = 0,
+    FRIBO_E_DT_TOO_SMALL = 1,
+    FRIBO_E_CFL_TOO_LARGE = 2,
+    FRIBO_E_MARKER_OUTSIDE = 3,
+    FRIBO_E_PARTITION_UNITY = 4,
+    FRIBO_E_ARG = 5,
+    FRIBO_E_HIP = 6,
+    FRIBO_E_LIMIT = 7,
+};
+
+// Kernel launchers (each .hip file owns its kernels).
+void launch_cfl(const FluidDev& F, hipStream_t st);
+void launch_bc(const FluidDev& F, int n, int ns, double dt_in, double dt_o, hipStream_t st);
+void launch_rhs(const FluidDev& F, hipStream_t st);
+void launch_div(const FluidDev& F, hipStream_t st);
+void launch_upd(const FluidDev& F, hipStream_t st);
+void launch_mean(const FluidDev& F, hipStream_t st);
+void launch_mls_setup(const FluidDev& F, hipStream_t st);
+void launch_mls_force(const FluidDev& F, hipStream_t st);
+void launch_jsolve_helm(const FluidDev& F, hipStream_t st);
+void launch_jsolve_poisson(const FluidDev& F, hipStream_t st);
+void launch_ksolve(const FluidDev& F, hipStream_t st);
+void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
+                    double* C, int ldc, const DevScalars* S, hipStream_t st);
+size_t jsolve_lds_bytes(int n);

@@ -1,0 +1,398 @@
+// Moving-least-squares immersed-boundary kernels (gfx950).
+//
+// Substep 1 of every step (fluid_tsch.F90:116-118, forcing_MLS.F90:59-96):
+//   k_mls_idx   lgrdistribution.F90:39-75  nearest Eulerian nodes of every
+//               marker (Fortran MINLOC: first index of the minimum), one wave
+//               per (marker, search), exact FP64 |x - p| comparisons
+//   k_mls_sup   MLS_SD_ind (MLSint.F90:4-104), MLS_SF + weight_function +
+//               compute_AB + GEsolve (MLSint.F90:115-166,284-486,
+//               utils_math.f90:637-725) and the momentum coefficient cl
+//               (forcing_MLS.F90:72-81); one thread per (marker, direction)
+//   k_mls_sort  orders the 9 contributions of every marker by (cell, marker)
+//               (bitonic sort in LDS) and cuts the sorted list into per-cell
+//               segments
+// Every substep (interp_spread, forcing_MLS.F90:135-182):
+//   k_mls_interp  u_m = sum phi q*f, f = (V - u_m)/dt, contribution cl*phi*f
+//   k_mls_spread  one thread per touched cell sums its contributions in
+//                 marker order (the order the reference accumulates fey/fez
+//                 in), zeroes plane k = n3m (the single-rank ghost update of
+//                 mpi_routines.F90:324-339) and adds f*dt to rhs2/rhs3
+//                 (helsolve.F90:20-49).  No atomics, bit-reproducible.
+#include "fribo_internal.h"
+
+#define LMCC(n, c) F.lmcc[(size_t)((n) - 1) + (size_t)((c) - 1) * F.nlm]
+#define LMVEL(n, c) F.lmvel[(size_t)((n) - 1) + (size_t)((c) - 1) * F.nlm]
+#define LMIND(n, d, c) F.lmind[(size_t)((n) - 1) + (size_t)((d) - 1) * F.nlm + (size_t)((c) - 1) * 2 * F.nlm]
+#define EULINS(n, d, c, m)                                                                   \
+    F.eulins[(size_t)((n) - 1) + (size_t)((d) - 1) * F.nlm + (size_t)((c) - 1) * 2 * F.nlm + \
+             (size_t)((m) - 1) * 4 * F.nlm]
+#define PHI(n, d, m) F.phi[(size_t)((n) - 1) + (size_t)((d) - 1) * F.nlm + (size_t)((m) - 1) * 2 * F.nlm]
+#define PHIPRED(n, d, m) F.phipred[(size_t)((n) - 1) + (size_t)((d) - 1) * F.nlm + (size_t)((m) - 1) * 2 * F.nlm]
+#define CL(n, d) F.cl[(size_t)((n) - 1) + (size_t)((d) - 1) * F.nlm]
+
+static const unsigned long long KEY_NONE = ~0ull;
+#define ENT_BITS 20
+#define ENT_MASK ((1ull << ENT_BITS) - 1)
+
+// Fortran MINLOC(|x(1:len) - v|) over a 1-based array, one wave.
+__device__ int wave_minloc(const double* x, int len, double v)
+{
+    const int lane = threadIdx.x;
+    double best = 0.0;
+    int bi = 0x7fffffff;
+    for (int i = lane + 1; i <= len; i += 64) {
+        double a = fabs(x[i] - v);
+        if (bi == 0x7fffffff || a < best) {
+            best = a;
+            bi = i;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        double ob = __shfl_xor(best, off, 64);
+        int oi = __shfl_xor(bi, off, 64);
+        bool take = (oi != 0x7fffffff) && (bi == 0x7fffffff || ob < best || (ob == best && oi < bi));
+        if (take) {
+            best = ob;
+            bi = oi;
+        }
+    }
+    return bi;
+}
+
+__global__ __launch_bounds__(64) void k_mls_idx(const FluidDev F)
+{
+    if (F.S->err) return;
+    const int n = blockIdx.x + 1, s = blockIdx.y;
+    const double y = LMCC(n, 1), z = LMCC(n, 2);
+    int ind;
+    if (s == 0) {
+        ind = wave_minloc(F.rc, F.n2, y);
+    } else if (s == 1) {
+        if (z < F.zm[1] && z > 0.0) ind = 1;
+        else if (z > F.zm[F.n3m] && z < F.alx3) ind = F.n3m;
+        else ind = wave_minloc(F.zm, F.n3, z);
+    } else if (s == 2) {
+        if (y < F.rm[1] && y > 0.0) ind = 1;
+        else if (y > F.rm[F.n2m] && y < F.rext2) ind = F.n2m;
+        else ind = wave_minloc(F.rm, F.n2, y);
+    } else {
+        ind = wave_minloc(F.zz, F.n3, z);
+    }
+    if (threadIdx.x == 0) {
+        int d = (s < 2) ? 1 : 2, c = (s & 1) + 1;
+        LMIND(n, d, c) = ind;
+    }
+}
+
+// utils_math.f90:637-725, 3x3 scaled partial pivoting; returns info
+__device__ int gesolve3(const double ai[3][3], const double bi[3], double x[3])
+{
+    double a[3][3], b[3], sc[3];
+    for (int i = 0; i < 3; ++i) {
+        b[i] = bi[i];
+        for (int j = 0; j < 3; ++j) a[i][j] = ai[i][j];
+    }
+    for (int k = 0; k < 2; ++k) {
+        for (int i = k; i < 3; ++i) {
+            sc[i] = 0.0;
+            for (int j = k; j < 3; ++j) sc[i] = fmax(sc[i], fabs(a[i][j]));
+        }
+        double pivot = fabs(a[k][k] / sc[k]);
+        int l = k;
+        for (int j = k + 1; j < 3; ++j)
+            if (fabs(a[j][k] / sc[j]) > pivot) {
+                pivot = fabs(a[j][k] / sc[j]);
+                l = j;
+            }
+        if (pivot == 0.0) return 0;
+        if (l != k) {
+            for (int j = k; j < 3; ++j) {
+                double st = a[k][j];
+                a[k][j] = a[l][j];
+                a[l][j] = st;
+            }
+            double st = b[k];
+            b[k] = b[l];
+            b[l] = st;
+        }
+        for (int i = k + 1; i < 3; ++i) {
+            double c = a[i][k] / a[k][k];
+            a[i][k] = 0.0;
+            b[i] = b[i] - c * b[k];
+            for (int j = k + 1; j < 3; ++j) a[i][j] = a[i][j] - c * a[k][j];
+        }
+    }
+    x[2] = b[2] / a[2][2];
+    for (int i = 1; i >= 0; --i) {
+        double c = 0.0;
+        for (int j = i + 1; j < 3; ++j) c = c + a[i][j] * x[j];
+        x[i] = (b[i] - c) / a[i][i];
+    }
+    return 1;
+}
+
+// MLSint.F90:406-486
+__device__ double weight1d(double dif, double sdd, int wfun, double radiusSD)
+{
+    const double wcon = 0.3;
+    double sdds = radiusSD * sdd;
+    double rcv = fabs(dif) / sdds;
+    double rc2 = rcv * rcv, rc3 = rcv * rcv * rcv, rc4 = rcv * rcv * rcv * rcv;
+    if (wfun == 1) {
+        if (rcv > 1.0) return 0.0;
+        if (rcv <= 0.5) return (2.0 / 3.0) - 4.0 * rc2 + 4.0 * rc3;
+        return (4.0 / 3.0) - 4.0 * rcv + 4.0 * rc2 - (4.0 / 3.0) * rc3;
+    } else if (wfun == 2) {
+        double t = rcv / wcon;
+        return (rcv <= 1.0) ? exp(-(t * t)) : 0.0;
+    } else if (wfun == 3) {
+        return (rcv <= 1.0) ? 1.0 - 6.0 * rc2 + 8.0 * rc3 - 3.0 * rc4 : 0.0;
+    } else if (wfun == 4) {
+        if (rcv > 1.0) return 0.0;
+        if (rcv < 0.5) return 1.0 / 3.0 * (1.0 + sqrt(-3.0 * rc2 + 1.0));
+        double om = 1.0 - rcv;
+        return 1.0 / 6.0 * (5.0 - 3.0 * rcv - sqrt(-3.0 * (om * om) + 1.0));
+    }
+    return 0.0;
+}
+
+__global__ __launch_bounds__(64) void k_mls_sup(const FluidDev F)
+{
+    DevScalars* S = F.S;
+    if (S->err) return;
+    const int tid = blockIdx.x * 64 + threadIdx.x;
+    if (tid >= 2 * F.nlm) return;
+    const int n = tid % F.nlm + 1, d = tid / F.nlm + 1;
+    const double pos[2] = {LMCC(n, 1), LMCC(n, 2)};
+    unsigned long long* keys = F.keys + (size_t)(d - 1) * F.maxkeys;
+    const bool act = (F.zz[1] < pos[1]) && (pos[1] < F.zz[F.n3]);
+    if (d == 1) F.active[n - 1] = act ? 1 : 0;
+    if (!act) {
+        for (int m = 1; m <= FRIBO_DIMSD; ++m) {
+            PHI(n, d, m) = 0.0;
+            PHIPRED(n, d, m) = 0.0;
+            keys[(size_t)(n - 1) * FRIBO_DIMSD + (m - 1)] = KEY_NONE;
+        }
+        CL(n, d) = 0.0;
+        return;
+    }
+    // MLS_SD_ind
+    const int i1 = LMIND(n, d, 1), i2 = LMIND(n, d, 2);
+    const double* ya = (d == 1) ? F.rc : F.rm;
+    const double* za = (d == 1) ? F.zm : F.zz;
+    double sdc[2][FRIBO_DIMSD], sdd[2][FRIBO_DIMSD];
+    int eul[2][FRIBO_DIMSD];
+    int m = 0;
+    for (int j = 1; j <= 3; ++j)
+        for (int k = 1; k <= 3; ++k) {
+            int jj = (i1 == 1) ? j + 1 : ((i1 == F.n2m) ? j - 1 : j);
+            int kk = (i2 == 1) ? k + 1 : ((i2 == F.n3m) ? k - 1 : k);
+            int ej = i1 + jj - 2, ek = i2 + kk - 2;
+            sdc[0][m] = ya[ej];
+            sdd[0][m] = ya[ej + 1] - ya[ej];
+            sdc[1][m] = za[ek];
+            sdd[1][m] = za[ek + 1] - za[ek];
+            eul[0][m] = ej;
+            eul[1][m] = ek;
+            ++m;
+        }
+    // MLS_SF
+    double w[FRIBO_DIMSD], A[3][3], pk[3][FRIBO_DIMSD];
+    for (int i = 0; i < FRIBO_DIMSD; ++i) {
+        pk[0][i] = 1.0;
+        pk[1][i] = sdc[0][i];
+        pk[2][i] = sdc[1][i];
+        w[i] = weight1d(pos[0] - sdc[0][i], sdd[0][i], F.wfun, F.radiusSD) *
+               weight1d(pos[1] - sdc[1][i], sdd[1][i], F.wfun, F.radiusSD);
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) A[a][b] = 0.0;
+    for (int i = 0; i < FRIBO_DIMSD; ++i)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) A[a][b] = A[a][b] + w[i] * (pk[a][i] * pk[b][i]);
+    const double gps[3] = {1.0, pos[0], pos[1]};
+    double ptA[3] = {0.0, 0.0, 0.0};
+    gesolve3(A, gps, ptA);
+    double phl[FRIBO_DIMSD], sumphi = 0.0;
+    for (int i = 0; i < FRIBO_DIMSD; ++i)
+        phl[i] = ptA[0] * (pk[0][i] * w[i]) + ptA[1] * (pk[1][i] * w[i]) + ptA[2] * (pk[2][i] * w[i]);
+    for (int i = 0; i < FRIBO_DIMSD; ++i) sumphi = sumphi + phl[i];
+    if (fabs(1.0 - fabs(sumphi)) > 1.0e-10) atomicCAS(&S->err, 0, FRIBO_E_PARTITION_UNITY);
+    // forcing_MLS.F90:72-81
+    double dvel = 0.0, lgr_h = 0.0;
+    for (int i = 0; i < FRIBO_DIMSD; ++i) {
+        double vol = sdd[0][i] * sdd[1][i];
+        dvel = dvel + phl[i] * vol;
+        lgr_h = lgr_h + 1.0 / 2.0 * phl[i] * (sdd[0][i] + sdd[1][i]);
+    }
+    CL(n, d) = F.lmA[n - 1] * lgr_h / dvel;
+    for (int i = 0; i < FRIBO_DIMSD; ++i) {
+        PHI(n, d, i + 1) = phl[i];
+        PHIPRED(n, d, i + 1) = phl[i];
+        EULINS(n, d, 1, i + 1) = eul[0][i];
+        EULINS(n, d, 2, i + 1) = eul[1][i];
+        // supports past the last node exist only for markers within half a
+        // cell of the top wall / outflow, where the reference writes outside
+        // its arrays; they are not spread
+        const bool inb = eul[0][i] >= 1 && eul[0][i] <= F.n2 && eul[1][i] >= 1 && eul[1][i] <= F.n3;
+        unsigned long long cell = (unsigned long long)(eul[0][i] - 1) + (unsigned long long)(eul[1][i] - 1) * F.n2;
+        keys[(size_t)(n - 1) * FRIBO_DIMSD + i] =
+            inb ? ((cell << ENT_BITS) | (unsigned long long)((n - 1) * FRIBO_DIMSD + i)) : KEY_NONE;
+    }
+}
+
+// One workgroup per direction: bitonic sort of the keys in LDS, then
+// segment heads by a block-wide scan.
+__global__ __launch_bounds__(1024) void k_mls_sort(const FluidDev F)
+{
+    extern __shared__ unsigned long long skeys[];
+    __shared__ int scan[1024];
+    __shared__ int base_sh;
+    DevScalars* S = F.S;
+    if (S->err) return;
+    const int d = blockIdx.x;
+    const int t = threadIdx.x;
+    const int nk = F.maxkeys;              // power of two
+    const int nreal = F.nlm * FRIBO_DIMSD;
+    unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
+    for (int i = t; i < nk; i += 1024) skeys[i] = (i < nreal) ? gk[i] : KEY_NONE;
+    __syncthreads();
+    for (int size = 2; size <= nk; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = t; i < nk; i += 1024) {
+                int pj = i ^ stride;
+                if (pj > i) {
+                    bool up = ((i & size) == 0);
+                    unsigned long long a = skeys[i], b = skeys[pj];
+                    if ((a > b) == up) {
+                        skeys[i] = b;
+                        skeys[pj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = t; i < nk; i += 1024) gk[i] = skeys[i];
+    // segments
+    int* sst = F.segstart + (size_t)d * (F.maxkeys + 1);
+    int* scl = F.segcell + (size_t)d * F.maxkeys;
+    if (t == 0) base_sh = 0;
+    __syncthreads();
+    int nvalid_local = 0;
+    for (int c0 = 0; c0 < nk; c0 += 1024) {
+        int i = c0 + t;
+        unsigned long long k = (i < nk) ? skeys[i] : KEY_NONE;
+        int head = 0;
+        if (k != KEY_NONE) {
+            ++nvalid_local;
+            head = (i == 0) || ((skeys[i - 1] >> ENT_BITS) != (k >> ENT_BITS));
+        }
+        scan[t] = head;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int v = (t >= off) ? scan[t - off] : 0;
+            __syncthreads();
+            scan[t] += v;
+            __syncthreads();
+        }
+        int base = base_sh;
+        if (head) {
+            int sid = base + scan[t] - 1;
+            sst[sid] = i;
+            scl[sid] = (int)(k >> ENT_BITS);
+        }
+        __syncthreads();
+        if (t == 1023) base_sh = base + scan[1023];
+        __syncthreads();
+    }
+    // count valid entries
+    scan[t] = nvalid_local;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) scan[t] += scan[t + s];
+        __syncthreads();
+    }
+    if (t == 0) {
+        int nseg = base_sh;
+        sst[nseg] = scan[0];
+        S->nseg[d] = nseg;
+        S->nent[d] = scan[0];
+    }
+}
+
+void launch_mls_setup(const FluidDev& F, hipStream_t st)
+{
+    if (F.nlm <= 0) return;
+    hipLaunchKernelGGL(k_mls_idx, dim3(F.nlm, 4), dim3(64), 0, st, F);
+    hipLaunchKernelGGL(k_mls_sup, dim3((2 * F.nlm + 63) / 64), dim3(64), 0, st, F);
+    hipLaunchKernelGGL(k_mls_sort, dim3(2), dim3(1024), (size_t)F.maxkeys * sizeof(unsigned long long), st, F);
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_mls_interp(const FluidDev F)
+{
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    const int tid = blockIdx.x * 64 + threadIdx.x;
+    if (tid >= 2 * F.nlm) return;
+    const int n = tid % F.nlm + 1, d = tid / F.nlm + 1;
+    if (!F.active[n - 1]) return;
+    const double* q = (d == 1) ? F.q2 : F.q3;
+    const double* r = (d == 1) ? F.rhs2 : F.rhs3;
+    double u = 0.0;
+    for (int m = 1; m <= FRIBO_DIMSD; ++m) {
+        int j = EULINS(n, d, 1, m), k = EULINS(n, d, 2, m);
+        // q2f is written for 2<=j<=n2m (INVTR.F90:145-149), q3f for
+        // 2<=k<=n3m (INVTR.F90:273-277); elsewhere it keeps its zero start
+        bool def = (d == 1) ? (j >= 2 && j <= F.n2m && k >= 1 && k <= F.n3m)
+                            : (j >= 1 && j <= F.n2m && k >= 2 && k <= F.n3m);
+        double qf = 0.0;
+        if (def) {
+            size_t c = fidx(F, j, k);
+            qf = q[c] + r[c];
+        }
+        u = u + PHI(n, d, m) * qf;
+    }
+    const double f = (LMVEL(n, d) - u) / S->dt;
+    double* con = F.contrib + (size_t)(d - 1) * F.nlm * FRIBO_DIMSD;
+    const double cl = CL(n, d);
+    for (int m = 1; m <= FRIBO_DIMSD; ++m) con[(size_t)(n - 1) * FRIBO_DIMSD + (m - 1)] = cl * PHI(n, d, m) * f;
+}
+
+__global__ __launch_bounds__(64) void k_mls_spread(const FluidDev F)
+{
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    const int d = blockIdx.y;
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= S->nseg[d]) return;
+    const int* sst = F.segstart + (size_t)d * (F.maxkeys + 1);
+    const unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
+    const double* con = F.contrib + (size_t)d * F.nlm * FRIBO_DIMSD;
+    double acc = 0.0;
+    for (int i = sst[s]; i < sst[s + 1]; ++i) acc = acc + con[gk[i] & ENT_MASK];
+    const int cell = F.segcell[(size_t)d * F.maxkeys + s];
+    const int j = cell % F.n2 + 1, k = cell / F.n2 + 1;
+    const double fval = (k == F.n3m) ? 0.0 : acc;
+    F.segval[(size_t)d * F.maxkeys + s] = fval;
+    if (k == F.n3m) return;
+    if (d == 0) {
+        if (j >= 2 && j <= F.n2m && k >= 1) {
+            size_t c = fidx(F, j, k);
+            F.rhs2[c] = F.rhs2[c] + fval * S->dt;
+        }
+    } else {
+        if (j >= 1 && j <= F.n2m && k >= 2) {
+            size_t c = fidx(F, j, k);
+            F.rhs3[c] = F.rhs3[c] + fval * S->dt;
+        }
+    }
+}
+
+void launch_mls_force(const FluidDev& F, hipStream_t st)
+{
+    if (F.nlm <= 0) return;
+    hipLaunchKernelGGL(k_mls_interp, dim3((2 * F.nlm + 63) / 64), dim3(64), 0, st, F);
+    hipLaunchKernelGGL(k_mls_spread, dim3((F.nlm * FRIBO_DIMSD + 63) / 64, 2), dim3(64), 0, st, F);
+}

@@ -1,0 +1,508 @@
+// Staggered-grid stencil kernels of the fluid step (gfx950).
+//
+//   k_cfl   cfl.F90:4-51                  -> per-block partial max / sum
+//   k_bc    fluid_tsch.F90:34-77 (dt control, RK coefficients),
+//           iobcs.F90:5-77 (boucdq), and the line-invariant factors of the
+//           partitioned k-direction solves of solve_fluid.F90:107-255
+//   k_rhs   HDNL.F90:5-171 + INVTR.F90:4-280 fused: one pass over q2,q3,pr,
+//           ru2,ru3 writes ru2,ru3,rhs2,rhs3 (the convective terms dph/qcap
+//           stay in registers)
+//   k_div   divergence.F90:15-42
+//   k_upd   updvp.F90:5-105 (updvp + prcalc) + partial sums of pr_submean
+//   k_mean  updvp.F90:112-144 (pr_submean) + iobcs.F90:81-108 (boucqt)
+//
+// The arithmetic of every formula keeps the reference's operation order;
+// the library is compiled with -ffp-contract=off so these kernels round
+// exactly like the FP64 reference loops.  Deviations from the reference's
+// sequential sums: the CFL/pressure-mean/outflow-flux reductions use a fixed
+// tree order (deterministic run to run).
+#include "fribo_internal.h"
+
+#define BLK_J 64
+#define BLK_K 4
+
+template <int NT>
+__device__ inline void block_reduce_sum(double* sh, double v)
+{
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t] + sh[t + s];
+        __syncthreads();
+    }
+}
+
+template <int NT>
+__device__ inline void block_reduce_max(double* sh, double v)
+{
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// cfl.F90:19-38 -- partial max of the local CFL and partial volume sum of q3
+__global__ __launch_bounds__(256) void k_cfl(const FluidDev F)
+{
+    __shared__ double sh[256];
+    const long ncell = (long)F.n2m * F.n3m;
+    double m = 1.0e-8, s = 0.0;
+    for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (long)gridDim.x * 256) {
+        int j = (int)(c % F.n2m) + 1;
+        int k = (int)(c / F.n2m) + 1;
+        double q2a = F.q2[fidx(F, j, k)], q2b = F.q2[fidx(F, j + 1, k)];
+        double q3a = F.q3[fidx(F, j, k)], q3b = F.q3[fidx(F, j, k + 1)];
+        double qcf = fabs((q2a + q2b) * 0.5 * F.udx2m[j]) + fabs((q3a + q3b) * 0.5 * F.udx3m[k]);
+        m = fmax(m, qcf);
+        s = s + q3a * (F.zz[k + 1] - F.zz[k]) * (F.rc[j + 1] - F.rc[j]);
+    }
+    block_reduce_max<256>(sh, m);
+    double bm = sh[0];
+    __syncthreads();
+    block_reduce_sum<256>(sh, s);
+    if (threadIdx.x == 0) {
+        F.cfl_part[2 * blockIdx.x] = bm;
+        F.cfl_part[2 * blockIdx.x + 1] = sh[0];
+    }
+}
+
+void launch_cfl(const FluidDev& F, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cfl, dim3(FRIBO_NPART), dim3(256), 0, st, F);
+}
+
+// ---------------------------------------------------------------------------
+// Line-invariant factors of one partitioned k-system (run by one wave).
+// Row r couples a[r] x[r-1] + x[r] + c[r] x[r+1]; chunk p = rows
+// [cs[p], cs[p+1]), interior rows cs[p]..e-1, separator e = cs[p+1]-1.
+__device__ void ksys_prepare(const KSys& K, int lane)
+{
+    double alpha = 0.0, beta = 1.0, gamma = 0.0;
+    double vs = 0.0, ws = 0.0;
+    int p = lane;
+    if (p < K.nch) {
+        int s = K.cs[p], e = K.cs[p + 1] - 1;
+        double cpp = 0.0, vpp = 0.0, gg = 1.0, wlast = 0.0;
+        for (int i = s; i <= e - 1; ++i) {
+            double den = 1.0 - K.a[i] * cpp;
+            double iv = 1.0 / den;
+            double cpi = K.c[i] * iv;
+            double vpi = (i == s) ? (-K.a[i]) * iv : (-K.a[i] * vpp) * iv;
+            K.inv[i] = iv;
+            K.cp[i] = cpi;
+            K.vp[i] = vpi;
+            K.g[i] = gg;
+            gg = -cpi * gg;
+            cpp = cpi;
+            vpp = vpi;
+            if (i == e - 1) wlast = (-K.c[i]) * iv;
+        }
+        K.wl[p] = wlast;
+        // backward sweeps for v and w at the first interior row
+        double vb = vpp, wb = wlast;   // values at e-1
+        for (int i = e - 2; i >= s; --i) {
+            vb = K.vp[i] - K.cp[i] * vb;
+            wb = -K.cp[i] * wb;
+        }
+        vs = vb;
+        ws = wb;
+        // separator row e: v, w at row e-1 are vpp, wlast
+        double ae = K.a[e];
+        alpha = ae * vpp;
+        beta = 1.0 + ae * wlast;    // + c_e * v^{p+1}_s added below
+        gamma = 0.0;                // c_e * w^{p+1}_s added below
+    }
+    // exchange v_s, w_s with lane p+1
+    double vs_n = __shfl_down(vs, 1, 64);
+    double ws_n = __shfl_down(ws, 1, 64);
+    if (p < K.nch) {
+        int e = K.cs[p + 1] - 1;
+        double ce = K.c[e];
+        if (p + 1 < K.nch) {
+            beta = beta + ce * vs_n;
+            gamma = ce * ws_n;
+        }
+    }
+    // PCR on the (alpha, beta, gamma) rows; record the multipliers
+    int li = 0;
+    for (int h = 1; h < 64; h <<= 1, ++li) {
+        double am = __shfl_up(alpha, h, 64), bm = __shfl_up(beta, h, 64), gm = __shfl_up(gamma, h, 64);
+        double ap = __shfl_down(alpha, h, 64), bp = __shfl_down(beta, h, 64), gp = __shfl_down(gamma, h, 64);
+        if (lane - h < 0) { am = 0.0; bm = 1.0; gm = 0.0; }
+        if (lane + h >= 64) { ap = 0.0; bp = 1.0; gp = 0.0; }
+        double m1 = alpha / bm, m2 = gamma / bp;
+        K.k1[li * 64 + lane] = m1;
+        K.k2[li * 64 + lane] = m2;
+        double na = -m1 * am;
+        double ng = -m2 * gp;
+        double nb = beta - m1 * gm - m2 * ap;
+        alpha = na;
+        gamma = ng;
+        beta = nb;
+    }
+    K.ib[lane] = 1.0 / beta;
+}
+
+// Row coefficients of the two k-systems for this substep (betadx known).
+// q2 (solve_fluid.F90:138-152): rows kc = 1..n3m.
+// q3 (solve_fluid.F90:215-231): interior rows kc = 2..n3m of the n3 system.
+__device__ void ksys_rows(const FluidDev& F, const KSys& K, double betadx, int t, int nt)
+{
+    for (int r = t; r < K.n; r += nt) {
+        int kc = r + K.kofs;
+        double b = 1.0 / (1.0 - F.ac3sk[kc] * betadx);
+        double am = -F.am3sk[kc] * betadx * b;
+        double ap = -F.ap3sk[kc] * betadx * b;
+        K.S[r] = b;
+        if (r == 0) K.bc[0] = am;
+        if (r == K.n - 1) K.bc[1] = ap;
+        K.a[r] = (r == 0) ? 0.0 : am;
+        K.c[r] = (r == K.n - 1) ? 0.0 : ap;
+    }
+}
+
+// One workgroup: finalize CFL/dt (substep 1), RK coefficients, boucdq and
+// the k-system factors.
+__global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int ns, double dt_in,
+                                             double dt_o_in)
+{
+    __shared__ double sh[1024];
+    __shared__ double sh2[1024];
+    DevScalars* S = F.S;
+    const int t = threadIdx.x;
+    if (n == 1) {
+        double m = 1.0e-8, s = 0.0;
+        for (int i = t; i < FRIBO_NPART; i += 1024) {
+            m = fmax(m, F.cfl_part[2 * i]);
+            s = s + F.cfl_part[2 * i + 1];
+        }
+        block_reduce_max<1024>(sh, m);
+        block_reduce_sum<1024>(sh2, s);
+        if (t == 0) {
+            double cflm = sh[0];
+            double dt = dt_in;
+            int err = 0;
+            S->q3avg = sh2[0] / (F.rext2 * F.alx3);
+            if (!F.idtv) {
+                if (ns == 0) {
+                    cflm = cflm * dt;
+                } else if (ns > 0) {
+                    dt = F.cflmax / cflm;
+                    if (dt > F.dtmax) dt = F.dtmax;
+                }
+                if (dt < 1.0e-8) err = FRIBO_E_DT_TOO_SMALL;
+            } else {
+                cflm = cflm * dt;
+                if (cflm > F.cfllim) err = FRIBO_E_CFL_TOO_LARGE;
+            }
+            S->cflm = cflm;
+            S->dt = dt;
+            S->dt_o = dt_o_in;
+            S->beta = dt / F.ren * 0.5;
+            if (err) S->err = err;
+        }
+        __syncthreads();
+    }
+    __shared__ double coef[4];
+    __shared__ int errflag;
+    if (t == 0) {
+        errflag = S->err;
+        double dt = S->dt, al, ga, ro;
+        if (F.nsst == 3) {
+            al = F.alm[n];
+            ga = F.gam[n];
+            ro = F.rom[n];
+        } else {
+            if (ns == 0) {
+                ga = 1.0;
+                ro = 0.0;
+            } else {
+                ga = 1.0 + 0.5 * dt / S->dt_o;
+                ro = -0.5 * dt / S->dt_o;
+            }
+            al = ga + ro;
+        }
+        S->al = al;
+        S->ga = ga;
+        S->ro = ro;
+        S->alre = al / F.ren;
+        S->betadx = S->beta * al;
+        S->ugkks = 1.0 / (F.g3rm[1] * F.g3rc[1]) * F.dx3q * 2.0 * S->betadx;
+        S->ugkkn = 1.0 / (F.g3rm[F.n3m] * F.g3rc[F.n3]) * F.dx3q * 2.0 * S->betadx;
+        coef[0] = dt;
+        coef[1] = ga;
+        coef[2] = ro;
+        coef[3] = S->betadx;
+    }
+    __syncthreads();
+    if (errflag) return;
+    const double dt = coef[0], ga = coef[1], ro = coef[2], betadx = coef[3];
+
+    // boucdq (iobcs.F90:22-50)
+    for (int j = 1 + t; j <= F.n2; j += 1024) {
+        F.dqb2s[j] = 0.0;
+        F.dqb3s[j] = 0.0;
+    }
+    const double udx3c_n3 = F.udx3c[F.n3];
+    for (int j = 1 + t; j <= F.n2m; j += 1024) {
+        double dq2x2 = (F.qb2n[j] - F.q2[fidx(F, j, F.n3m)]) * 2.0 * udx3c_n3;
+        F.dqb2n[j] = -dt * (ga * dq2x2 + ro * F.dq2x2o[j]) * F.cou;
+        double dq3x2 = (F.qb3n[j] - F.q3[fidx(F, j, F.n3m)]) * udx3c_n3;
+        F.dqb3n[j] = -dt * (ga * dq3x2 + ro * F.dq3x2o[j]) * F.cou;
+        F.dq2x2o[j] = dq2x2;
+        F.dq3x2o[j] = dq3x2;
+    }
+    __syncthreads();
+    // iobcs.F90:54-74: global mass balance at the outflow
+    double area = 0.0, qout = 0.0;
+    for (int j = 1 + t; j <= F.n2m; j += 1024) {
+        double darea = F.g2rm[j] / F.dx2;
+        area = area + darea;
+        qout = qout + F.dqb3n[j] * darea;
+    }
+    block_reduce_sum<1024>(sh, area);
+    block_reduce_sum<1024>(sh2, qout);
+    const double tarea = sh[0], tqout = sh2[0];
+    const double qinf = 0.0 * tarea;   // dqb3s == 0 (steady inflow)
+    const double cor = (qinf - tqout) / tarea;
+    for (int j = 1 + t; j <= F.n2m; j += 1024) F.dqb3n[j] = F.dqb3n[j] + cor;
+    if (t == 0) {
+        S->qinf = qinf;
+        S->qout = tqout;
+        S->area = tarea;
+    }
+
+    // k-direction systems for this substep
+    ksys_rows(F, F.ks[0], betadx, t, 1024);
+    ksys_rows(F, F.ks[1], betadx, t, 1024);
+    __syncthreads();
+    __threadfence_block();
+    if (t < 64) ksys_prepare(F.ks[0], t);
+    else if (t < 128) ksys_prepare(F.ks[1], t - 64);
+    // q2 wall rows: q2(1,:) = q2(n2,:) = 0 on the ghost planes (helsolve.F90:32-33)
+    if (t >= 128 && t < 128 + 6) {
+        int kk = t - 128;
+        int k = (kk < 3) ? kk - 2 : F.n3m + 1 + (kk - 3);
+        F.q2[fidx(F, 1, k)] = 0.0;
+        F.q2[fidx(F, F.n2, k)] = 0.0;
+    }
+}
+
+void launch_bc(const FluidDev& F, int n, int ns, double dt_in, double dt_o, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bc, dim3(1), dim3(1024), 0, st, F, n, ns, dt_in, dt_o);
+}
+
+// ---------------------------------------------------------------------------
+// Fused HDNL + INVTR.  Thread (j,k): q2 point (rc(j), zm(k)) for 2<=j<=n2m and
+// q3 point (rm(j), zz(k)) for 2<=k<=n3m.
+__global__ __launch_bounds__(BLK_J* BLK_K) void k_rhs(const FluidDev F)
+{
+    const int j = blockIdx.x * BLK_J + threadIdx.x + 1;
+    const int k = blockIdx.y * BLK_K + threadIdx.y + 1;
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    if (j > F.n2m || k > F.n3m) return;
+    const double dt = S->dt, ga = S->ga, ro = S->ro, al = S->al, alre = S->alre, pend = S->pend;
+    const int n2m = F.n2m, n3m = F.n3m;
+    const double* q2 = F.q2;
+    const double* q3 = F.q3;
+    const double* pr = F.pr;
+#define Q2(a, b) q2[fidx(F, a, b)]
+#define Q3(a, b) q3[fidx(F, a, b)]
+#define PRV(a, b) (pr[fidx(F, a, b)] - pend)
+
+    // ---- q2: hdnl2 (HDNL.F90:35-107) + invtr2 (INVTR.F90:45-147)
+    if (j >= 2) {
+        const int jm = j - 1, jp = j + 1;
+        double a = Q2(jp, k) + Q2(j, k), b = Q2(jm, k) + Q2(j, k);
+        double h22 = (a * a - b * b) * F.udx2c[j] * 0.25;
+        double dphv = -h22;
+        double h23;
+        if (k >= 2 && k <= n3m - 1) {
+            h23 = ((Q3(j, k + 1) + Q3(jm, k + 1)) * (Q2(j, k + 1) + Q2(j, k)) -
+                   (Q3(j, k) + Q3(jm, k)) * (Q2(j, k) + Q2(j, k - 1))) * F.udx3m[k] * 0.25;
+        } else if (k == n3m) {
+            h23 = (F.qb2n[j] * (F.qb3n[j] + F.qb3n[jm]) -
+                   (Q3(j, k) + Q3(jm, k)) * (Q2(j, k) + Q2(j, k - 1)) * 0.5) * F.udx3m[k] * 0.5;
+        } else {  // k == 1
+            h23 = ((Q3(j, 2) + Q3(jm, 2)) * (Q2(j, 1) + Q2(j, 2)) * 0.5 -
+                   F.qb2s[j] * (F.qb3s[j] + F.qb3s[jm])) * F.udx3m[1] * 0.5;
+        }
+        dphv = dphv - h23;
+        const double ammj = F.am2j[j], accj = F.ac2j[j], appj = F.ap2j[j];
+        double d22q2 = Q2(jp, k) * appj + Q2(j, k) * accj + Q2(jm, k) * ammj;
+        double d33q2, dpx22;
+        if (k >= 2 && k <= n3m - 1) {
+            d33q2 = Q2(j, k + 1) * F.ap3sk[k] + Q2(j, k) * F.ac3sk[k] + Q2(j, k - 1) * F.am3sk[k];
+            double udx2 = al * F.udx2c[j];
+            dpx22 = (PRV(j, k) - PRV(jm, k)) * udx2;
+        } else if (k == 1) {
+            double codea = F.dx3q / (F.g3rc[2] * F.g3rm[1]);
+            double codeb = F.dx3q / (F.g3rc[1] * F.g3rm[1]);
+            d33q2 = ((Q2(j, 2) - Q2(j, 1)) * codea - (Q2(j, 1) - F.qb2s[j]) * codeb * 2.0);
+            dpx22 = (PRV(j, k) - PRV(jm, k)) * F.udx2c[j] * al;
+        } else {
+            double codea = F.dx3q / (F.g3rc[k + 1] * F.g3rm[k]);
+            double codeb = F.dx3q / (F.g3rc[k] * F.g3rm[k]);
+            d33q2 = ((F.qb2n[j] - Q2(j, k)) * codea * 2.0 - (Q2(j, k) - Q2(j, k - 1)) * codeb);
+            dpx22 = (PRV(j, k) - PRV(jm, k)) * F.udx2c[j] * al;
+        }
+        double dcq2 = d22q2 + d33q2;
+        const size_t c = fidx(F, j, k);
+        F.rhs2[c] = (ga * dphv + ro * F.ru2[c] + alre * dcq2 - dpx22) * dt;
+        F.ru2[c] = dphv;
+    }
+
+    // ---- q3: hdnl3 (HDNL.F90:195-220) + invtr3 (INVTR.F90:188-269)
+    if (k >= 2) {
+        double qcapv = 0.0;  // qcap(:,n3m) is never written (stays 0 from initia)
+        if (k <= n3m - 1) {
+            const int jmm = F.jmv[j], jpp = F.jpv[j], jp = j + 1, km = k - 1, kp = k + 1;
+            double h32 = (((Q2(jp, k) + Q2(jp, km)) * (Q3(jpp, k) + Q3(j, k))) -
+                          ((Q2(j, k) + Q2(j, km)) * (Q3(j, k) + Q3(jmm, k)))) * F.udx2m[j] * 0.25;
+            double a = Q3(j, kp) + Q3(j, k), b = Q3(j, k) + Q3(j, km);
+            double h33 = (a * a - b * b) * F.udx3c[k] * 0.25;
+            qcapv = -(h32 + h33);
+        }
+        const double udx3 = al * F.udx3c[k];
+        const double amm = F.am3ck[k], acc = F.ac3ck[k], app = F.ap3ck[k];
+        const double ammj = F.am3j[j], accj = F.ac3j[j], appj = F.ap3j[j];
+        double dq32;
+        if (j == 1) {
+            dq32 = Q3(2, k) * appj + Q3(1, k) * accj + F.q3lw * ammj;
+        } else if (j == n2m) {
+            dq32 = F.q3uw * appj + Q3(j, k) * accj + Q3(j - 1, k) * ammj;
+        } else {
+            dq32 = Q3(j + 1, k) * appj + Q3(j, k) * accj + Q3(j - 1, k) * ammj;
+        }
+        double dq33 = Q3(j, k + 1) * app + Q3(j, k) * acc + Q3(j, k - 1) * amm;
+        double dcq3 = dq32 + dq33;
+        double dpx33 = (PRV(j, k) - PRV(j, k - 1)) * udx3;
+        const size_t c = fidx(F, j, k);
+        F.rhs3[c] = (ga * qcapv + ro * F.ru3[c] + alre * dcq3 - dpx33) * dt;
+        F.ru3[c] = qcapv;
+    }
+#undef Q2
+#undef Q3
+#undef PRV
+}
+
+void launch_rhs(const FluidDev& F, hipStream_t st)
+{
+    dim3 g((F.n2m + BLK_J - 1) / BLK_J, (F.n3m + BLK_K - 1) / BLK_K);
+    hipLaunchKernelGGL(k_rhs, g, dim3(BLK_J, BLK_K), 0, st, F);
+}
+
+// ---------------------------------------------------------------------------
+// divergence.F90:26-39
+__global__ __launch_bounds__(BLK_J* BLK_K) void k_div(const FluidDev F)
+{
+    const int j = blockIdx.x * BLK_J + threadIdx.x + 1;
+    const int k = blockIdx.y * BLK_K + threadIdx.y + 1;
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    if (j > F.n2m || k > F.n3m) return;
+    const double usdtal = 1.0 / (S->dt * S->al);
+    double dq = (F.q2[fidx(F, j + 1, k)] - F.q2[fidx(F, j, k)]) * F.udx2m[j] +
+                (F.q3[fidx(F, j, k + 1)] - F.q3[fidx(F, j, k)]) * F.udx3m[k];
+    F.dph[fidx(F, j, k)] = dq * usdtal;
+}
+
+void launch_div(const FluidDev& F, hipStream_t st)
+{
+    dim3 g((F.n2m + BLK_J - 1) / BLK_J, (F.n3m + BLK_K - 1) / BLK_K);
+    hipLaunchKernelGGL(k_div, g, dim3(BLK_J, BLK_K), 0, st, F);
+}
+
+// ---------------------------------------------------------------------------
+// updvp.F90:27-56 (velocity correction) + 83-102 (prcalc) + partial sums of
+// the pressure mean (updvp.F90:127-133).  The stored pressure is "raw": the
+// previous mean (S->pend) is subtracted on read.
+__global__ __launch_bounds__(BLK_J* BLK_K) void k_upd(const FluidDev F)
+{
+    __shared__ double sh[BLK_J * BLK_K];
+    const int tid = threadIdx.y * BLK_J + threadIdx.x;
+    const int j = blockIdx.x * BLK_J + threadIdx.x + 1;
+    const int k = blockIdx.y * BLK_K + threadIdx.y + 1;
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    double part = 0.0;
+    if (j <= F.n2m && k <= F.n3m) {
+        const double al = S->al, dt = S->dt, pend = S->pend;
+        const double* dph = F.dph;
+        const size_t c = fidx(F, j, k);
+        const int jm = F.jmv[j], jp = F.jpv[j], km = F.kmv[k], kp = F.kpv[k];
+        const double dc = dph[c];
+        // q2
+        double usujm = al * dt * F.udx2c[j];
+        F.q2[c] = F.q2[c] - (dc - dph[fidx(F, jm, k)]) * usujm;
+        // q3
+        if (k >= 2) {
+            double usukm = al * dt * F.udx3c[k];
+            F.q3[c] = F.q3[c] - (dc - dph[fidx(F, j, k - 1)]) * usukm;
+        }
+        // prcalc
+        const double be = al * S->beta;
+        double lap = (dph[fidx(F, jp, k)] * F.apphj[j] + dc * F.acphj[j] + dph[fidx(F, jm, k)] * F.amphj[j]) +
+                     (dph[fidx(F, j, kp)] * F.apphk[k] + dc * F.acphk[k] + dph[fidx(F, j, km)] * F.amphk[k]);
+        double prv = (F.pr[c] - pend) + dc - be * lap;
+        F.pr[c] = prv;
+        part = prv * (F.zz[k + 1] - F.zz[k]) * (F.rc[j + 1] - F.rc[j]);
+    }
+    block_reduce_sum<BLK_J * BLK_K>(sh, part);
+    if (tid == 0) F.pr_part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
+void launch_upd(const FluidDev& F, hipStream_t st)
+{
+    dim3 g((F.n2m + BLK_J - 1) / BLK_J, (F.n3m + BLK_K - 1) / BLK_K);
+    hipLaunchKernelGGL(k_upd, g, dim3(BLK_J, BLK_K), 0, st, F);
+}
+
+// ---------------------------------------------------------------------------
+// pr_submean (updvp.F90:125-141) and boucqt (iobcs.F90:300-312).  The mean
+// is applied directly to the pressure entries no kernel of the step reads
+// (row j=n2 and the ghost planes) and kept pending for the interior.
+__global__ __launch_bounds__(1024) void k_mean(const FluidDev F)
+{
+    __shared__ double sh[1024];
+    DevScalars* S = F.S;
+    if (S->err) return;
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int i = t; i < F.npr_part; i += 1024) s = s + F.pr_part[i];
+    block_reduce_sum<1024>(sh, s);
+    const double pravg = sh[0] / (F.rext2 * F.alx3);
+    // row j = n2, planes -2..n3m+3
+    for (int k = -2 + t; k <= F.n3m + 3; k += 1024) {
+        size_t c = fidx(F, F.n2, k);
+        F.pr[c] = F.pr[c] - pravg;
+    }
+    // ghost planes, rows 1..n2m
+    const int ng = 6 * F.n2m;
+    for (int i = t; i < ng; i += 1024) {
+        int kk = i / F.n2m, j = i % F.n2m + 1;
+        int k = (kk < 3) ? kk - 2 : F.n3m + 1 + (kk - 3);
+        size_t c = fidx(F, j, k);
+        F.pr[c] = F.pr[c] - pravg;
+    }
+    for (int j = 1 + t; j <= F.n2m; j += 1024) {
+        F.qb2s[j] = F.qb2s[j] + F.dqb2s[j];
+        F.qb3s[j] = F.qb3s[j] + F.dqb3s[j];
+        F.qb2n[j] = F.qb2n[j] + F.dqb2n[j];
+        F.qb3n[j] = F.qb3n[j] + F.dqb3n[j];
+    }
+    __syncthreads();
+    if (t == 0) S->pend = pravg;
+}
+
+void launch_mean(const FluidDev& F, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mean, dim3(1), dim3(1024), 0, st, F);
+}

@@ -1,0 +1,221 @@
+"""ctypes binding of libfribo_gpu.so -- the fluid step as the reference drives it.
+
+``FluidStep`` mirrors the reference's calling pattern (main.F90:79-101):
+upload the module state once, then per time step set the marker kinematics
+produced by rigid() and call ``fluid_tsch(ns, dt, dt_o)``; errors that the
+reference sends to stopcase (fluid_tsch.F90:43-54, lgrdistribution.F90:32-36,
+MLSint.F90:157-163) raise ``FriboError`` with the same code the C ABI returns.
+
+The library must be built (``python -m fribois2d_amd.build``); there is no
+fallback path: importing this module without it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfribo_gpu.so")
+
+ERR_NAMES = {1: "DT_TOO_SMALL", 2: "CFL_TOO_LARGE", 3: "MARKER_OUTSIDE", 4: "PARTITION_UNITY",
+             5: "ARG", 6: "HIP", 7: "LIMIT"}
+
+EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state", "fribo_set_markers",
+           "fribo_fluid_tsch", "fribo_cfl_dt", "fribo_substep", "fribo_array_size", "fribo_get_array",
+           "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_set_profiling", "fribo_get_profile",
+           "fribo_reset_profile", "fribo_profile_name"]
+
+NPROF = 12
+
+
+class FriboError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fribo error {code} ({ERR_NAMES.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("n2", "n3", "nsst", "idtv", "ibact", "forst", "wfun",
+                                          "inslws", "inslwn")] + \
+               [(n, C.c_double) for n in ("alx3", "rext2", "visc", "cou", "q3uw", "q3lw", "dtmax",
+                                           "cflmax", "cfllim", "radiusSD")]
+
+    @classmethod
+    def from_fluid(cls, fp):
+        p = cls()
+        for f, t in cls._fields_:
+            v = getattr(fp, f)
+            setattr(p, f, int(v) if t is C.c_int32 else float(v))
+        return p
+
+
+_lib = None
+
+
+def load():
+    """Load libfribo_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: build it with `python -m fribois2d_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    L.fribo_init.argtypes = [C.POINTER(Params), dp, dp, dp, dp, C.c_int, dp, C.POINTER(vp)]
+    L.fribo_finalize.argtypes = [vp]
+    L.fribo_set_state.argtypes = [vp] + [dp] * 11
+    L.fribo_get_state.argtypes = [vp] + [dp] * 11
+    L.fribo_set_markers.argtypes = [vp, C.c_int32, dp, dp, dp]
+    L.fribo_fluid_tsch.argtypes = [vp, C.c_int32, dp, C.c_double, dp]
+    L.fribo_cfl_dt.argtypes = [vp, C.c_int32, dp, C.c_double, dp]
+    L.fribo_substep.argtypes = [vp, C.c_int32, C.c_int32]
+    L.fribo_array_size.argtypes = [vp, C.c_char_p]
+    L.fribo_array_size.restype = C.c_long
+    L.fribo_get_array.argtypes = [vp, C.c_char_p, vp, C.c_long]
+    L.fribo_sync.argtypes = [vp]
+    L.fribo_last_error.restype = C.c_char_p
+    L.fribo_test_gemm.argtypes = [C.c_int32, C.c_int32, C.c_int32, dp, C.c_int32, dp, C.c_int32, dp, C.c_int32]
+    L.fribo_set_profiling.argtypes = [vp, C.c_int32]
+    L.fribo_get_profile.argtypes = [vp, dp, C.POINTER(C.c_int32), C.c_int32]
+    L.fribo_reset_profile.argtypes = [vp]
+    L.fribo_profile_name.argtypes = [C.c_int32]
+    L.fribo_profile_name.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise FriboError(rc, load().fribo_last_error().decode())
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+_INT = {"lmind", "eulins"}
+
+
+class FluidStep:
+    """One fluid instance on the GPU (reference-layout host interface)."""
+
+    def __init__(self, params, rc, zz, zmm, zmmt, wr):
+        L = load()
+        self.params = params
+        self.n2, self.n3 = params.n2, params.n3
+        p = Params.from_fluid(params)
+        zmm = np.asfortranarray(zmm, dtype=np.float64)
+        zmmt = np.asfortranarray(zmmt, dtype=np.float64)
+        h = C.c_void_p()
+        rc = np.ascontiguousarray(rc, dtype=np.float64)
+        zz = np.ascontiguousarray(zz, dtype=np.float64)
+        wr = np.ascontiguousarray(wr, dtype=np.float64)
+        _check(L.fribo_init(C.byref(p), _dp(rc), _dp(zz), _dp(zmm), _dp(zmmt), zmm.shape[0], _dp(wr), C.byref(h)))
+        self.h = h
+        self.nlm = 0
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().fribo_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ------------------------------------------------------------
+    def set_state(self, st):
+        keys = ("q2", "q3", "pr", "ru2", "ru3", "qb2s", "qb3s", "qb2n", "qb3n", "dq2x2o", "dq3x2o")
+        arrs = [None if st.get(k) is None else np.asfortranarray(st[k], dtype=np.float64) for k in keys]
+        _check(load().fribo_set_state(self.h, *[_dp(a) for a in arrs]))
+
+    def get_state(self):
+        n2, n3m = self.n2, self.n3 - 1
+        st = {k: np.zeros((n2, n3m + 6), order="F") for k in ("q2", "q3", "pr")}
+        st.update({k: np.zeros((n2, n3m), order="F") for k in ("ru2", "ru3")})
+        st.update({k: np.zeros(n2) for k in ("qb2s", "qb3s", "qb2n", "qb3n", "dq2x2o", "dq3x2o")})
+        keys = ("q2", "q3", "pr", "ru2", "ru3", "qb2s", "qb3s", "qb2n", "qb3n", "dq2x2o", "dq3x2o")
+        _check(load().fribo_get_state(self.h, *[_dp(st[k]) for k in keys]))
+        return st
+
+    def set_markers(self, lmcc, lmvel, lmA):
+        lmcc = np.asfortranarray(lmcc, dtype=np.float64)
+        lmvel = np.asfortranarray(lmvel, dtype=np.float64)
+        lmA = np.ascontiguousarray(lmA, dtype=np.float64)
+        self._mk = (lmcc, lmvel, lmA)
+        self.nlm = lmA.size
+        _check(load().fribo_set_markers(self.h, lmA.size, _dp(lmcc), _dp(lmvel), _dp(lmA)))
+
+    # -- stepping -----------------------------------------------------------
+    def fluid_tsch(self, ns, dt, dt_o=None):
+        d = C.c_double(dt)
+        cf = C.c_double(0.0)
+        _check(load().fribo_fluid_tsch(self.h, ns, C.byref(d), float(dt if dt_o is None else dt_o), C.byref(cf)))
+        return d.value, cf.value
+
+    def fluid_tsch_rc(self, ns, dt, dt_o=None):
+        """Like fluid_tsch but returns (code, dt, cflm) instead of raising."""
+        d = C.c_double(dt)
+        cf = C.c_double(0.0)
+        rc = load().fribo_fluid_tsch(self.h, ns, C.byref(d), float(dt if dt_o is None else dt_o), C.byref(cf))
+        return rc, d.value, cf.value
+
+    def cfl_dt(self, ns, dt, dt_o=None):
+        d = C.c_double(dt)
+        cf = C.c_double(0.0)
+        _check(load().fribo_cfl_dt(self.h, ns, C.byref(d), float(dt if dt_o is None else dt_o), C.byref(cf)))
+        return d.value, cf.value
+
+    def substep(self, n, ns):
+        _check(load().fribo_substep(self.h, n, ns))
+
+    def sync(self):
+        _check(load().fribo_sync(self.h))
+
+    # -- arrays -------------------------------------------------------------
+    def get(self, name):
+        L = load()
+        n = L.fribo_array_size(self.h, name.encode())
+        if n <= 0:
+            raise KeyError(name)
+        out = np.zeros(n, dtype=np.int32 if name in _INT else np.float64)
+        _check(L.fribo_get_array(self.h, name.encode(), out.ctypes.data_as(C.c_void_p), n))
+        n2, n3m = self.n2, self.n3 - 1
+        if n == n2 * (n3m + 6) and name not in _INT:
+            return out.reshape((n2, n3m + 6), order="F")
+        if n == n2 * (n3m + 5) and name in ("fey", "fez"):
+            return out.reshape((n2, n3m + 5), order="F")
+        if n == n2 * n3m and name in ("ru2", "ru3", "rhs2", "rhs3"):
+            return out.reshape((n2, n3m), order="F")
+        return out
+
+    # -- profiling ----------------------------------------------------------
+    def set_profiling(self, on=True):
+        _check(load().fribo_set_profiling(self.h, 1 if on else 0))
+
+    def reset_profile(self):
+        _check(load().fribo_reset_profile(self.h))
+
+    def profile(self):
+        L = load()
+        ms = np.zeros(NPROF)
+        cnt = (C.c_int32 * NPROF)()
+        _check(L.fribo_get_profile(self.h, _dp(ms), cnt, NPROF))
+        return {L.fribo_profile_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(NPROF)}
+
+
+def gemm_nt(A, B):
+    """C = A @ B.T on the GPU through the library's MFMA kernel (test hook)."""
+    L = load()
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    M, K = A.shape
+    N = B.shape[0]
+    Cm = np.zeros((M, N), order="F")
+    _check(L.fribo_test_gemm(M, N, K, _dp(A), M, _dp(B), N, _dp(Cm), M))
+    return Cm

@@ -1,0 +1,134 @@
+/*
+ * libfribo_gpu -- MI355X (gfx950) fractional-step fluid step of FRiBoIS2D.
+ *
+ * C ABI (plain pointers and sizes) that replaces the body of the reference's
+ * per-time-step routine
+ *
+ *     SUBROUTINE fluid_tsch(ns,str)            src/fluid_tsch.F90:4-204
+ *
+ * together with everything it calls on the fluid side (cfl, boucdq/boucqt,
+ * hdnl2/hdnl3, invtr2/invtr3, lgrdistribution, forcing_MLS, helsolve and the
+ * solq* sweeps, divg, phcalc_outflow, updvp, prcalc, pr_submean).  The
+ * Fortran driver keeps its module state in host arrays with the reference
+ * extents; it uploads them once, calls fribo_fluid_tsch once per step and
+ * copies back only what its host consumers read.  The Fortran interface
+ * module is fribois2d_amd/fortran/fribo_gpu_mod.f90.
+ *
+ * Host array layouts are exactly the reference's (Fortran column-major, one
+ * MPI rank so kstart=1, kend=n3m):
+ *   q2, q3, pr          (1:n2, -2:n3m+3)     mpi_routines.F90:428-441
+ *   ru2, ru3            (1:n2, 1:n3m)        mpi_routines.F90:452-458
+ *   qb2s ... dq3x2o     (1:n2)               utils_proc.F90:277-284
+ *   lmcc, lmvel         (nlm, 2)  [y, z]     declarations.F90:258-262
+ *   lmA                 (nlm)
+ *   lmind               (nlm, 2, 2)          lgrdistribution.F90:46-72
+ *   eulins              (nlm, 2, 2, 9)       forcing_MLS.F90:94
+ *   phi, phipred        (nlm, 2, 9)          forcing_MLS.F90:69,110
+ *   cl                  (nlm, 2)             forcing_MLS.F90:81
+ *   fey, fez            (1:n2, -2:n3m+2)     utils_proc.F90:310-320
+ *   zmm, zmmt           (ldz, *) leading dimension ldz (n3 in the reference)
+ *
+ * Every call returns 0 on success or one of the FRIBO_ERR_* codes; the
+ * codes that the reference turns into stopcase/MPI_ABORT are listed with the
+ * reference line that raises them.  fribo_last_error() gives a message.
+ * All calls of one handle must come from one host thread.
+ */
+#ifndef FRIBO_GPU_H
+#define FRIBO_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FRIBO_OK = 0,
+    FRIBO_ERR_DT_TOO_SMALL = 1,     /* fluid_tsch.F90:43-47   dt < 1e-8            */
+    FRIBO_ERR_CFL_TOO_LARGE = 2,    /* fluid_tsch.F90:50-54   cflm*dt > cfllim     */
+    FRIBO_ERR_MARKER_OUTSIDE = 3,   /* lgrdistribution.F90:32-36                   */
+    FRIBO_ERR_PARTITION_UNITY = 4,  /* MLSint.F90:157-163     |1-sum(phi)| > 1e-10 */
+    FRIBO_ERR_ARG = 5,              /* invalid argument / handle                   */
+    FRIBO_ERR_HIP = 6,              /* HIP runtime failure                         */
+    FRIBO_ERR_LIMIT = 7             /* size beyond what this build supports        */
+};
+
+/* Scalars of fluid_par.in / global_par.in that the fluid step reads
+ * (utils_proc.F90:59-83,142-170).  Logicals are 0/1. */
+typedef struct {
+    int32_t n2, n3;
+    int32_t nsst, idtv, ibact, forst, wfun, inslws, inslwn;
+    double alx3, rext2, visc, cou, q3uw, q3lw, dtmax, cflmax, cfllim, radiusSD;
+} fribo_params;
+
+typedef struct fribo_ctx* fribo_handle;
+
+/* Set-up: fluid_pre.F90:27-70 scalars, indic/cordin/coetar/tridiag_matrices
+ * metrics (indic.F90:4-44, cordin.F90:105-215, coetar.F90:5-144,
+ * phini.F90:109-145) from the node coordinates rc(1:n2) and zz(1:n3), and
+ * the phini eigen-decomposition zmm, zmmt, wr (phini.F90:35-92) computed by
+ * the host.  Allocates all device memory. */
+int fribo_init(const fribo_params* p, const double* rc, const double* zz, const double* zmm,
+               const double* zmmt, int ldz, const double* wr, fribo_handle* out);
+int fribo_finalize(fribo_handle h);
+
+/* Copy the prognostic state host -> device / device -> host (reference
+ * layouts above).  NULL pointers are skipped. */
+int fribo_set_state(fribo_handle h, const double* q2, const double* q3, const double* pr,
+                    const double* ru2, const double* ru3, const double* qb2s, const double* qb3s,
+                    const double* qb2n, const double* qb3n, const double* dq2x2o,
+                    const double* dq3x2o);
+int fribo_get_state(fribo_handle h, double* q2, double* q3, double* pr, double* ru2, double* ru3,
+                    double* qb2s, double* qb3s, double* qb2n, double* qb3n, double* dq2x2o,
+                    double* dq3x2o);
+
+/* Marker positions / velocities / lengths as rigid() leaves them
+ * (rigid.F90:4-138, read_rigid utils_proc.F90:560-565). */
+int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc, const double* lmvel,
+                      const double* lmA);
+
+/* One time step: fluid_tsch.F90:25-166.  dt is in/out (CFL control may
+ * change it), dt_o is the previous step's dt (used by nsst=1), cflm out. */
+int fribo_fluid_tsch(fribo_handle h, int32_t ns, double* dt, double dt_o, double* cflm);
+
+/* The same step in two parts, for stage-by-stage checks: CFL and dt control
+ * (fluid_tsch.F90:30-57), then one substep n=1..nsst (fluid_tsch.F90:62-166). */
+int fribo_cfl_dt(fribo_handle h, int32_t ns, double* dt, double dt_o, double* cflm);
+int fribo_substep(fribo_handle h, int32_t n, int32_t ns);
+
+/* Named arrays in the reference layout: "q2","q3","pr","dph","ru2","ru3",
+ * "rhs2","rhs3" (rhs2/rhs3 include the forcing term f*dt at the IB
+ * support cells, i.e. the rhs helsolve.F90:22,47 builds), "fey","fez",
+ * "qb2s","qb3s","qb2n","qb3n","dqb2s","dqb3s","dqb2n","dqb3n","dq2x2o",
+ * "dq3x2o", the 1-D metric arrays ("rc","rm","zz","zm","g2rc","g2rm",
+ * "g3rc","g3rm","udx2c","udx2m","udx3c","udx3m","am2j","ac2j","ap2j",
+ * "am3j","ac3j","ap3j","am3ck","ac3ck","ap3ck","am3sk","ac3sk","ap3sk",
+ * "amphj","acphj","apphj","amphk","acphk","apphk"), and the marker arrays
+ * "lmind","eulins" (int32), "phi","phipred","cl".  "scalars" returns
+ * {dt, beta, al, ga, ro, cflm, q3avg, qinf, qout, dx2, dx3, ren}. */
+long fribo_array_size(fribo_handle h, const char* name);
+int fribo_get_array(fribo_handle h, const char* name, void* out, long n);
+
+int fribo_sync(fribo_handle h);
+const char* fribo_last_error(void);
+
+/* Diagnostic entry point (no reference counterpart): C = A * B^T in FP64 on
+ * device buffers of the library (used by the GEMM parity test).  A is MxK
+ * (lda), B is NxK (ldb), C is MxN (ldc), all column-major host arrays. */
+int fribo_test_gemm(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
+                    int32_t ldb, double* C, int32_t ldc);
+
+/* Diagnostic timing (no reference counterpart): with profiling on, every
+ * kernel launch of the step is bracketed by HIP events on the library's
+ * stream; fribo_get_profile returns, per kernel class, the summed device
+ * time (ms) and the number of launches since the last reset. */
+#define FRIBO_NPROF 12
+int fribo_set_profiling(fribo_handle h, int32_t on);
+int fribo_get_profile(fribo_handle h, double* ms, int32_t* launches, int32_t nclass);
+int fribo_reset_profile(fribo_handle h);
+const char* fribo_profile_name(int32_t cls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

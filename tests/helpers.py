@@ -1,0 +1,79 @@
+"""Shared set-up for the parity tests: one case, the oracle and the GPU step
+fed with identical inputs."""
+import os
+
+import numpy as np
+
+from fribois2d_amd import cases
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def case(name):
+    if name in ("osccyl", "gallrect"):
+        return cases.reference_example(os.path.join(GOLDEN, name), name)
+    return cases.make_case(name)
+
+
+def decomposition(c, method="sym"):
+    return cases.phini(c.zz, c.params.alx3, method)
+
+
+def oracle_for(c, dec=None, state=None, params=None):
+    from oracle import Oracle
+    p = params or c.params
+    zmm, zmmt, wr = dec or decomposition(c)
+    o = Oracle(p, c.rc, c.zz, zmm, zmmt, wr)
+    o.set_state(state if state is not None else cases.initial_state(p, c.seed))
+    return o
+
+
+def gpu_for(c, dec=None, state=None, params=None):
+    from fribois2d_amd.gpu import FluidStep
+    p = params or c.params
+    zmm, zmmt, wr = dec or decomposition(c)
+    g = FluidStep(p, c.rc, c.zz, zmm, zmmt, wr)
+    g.set_state(state if state is not None else cases.initial_state(p, c.seed))
+    return g
+
+
+def relmax(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
+
+
+def divergence(st, c):
+    """max |div u| over cells (divergence.F90:28-39 without the 1/(dt al))."""
+    n2m, n3m = c.params.n2 - 1, c.params.n3 - 1
+    rc, zz = c.rc, c.zz
+    dx2 = n2m / c.params.rext2
+    dx3 = n3m / c.params.alx3
+    udx2m = dx2 / ((rc[1:] - rc[:-1]) * dx2)
+    udx3m = dx3 / ((zz[1:] - zz[:-1]) * dx3)
+    q2, q3 = st["q2"], st["q3"]
+    d = (q2[1:n2m + 1, 3:n3m + 3] - q2[0:n2m, 3:n3m + 3]) * udx2m[:, None] + \
+        (q3[0:n2m, 4:n3m + 4] - q3[0:n2m, 3:n3m + 3]) * udx3m[None, :]
+    return float(np.max(np.abs(d)))
+
+
+def run_steps(solver, c, nsteps, dt=None, t0=0.0, ns0=0, hist=None):
+    """Advance with prescribed marker motion; returns (t, dt)."""
+    t = t0
+    dt = c.params.dt0 if dt is None else dt
+    dto = dt
+    for ns in range(ns0, ns0 + nsteps):
+        if c.params.ibact and c.verts is not None:
+            solver.set_markers(*c.markers(t))
+        out = solver.fluid_tsch(ns, dt, dto)
+        if isinstance(out, tuple) and len(out) == 3:
+            rc, dtn, cf = out
+            assert rc == 0, f"oracle error {rc} at step {ns}"
+        else:
+            dtn, cf = out
+        if hist is not None:
+            hist.append((ns, dtn, cf))
+        t += dtn
+        dto = dtn
+        dt = dtn
+    return t, dt

@@ -1,0 +1,196 @@
+"""GPU parity: libfribo_gpu (HIP, gfx950) against the CPU oracle.
+
+Bars (written per test):
+* integer work (marker->cell indices, MLS supports): bit-exact;
+* stencil outputs of one substep (ru2, ru3 and the right-hand sides):
+  identical operation order in both codes, compared at 1e-14 relative;
+* solver outputs (tridiagonal sweeps, Poisson transform): different but exact
+  elimination orders, 1e-11 relative after a substep, 1e-10 relative to the
+  field maximum after multi-step runs (the north-star velocity/pressure bar).
+"""
+import numpy as np
+import pytest
+
+from fribois2d_amd import cases
+from helpers import case, decomposition, divergence, gpu_for, oracle_for, relmax, run_steps
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu():
+    from fribois2d_amd import gpu
+    return gpu
+
+
+def test_gemm_matches_numpy():
+    g = _gpu()
+    rng = np.random.default_rng(7)
+    for (M, N, K) in [(16, 16, 4), (100, 77, 53), (288, 384, 384), (1, 65, 17), (130, 3, 200)]:
+        A = rng.standard_normal((M, K))
+        B = rng.standard_normal((N, K))
+        C = g.gemm_nt(A, B)
+        ref = A @ B.T
+        scale = np.abs(A) @ np.abs(B).T
+        err = np.max(np.abs(C - ref) / scale)
+        print(f"gemm {M}x{N}x{K}: max err/scale = {err:.3e}", flush=True)
+        assert err < 1e-14
+
+
+def test_metrics_bit_exact():
+    c = case("cyl64")
+    o = oracle_for(c)
+    g = gpu_for(c)
+    for name in ("rm", "zm", "g2rc", "g2rm", "g3rc", "g3rm", "udx2c", "udx2m", "udx3c", "udx3m", "am2j", "ac2j",
+                 "ap2j", "am3j", "ac3j", "ap3j", "am3ck", "ac3ck", "ap3ck", "am3sk", "ac3sk", "ap3sk", "amphj",
+                 "acphj", "apphj", "amphk", "acphk", "apphk"):
+        np.testing.assert_array_equal(g.get(name), o.get(name), err_msg=name)
+
+
+@pytest.mark.parametrize("name", ["tiny", "cyl64"])
+def test_substep_stages(name):
+    c = case(name)
+    dec = decomposition(c)
+    o = oracle_for(c, dec)
+    g = gpu_for(c, dec)
+    lm = c.markers(0.0)
+    o.set_markers(*lm)
+    g.set_markers(*lm)
+    rc, dto, cfo = o.cfl_dt(0, c.params.dt0)
+    assert rc == 0
+    dtg, cfg = g.cfl_dt(0, c.params.dt0)
+    assert dtg == dto
+    assert abs(cfg - cfo) <= 1e-15 * abs(cfo)
+    n2, n3m = c.params.n2, c.params.n3 - 1
+    n2m = n2 - 1
+    for n in range(1, c.params.nsst + 1):
+        print(f"{name}: substep {n}", flush=True)
+        assert o.substep(n, 0) == 0
+        g.substep(n, 0)
+        if n == 1:
+            np.testing.assert_array_equal(g.get("lmind"), o.get("lmind"))
+            np.testing.assert_array_equal(g.get("eulins"), o.get("eulins"))
+            assert relmax(g.get("phi"), o.get("phi")) <= 1e-14
+            assert relmax(g.get("cl"), o.get("cl")) <= 1e-14
+        # stencil outputs: identical operation order
+        np.testing.assert_array_equal(g.get("ru2")[1:n2m], o.get("ru2")[1:n2m])
+        np.testing.assert_array_equal(g.get("ru3")[:n2m, 1:], o.get("ru3")[:n2m, 1:])
+        dt = o.get("scalars")[0]
+        fey, fez = o.get("fey"), o.get("fez")
+        exp2 = o.get("rhs2") + fey[:, 3:3 + n3m] * dt
+        exp3 = o.get("rhs3") + fez[:, 3:3 + n3m] * dt
+        assert relmax(g.get("rhs2")[1:n2m], exp2[1:n2m]) <= 1e-14
+        assert relmax(g.get("rhs3")[:n2m, 1:], exp3[:n2m, 1:]) <= 1e-14
+        assert relmax(g.get("fey"), fey) <= 1e-13
+        assert relmax(g.get("fez"), fez) <= 1e-13
+        for f in ("q2", "q3", "pr", "dph"):
+            e = relmax(g.get(f), o.get(f))
+            print(f"  {f}: {e:.2e}", flush=True)
+            assert e <= 1e-11, f
+        for f in ("qb2n", "qb3n", "dq2x2o", "dq3x2o"):
+            assert relmax(g.get(f), o.get(f)) <= 1e-12, f
+
+
+@pytest.mark.parametrize("name,nsteps", [("cyl64", 40), ("cyl384", 4)])
+def test_multistep(name, nsteps):
+    c = case(name)
+    dec = decomposition(c)
+    o = oracle_for(c, dec)
+    g = gpu_for(c, dec)
+    ho, hg = [], []
+    run_steps(o, c, nsteps, hist=ho)
+    run_steps(g, c, nsteps, hist=hg)
+    dto = np.array([h[1] for h in ho])
+    dtg = np.array([h[1] for h in hg])
+    assert relmax(dtg, dto) <= 1e-12
+    so, sg = {k: o.get(k) for k in ("q2", "q3", "pr")}, g.get_state()
+    for f in ("q2", "q3", "pr"):
+        e = relmax(sg[f], so[f])
+        print(f"{name} {nsteps} steps {f}: {e:.2e}", flush=True)
+        assert e <= 1e-10, f
+    for f in ("ru2", "ru3"):
+        assert relmax(sg[f], o.get(f)) <= 1e-10, f
+
+
+@pytest.mark.parametrize("name", ["osccyl", "gallrect"])
+def test_reference_examples(name):
+    """The reference's own example inputs (grids and markers of test_*1DOF)."""
+    c = case(name)
+    dec = decomposition(c)
+    o = oracle_for(c, dec)
+    g = gpu_for(c, dec)
+    lm = c.markers(0.0)
+    o.set_markers(*lm)
+    g.set_markers(*lm)
+    assert lm[0].shape[0] == {"osccyl": 377, "gallrect": 798}[name]
+    rc, dto, _ = o.fluid_tsch(0, 1e-3)
+    assert rc == 0
+    dtg, _ = g.fluid_tsch(0, 1e-3)
+    np.testing.assert_array_equal(g.get("lmind"), o.get("lmind"))
+    np.testing.assert_array_equal(g.get("eulins"), o.get("eulins"))
+    sg = g.get_state()
+    for f in ("q2", "q3", "pr"):
+        e = relmax(sg[f], o.get(f))
+        print(f"{name} {f}: {e:.2e}", flush=True)
+        assert e <= 1e-10, f
+
+
+def test_large_properties():
+    """2048x1536 rectangle: projection, determinism, no NaN (sizes the oracle cannot run)."""
+    c = case("rect2048")
+    dec = decomposition(c)
+    g1 = gpu_for(c, dec)
+    g2 = gpu_for(c, dec)
+    run_steps(g1, c, 2)
+    run_steps(g2, c, 2)
+    s1, s2 = g1.get_state(), g2.get_state()
+    for f in ("q2", "q3", "pr"):
+        assert np.all(np.isfinite(s1[f]))
+        np.testing.assert_array_equal(s1[f], s2[f])
+    dx = np.min(np.diff(c.zz))
+    umax = max(np.max(np.abs(s1["q2"])), np.max(np.abs(s1["q3"])))
+    d = divergence(s1, c)
+    print(f"rect2048 max|div| = {d:.3e} (umax/dx = {umax / dx:.3e})", flush=True)
+    assert d <= 1e-11 * umax / dx
+
+
+def test_uniform_flow_is_steady():
+    """Without a body, uniform inflow between free-slip walls is an exact solution."""
+    c = case("cyl64")
+    p = cases.FluidParams(**{**c.params.__dict__, "ibact": False})
+    st = cases.initial_state(p, None)
+    dec = decomposition(c)
+    g = gpu_for(c, dec, state=st, params=p)
+    for ns in range(5):
+        g.fluid_tsch(ns, 1e-3)
+    s = g.get_state()
+    n2m, n3m = p.n2 - 1, p.n3 - 1
+    assert np.max(np.abs(s["q3"][:n2m, 2:n3m + 4] - 1.0)) < 1e-12
+    assert np.max(np.abs(s["q2"])) < 1e-12
+
+
+def test_error_codes():
+    gpu = _gpu()
+    c = case("tiny")
+    dec = decomposition(c)
+    # marker outside the domain (lgrdistribution.F90:32-36)
+    g = gpu_for(c, dec)
+    lmcc, lmvel, lmA = c.markers(0.0)
+    bad = lmcc.copy()
+    bad[0, 1] = -0.5
+    g.set_markers(bad, lmvel, lmA)
+    rc, _, _ = g.fluid_tsch_rc(0, 1e-3)
+    assert rc == 3
+    # dt below 1e-8 under CFL control (fluid_tsch.F90:43-47)
+    p = cases.FluidParams(**{**c.params.__dict__, "cflmax": 1e-12})
+    g = gpu_for(c, dec, params=p)
+    g.set_markers(lmcc, lmvel, lmA)
+    rc, _, _ = g.fluid_tsch_rc(1, 1e-3)
+    assert rc == 1
+    # CFL above cfllim with fixed dt (fluid_tsch.F90:50-54)
+    p = cases.FluidParams(**{**c.params.__dict__, "idtv": True})
+    g = gpu_for(c, dec, params=p)
+    g.set_markers(lmcc, lmvel, lmA)
+    rc, _, _ = g.fluid_tsch_rc(0, 10.0)
+    assert rc == 2
+    with pytest.raises(gpu.FriboError):
+        g.fluid_tsch(0, 10.0)
