@@ -1,0 +1,140 @@
+"""Throughput of the FRiBoIS2D fluid step on one MI355X.
+
+One "step" is one call of the drop-in fluid_tsch (CFL/dt control + 3 RK
+substeps of HDNL/INVTR, MLS forcing, ADI Helmholtz, divergence, eigen-transform
+Poisson solve and projection) on the 2048x1536-cell rectangle workload
+(chord 1, thickness 0.25, Re=200, 1000 Lagrangian markers pitching about the
+centre, CFL 0.3), fields resident in HBM.  Marker kinematics for each step are
+produced on the host (as rigid() does) and passed through the C ABI inside the
+timed region.
+
+Prints one JSON line: Mcell-steps/s, the roofline of the dominant kernel
+(the FP64 MFMA eigen-transform GEMM) measured with HIP events on the library's
+stream over the timed steps, and the CPU oracle timed on one step of the same
+workload on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_MFMA_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def pmc_traffic(name):
+    """HBM bytes per dispatch of `name` from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return d.get(name, {}).get("bytes_per_dispatch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="rect2048")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from fribois2d_amd import cases
+    from fribois2d_amd.gpu import FluidStep
+
+    c = cases.make_case(args.workload)
+    p = c.params
+    n2m, n3m = p.n2 - 1, p.n3 - 1
+    cells = n2m * n3m
+    zmm, zmmt, wr = cases.phini(c.zz, p.alx3)
+    st0 = cases.initial_state(p, c.seed)
+    g = FluidStep(p, c.rc, c.zz, zmm, zmmt, wr)
+    g.set_state(st0)
+
+    t, dt = 0.0, p.dt0
+    ns = 0
+
+    def step():
+        nonlocal t, dt, ns
+        g.set_markers(*c.markers(t))
+        dtn, _ = g.fluid_tsch(ns, dt, dt)
+        t += dtn
+        dt = dtn
+        ns += 1
+
+    for _ in range(args.warmup):
+        step()
+    g.sync()
+    g.set_profiling(True)
+    g.reset_profile()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    g.sync()
+    el = time.perf_counter() - t0
+    prof = g.profile()
+    g.set_profiling(False)
+    ms_step = el * 1e3 / args.steps
+    value = cells * args.steps / el / 1e6
+
+    gemm_ms, gemm_n = prof["gemm"]
+    gemm_avg_s = gemm_ms / gemm_n / 1e3
+    gemm_flops = 2.0 * n2m * n3m * n3m
+    achieved = gemm_flops / gemm_avg_s / 1e12
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_MFMA_TFLOPS, 4), "traffic": pmc_traffic("k_gemm_nt"),
+            "kernel": "k_gemm_nt", "avg_ms": round(gemm_avg_s * 1e3, 4)}
+    kernels = {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches_per_step": v[1] / args.steps}
+               for k, v in prof.items() if v[1]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        o = Oracle(p, c.rc, c.zz, zmm, zmmt, wr)
+        o.set_state(st0)
+        o.set_markers(*c.markers(0.0))
+        t1 = time.perf_counter()
+        rc, _, _ = o.fluid_tsch(0, p.dt0, p.dt0)
+        cel = time.perf_counter() - t1
+        cpu = {"value": round(cells / cel / 1e6, 4), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"1 time step (3 substeps) of the same {args.workload} workload, single-threaded C "
+                         f"oracle, {cel:.1f} s"}
+
+    out = {
+        "metric": "Mcell-steps/s (plain FP64 GPU fluid step)",
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated grid and markers, seeded 1e-3 velocity perturbation)",
+        "config": {"workload": f"{args.workload}: {n3m}x{n2m} cells, {c.nlm} markers, Re={1.0 / p.visc:g}, CFL {p.cflmax}",
+                   "precision": "plain FP64"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -667,7 +667,8 @@ extern "C" long fribo_array_size(fribo_handle h, const char* name)
     const FluidDev& F = h->F;
     std::string s(name);
     const long g = (long)F.n2 * (F.n3m + 6), r = (long)F.n2 * F.n3m;
-    if (s == "q2" || s == "q3" || s == "pr" || s == "dph") return g;
+    if (s == "q2" || s == "q3" || s == "pr" || s == "dph" || s == "sA" || s == "sB") return g;
+    if (s == "hmn") return (long)F.n2m * F.n3m;
     if (s == "ru2" || s == "ru3" || s == "rhs2" || s == "rhs3") return r;
     if (s == "fey" || s == "fez") return (long)F.n2 * (F.n3m + 5);
     if (s == "qb2s" || s == "qb3s" || s == "qb2n" || s == "qb3n" || s == "dqb2s" || s == "dqb3s" || s == "dqb2n" ||
@@ -700,9 +701,12 @@ extern "C" int fribo_get_array(fribo_handle h, const char* name, void* out, long
         memcpy(o, v, sizeof(v));
         return FRIBO_OK;
     }
-    if (s == "q2" || s == "q3" || s == "dph") {
-        double* src = s == "q2" ? F.q2 : (s == "q3" ? F.q3 : F.dph);
+    if (s == "q2" || s == "q3" || s == "dph" || s == "sA" || s == "sB") {
+        double* src = s == "q2" ? F.q2 : (s == "q3" ? F.q3 : (s == "dph" ? F.dph : (s == "sA" ? F.sA : F.sB)));
         e = get_field(h, (double*)out, src, -2, F.nk);
+    } else if (s == "hmn") {
+        HIPCHK(hipMemcpy2DAsync(out, F.n2m * sizeof(double), F.hmn, F.P * sizeof(double), F.n2m * sizeof(double),
+                                F.n3m, hipMemcpyDeviceToHost, h->st));
     } else if (s == "pr") {
         return fribo_get_state(h, nullptr, nullptr, (double*)out, nullptr, nullptr, nullptr, nullptr, nullptr,
                                nullptr, nullptr, nullptr);

