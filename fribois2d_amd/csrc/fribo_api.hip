@@ -189,31 +189,70 @@ static void host_metrics(fribo_ctx* c, const double* rc_in, const double* zz_in,
     }
 }
 
-static int setup_ksys(fribo_ctx* c, KSys& K, int n, int kofs)
+static int upload1d(fribo_ctx* c, double** dst, const std::vector<double>& h);
+
+// Partition rows [0,n) into nch = min(64, n/2) chunks starting at floor(p*n/nch).
+static std::vector<int> chunk_starts(int n, int* nch_out, int* maxm_out)
 {
-    K.n = n;
-    K.kofs = kofs;
     int nch = n / 2;
     if (nch > FRIBO_KMAXCH) nch = FRIBO_KMAXCH;
     if (nch < 1) nch = 1;
-    K.nch = nch;
     std::vector<int> cs(nch + 1);
     int maxm = 0;
     for (int p = 0; p <= nch; ++p) cs[p] = (int)(((long)p * n) / nch);
     for (int p = 0; p < nch; ++p) maxm = std::max(maxm, cs[p + 1] - cs[p]);
-    K.maxm = maxm;
-    if (maxm - 1 > 128) return set_err(FRIBO_ERR_LIMIT, "k-lines longer than 8256 rows are not supported");
+    *nch_out = nch;
+    *maxm_out = maxm;
+    return cs;
+}
+
+static int setup_ksys(fribo_ctx* c, KSys& K, int n, int kofs, int src, int inter)
+{
+    K.n = n;
+    K.kofs = kofs;
+    K.src = src;
+    K.inter = inter;
+    std::vector<int> cs = chunk_starts(n, &K.nch, &K.maxm);
+    if (K.maxm - 1 > (inter ? 64 : 128))
+        return set_err(FRIBO_ERR_LIMIT, "tridiagonal lines too long for this build");
     int rc;
-    if ((rc = dalloc(c, &K.cs, nch + 1))) return rc;
-    HIPCHK(hipMemcpy(K.cs, cs.data(), (nch + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if ((rc = dalloc(c, &K.cs, K.nch + 1))) return rc;
+    HIPCHK(hipMemcpy(K.cs, cs.data(), (K.nch + 1) * sizeof(int), hipMemcpyHostToDevice));
+    const size_t len = inter ? (size_t)K.maxm * 64 : (size_t)n;
     double** arrs[] = {&K.a, &K.c, &K.S, &K.cp, &K.inv, &K.vp, &K.g};
     for (auto a : arrs)
-        if ((rc = dalloc(c, a, n))) return rc;
+        if ((rc = dalloc(c, a, len))) return rc;
     if ((rc = dalloc(c, &K.wl, 64))) return rc;
     if ((rc = dalloc(c, &K.k1, 6 * 64))) return rc;
     if ((rc = dalloc(c, &K.k2, 6 * 64))) return rc;
     if ((rc = dalloc(c, &K.ib, 64))) return rc;
     if ((rc = dalloc(c, &K.bc, 2))) return rc;
+    return FRIBO_OK;
+}
+
+// Poisson j-lines: fixed partition and lane-interleaved amphj/acphj/apphj.
+static int setup_psys(fribo_ctx* c, PSys& Q, int n, const std::vector<double>& am, const std::vector<double>& ac,
+                      const std::vector<double>& ap)
+{
+    Q.n = n;
+    std::vector<int> cs = chunk_starts(n, &Q.nch, &Q.maxm);
+    if (Q.maxm - 1 > 64) return set_err(FRIBO_ERR_LIMIT, "Poisson lines too long for this build");
+    int rc;
+    if ((rc = dalloc(c, &Q.cs, Q.nch + 1))) return rc;
+    HIPCHK(hipMemcpy(Q.cs, cs.data(), (Q.nch + 1) * sizeof(int), hipMemcpyHostToDevice));
+    const size_t len = (size_t)Q.maxm * 64;
+    std::vector<double> ham(len, 0.0), hac(len, 1.0), hap(len, 0.0);
+    for (int p = 0; p < Q.nch; ++p)
+        for (int r = cs[p]; r < cs[p + 1]; ++r) {
+            const size_t ix = (size_t)(r - cs[p]) * 64 + p;
+            const int j = r + 1;
+            ham[ix] = am[j];
+            hac[ix] = ac[j];
+            hap[ix] = ap[j];
+        }
+    if ((rc = upload1d(c, &Q.am, ham))) return rc;
+    if ((rc = upload1d(c, &Q.ac, hac))) return rc;
+    if ((rc = upload1d(c, &Q.ap, hap))) return rc;
     return FRIBO_OK;
 }
 
@@ -270,10 +309,6 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
         F.gam[1] = 1.5; F.rom[1] = -0.50;
     }
     for (int n = 1; n <= p->nsst; ++n) F.alm[n] = F.gam[n] + F.rom[n];
-    if (jsolve_lds_bytes(F.n2) > c->lds_limit) {
-        delete c;
-        return set_err(FRIBO_ERR_LIMIT, "n2 too large for one j-line in LDS");
-    }
 
     std::vector<int> jmv, jpv, kmv, kpv;
     host_metrics(c, rc, zz, jmv, jpv, kmv, kpv);
@@ -319,8 +354,17 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     if (!e) e = dalloc(c, &F.cfl_part, 2 * FRIBO_NPART);
     F.npr_part = ((F.n2m + 63) / 64) * ((F.n3m + 3) / 4);
     if (!e) e = dalloc(c, &F.pr_part, F.npr_part);
-    if (!e) e = setup_ksys(c, F.ks[0], F.n3m, 1);
-    if (!e) e = setup_ksys(c, F.ks[1], F.n3m - 1, 2);
+    if (!e) e = setup_ksys(c, F.ks[0], F.n3m, 1, 0, 0);
+    if (!e) e = setup_ksys(c, F.ks[1], F.n3m - 1, 2, 0, 0);
+    if (!e) e = setup_ksys(c, F.js[0], F.n2 - 2, 2, 1, 1);
+    if (!e) e = setup_ksys(c, F.js[1], F.n2m, 1, 2, 1);
+    if (!e) e = setup_psys(c, F.ps, F.n2m, c->h1[A_AMPHJ], c->h1[A_ACPHJ], c->h1[A_APPHJ]);
+    if (!e) {
+        int mm = std::max(std::max(F.js[0].maxm, F.js[1].maxm), F.ps.maxm);
+        F.jstride = mm | 1;
+        if (jsolve_lds_bytes(F) > c->lds_limit || sysprep_lds_bytes(F) + 17000 > c->lds_limit)
+            e = set_err(FRIBO_ERR_LIMIT, "grid too large for the LDS staging of this build");
+    }
     if (!e) e = dalloc(c, &F.kYs, (size_t)2 * FRIBO_KMAXCH * F.n2m);
     if (!e) e = dalloc(c, &F.kYe, (size_t)2 * FRIBO_KMAXCH * F.n2m);
     if (!e) e = dalloc(c, &F.kDe, (size_t)2 * FRIBO_KMAXCH * F.n2m);
@@ -449,10 +493,10 @@ extern "C" int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc
         HIPCHK(hipStreamSynchronize(h->st));
         for (void* q : h->mk_allocs) hipFree(q);
         h->mk_allocs.clear();
-        int maxkeys = 1024;
+        int maxkeys = 2048;
         while (maxkeys < nlm * FRIBO_DIMSD) maxkeys <<= 1;
-        if ((size_t)maxkeys * sizeof(unsigned long long) > h->lds_limit || nlm >= (1 << 20) / FRIBO_DIMSD)
-            return set_err(FRIBO_ERR_LIMIT, "too many markers for the in-LDS contribution sort");
+        if (nlm >= (1 << 20) / FRIBO_DIMSD)
+            return set_err(FRIBO_ERR_LIMIT, "too many markers (at most 116508)");
         F.maxkeys = maxkeys;
         auto* L = &h->mk_allocs;
         int e = 0;
@@ -467,8 +511,6 @@ extern "C" int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc
         if (!e) e = dalloc(h, &F.active, (size_t)nlm, L);
         if (!e) e = dalloc(h, &F.contrib, 2 * FRIBO_DIMSD * (size_t)nlm, L);
         if (!e) e = dalloc(h, &F.keys, 2 * (size_t)maxkeys, L);
-        if (!e) e = dalloc(h, &F.segstart, 2 * ((size_t)maxkeys + 1), L);
-        if (!e) e = dalloc(h, &F.segcell, 2 * (size_t)maxkeys, L);
         if (!e) e = dalloc(h, &F.segval, 2 * (size_t)maxkeys, L);
         if (e) return e;
         h->nlm_alloc = nlm;
@@ -717,15 +759,20 @@ extern "C" int fribo_get_array(fribo_handle h, const char* name, void* out, long
         const int d = (s == "fey") ? 0 : 1;
         double* o = (double*)out;
         std::fill(o, o + need, 0.0);
-        if ((e = read_scalars(h))) return e;
-        const int nseg = h->hs.nseg[d];
-        if (F.nlm > 0 && F.ibact && nseg > 0) {
-            std::vector<int> cell(nseg);
-            std::vector<double> val(nseg);
-            HIPCHK(hipMemcpy(cell.data(), F.segcell + (size_t)d * F.maxkeys, nseg * sizeof(int), hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(val.data(), F.segval + (size_t)d * F.maxkeys, nseg * sizeof(double), hipMemcpyDeviceToHost));
-            for (int i = 0; i < nseg; ++i) {
-                int j = cell[i] % F.n2 + 1, k = cell[i] / F.n2 + 1;
+        const int nreal = F.nlm * FRIBO_DIMSD;
+        if (F.nlm > 0 && F.ibact) {
+            // segment heads of the sorted (cell, entry) keys carry the cell value
+            std::vector<unsigned long long> key(nreal);
+            std::vector<double> val(nreal);
+            HIPCHK(hipMemcpy(key.data(), F.keys + (size_t)d * F.maxkeys, nreal * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(val.data(), F.segval + (size_t)d * F.maxkeys, nreal * sizeof(double),
+                             hipMemcpyDeviceToHost));
+            for (int i = 0; i < nreal; ++i) {
+                if (key[i] == ~0ull) break;
+                const unsigned long long cell = key[i] >> 20;
+                if (i > 0 && (key[i - 1] >> 20) == cell) continue;
+                const int j = (int)(cell % F.n2) + 1, k = (int)(cell / F.n2) + 1;
                 o[(size_t)(j - 1) + (size_t)(k + 2) * F.n2] = val[i];
             }
         }

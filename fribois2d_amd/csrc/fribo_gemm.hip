@@ -8,100 +8,132 @@
 // column-major layout.  Every K-slice of both operands is therefore a set of
 // contiguous columns and loads fully coalesced.
 //
-// Tiling: 64x64 output tile per 256-thread workgroup (2x2 waves of 32x32),
-// BK = 16, v_mfma_f64_16x16x4_f64 (A fragment: lane l holds A[l%16][l/16];
-// B fragment: B[l/16][l%16]; accumulator element v of lane l is
-// C[l/16 + 4v][l%16]).  The next K-slice is fetched into registers while the
-// current one is consumed from LDS.
+// v_mfma_f64_16x16x4_f64 fragment layout: A fragment lane l holds
+// A[l%16][l/16]; B fragment lane l holds B[l/16][l%16]; accumulator element v
+// of lane l is C[l/16 + 4v][l%16].
+//
+// Tiling (template): BM x BN output tile per workgroup of WM x WN waves, each
+// wave owning (BM/WM) x (BN/WN) as 16x16 MFMA blocks; BK = 16 with two LDS
+// buffers (one barrier per K-slice: the next slice is fetched into registers
+// while the current one is multiplied, then stored into the other buffer).
+// LDS rows are padded to 16 (mod 32) doubles so the four k-rows a fragment
+// read touches fall in disjoint bank halves.  The large-problem shape
+// 96 x 128 gives exactly 256 workgroups (one per CU) for the 1536 x 2048
+// transforms of the 2048x1536-cell grid.
 #include "fribo_internal.h"
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-#define GBM 64
-#define GBN 64
 #define GBK 16
-#define GPAD 16
 
-__global__ __launch_bounds__(256) void k_gemm_nt(int M, int N, int K, const double* __restrict__ A, int lda,
-                                                 const double* __restrict__ B, int ldb, double* __restrict__ C,
-                                                 int ldc, const DevScalars* S)
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, const double* __restrict__ A, int lda,
+                                                          const double* __restrict__ B, int ldb,
+                                                          double* __restrict__ C, int ldc, const DevScalars* S)
 {
+    constexpr int NT = 64 * WM * WN;
+    constexpr int TM = BM / WM, TN = BN / WN;     // wave tile
+    constexpr int RM = TM / 16, RN = TN / 16;     // MFMA blocks per wave
+    constexpr int LA = BM + ((16 - BM % 32) + 32) % 32;
+    constexpr int LB = BN + ((16 - BN % 32) + 32) % 32;
+    constexpr int NA = (BM * GBK + NT - 1) / NT;  // A elements per thread per slice
+    constexpr int NB = (BN * GBK + NT - 1) / NT;
+    static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile");
     if (S && S->err) return;
-    __shared__ double As[GBK][GBM + GPAD];
-    __shared__ double Bs[GBK][GBN + GPAD];
+    __shared__ double As[2][GBK][LA];
+    __shared__ double Bs[2][GBK][LB];
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int i0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
-    // loader mapping: column kk = t/16, rows (t%16)*4 .. +3
-    const int lk = t >> 4, lr = (t & 15) * 4;
-    double ra[4], rb[4];
+    const int wm = wave % WM, wn = wave / WM;
+    const int i0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    double ra[NA], rb[NB];
 
     auto load = [&](int k0) {
-        const int kk = k0 + lk;
-        const bool kin = kk < K;
-        const double* pa = A + (size_t)kk * lda + i0 + lr;
-        const double* pb = B + (size_t)kk * ldb + n0 + lr;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            ra[v] = (kin && i0 + lr + v < M) ? pa[v] : 0.0;
-            rb[v] = (kin && n0 + lr + v < N) ? pb[v] : 0.0;
+        for (int u = 0; u < NA; ++u) {
+            int e = t + u * NT;
+            int kk = e / BM, r = e % BM;
+            bool ok = (e < BM * GBK) && (k0 + kk < K) && (i0 + r < M);
+            ra[u] = ok ? A[(size_t)(k0 + kk) * lda + i0 + r] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            int e = t + u * NT;
+            int kk = e / BN, r = e % BN;
+            bool ok = (e < BN * GBK) && (k0 + kk < K) && (n0 + r < N);
+            rb[u] = ok ? B[(size_t)(k0 + kk) * ldb + n0 + r] : 0.0;
         }
     };
-    auto store = [&]() {
+    auto store = [&](int buf) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            As[lk][lr + v] = ra[v];
-            Bs[lk][lr + v] = rb[v];
+        for (int u = 0; u < NA; ++u) {
+            int e = t + u * NT;
+            if (e < BM * GBK) As[buf][e / BM][e % BM] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            int e = t + u * NT;
+            if (e < BN * GBK) Bs[buf][e / BN][e % BN] = rb[u];
         }
     };
 
-    f64x4 acc[2][2];
+    f64x4 acc[RM][RN];
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < RM; ++r)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) acc[r][c] = (f64x4){0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < RN; ++c) acc[r][c] = (f64x4){0.0, 0.0, 0.0, 0.0};
 
     load(0);
-    store();
+    store(0);
     __syncthreads();
+    int buf = 0;
     for (int k0 = 0; k0 < K; k0 += GBK) {
         const bool more = k0 + GBK < K;
         if (more) load(k0 + GBK);
 #pragma unroll
         for (int k4 = 0; k4 < GBK; k4 += 4) {
             const int kr = k4 + (lane >> 4);
-            double a0 = As[kr][wm * 32 + (lane & 15)];
-            double a1 = As[kr][wm * 32 + 16 + (lane & 15)];
-            double b0 = Bs[kr][wn * 32 + (lane & 15)];
-            double b1 = Bs[kr][wn * 32 + 16 + (lane & 15)];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            double a[RM], b[RN];
+#pragma unroll
+            for (int r = 0; r < RM; ++r) a[r] = As[buf][kr][wm * TM + r * 16 + (lane & 15)];
+#pragma unroll
+            for (int c = 0; c < RN; ++c) b[c] = Bs[buf][kr][wn * TN + c * 16 + (lane & 15)];
+#pragma unroll
+            for (int r = 0; r < RM; ++r)
+#pragma unroll
+                for (int c = 0; c < RN; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
         }
+        if (more) store(buf ^ 1);
         __syncthreads();
-        if (more) {
-            store();
-            __syncthreads();
-        }
+        buf ^= 1;
     }
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < RM; ++r)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const int col = n0 + wn * 32 + c * 16 + (lane & 15);
+        for (int c = 0; c < RN; ++c) {
+            const int col = n0 + wn * TN + c * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = i0 + wm * 32 + r * 16 + (lane >> 4) + 4 * v;
+                const int row = i0 + wm * TM + r * 16 + (lane >> 4) + 4 * v;
                 if (row < M && col < N) C[(size_t)col * ldc + row] = acc[r][c][v];
             }
         }
 }
 
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
 void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                     const DevScalars* S, hipStream_t st)
 {
-    dim3 g((M + GBM - 1) / GBM, (N + GBN - 1) / GBN);
-    hipLaunchKernelGGL(k_gemm_nt, g, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, S);
+    if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
+        hipLaunchKernelGGL((k_gemm_nt<96, 128, 2, 4>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M, N, K,
+                           A, lda, B, ldb, C, ldc, S);
+    } else if ((long)cdiv(M, 64) * cdiv(N, 64) >= 96) {
+        hipLaunchKernelGGL((k_gemm_nt<64, 64, 2, 2>), dim3(cdiv(M, 64), cdiv(N, 64)), dim3(256), 0, st, M, N, K, A,
+                           lda, B, ldb, C, ldc, S);
+    } else {
+        hipLaunchKernelGGL((k_gemm_nt<32, 32, 2, 2>), dim3(cdiv(M, 32), cdiv(N, 32)), dim3(256), 0, st, M, N, K, A,
+                           lda, B, ldb, C, ldc, S);
+    }
 }

@@ -21,32 +21,44 @@ struct DevScalars {
     double qinf, qout, area;
     double ugkks, ugkkn;
     int err;                 // first error code raised on the device
-    int nseg[2];             // MLS segments per direction
-    int nent[2];             // valid sorted MLS entries per direction
-    int pad;
+    int pad[5];
 };
 
-// Precomputed, line-invariant factors of one partitioned k-direction system
+// Precomputed, line-invariant factors of one partitioned tridiagonal system
 // (rows r = 0..n-1, normalised diagonal 1).  Chunk p covers rows
-// [cs[p], cs[p+1]); its last row is the separator X_p.
+// [cs[p], cs[p+1]); its last row is the separator X_p, the others are its
+// interior rows.  Row arrays are stored either by row (k-direction kernels,
+// whose waves read one row at a time) or lane-interleaved, row t of chunk p
+// at [t*64 + p] (j-direction kernels, whose lanes own the chunks).
 struct KSys {
     int n;                   // rows of the system
     int nch;                 // chunks (<= FRIBO_KMAXCH)
-    int kofs;                // k index of row 0
+    int kofs;                // k (or j) index of row 0
     int maxm;                // largest chunk length
+    int inter;               // 1: lane-interleaved row arrays
+    int src;                 // coefficient source: 0 am3sk.., 1 am2j.., 2 am3j..
     int* cs;                 // [nch+1] chunk starts
-    double* a;               // [n] sub-diagonal (0 at row 0)
-    double* c;               // [n] super-diagonal (0 at row n-1)
-    double* S;               // [n] rhs scaling 1/(1 - betadx*ac)
-    double* cp;              // [n] forward multipliers c'
-    double* inv;             // [n] 1/pivot
-    double* vp;              // [n] forward left-coupling vector v'
-    double* g;               // [n] weights giving y_s from y' (sum g_i y'_i)
-    double* wl;              // [nch] w' at the last interior row
+    double* a;               // sub-diagonal (0 at row 0)
+    double* c;               // super-diagonal (0 at row n-1)
+    double* S;               // rhs scaling 1/(1 - betadx*ac)
+    double* cp;              // forward multipliers c'
+    double* inv;             // 1/pivot
+    double* vp;              // forward left-coupling vector v'
+    double* g;               // weights giving y_s from y' (sum g_i y'_i)
+    double* wl;              // [64] w' at the last interior row of each chunk
     double* k1;              // [6*64] PCR multipliers (lower)
     double* k2;              // [6*64] PCR multipliers (upper)
     double* ib;              // [64] 1/beta after PCR
     double* bc;              // [2] couplings of rows 0 / n-1 to the Dirichlet ends
+};
+
+// Fixed partition of the Poisson j-lines (n2m rows; the per-mode matrices
+// are factored in registers by the line kernel).  Coefficients of row t of
+// chunk p are lane-interleaved at [t*64 + p].
+struct PSys {
+    int n, nch, maxm, stride;
+    int* cs;                 // [nch+1]
+    double *am, *ac, *ap;    // amphj, acphj, apphj, interleaved
 };
 
 struct FluidDev {
@@ -77,8 +89,12 @@ struct FluidDev {
     double *cfl_part;        // [2*FRIBO_NPART]
     double *pr_part;         // [npr_part]
     int npr_part;
-    // k-direction systems: [0] q2, [1] q3
+    // Helmholtz systems: ks[0] q2 along k, ks[1] q3 along k,
+    // js[0] q2 along j, js[1] q3 along j; Poisson j-lines
     KSys ks[2];
+    KSys js[2];
+    PSys ps;
+    int jstride;             // LDS chunk stride of the j-line kernels (odd)
     double *kYs, *kYe, *kDe, *kX;  // [2][64][n2m] reduced-system data
     // markers
     double *lmcc, *lmvel, *lmA;    // (nlm,2) column-major
@@ -88,9 +104,7 @@ struct FluidDev {
     int *active;                   // [nlm]
     double *contrib;               // [2][nlm*9]
     unsigned long long *keys;      // [2][maxkeys]
-    int *segstart;                 // [2][maxkeys+1]
-    int *segcell;                  // [2][maxkeys]
-    double *segval;                // [2][maxkeys]
+    double *segval;                // [2][maxkeys] cell value at segment heads
     int maxkeys;
     DevScalars* S;
 };
@@ -126,4 +140,6 @@ void launch_jsolve_poisson(const FluidDev& F, hipStream_t st);
 void launch_ksolve(const FluidDev& F, hipStream_t st);
 void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc, const DevScalars* S, hipStream_t st);
-size_t jsolve_lds_bytes(int n);
+size_t jsolve_lds_bytes(const FluidDev& F);
+size_t sysprep_lds_bytes(const FluidDev& F);
+void launch_bc(const FluidDev& F, int n, int ns, double dt_in, double dt_o, hipStream_t st);

@@ -9,8 +9,8 @@
 //               utils_math.f90:637-725) and the momentum coefficient cl
 //               (forcing_MLS.F90:72-81); one thread per (marker, direction)
 //   k_mls_sort  orders the 9 contributions of every marker by (cell, marker)
-//               (bitonic sort in LDS) and cuts the sorted list into per-cell
-//               segments
+//               (bitonic sort: LDS blocks plus global merge passes); equal
+//               cells form contiguous segments
 // Every substep (interp_spread, forcing_MLS.F90:135-182):
 //   k_mls_interp  u_m = sum phi q*f, f = (V - u_m)/dt, contribution cl*phi*f
 //   k_mls_spread  one thread per touched cell sums its contributions in
@@ -241,83 +241,60 @@ __global__ __launch_bounds__(64) void k_mls_sup(const FluidDev F)
     }
 }
 
-// One workgroup per direction: bitonic sort of the keys in LDS, then
-// segment heads by a block-wide scan.
-__global__ __launch_bounds__(1024) void k_mls_sort(const FluidDev F)
+// Bitonic sort of the (cell, entry) keys of each direction, ascending.
+// Blocks of SORT_BLK keys are sorted (or merged) in LDS; the strides of the
+// larger merge stages run as global compare-exchange passes, one thread per
+// pair.  Key order is unique, so the result is deterministic.
+#define SORT_BLK 2048
+
+__global__ __launch_bounds__(1024) void k_mls_sort_local(const FluidDev F, int size0, int first)
 {
-    extern __shared__ unsigned long long skeys[];
-    __shared__ int scan[1024];
-    __shared__ int base_sh;
-    DevScalars* S = F.S;
-    if (S->err) return;
-    const int d = blockIdx.x;
-    const int t = threadIdx.x;
-    const int nk = F.maxkeys;              // power of two
+    __shared__ unsigned long long sk[SORT_BLK];
+    if (F.S->err) return;
+    const int d = blockIdx.y, t = threadIdx.x;
+    const int base = blockIdx.x * SORT_BLK;
     const int nreal = F.nlm * FRIBO_DIMSD;
     unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
-    for (int i = t; i < nk; i += 1024) skeys[i] = (i < nreal) ? gk[i] : KEY_NONE;
+    for (int i = t; i < SORT_BLK; i += 1024) {
+        const int gi = base + i;
+        sk[i] = (!first || gi < nreal) ? gk[gi] : KEY_NONE;
+    }
     __syncthreads();
-    for (int size = 2; size <= nk; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = t; i < nk; i += 1024) {
-                int pj = i ^ stride;
-                if (pj > i) {
-                    bool up = ((i & size) == 0);
-                    unsigned long long a = skeys[i], b = skeys[pj];
-                    if ((a > b) == up) {
-                        skeys[i] = b;
-                        skeys[pj] = a;
-                    }
+    // first pass: full sort of the block (sizes 2..SORT_BLK); later passes:
+    // the strides < SORT_BLK of merge stage size0
+    const int smin = first ? 2 : size0, smax = first ? SORT_BLK : size0;
+    for (int size = smin; size <= smax; size <<= 1) {
+        for (int stride = (size < SORT_BLK ? size : SORT_BLK) >> 1; stride > 0; stride >>= 1) {
+            for (int q = t; q < SORT_BLK / 2; q += 1024) {
+                const int i = (q / stride) * 2 * stride + (q % stride);
+                const int pj = i + stride;
+                const bool up = (((base + i) & size) == 0);
+                const unsigned long long x = sk[i], y = sk[pj];
+                if ((x > y) == up) {
+                    sk[i] = y;
+                    sk[pj] = x;
                 }
             }
             __syncthreads();
         }
     }
-    for (int i = t; i < nk; i += 1024) gk[i] = skeys[i];
-    // segments
-    int* sst = F.segstart + (size_t)d * (F.maxkeys + 1);
-    int* scl = F.segcell + (size_t)d * F.maxkeys;
-    if (t == 0) base_sh = 0;
-    __syncthreads();
-    int nvalid_local = 0;
-    for (int c0 = 0; c0 < nk; c0 += 1024) {
-        int i = c0 + t;
-        unsigned long long k = (i < nk) ? skeys[i] : KEY_NONE;
-        int head = 0;
-        if (k != KEY_NONE) {
-            ++nvalid_local;
-            head = (i == 0) || ((skeys[i - 1] >> ENT_BITS) != (k >> ENT_BITS));
-        }
-        scan[t] = head;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int v = (t >= off) ? scan[t - off] : 0;
-            __syncthreads();
-            scan[t] += v;
-            __syncthreads();
-        }
-        int base = base_sh;
-        if (head) {
-            int sid = base + scan[t] - 1;
-            sst[sid] = i;
-            scl[sid] = (int)(k >> ENT_BITS);
-        }
-        __syncthreads();
-        if (t == 1023) base_sh = base + scan[1023];
-        __syncthreads();
-    }
-    // count valid entries
-    scan[t] = nvalid_local;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (t < s) scan[t] += scan[t + s];
-        __syncthreads();
-    }
-    if (t == 0) {
-        int nseg = base_sh;
-        sst[nseg] = scan[0];
-        S->nseg[d] = nseg;
-        S->nent[d] = scan[0];
+    for (int i = t; i < SORT_BLK; i += 1024) gk[base + i] = sk[i];
+}
+
+__global__ __launch_bounds__(256) void k_mls_sort_global(const FluidDev F, int size, int stride)
+{
+    if (F.S->err) return;
+    const int d = blockIdx.y;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= F.maxkeys / 2) return;
+    unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
+    const int i = (q / stride) * 2 * stride + (q % stride);
+    const int pj = i + stride;
+    const bool up = ((i & size) == 0);
+    const unsigned long long x = gk[i], y = gk[pj];
+    if ((x > y) == up) {
+        gk[i] = y;
+        gk[pj] = x;
     }
 }
 
@@ -326,7 +303,14 @@ void launch_mls_setup(const FluidDev& F, hipStream_t st)
     if (F.nlm <= 0) return;
     hipLaunchKernelGGL(k_mls_idx, dim3(F.nlm, 4), dim3(64), 0, st, F);
     hipLaunchKernelGGL(k_mls_sup, dim3((2 * F.nlm + 63) / 64), dim3(64), 0, st, F);
-    hipLaunchKernelGGL(k_mls_sort, dim3(2), dim3(1024), (size_t)F.maxkeys * sizeof(unsigned long long), st, F);
+    const int nblk = F.maxkeys / SORT_BLK;
+    hipLaunchKernelGGL(k_mls_sort_local, dim3(nblk, 2), dim3(1024), 0, st, F, 0, 1);
+    for (int size = 2 * SORT_BLK; size <= F.maxkeys; size <<= 1) {
+        for (int stride = size / 2; stride >= SORT_BLK; stride >>= 1)
+            hipLaunchKernelGGL(k_mls_sort_global, dim3((F.maxkeys / 2 + 255) / 256, 2), dim3(256), 0, st, F, size,
+                               stride);
+        hipLaunchKernelGGL(k_mls_sort_local, dim3(nblk, 2), dim3(1024), 0, st, F, size, 0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -360,31 +344,40 @@ __global__ __launch_bounds__(64) void k_mls_interp(const FluidDev F)
     for (int m = 1; m <= FRIBO_DIMSD; ++m) con[(size_t)(n - 1) * FRIBO_DIMSD + (m - 1)] = cl * PHI(n, d, m) * f;
 }
 
+// One thread per sorted entry; the first entry of each cell (segment head)
+// sums the cell's contributions in marker order.
 __global__ __launch_bounds__(64) void k_mls_spread(const FluidDev F)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const int d = blockIdx.y;
-    const int s = blockIdx.x * 64 + threadIdx.x;
-    if (s >= S->nseg[d]) return;
-    const int* sst = F.segstart + (size_t)d * (F.maxkeys + 1);
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= F.nlm * FRIBO_DIMSD) return;
     const unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
+    const unsigned long long key = gk[i];
+    if (key == KEY_NONE) return;
+    const unsigned long long cell = key >> ENT_BITS;
+    if (i > 0 && (gk[i - 1] >> ENT_BITS) == cell) return;
     const double* con = F.contrib + (size_t)d * F.nlm * FRIBO_DIMSD;
+    const int nreal = F.nlm * FRIBO_DIMSD;
     double acc = 0.0;
-    for (int i = sst[s]; i < sst[s + 1]; ++i) acc = acc + con[gk[i] & ENT_MASK];
-    const int cell = F.segcell[(size_t)d * F.maxkeys + s];
-    const int j = cell % F.n2 + 1, k = cell / F.n2 + 1;
+    for (int e = i; e < nreal; ++e) {
+        const unsigned long long ke = gk[e];
+        if (ke == KEY_NONE || (ke >> ENT_BITS) != cell) break;
+        acc = acc + con[ke & ENT_MASK];
+    }
+    const int j = (int)(cell % F.n2) + 1, k = (int)(cell / F.n2) + 1;
     const double fval = (k == F.n3m) ? 0.0 : acc;
-    F.segval[(size_t)d * F.maxkeys + s] = fval;
+    F.segval[(size_t)d * F.maxkeys + i] = fval;
     if (k == F.n3m) return;
     if (d == 0) {
         if (j >= 2 && j <= F.n2m && k >= 1) {
-            size_t c = fidx(F, j, k);
+            const size_t c = fidx(F, j, k);
             F.rhs2[c] = F.rhs2[c] + fval * S->dt;
         }
     } else {
         if (j >= 1 && j <= F.n2m && k >= 2) {
-            size_t c = fidx(F, j, k);
+            const size_t c = fidx(F, j, k);
             F.rhs3[c] = F.rhs3[c] + fval * S->dt;
         }
     }

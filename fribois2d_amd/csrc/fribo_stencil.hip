@@ -77,103 +77,18 @@ void launch_cfl(const FluidDev& F, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// Line-invariant factors of one partitioned k-system (run by one wave).
-// Row r couples a[r] x[r-1] + x[r] + c[r] x[r+1]; chunk p = rows
-// [cs[p], cs[p+1]), interior rows cs[p]..e-1, separator e = cs[p+1]-1.
-__device__ void ksys_prepare(const KSys& K, int lane)
-{
-    double alpha = 0.0, beta = 1.0, gamma = 0.0;
-    double vs = 0.0, ws = 0.0;
-    int p = lane;
-    if (p < K.nch) {
-        int s = K.cs[p], e = K.cs[p + 1] - 1;
-        double cpp = 0.0, vpp = 0.0, gg = 1.0, wlast = 0.0;
-        for (int i = s; i <= e - 1; ++i) {
-            double den = 1.0 - K.a[i] * cpp;
-            double iv = 1.0 / den;
-            double cpi = K.c[i] * iv;
-            double vpi = (i == s) ? (-K.a[i]) * iv : (-K.a[i] * vpp) * iv;
-            K.inv[i] = iv;
-            K.cp[i] = cpi;
-            K.vp[i] = vpi;
-            K.g[i] = gg;
-            gg = -cpi * gg;
-            cpp = cpi;
-            vpp = vpi;
-            if (i == e - 1) wlast = (-K.c[i]) * iv;
-        }
-        K.wl[p] = wlast;
-        // backward sweeps for v and w at the first interior row
-        double vb = vpp, wb = wlast;   // values at e-1
-        for (int i = e - 2; i >= s; --i) {
-            vb = K.vp[i] - K.cp[i] * vb;
-            wb = -K.cp[i] * wb;
-        }
-        vs = vb;
-        ws = wb;
-        // separator row e: v, w at row e-1 are vpp, wlast
-        double ae = K.a[e];
-        alpha = ae * vpp;
-        beta = 1.0 + ae * wlast;    // + c_e * v^{p+1}_s added below
-        gamma = 0.0;                // c_e * w^{p+1}_s added below
-    }
-    // exchange v_s, w_s with lane p+1
-    double vs_n = __shfl_down(vs, 1, 64);
-    double ws_n = __shfl_down(ws, 1, 64);
-    if (p < K.nch) {
-        int e = K.cs[p + 1] - 1;
-        double ce = K.c[e];
-        if (p + 1 < K.nch) {
-            beta = beta + ce * vs_n;
-            gamma = ce * ws_n;
-        }
-    }
-    // PCR on the (alpha, beta, gamma) rows; record the multipliers
-    int li = 0;
-    for (int h = 1; h < 64; h <<= 1, ++li) {
-        double am = __shfl_up(alpha, h, 64), bm = __shfl_up(beta, h, 64), gm = __shfl_up(gamma, h, 64);
-        double ap = __shfl_down(alpha, h, 64), bp = __shfl_down(beta, h, 64), gp = __shfl_down(gamma, h, 64);
-        if (lane - h < 0) { am = 0.0; bm = 1.0; gm = 0.0; }
-        if (lane + h >= 64) { ap = 0.0; bp = 1.0; gp = 0.0; }
-        double m1 = alpha / bm, m2 = gamma / bp;
-        K.k1[li * 64 + lane] = m1;
-        K.k2[li * 64 + lane] = m2;
-        double na = -m1 * am;
-        double ng = -m2 * gp;
-        double nb = beta - m1 * gm - m2 * ap;
-        alpha = na;
-        gamma = ng;
-        beta = nb;
-    }
-    K.ib[lane] = 1.0 / beta;
-}
+// Step scalars shared by every workgroup of k_bc (fluid_tsch.F90:34-77): each
+// workgroup reduces the CFL partials itself (a max, exact in any order) so the
+// system-preparation workgroups need no inter-workgroup synchronisation.
+struct StepScalars {
+    double dt, cflm, q3sum, beta, al, ga, ro, betadx;
+    int err;
+};
 
-// Row coefficients of the two k-systems for this substep (betadx known).
-// q2 (solve_fluid.F90:138-152): rows kc = 1..n3m.
-// q3 (solve_fluid.F90:215-231): interior rows kc = 2..n3m of the n3 system.
-__device__ void ksys_rows(const FluidDev& F, const KSys& K, double betadx, int t, int nt)
+__device__ StepScalars step_scalars(const FluidDev& F, const DevScalars* S, int n, int ns, double dt_in,
+                                    double dt_o, double* sh, double* sh2)
 {
-    for (int r = t; r < K.n; r += nt) {
-        int kc = r + K.kofs;
-        double b = 1.0 / (1.0 - F.ac3sk[kc] * betadx);
-        double am = -F.am3sk[kc] * betadx * b;
-        double ap = -F.ap3sk[kc] * betadx * b;
-        K.S[r] = b;
-        if (r == 0) K.bc[0] = am;
-        if (r == K.n - 1) K.bc[1] = ap;
-        K.a[r] = (r == 0) ? 0.0 : am;
-        K.c[r] = (r == K.n - 1) ? 0.0 : ap;
-    }
-}
-
-// One workgroup: finalize CFL/dt (substep 1), RK coefficients, boucdq and
-// the k-system factors.
-__global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int ns, double dt_in,
-                                             double dt_o_in)
-{
-    __shared__ double sh[1024];
-    __shared__ double sh2[1024];
-    DevScalars* S = F.S;
+    __shared__ StepScalars out;
     const int t = threadIdx.x;
     if (n == 1) {
         double m = 1.0e-8, s = 0.0;
@@ -183,11 +98,13 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int ns, do
         }
         block_reduce_max<1024>(sh, m);
         block_reduce_sum<1024>(sh2, s);
-        if (t == 0) {
-            double cflm = sh[0];
-            double dt = dt_in;
-            int err = 0;
-            S->q3avg = sh2[0] / (F.rext2 * F.alx3);
+    }
+    if (t == 0) {
+        StepScalars r;
+        r.err = 0;
+        if (n == 1) {
+            double cflm = sh[0], dt = dt_in;
+            r.q3sum = sh2[0];
             if (!F.idtv) {
                 if (ns == 0) {
                     cflm = cflm * dt;
@@ -195,24 +112,22 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int ns, do
                     dt = F.cflmax / cflm;
                     if (dt > F.dtmax) dt = F.dtmax;
                 }
-                if (dt < 1.0e-8) err = FRIBO_E_DT_TOO_SMALL;
+                if (dt < 1.0e-8) r.err = FRIBO_E_DT_TOO_SMALL;
             } else {
                 cflm = cflm * dt;
-                if (cflm > F.cfllim) err = FRIBO_E_CFL_TOO_LARGE;
+                if (cflm > F.cfllim) r.err = FRIBO_E_CFL_TOO_LARGE;
             }
-            S->cflm = cflm;
-            S->dt = dt;
-            S->dt_o = dt_o_in;
-            S->beta = dt / F.ren * 0.5;
-            if (err) S->err = err;
+            r.dt = dt;
+            r.cflm = cflm;
+            r.beta = dt / F.ren * 0.5;
+        } else {
+            r.dt = S->dt;
+            r.cflm = S->cflm;
+            r.q3sum = 0.0;
+            r.beta = S->beta;
+            if (S->err) r.err = S->err;
         }
-        __syncthreads();
-    }
-    __shared__ double coef[4];
-    __shared__ int errflag;
-    if (t == 0) {
-        errflag = S->err;
-        double dt = S->dt, al, ga, ro;
+        double al, ga, ro;
         if (F.nsst == 3) {
             al = F.alm[n];
             ga = F.gam[n];
@@ -222,26 +137,173 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int ns, do
                 ga = 1.0;
                 ro = 0.0;
             } else {
-                ga = 1.0 + 0.5 * dt / S->dt_o;
-                ro = -0.5 * dt / S->dt_o;
+                ga = 1.0 + 0.5 * r.dt / dt_o;
+                ro = -0.5 * r.dt / dt_o;
             }
             al = ga + ro;
         }
-        S->al = al;
-        S->ga = ga;
-        S->ro = ro;
-        S->alre = al / F.ren;
-        S->betadx = S->beta * al;
-        S->ugkks = 1.0 / (F.g3rm[1] * F.g3rc[1]) * F.dx3q * 2.0 * S->betadx;
-        S->ugkkn = 1.0 / (F.g3rm[F.n3m] * F.g3rc[F.n3]) * F.dx3q * 2.0 * S->betadx;
-        coef[0] = dt;
-        coef[1] = ga;
-        coef[2] = ro;
-        coef[3] = S->betadx;
+        r.al = al;
+        r.ga = ga;
+        r.ro = ro;
+        r.betadx = r.beta * al;
+        out = r;
     }
     __syncthreads();
-    if (errflag) return;
-    const double dt = coef[0], ga = coef[1], ro = coef[2], betadx = coef[3];
+    return out;
+}
+
+// Line-invariant factors of one partitioned Helmholtz system for this
+// substep (solve_fluid.F90:77-87,126-143,185-200,260-278: rows normalised by
+// 1/(1 - betadx*ac)).  Row coefficients go to LDS; lane p of the first wave
+// then eliminates chunk p (forward multipliers, left-coupling vector and the
+// weights that turn forward values into first-row values), and the wave
+// builds the separator system and its PCR multipliers.
+__device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, double* lds)
+{
+    const int t = threadIdx.x;
+    const int n = K.n;
+    double* la = lds;
+    double* lc = lds + n;
+    double* lS = lds + 2 * n;
+    const double *am, *ac, *ap;
+    if (K.src == 0) { am = F.am3sk; ac = F.ac3sk; ap = F.ap3sk; }
+    else if (K.src == 1) { am = F.am2j; ac = F.ac2j; ap = F.ap2j; }
+    else { am = F.am3j; ac = F.ac3j; ap = F.ap3j; }
+    for (int r = t; r < n; r += blockDim.x) {
+        const int idx = r + K.kofs;
+        const double b = 1.0 / (1.0 - ac[idx] * betadx);
+        const double amv = -am[idx] * betadx * b;
+        const double apv = -ap[idx] * betadx * b;
+        la[r] = (r == 0) ? 0.0 : amv;
+        lc[r] = (r == n - 1) ? 0.0 : apv;
+        lS[r] = b;
+        if (r == 0) K.bc[0] = amv;
+        if (r == n - 1) K.bc[1] = apv;
+        if (!K.inter) {
+            K.S[r] = b;
+            K.a[r] = la[r];
+            K.c[r] = lc[r];
+        }
+    }
+    __syncthreads();
+    if (t >= 64) return;
+    const int p = t;
+    double alpha = 0.0, beta = 1.0, gamma = 0.0, vs = 0.0, ws = 0.0, ce = 0.0;
+    if (p < K.nch) {
+        const int s = K.cs[p], e = K.cs[p + 1] - 1;
+        double cpp = 0.0, vpp = 1.0, gg = 1.0, wl = 0.0, glast = 1.0, vsum = 0.0;
+        for (int i = s; i <= e - 1; ++i) {
+            const double a = la[i], c = lc[i];
+            const double iv = 1.0 / (1.0 - a * cpp);
+            const double cpi = c * iv;
+            const double vpi = (-a * vpp) * iv;
+            const int idx = K.inter ? (i - s) * 64 + p : i;
+            K.inv[idx] = iv;
+            K.cp[idx] = cpi;
+            K.vp[idx] = vpi;
+            K.g[idx] = gg;
+            if (K.inter) {
+                K.S[idx] = lS[i];
+                K.a[idx] = a;
+                K.c[idx] = c;
+            }
+            vsum = vsum + gg * vpi;
+            if (i == e - 1) {
+                wl = (-c) * iv;
+                glast = gg;
+            }
+            gg = -cpi * gg;
+            cpp = cpi;
+            vpp = vpi;
+        }
+        if (K.inter) {
+            const int idx = (e - s) * 64 + p;
+            K.S[idx] = lS[e];
+            K.a[idx] = la[e];
+            K.c[idx] = lc[e];
+        }
+        K.wl[p] = wl;
+        vs = vsum;
+        ws = glast * wl;
+        const double ae = la[e];
+        ce = lc[e];
+        alpha = ae * vpp;
+        beta = 1.0 + ae * wl;
+    }
+    const double vs_n = __shfl_down(vs, 1, 64);
+    const double ws_n = __shfl_down(ws, 1, 64);
+    if (p + 1 < K.nch) {
+        beta = beta + ce * vs_n;
+        gamma = ce * ws_n;
+    }
+    int li = 0;
+    for (int h = 1; h < 64; h <<= 1, ++li) {
+        double amn = __shfl_up(alpha, h, 64), bm = __shfl_up(beta, h, 64), gm = __shfl_up(gamma, h, 64);
+        double apn = __shfl_down(alpha, h, 64), bp = __shfl_down(beta, h, 64), gp = __shfl_down(gamma, h, 64);
+        if (p - h < 0) { amn = 0.0; bm = 1.0; gm = 0.0; }
+        if (p + h >= 64) { apn = 0.0; bp = 1.0; gp = 0.0; }
+        const double m1 = alpha / bm, m2 = gamma / bp;
+        K.k1[li * 64 + p] = m1;
+        K.k2[li * 64 + p] = m2;
+        const double na = -m1 * amn, ng = -m2 * gp;
+        beta = beta - m1 * gm - m2 * apn;
+        alpha = na;
+        gamma = ng;
+    }
+    K.ib[p] = 1.0 / beta;
+}
+
+size_t sysprep_lds_bytes(const FluidDev& F)
+{
+    int n = F.ks[0].n;
+    for (int i = 0; i < 2; ++i) {
+        if (F.ks[i].n > n) n = F.ks[i].n;
+        if (F.js[i].n > n) n = F.js[i].n;
+    }
+    return (size_t)3 * n * sizeof(double);
+}
+
+// Workgroup 0: finalize CFL/dt (substep 1), RK coefficients, boucdq.
+// Workgroups 1..4: factors of the four Helmholtz systems of this substep.
+__global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int ns, double dt_in, double dt_o_in)
+{
+    extern __shared__ double dyn[];
+    __shared__ double sh[1024];
+    __shared__ double sh2[1024];
+    DevScalars* S = F.S;
+    const int t = threadIdx.x;
+    const StepScalars r = step_scalars(F, S, n, ns, dt_in, dt_o_in, sh, sh2);
+    if (r.err) {
+        if (blockIdx.x == 0 && t == 0 && n == 1) {
+            S->cflm = r.cflm;
+            S->dt = r.dt;
+            S->err = r.err;
+        }
+        return;
+    }
+    if (blockIdx.x > 0) {
+        const int sys = blockIdx.x - 1;
+        const KSys& K = (sys < 2) ? F.ks[sys] : F.js[sys - 2];
+        sys_prepare(F, K, r.betadx, dyn);
+        return;
+    }
+    if (t == 0) {
+        if (n == 1) {
+            S->q3avg = r.q3sum / (F.rext2 * F.alx3);
+            S->cflm = r.cflm;
+            S->dt = r.dt;
+            S->dt_o = dt_o_in;
+            S->beta = r.beta;
+        }
+        S->al = r.al;
+        S->ga = r.ga;
+        S->ro = r.ro;
+        S->alre = r.al / F.ren;
+        S->betadx = r.betadx;
+        S->ugkks = 1.0 / (F.g3rm[1] * F.g3rc[1]) * F.dx3q * 2.0 * r.betadx;
+        S->ugkkn = 1.0 / (F.g3rm[F.n3m] * F.g3rc[F.n3]) * F.dx3q * 2.0 * r.betadx;
+    }
+    const double dt = r.dt, ga = r.ga, ro = r.ro;
 
     // boucdq (iobcs.F90:22-50)
     for (int j = 1 + t; j <= F.n2; j += 1024) {
@@ -276,18 +338,9 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int ns, do
         S->qout = tqout;
         S->area = tarea;
     }
-
-    // k-direction systems for this substep
-    ksys_rows(F, F.ks[0], betadx, t, 1024);
-    ksys_rows(F, F.ks[1], betadx, t, 1024);
-    __syncthreads();
-    __threadfence_block();
-    if (t < 64) ksys_prepare(F.ks[0], t);
-    else if (t < 128) ksys_prepare(F.ks[1], t - 64);
     // q2 wall rows: q2(1,:) = q2(n2,:) = 0 on the ghost planes (helsolve.F90:32-33)
-    if (t >= 128 && t < 128 + 6) {
-        int kk = t - 128;
-        int k = (kk < 3) ? kk - 2 : F.n3m + 1 + (kk - 3);
+    if (t < 6) {
+        int k = (t < 3) ? t - 2 : F.n3m + 1 + (t - 3);
         F.q2[fidx(F, 1, k)] = 0.0;
         F.q2[fidx(F, F.n2, k)] = 0.0;
     }
@@ -295,7 +348,7 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int ns, do
 
 void launch_bc(const FluidDev& F, int n, int ns, double dt_in, double dt_o, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_bc, dim3(1), dim3(1024), 0, st, F, n, ns, dt_in, dt_o);
+    hipLaunchKernelGGL(k_bc, dim3(5), dim3(1024), sysprep_lds_bytes(F), st, F, n, ns, dt_in, dt_o);
 }
 
 // ---------------------------------------------------------------------------

@@ -18,85 +18,170 @@
 // precomputed multipliers) -> chunk back-substitution fused with q += dq.
 #include "fribo_internal.h"
 
-size_t jsolve_lds_bytes(int n) { return (size_t)3 * n * sizeof(double); }
+size_t jsolve_lds_bytes(const FluidDev& F) { return (size_t)64 * F.jstride * sizeof(double); }
 
-// Coefficients of row r of a j-line (normalised diagonal 1).
-struct JCoef {
-    const double *am, *ac, *ap;
-    double s0, s1, s2;   // S = 1/(s0 + s1*ac[j]),  a = (s2*am[j])*S,  c = (s2*ap[j])*S
-    int jofs;            // j index of row 0
-    int n;
-    __device__ inline void row(int r, double& a, double& c, double& S) const
-    {
-        int j = r + jofs;
-        S = 1.0 / (s0 + s1 * ac[j]);
-        a = (r == 0) ? 0.0 : s2 * am[j] * S;
-        c = (r == n - 1) ? 0.0 : s2 * ap[j] * S;
-    }
-};
+// Row r of a j-line belongs to chunk owner(r) when chunk starts are
+// floor(p*n/nch).
+__device__ inline int chunk_owner(int r, int n, int nch) { return (int)((((long)r + 1) * nch - 1) / n); }
 
-// Partitioned solve of one line held in LDS (D: scaled rhs in, solution out).
-__device__ void wave_line_solve(const JCoef& Q, double* C, double* D, double* V, int lane, bool pin_last)
+// Coalesced global <-> LDS staging of one line in chunk layout: row t of
+// chunk p at lds[p*stride + t] (odd stride: conflict-free lane access).
+__device__ inline void line_to_lds(const double* __restrict__ src, double* lds, const int* cs, int n, int nch,
+                                   int stride, int lane)
 {
-    const int n = Q.n;
-    int P = n / 2;
-    if (P > 64) P = 64;
-    if (P < 1) P = 1;
-    const int p = lane;
-    const bool act = p < P;
-    double yp = 0.0, vpp = 1.0, wl = 0.0, ys = 0.0, vs = 0.0, ws = 0.0;
-    double alpha = 0.0, beta = 1.0, gamma = 0.0, delta = 0.0;
-    double ae = 0.0, ce = 0.0, de = 0.0;
-    int s = 0, e = 0;
-    if (act) {
-        s = (int)(((long)p * n) / P);
-        e = (int)(((long)(p + 1) * n) / P) - 1;
-        double cpp = 0.0;
-        vpp = 1.0;
-        yp = 0.0;
-        for (int i = s; i <= e - 1; ++i) {
-            double a, c, S;
-            Q.row(i, a, c, S);
-            double iv = 1.0 / (1.0 - a * cpp);
-            double cpi = c * iv;
-            double ypi = (D[i] - a * yp) * iv;
-            double vpi = (-a * vpp) * iv;
-            C[i] = cpi;
-            D[i] = ypi;
-            V[i] = vpi;
-            cpp = cpi;
-            yp = ypi;
-            vpp = vpi;
-            if (i == e - 1) wl = (-c) * iv;
-        }
-        double yb = yp, vb = vpp, wb = wl;
-        for (int i = e - 2; i >= s; --i) {
-            yb = D[i] - C[i] * yb;
-            vb = V[i] - C[i] * vb;
-            wb = -C[i] * wb;
-        }
-        ys = yb;
-        vs = vb;
-        ws = wb;
-        double S;
-        Q.row(e, ae, ce, S);
-        de = D[e];
+    for (int r = lane; r < n; r += 64) {
+        const int p = chunk_owner(r, n, nch);
+        lds[p * stride + (r - cs[p])] = src[r];
     }
-    double ys_n = __shfl_down(ys, 1, 64);
-    double vs_n = __shfl_down(vs, 1, 64);
-    double ws_n = __shfl_down(ws, 1, 64);
-    if (act) {
-        const bool nxt = (p + 1) < P;
-        alpha = ae * vpp;
-        beta = 1.0 + ae * wl + (nxt ? ce * vs_n : 0.0);
-        gamma = nxt ? ce * ws_n : 0.0;
-        delta = de - ae * yp - (nxt ? ce * ys_n : 0.0);
-        if (pin_last && p == P - 1) {
-            alpha = 0.0;
-            beta = 1.0;
-            gamma = 0.0;
-            delta = 0.0;
+}
+
+__device__ inline void lds_to_line(double* __restrict__ dst, const double* lds, const int* cs, int n, int nch,
+                                   int stride, int lane)
+{
+    for (int r = lane; r < n; r += 64) {
+        const int p = chunk_owner(r, n, nch);
+        dst[r] = lds[p * stride + (r - cs[p])];
+    }
+}
+
+// Helmholtz j-lines (solq2j / solq3j): q2 for k = 1..n3m (rows j = 2..n2m;
+// rows 1 and n2 are the Dirichlet identity rows with zero rhs), then q3 for
+// k = 2..n3m (rows j = 1..n2m).  All factors are line-invariant (k_bc); the
+// wave only runs the rhs through them.
+template <int MAXR>
+__global__ __launch_bounds__(64) void k_jsolve_h(const FluidDev F)
+{
+    extern __shared__ double lds[];
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    const int lane = threadIdx.x, line = blockIdx.x;
+    const int f = line < F.n3m ? 0 : 1;
+    const int k = f == 0 ? line + 1 : line - F.n3m + 2;
+    const KSys& K = F.js[f];
+    const double* src = (f == 0 ? F.rhs2 : F.rhs3) + fidx(F, K.kofs, k);
+    double* dst = (f == 0 ? F.sA : F.sB) + fidx(F, K.kofs, k);
+    const int n = K.n, nch = K.nch, stride = F.jstride;
+    line_to_lds(src, lds, K.cs, n, nch, stride, lane);
+    __syncthreads();
+    const int p = lane;
+    double y[MAXR];
+    double yl = 0.0, acc = 0.0, de = 0.0, ae = 0.0, ce = 0.0;
+    int m = 0;
+    double* row = lds + p * stride;
+    if (p < nch) {
+        m = K.cs[p + 1] - K.cs[p];
+#pragma unroll
+        for (int t = 0; t < MAXR; ++t) {
+            if (t < m - 1) {
+                const int ix = t * 64 + p;
+                yl = (row[t] * K.S[ix] - K.a[ix] * yl) * K.inv[ix];
+                y[t] = yl;
+                acc = acc + K.g[ix] * yl;
+            }
         }
+        const int ix = (m - 1) * 64 + p;
+        de = row[m - 1] * K.S[ix];
+        ae = K.a[ix];
+        ce = K.c[ix];
+    }
+    const double ysn = __shfl_down(acc, 1, 64);
+    double delta = (p < nch) ? de - ae * yl - ((p + 1 < nch) ? ce * ysn : 0.0) : 0.0;
+    int li = 0;
+    for (int h = 1; h < 64; h <<= 1, ++li) {
+        double dm = __shfl_up(delta, h, 64), dp = __shfl_down(delta, h, 64);
+        if (p - h < 0) dm = 0.0;
+        if (p + h >= 64) dp = 0.0;
+        delta = delta - K.k1[li * 64 + p] * dm - K.k2[li * 64 + p] * dp;
+    }
+    const double X = delta * K.ib[p];
+    double Xm = __shfl_up(X, 1, 64);
+    if (p == 0) Xm = 0.0;
+    if (p < nch) {
+        double x = 0.0;
+#pragma unroll
+        for (int t = MAXR - 1; t >= 0; --t) {
+            if (t < m - 1) {
+                const int ix = t * 64 + p;
+                if (t == m - 2) x = y[t] + K.vp[ix] * Xm + K.wl[p] * X;
+                else x = (y[t] + K.vp[ix] * Xm) - K.cp[ix] * x;
+                row[t] = x;
+            }
+        }
+        row[m - 1] = X;
+    }
+    __syncthreads();
+    lds_to_line(dst, lds, K.cs, n, nch, stride, lane);
+}
+
+// Poisson j-lines (phcalc_outflow.F90:34-56): mode m solves
+// (A_y + wr(m)) v = hmn(:,m); each lane factors its chunk in registers
+// (unnormalised rows amphj, acphj+wr, apphj -- the same system the reference
+// normalises by 1/(acphj+wr)), the separators are solved by PCR, and the
+// singular Neumann mode is gauge-fixed by pinning its last unknown to 0.
+template <int MAXR>
+__global__ __launch_bounds__(64) void k_jsolve_p(const FluidDev F)
+{
+    extern __shared__ double lds[];
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    const int lane = threadIdx.x, mode = blockIdx.x + 1;
+    const PSys& Q = F.ps;
+    double* col = F.hmn + (size_t)(mode - 1) * F.P;
+    const double wrm = F.wr[mode];
+    const bool pin = (mode == F.null_mode);
+    const int n = Q.n, nch = Q.nch, stride = F.jstride;
+    line_to_lds(col, lds, Q.cs, n, nch, stride, lane);
+    __syncthreads();
+    const int p = lane;
+    double cp[MAXR], vp[MAXR], y[MAXR];
+    double cpp = 0.0, yl = 0.0, vl = 1.0, gg = 1.0, ys = 0.0, vs = 0.0, ws = 0.0, wl = 0.0;
+    double alpha = 0.0, beta = 1.0, gamma = 0.0, delta = 0.0, ae = 0.0, ce = 0.0, de = 0.0;
+    int m = 0;
+    double* row = lds + p * stride;
+    if (p < nch) {
+        m = Q.cs[p + 1] - Q.cs[p];
+        double glast = 1.0;
+#pragma unroll
+        for (int t = 0; t < MAXR; ++t) {
+            if (t < m - 1) {
+                const int ix = t * 64 + p;
+                const double a = Q.am[ix], b = Q.ac[ix] + wrm, c = Q.ap[ix];
+                const double iv = 1.0 / (b - a * cpp);
+                cpp = c * iv;
+                yl = (row[t] - a * yl) * iv;
+                vl = (-a * vl) * iv;
+                cp[t] = cpp;
+                vp[t] = vl;
+                y[t] = yl;
+                ys = ys + gg * yl;
+                vs = vs + gg * vl;
+                if (t == m - 2) {
+                    wl = (-c) * iv;
+                    glast = gg;
+                }
+                gg = -cpp * gg;
+            }
+        }
+        ws = glast * wl;
+        const int ix = (m - 1) * 64 + p;
+        ae = Q.am[ix];
+        ce = Q.ap[ix];
+        de = row[m - 1];
+        alpha = ae * vl;
+        beta = (Q.ac[ix] + wrm) + ae * wl;
+        delta = de - ae * yl;
+    }
+    const double ysn = __shfl_down(ys, 1, 64), vsn = __shfl_down(vs, 1, 64), wsn = __shfl_down(ws, 1, 64);
+    if (p + 1 < nch) {
+        beta = beta + ce * vsn;
+        gamma = ce * wsn;
+        delta = delta - ce * ysn;
+    }
+    if (pin && p == nch - 1) {
+        alpha = 0.0;
+        beta = 1.0;
+        gamma = 0.0;
+        delta = 0.0;
     }
     for (int h = 1; h < 64; h <<= 1) {
         double am = __shfl_up(alpha, h, 64), bm = __shfl_up(beta, h, 64);
@@ -105,99 +190,55 @@ __device__ void wave_line_solve(const JCoef& Q, double* C, double* D, double* V,
         double gp = __shfl_down(gamma, h, 64), dp = __shfl_down(delta, h, 64);
         if (lane - h < 0) { am = 0.0; bm = 1.0; gm = 0.0; dm = 0.0; }
         if (lane + h >= 64) { ap = 0.0; bp = 1.0; gp = 0.0; dp = 0.0; }
-        double m1 = alpha / bm, m2 = gamma / bp;
-        double na = -m1 * am, ng = -m2 * gp;
-        double nb = beta - m1 * gm - m2 * ap;
-        double nd = delta - m1 * dm - m2 * dp;
+        const double m1 = alpha / bm, m2 = gamma / bp;
+        const double na = -m1 * am, ng = -m2 * gp;
+        const double nb = beta - m1 * gm - m2 * ap;
+        const double nd = delta - m1 * dm - m2 * dp;
         alpha = na;
         beta = nb;
         gamma = ng;
         delta = nd;
     }
-    double X = delta / beta;
+    const double X = delta / beta;
     double Xm = __shfl_up(X, 1, 64);
     if (p == 0) Xm = 0.0;
-    if (act) {
-        double x = yp + vpp * Xm + wl * X;
-        D[e - 1] = x;
-        for (int i = e - 2; i >= s; --i) {
-            x = (D[i] + V[i] * Xm) - C[i] * x;
-            D[i] = x;
+    if (p < nch) {
+        double x = 0.0;
+#pragma unroll
+        for (int t = MAXR - 1; t >= 0; --t) {
+            if (t < m - 1) {
+                if (t == m - 2) x = y[t] + vp[t] * Xm + wl * X;
+                else x = (y[t] + vp[t] * Xm) - cp[t] * x;
+                row[t] = x;
+            }
         }
-        D[e] = X;
-    }
-}
-
-// MODE 0: Helmholtz lines (q2 for k=1..n3m, then q3 for k=2..n3m).
-// MODE 1: Poisson modes m=1..n3m on the transformed field hmn.
-template <int MODE>
-__global__ __launch_bounds__(64) void k_jsolve(const FluidDev F)
-{
-    extern __shared__ double lds[];
-    const DevScalars* S = F.S;
-    if (S->err) return;
-    const int lane = threadIdx.x;
-    const int line = blockIdx.x;
-    JCoef Q;
-    const double* src;
-    double* dst;
-    bool pin = false;
-    if (MODE == 0) {
-        const double betadx = S->betadx;
-        Q.s0 = 1.0;
-        Q.s1 = -betadx;
-        Q.s2 = -betadx;
-        if (line < F.n3m) {
-            int k = line + 1;
-            Q.am = F.am2j; Q.ac = F.ac2j; Q.ap = F.ap2j;
-            Q.jofs = 2;
-            Q.n = F.n2 - 2;
-            src = F.rhs2 + fidx(F, 2, k);
-            dst = F.sA + fidx(F, 2, k);
-        } else {
-            int k = line - F.n3m + 2;
-            Q.am = F.am3j; Q.ac = F.ac3j; Q.ap = F.ap3j;
-            Q.jofs = 1;
-            Q.n = F.n2m;
-            src = F.rhs3 + fidx(F, 1, k);
-            dst = F.sB + fidx(F, 1, k);
-        }
-    } else {
-        const int m = line + 1;
-        Q.s0 = F.wr[m];
-        Q.s1 = 1.0;
-        Q.s2 = 1.0;
-        Q.am = F.amphj; Q.ac = F.acphj; Q.ap = F.apphj;
-        Q.jofs = 1;
-        Q.n = F.n2m;
-        src = F.hmn + (size_t)line * F.P;
-        dst = F.hmn + (size_t)line * F.P;
-        pin = (m == F.null_mode);
-    }
-    const int n = Q.n;
-    double* C = lds;
-    double* D = lds + n;
-    double* V = lds + 2 * n;
-    for (int r = lane; r < n; r += 64) {
-        double a, c, Sr;
-        Q.row(r, a, c, Sr);
-        D[r] = src[r] * Sr;
+        row[m - 1] = X;
     }
     __syncthreads();
-    wave_line_solve(Q, C, D, V, lane, pin);
-    __syncthreads();
-    for (int r = lane; r < n; r += 64) dst[r] = D[r];
+    lds_to_line(col, lds, Q.cs, n, nch, stride, lane);
 }
 
 void launch_jsolve_helm(const FluidDev& F, hipStream_t st)
 {
-    int nlines = F.n3m + (F.n3m - 1);
-    hipLaunchKernelGGL(k_jsolve<0>, dim3(nlines), dim3(64), jsolve_lds_bytes(F.n2), st, F);
+    const int nlines = F.n3m + (F.n3m - 1);
+    const int mr = (F.js[0].maxm > F.js[1].maxm ? F.js[0].maxm : F.js[1].maxm) - 1;
+    const size_t lds = jsolve_lds_bytes(F);
+    if (mr <= 8) hipLaunchKernelGGL(k_jsolve_h<8>, dim3(nlines), dim3(64), lds, st, F);
+    else if (mr <= 16) hipLaunchKernelGGL(k_jsolve_h<16>, dim3(nlines), dim3(64), lds, st, F);
+    else if (mr <= 24) hipLaunchKernelGGL(k_jsolve_h<24>, dim3(nlines), dim3(64), lds, st, F);
+    else if (mr <= 32) hipLaunchKernelGGL(k_jsolve_h<32>, dim3(nlines), dim3(64), lds, st, F);
+    else hipLaunchKernelGGL(k_jsolve_h<64>, dim3(nlines), dim3(64), lds, st, F);
 }
 
 void launch_jsolve_poisson(const FluidDev& F, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_jsolve<1>, dim3(F.n3m), dim3(64), jsolve_lds_bytes(F.n2m), st, F);
+    const int mr = F.ps.maxm - 1;
+    const size_t lds = jsolve_lds_bytes(F);
+    if (mr <= 8) hipLaunchKernelGGL(k_jsolve_p<8>, dim3(F.n3m), dim3(64), lds, st, F);
+    else if (mr <= 16) hipLaunchKernelGGL(k_jsolve_p<16>, dim3(F.n3m), dim3(64), lds, st, F);
+    else if (mr <= 24) hipLaunchKernelGGL(k_jsolve_p<24>, dim3(F.n3m), dim3(64), lds, st, F);
+    else if (mr <= 32) hipLaunchKernelGGL(k_jsolve_p<32>, dim3(F.n3m), dim3(64), lds, st, F);
+    else hipLaunchKernelGGL(k_jsolve_p<64>, dim3(F.n3m), dim3(64), lds, st, F);
 }
 
 // ---------------------------------------------------------------------------
@@ -275,48 +316,47 @@ __global__ __launch_bounds__(64) void k_ksolve3(const FluidDev F)
     const KSys& K = F.ks[f];
     const int j = blockIdx.x * 64 + threadIdx.x + 1;
     if (p >= K.nch || j > F.n2m) return;
-    const double* src = f == 0 ? F.sA : F.sB;
-    double* q = f == 0 ? F.q2 : F.q3;
+    const double* __restrict__ src = f == 0 ? F.sA : F.sB;
+    double* __restrict__ q = f == 0 ? F.q2 : F.q3;
     const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;   // m interior rows
     const size_t base = (size_t)f * FRIBO_KMAXCH * F.n2m + (j - 1);
     const double X = F.kX[base + (size_t)p * F.n2m];
     const double Xm = (p > 0) ? F.kX[base + (size_t)(p - 1) * F.n2m] : 0.0;
-    double y[MAXM];
+    const bool zero_row = (f == 0 && j == 1);
+    double y[MAXM], qv[MAXM];
     double yp = 0.0;
 #pragma unroll
     for (int t = 0; t < MAXM; ++t) {
         if (t < m) {
-            int i = s + t;
-            double d = krhs(F, K, f, j, i, src, S);
+            const int i = s + t;
+            const double d = krhs(F, K, f, j, i, src, S);
+            qv[t] = q[fidx(F, j, i + K.kofs)];
             yp = (d - K.a[i] * yp) * K.inv[i];
             y[t] = yp;
         }
     }
-    const bool zero_row = (f == 0 && j == 1);
+    const size_t ce = fidx(F, j, e + K.kofs);
+    const double qe = q[ce];
     double x = 0.0;
 #pragma unroll
     for (int t = MAXM - 1; t >= 0; --t) {
         if (t < m) {
-            int i = s + t;
+            const int i = s + t;
             if (t == m - 1) x = y[t] + K.vp[i] * Xm + K.wl[p] * X;
             else x = (y[t] + K.vp[i] * Xm) - K.cp[i] * x;
-            size_t c = fidx(F, j, i + K.kofs);
-            q[c] = zero_row ? 0.0 : q[c] + x;
-            if (f == 0 && j == 1) q[fidx(F, F.n2, i + K.kofs)] = 0.0;
+            q[fidx(F, j, i + K.kofs)] = zero_row ? 0.0 : qv[t] + x;
+            if (zero_row) q[fidx(F, F.n2, i + K.kofs)] = 0.0;
         }
     }
-    {
-        size_t c = fidx(F, j, e + K.kofs);
-        q[c] = zero_row ? 0.0 : q[c] + X;
-        if (f == 0 && j == 1) q[fidx(F, F.n2, e + K.kofs)] = 0.0;
-    }
+    q[ce] = zero_row ? 0.0 : qe + X;
+    if (zero_row) q[fidx(F, F.n2, e + K.kofs)] = 0.0;
     if (f == 1) {
         if (p == 0) {
-            size_t c = fidx(F, j, 1);
+            const size_t c = fidx(F, j, 1);
             q[c] = q[c] + F.dqb3s[j];
         }
         if (p == K.nch - 1) {
-            size_t c = fidx(F, j, F.n3);
+            const size_t c = fidx(F, j, F.n3);
             q[c] = q[c] + F.dqb3n[j];
         }
     }

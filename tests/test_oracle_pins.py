@@ -1,0 +1,314 @@
+"""Pins of the CPU oracle (CPU only, no GPU).
+
+The reference ships no golden outputs, so each restated piece is checked
+against something that fixes it independently: LAPACK's own routines (scipy),
+brute force, closed forms, invariants, and the reference's own input files.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+from scipy.linalg import lapack
+
+import oracle as orc
+from fribois2d_amd import cases
+from helpers import GOLDEN, case, decomposition, divergence, oracle_for
+
+
+# -- input generators against the reference's own example inputs ------------
+
+@pytest.mark.parametrize("name,zint,znint,yint,ynint,n3,n2", [
+    ("osccyl", [13, 22], [30, 540, 241], [25, 35], [36, 600, 37], 811, 673),
+    ("gallrect", [17, 27], [25, 500, 206], [25, 35], [30, 500, 31], 731, 561),
+])
+def test_cubicstr_reproduces_shipped_grids(name, zint, znint, yint, ynint, n3, n2):
+    """preproc/cubicstr.m restatement vs the 14-decimal zcoord/ycoord files."""
+    zz = cases.cubicstr(zint, znint, 0.0, 40.0)
+    rc = cases.cubicstr(yint, ynint, 0.0, 60.0)
+    zf = cases.read_grid_file(os.path.join(GOLDEN, name, "zcoord.dat"), n3)
+    yf = cases.read_grid_file(os.path.join(GOLDEN, name, "ycoord.dat"), n2)
+    assert zz.size == n3 and rc.size == n2
+    assert np.max(np.abs(zz - zf)) < 1e-13
+    assert np.max(np.abs(rc - yf)) < 1e-13
+
+
+def test_marker_count_drops_closing_segment():
+    """utils_proc.F90:529-553: nlm = newlines - 1 (last line has no newline)."""
+    v1 = cases.read_lmdata(os.path.join(GOLDEN, "osccyl", "lmdata.in"))
+    v2 = cases.read_lmdata(os.path.join(GOLDEN, "gallrect", "lmdata.in"))
+    assert v1.shape[0] - 1 == 377
+    assert v2.shape[0] - 1 == 798
+
+
+def test_splitmix64_known_answer():
+    """splitmix64 (Vigna) first outputs for seed 1234567."""
+    mask = (1 << 64) - 1
+    x = 1234567
+    outs = []
+    for _ in range(3):
+        x = (x + 0x9E3779B97F4A7C15) & mask
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        outs.append(z ^ (z >> 31))
+    assert outs[0] == 6457827717110365317
+    assert outs[1] == 3203168211198807973
+    u = cases.splitmix64_uniform(1234567, 3)
+    assert np.array_equal(u, 2.0 * np.array([o >> 11 for o in outs]) / 9007199254740992.0 - 1.0)
+    assert np.array_equal(cases._splitmix64_uniform_np(99, 1000), cases.splitmix64_uniform(99, 1000))
+
+
+def test_initial_state_perturbs_only_interior_velocity():
+    p = cases.make_case("tiny").params
+    a = cases.initial_state(p, 5)
+    b = cases.initial_state(p, None)
+    d2, d3 = a["q2"] - b["q2"], a["q3"] - b["q3"]
+    n2m, n3m = p.n2 - 1, p.n3 - 1
+    assert np.max(np.abs(d2)) <= 1e-3 and np.max(np.abs(d3)) <= 1e-3
+    mask2 = np.zeros_like(d2, dtype=bool)
+    mask2[1:n2m, 3:n3m + 3] = True
+    mask3 = np.zeros_like(d3, dtype=bool)
+    mask3[0:n2m, 4:n3m + 3] = True
+    assert np.all(d2[~mask2] == 0) and np.all(d3[~mask3] == 0)
+    assert np.count_nonzero(d2[mask2]) > 0.99 * mask2.sum()
+
+
+# -- LAPACK / GEsolve restatements -------------------------------------------
+
+def test_dgttrf_dgttrs_match_lapack():
+    rng = np.random.default_rng(3)
+    for n, pivot in [(7, False), (50, False), (50, True), (3, False)]:
+        dl = rng.standard_normal(n - 1)
+        du = rng.standard_normal(n - 1)
+        d = rng.standard_normal(n) + (0.0 if pivot else 4.0 * np.sign(rng.standard_normal(n)))
+        b = rng.standard_normal(n)
+        x = orc.dgtsv(dl, d, du, b)
+        dl2, d2, du2, du22, ipiv, info = lapack.dgttrf(dl, d, du)
+        assert info == 0
+        xr, info = lapack.dgttrs(dl2, d2, du2, du22, ipiv, b)
+        assert info == 0
+        np.testing.assert_allclose(x, xr, rtol=1e-12, atol=1e-14 * np.max(np.abs(xr)))
+        A = np.diag(d) + np.diag(du, 1) + np.diag(dl, -1)
+        np.testing.assert_allclose(A @ x, b, atol=1e-11 * np.max(np.abs(b)) * np.linalg.cond(A))
+
+
+def test_gesolve_matches_numpy_and_pivots():
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        A = rng.standard_normal((3, 3))
+        b = rng.standard_normal(3)
+        x, info = orc.gesolve(A, b)
+        assert info == 1
+        np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    # zero leading pivot needs the row interchange
+    A = np.array([[0.0, 2.0, 1.0], [1.0, 1.0, 0.0], [3.0, 0.0, 1.0]])
+    x, info = orc.gesolve(A, np.array([1.0, 2.0, 3.0]))
+    np.testing.assert_allclose(A @ x, [1.0, 2.0, 3.0], atol=1e-14)
+    x, info = orc.gesolve(np.array([[0.0, 1.0, 2.0], [0.0, 3.0, 4.0], [0.0, 5.0, 6.0]]), np.ones(3))
+    assert info == 0
+
+
+# -- MLS ----------------------------------------------------------------------
+
+def _cubic_spline(r):
+    r = np.abs(r)
+    return np.where(r > 1.0, 0.0, np.where(r <= 0.5, 2 / 3 - 4 * r ** 2 + 4 * r ** 3,
+                                           4 / 3 - 4 * r + 4 * r ** 2 - 4 / 3 * r ** 3))
+
+
+def test_mls_weights_reproduce_linear_fields():
+    """phi from the oracle: partition of unity, linear reproduction, and equal to
+    a weighted least-squares fit computed independently with numpy."""
+    c = case("cyl64")
+    o = oracle_for(c)
+    lmcc, lmvel, lmA = c.markers(0.0)
+    o.set_markers(lmcc, lmvel, lmA)
+    assert o.cfl_dt(0, 1e-3)[0] == 0
+    assert o.substep(1, 0) == 0
+    nlm = lmA.size
+    phi = o.get("phi").reshape((nlm, 2, 9), order="F")
+    eul = o.get("eulins").reshape((nlm, 2, 2, 9), order="F")
+    rc, zz = c.rc, c.zz
+    rm = np.append(0.5 * (rc[:-1] + rc[1:]), rc[-1])
+    zm = np.append(0.5 * (zz[:-1] + zz[1:]), zz[-1])
+    for n in range(nlm):
+        for d in range(2):
+            ya, za = (rc, zm) if d == 0 else (rm, zz)
+            js, ks = eul[n, d, 0] - 1, eul[n, d, 1] - 1
+            y, z = ya[js], za[ks]
+            dy, dz = ya[js + 1] - ya[js], za[ks + 1] - za[ks]
+            ph = phi[n, d]
+            assert abs(ph.sum() - 1.0) < 1e-12
+            assert abs(ph @ y - lmcc[n, 0]) < 1e-12
+            assert abs(ph @ z - lmcc[n, 1]) < 1e-12
+            w = _cubic_spline((lmcc[n, 0] - y) / (1.6 * dy)) * _cubic_spline((lmcc[n, 1] - z) / (1.6 * dz))
+            P = np.stack([np.ones(9), y, z], axis=1)
+            A = P.T @ (w[:, None] * P)
+            gam = np.linalg.solve(A, np.array([1.0, lmcc[n, 0], lmcc[n, 1]]))
+            np.testing.assert_allclose(ph, (P @ gam) * w, rtol=1e-9, atol=1e-12)
+
+
+def test_lgrdistribution_is_fortran_minloc():
+    """Marker -> node indices on the reference's own OscCyl grid and markers,
+    against numpy.argmin (first occurrence, as Fortran MINLOC)."""
+    c = case("osccyl")
+    o = oracle_for(c)
+    lmcc, lmvel, lmA = c.markers(0.0)
+    o.set_markers(lmcc, lmvel, lmA)
+    assert o.cfl_dt(0, 1e-3)[0] == 0
+    assert o.substep(1, 0) == 0
+    nlm = lmA.size
+    lmind = o.get("lmind").reshape((nlm, 2, 2), order="F")
+    rc, zz = c.rc, c.zz
+    rm = np.append(0.5 * (rc[:-1] + rc[1:]), rc[-1])
+    zm = np.append(0.5 * (zz[:-1] + zz[1:]), zz[-1])
+    y, z = lmcc[:, 0], lmcc[:, 1]
+    exp11 = np.argmin(np.abs(rc[None, :] - y[:, None]), axis=1) + 1
+    exp12 = np.argmin(np.abs(zm[None, :] - z[:, None]), axis=1) + 1
+    exp21 = np.argmin(np.abs(rm[None, :] - y[:, None]), axis=1) + 1
+    exp22 = np.argmin(np.abs(zz[None, :] - z[:, None]), axis=1) + 1
+    np.testing.assert_array_equal(lmind[:, 0, 0], exp11)
+    np.testing.assert_array_equal(lmind[:, 0, 1], exp12)
+    np.testing.assert_array_equal(lmind[:, 1, 0], exp21)
+    np.testing.assert_array_equal(lmind[:, 1, 1], exp22)
+
+
+def test_support_shift_at_wall():
+    """MLS_SD_ind moves the 3x3 support inside the domain at j=1 (MLSint.F90:57-63)."""
+    c = case("tiny")
+    o = oracle_for(c)
+    lmcc = np.array([[0.05, 4.0], [3.0, 4.0]])
+    o.set_markers(lmcc, np.zeros_like(lmcc), np.array([0.1, 0.1]))
+    assert o.cfl_dt(0, 1e-3)[0] == 0
+    assert o.substep(1, 0) == 0
+    eul = o.get("eulins").reshape((2, 2, 2, 9), order="F")
+    lmind = o.get("lmind").reshape((2, 2, 2), order="F")
+    assert lmind[0, 1, 0] == 1                      # nearest rm node is cell 1
+    assert sorted(set(eul[0, 1, 0])) == [1, 2, 3]   # support shifted to j=1..3
+    j0 = lmind[1, 1, 0]
+    assert sorted(set(eul[1, 1, 0])) == [j0 - 1, j0, j0 + 1]
+
+
+# -- Poisson --------------------------------------------------------------------
+
+def _dense_laplacian(o, n2m, n3m):
+    amj, acj, apj = (o.get(k)[:n2m] for k in ("amphj", "acphj", "apphj"))
+    amk, ack, apk = (o.get(k)[:n3m] for k in ("amphk", "acphk", "apphk"))
+    Ay = np.diag(acj) + np.diag(apj[:-1], 1) + np.diag(amj[1:], -1)
+    Az = np.diag(ack) + np.diag(apk[:-1], 1) + np.diag(amk[1:], -1)
+    return np.kron(np.eye(n3m), Ay) + np.kron(Az, np.eye(n2m))
+
+
+@pytest.mark.parametrize("method", ["sym", "dgeev"])
+def test_poisson_matches_dense_solve(method):
+    """phcalc_outflow against a dense solve of the 5-point Neumann operator;
+    the answer is fixed up to the null-space constant."""
+    c = case("tiny")
+    c.zz = c.zz.copy()
+    c.zz[1:-1] += 0.05 * np.sin(np.arange(1, c.zz.size - 1))   # stretched, non-symmetric A_z
+    dec = decomposition(c, method)
+    o = oracle_for(c, dec)
+    n2m, n3m = c.params.n2 - 1, c.params.n3 - 1
+    L = _dense_laplacian(o, n2m, n3m)
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal((n2m, n3m))
+    # make it compatible: remove the component along the left null vector
+    g2rm = np.diff(c.rc) * (n2m / c.params.rext2)
+    g3rm = np.diff(c.zz) * (n3m / c.params.alx3)
+    wgt = np.outer(g2rm, g3rm)
+    rhs -= (np.sum(rhs * wgt) / np.sum(wgt))
+    x = orc.phcalc(o, rhs)
+    np.testing.assert_allclose(L @ x.ravel(order="F"), rhs.ravel(order="F"), atol=1e-9 * np.max(np.abs(rhs)))
+    xd = np.linalg.lstsq(L, rhs.ravel(order="F"), rcond=None)[0].reshape((n2m, n3m), order="F")
+    np.testing.assert_allclose(x - x.mean(), xd - xd.mean(), atol=1e-9 * np.max(np.abs(xd)))
+
+
+def test_poisson_independent_of_eigenbasis():
+    c = case("cyl64")
+    o1 = oracle_for(c, decomposition(c, "sym"))
+    o2 = oracle_for(c, decomposition(c, "dgeev"))
+    rng = np.random.default_rng(6)
+    n2m, n3m = c.params.n2 - 1, c.params.n3 - 1
+    rhs = rng.standard_normal((n2m, n3m))
+    x1, x2 = orc.phcalc(o1, rhs), orc.phcalc(o2, rhs)
+    d = (x1 - x1.mean()) - (x2 - x2.mean())
+    assert np.max(np.abs(d)) < 1e-8 * np.max(np.abs(x1))
+
+
+# -- whole-step invariants -----------------------------------------------------
+
+def test_projection_leaves_divergence_free_velocity():
+    c = case("cyl64")
+    o = oracle_for(c)
+    from helpers import run_steps
+    run_steps(o, c, 3)
+    st = {k: o.get(k) for k in ("q2", "q3")}
+    dx = np.min(np.diff(c.zz))
+    assert divergence(st, c) < 1e-11 * np.max(np.abs(st["q3"])) / dx
+
+
+def test_outflow_mass_balance():
+    """boucdq's correction makes outflow flux equal inflow flux (iobcs.F90:54-74)."""
+    c = case("cyl64")
+    o = oracle_for(c)
+    from helpers import run_steps
+    run_steps(o, c, 2)
+    n2m = c.params.n2 - 1
+    darea = np.diff(c.rc)
+    qout = np.sum(o.get("qb3n")[:n2m] * darea)
+    qinf = np.sum(o.get("qb3s")[:n2m] * darea)
+    assert abs(qout - qinf) < 1e-12 * abs(qinf)
+
+
+def _uniform_params(c, **kw):
+    d = dict(c.params.__dict__)
+    d.update(ibact=False, **kw)
+    return cases.FluidParams(**d)
+
+
+def test_cfl_closed_form():
+    """cfl.F90: uniform q3=1, q2=0 gives cflm = max_k udx3m(k); dt = cflmax/cflm."""
+    c = case("cyl64")
+    p = _uniform_params(c, dtmax=1.0)
+    o = oracle_for(c, params=p, state=cases.initial_state(p, None))
+    rc, dt, cf = o.cfl_dt(1, 1e-3)
+    assert rc == 0
+    udx3m = (c.params.n3 - 1) / c.params.alx3 / (np.diff(c.zz) * (c.params.n3 - 1) / c.params.alx3)
+    assert abs(cf - np.max(udx3m)) < 1e-13 * cf
+    assert dt == p.cflmax / cf
+
+
+def test_viscous_and_convective_terms_closed_form():
+    """invtr3/hdnl3 on a uniform grid: q3 = f(y) quadratic gives rhs3 =
+    dt*al/Re*f''; q3 = a + b z gives rhs3 = -2 ga q3 b dt (interior)."""
+    c = case("tiny")
+    p = _uniform_params(c, dtmax=1.0)
+    n2m, n3m = p.n2 - 1, p.n3 - 1
+    rm = 0.5 * (c.rc[:-1] + c.rc[1:])
+    # quadratic in y
+    st = cases.initial_state(p, None)
+    f = 0.1 * rm * (p.rext2 - rm)
+    st["q3"][:n2m, :] = f[:, None]
+    st["qb3s"][:n2m] = f
+    st["qb3n"][:n2m] = f
+    o = oracle_for(c, params=p, state=st)
+    rc, dt, _ = o.cfl_dt(0, 1e-3)
+    assert o.substep(1, 0) == 0
+    sc = o.get("scalars")
+    al, ga = sc[2], sc[3]
+    rhs3 = o.get("rhs3")
+    expect = dt * al * p.visc * (-0.2)
+    np.testing.assert_allclose(rhs3[1:n2m - 1, 2:n3m - 1], expect, rtol=1e-9)
+    # linear in z
+    st = cases.initial_state(p, None)
+    a, b = 1.0, 0.05
+    q3z = a + b * c.zz
+    st["q3"][:, 3:3 + p.n3] = q3z[None, :]      # planes k = 1..n3 hold q3(zz(k))
+    o = oracle_for(c, params=p, state=st)
+    o.cfl_dt(0, 1e-3)
+    assert o.substep(1, 0) == 0
+    rhs3 = o.get("rhs3")
+    k = np.arange(2, n3m)        # interior k (0-based column k-1)
+    expect = -2.0 * ga * q3z[k - 1] * b * dt
+    np.testing.assert_allclose(rhs3[2:n2m - 2, k - 1], np.broadcast_to(expect, (n2m - 4, k.size)), rtol=1e-9)
