@@ -18,8 +18,6 @@
 // tree order (deterministic run to run).
 #include "fribo_internal.h"
 
-#define BLK_J 64
-#define BLK_K 4
 
 // Fixed-order tree reductions over the NT threads of a block; t is the flat
 // thread index (threadIdx.x + blockDim.x * threadIdx.y).
@@ -352,170 +350,229 @@ void launch_bc(const FluidDev& F, int n, int ns, double dt_in, double dt_o, hipS
 }
 
 // ---------------------------------------------------------------------------
-// Fused HDNL + INVTR.  Thread (j,k): q2 point (rc(j), zm(k)) for 2<=j<=n2m and
-// q3 point (rm(j), zz(k)) for 2<=k<=n3m.
-__global__ __launch_bounds__(BLK_J* BLK_K) void k_rhs(const FluidDev F)
+// Marching layout shared by k_rhs, k_div and k_upd: one 64-lane wave owns
+// columns j = 62*bx + lane (lanes 1..62 produce output, lanes 0 and 63 are
+// the j-1 / j+1 halo) and marches MK planes k = MK*by+1 .. MK*by+MK, keeping
+// a register window of the planes it needs; j-neighbours come from the
+// adjacent lanes, so every field value is read from HBM once.
+#define MK 16
+#define WJ 62
+
+__device__ inline double up1(double v) { return __shfl_up(v, 1, 64); }
+__device__ inline double dn1(double v) { return __shfl_down(v, 1, 64); }
+
+// Fused HDNL + INVTR.  Cell (j,k): q2 point (rc(j), zm(k)) for 2<=j<=n2m and
+// q3 point (rm(j), zz(k)) for 2<=k<=n3m; operation order as in the reference
+// loops (HDNL.F90:35-220, INVTR.F90:45-277).
+__global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
 {
-    const int j = blockIdx.x * BLK_J + threadIdx.x + 1;
-    const int k = blockIdx.y * BLK_K + threadIdx.y + 1;
     const DevScalars* S = F.S;
     if (S->err) return;
-    if (j > F.n2m || k > F.n3m) return;
-    const double dt = S->dt, ga = S->ga, ro = S->ro, al = S->al, alre = S->alre, pend = S->pend;
+    const int lane = threadIdx.x;
+    const int j = WJ * blockIdx.x + lane;
+    const int k0 = MK * blockIdx.y + 1;
     const int n2m = F.n2m, n3m = F.n3m;
-    const double* q2 = F.q2;
-    const double* q3 = F.q3;
-    const double* pr = F.pr;
-#define Q2(a, b) q2[fidx(F, a, b)]
-#define Q3(a, b) q3[fidx(F, a, b)]
-#define PRV(a, b) (pr[fidx(F, a, b)] - pend)
-
-    // ---- q2: hdnl2 (HDNL.F90:35-107) + invtr2 (INVTR.F90:45-147)
-    if (j >= 2) {
-        const int jm = j - 1, jp = j + 1;
-        double a = Q2(jp, k) + Q2(j, k), b = Q2(jm, k) + Q2(j, k);
-        double h22 = (a * a - b * b) * F.udx2c[j] * 0.25;
-        double dphv = -h22;
-        double h23;
-        if (k >= 2 && k <= n3m - 1) {
-            h23 = ((Q3(j, k + 1) + Q3(jm, k + 1)) * (Q2(j, k + 1) + Q2(j, k)) -
-                   (Q3(j, k) + Q3(jm, k)) * (Q2(j, k) + Q2(j, k - 1))) * F.udx3m[k] * 0.25;
-        } else if (k == n3m) {
-            h23 = (F.qb2n[j] * (F.qb3n[j] + F.qb3n[jm]) -
-                   (Q3(j, k) + Q3(jm, k)) * (Q2(j, k) + Q2(j, k - 1)) * 0.5) * F.udx3m[k] * 0.5;
-        } else {  // k == 1
-            h23 = ((Q3(j, 2) + Q3(jm, 2)) * (Q2(j, 1) + Q2(j, 2)) * 0.5 -
-                   F.qb2s[j] * (F.qb3s[j] + F.qb3s[jm])) * F.udx3m[1] * 0.5;
+    if (k0 > n3m) return;
+    const int k1 = min(k0 + MK - 1, n3m);
+    const bool inj = (j >= 1 && j <= F.n2);
+    const bool out = (lane >= 1 && lane <= WJ && j <= n2m);
+    const double dt = S->dt, ga = S->ga, ro = S->ro, al = S->al, alre = S->alre, pend = S->pend;
+    const double* __restrict__ q2 = F.q2;
+    const double* __restrict__ q3 = F.q3;
+    const double* __restrict__ pr = F.pr;
+    const size_t P = F.P;
+    size_t c = inj ? fidx(F, j, k0) : 0;
+    // window: planes k-1, k (and k+1 loaded each step)
+    double q2m = inj ? q2[c - P] : 0.0, q2c = inj ? q2[c] : 0.0;
+    double q3m = inj ? q3[c - P] : 0.0, q3c = inj ? q3[c] : 0.0;
+    double prm = inj ? pr[c - P] - pend : 0.0;
+    // per-column coefficients
+    const int jj = out ? j : 1;
+    const double udx2c = F.udx2c[jj], udx2m = F.udx2m[jj];
+    const double am2 = F.am2j[jj], ac2 = F.ac2j[jj], ap2 = F.ap2j[jj];
+    const double am3 = F.am3j[jj], ac3 = F.ac3j[jj], ap3 = F.ap3j[jj];
+    const double qb2s = F.qb2s[jj], qb2n = F.qb2n[jj];
+    const double qb3s = F.qb3s[jj], qb3sm = F.qb3s[jj > 1 ? jj - 1 : 1];
+    const double qb3n = F.qb3n[jj], qb3nm = F.qb3n[jj > 1 ? jj - 1 : 1];
+    for (int k = k0; k <= k1; ++k, c += P) {
+        const double q2p = inj ? q2[c + P] : 0.0;
+        const double q3p = inj ? q3[c + P] : 0.0;
+        const double prc = inj ? pr[c] - pend : 0.0;
+        // neighbours in j
+        const double q2E = dn1(q2c), q2W = up1(q2c), q2Em = dn1(q2m), q2Ep = dn1(q2p);
+        const double q3W = up1(q3c), q3Wp = up1(q3p), q3E = dn1(q3c);
+        const double prW = up1(prc);
+        (void)q2Ep;
+        if (out) {
+            // ---- q2: hdnl2 + invtr2 (j >= 2)
+            if (j >= 2) {
+                double a = q2E + q2c, b = q2W + q2c;
+                double h22 = (a * a - b * b) * udx2c * 0.25;
+                double dphv = -h22;
+                double h23;
+                if (k >= 2 && k <= n3m - 1) {
+                    h23 = ((q3p + q3Wp) * (q2p + q2c) - (q3c + q3W) * (q2c + q2m)) * F.udx3m[k] * 0.25;
+                } else if (k == n3m) {
+                    h23 = (qb2n * (qb3n + qb3nm) - (q3c + q3W) * (q2c + q2m) * 0.5) * F.udx3m[k] * 0.5;
+                } else {
+                    h23 = ((q3p + q3Wp) * (q2c + q2p) * 0.5 - qb2s * (qb3s + qb3sm)) * F.udx3m[1] * 0.5;
+                }
+                dphv = dphv - h23;
+                double d22q2 = q2E * ap2 + q2c * ac2 + q2W * am2;
+                double d33q2, dpx22;
+                if (k >= 2 && k <= n3m - 1) {
+                    d33q2 = q2p * F.ap3sk[k] + q2c * F.ac3sk[k] + q2m * F.am3sk[k];
+                    double udx2 = al * udx2c;
+                    dpx22 = (prc - prW) * udx2;
+                } else if (k == 1) {
+                    double codea = F.dx3q / (F.g3rc[2] * F.g3rm[1]);
+                    double codeb = F.dx3q / (F.g3rc[1] * F.g3rm[1]);
+                    d33q2 = ((q2p - q2c) * codea - (q2c - qb2s) * codeb * 2.0);
+                    dpx22 = (prc - prW) * udx2c * al;
+                } else {
+                    double codea = F.dx3q / (F.g3rc[k + 1] * F.g3rm[k]);
+                    double codeb = F.dx3q / (F.g3rc[k] * F.g3rm[k]);
+                    d33q2 = ((qb2n - q2c) * codea * 2.0 - (q2c - q2m) * codeb);
+                    dpx22 = (prc - prW) * udx2c * al;
+                }
+                double dcq2 = d22q2 + d33q2;
+                const size_t o = fidx(F, j, k);
+                F.rhs2[o] = (ga * dphv + ro * F.ru2[o] + alre * dcq2 - dpx22) * dt;
+                F.ru2[o] = dphv;
+            }
+            // ---- q3: hdnl3 + invtr3 (k >= 2)
+            if (k >= 2) {
+                double qcapv = 0.0;   // qcap(:,n3m) is never written (stays 0 from initia)
+                if (k <= n3m - 1) {
+                    const double q3pp = (j == n2m) ? q3c : q3E;   // jpv clamps at n2m
+                    const double q3mm = (j == 1) ? q3c : q3W;     // jmv clamps at 1
+                    double h32 = (((q2E + q2Em) * (q3pp + q3c)) - ((q2c + q2m) * (q3c + q3mm))) * udx2m * 0.25;
+                    double a = q3p + q3c, b = q3c + q3m;
+                    double h33 = (a * a - b * b) * F.udx3c[k] * 0.25;
+                    qcapv = -(h32 + h33);
+                }
+                const double udx3 = al * F.udx3c[k];
+                double dq32;
+                if (j == 1) dq32 = q3E * ap3 + q3c * ac3 + F.q3lw * am3;
+                else if (j == n2m) dq32 = F.q3uw * ap3 + q3c * ac3 + q3W * am3;
+                else dq32 = q3E * ap3 + q3c * ac3 + q3W * am3;
+                double dq33 = q3p * F.ap3ck[k] + q3c * F.ac3ck[k] + q3m * F.am3ck[k];
+                double dcq3 = dq32 + dq33;
+                double dpx33 = (prc - prm) * udx3;
+                const size_t o = fidx(F, j, k);
+                F.rhs3[o] = (ga * qcapv + ro * F.ru3[o] + alre * dcq3 - dpx33) * dt;
+                F.ru3[o] = qcapv;
+            }
         }
-        dphv = dphv - h23;
-        const double ammj = F.am2j[j], accj = F.ac2j[j], appj = F.ap2j[j];
-        double d22q2 = Q2(jp, k) * appj + Q2(j, k) * accj + Q2(jm, k) * ammj;
-        double d33q2, dpx22;
-        if (k >= 2 && k <= n3m - 1) {
-            d33q2 = Q2(j, k + 1) * F.ap3sk[k] + Q2(j, k) * F.ac3sk[k] + Q2(j, k - 1) * F.am3sk[k];
-            double udx2 = al * F.udx2c[j];
-            dpx22 = (PRV(j, k) - PRV(jm, k)) * udx2;
-        } else if (k == 1) {
-            double codea = F.dx3q / (F.g3rc[2] * F.g3rm[1]);
-            double codeb = F.dx3q / (F.g3rc[1] * F.g3rm[1]);
-            d33q2 = ((Q2(j, 2) - Q2(j, 1)) * codea - (Q2(j, 1) - F.qb2s[j]) * codeb * 2.0);
-            dpx22 = (PRV(j, k) - PRV(jm, k)) * F.udx2c[j] * al;
-        } else {
-            double codea = F.dx3q / (F.g3rc[k + 1] * F.g3rm[k]);
-            double codeb = F.dx3q / (F.g3rc[k] * F.g3rm[k]);
-            d33q2 = ((F.qb2n[j] - Q2(j, k)) * codea * 2.0 - (Q2(j, k) - Q2(j, k - 1)) * codeb);
-            dpx22 = (PRV(j, k) - PRV(jm, k)) * F.udx2c[j] * al;
-        }
-        double dcq2 = d22q2 + d33q2;
-        const size_t c = fidx(F, j, k);
-        F.rhs2[c] = (ga * dphv + ro * F.ru2[c] + alre * dcq2 - dpx22) * dt;
-        F.ru2[c] = dphv;
+        q2m = q2c;
+        q2c = q2p;
+        q3m = q3c;
+        q3c = q3p;
+        prm = prc;
     }
-
-    // ---- q3: hdnl3 (HDNL.F90:195-220) + invtr3 (INVTR.F90:188-269)
-    if (k >= 2) {
-        double qcapv = 0.0;  // qcap(:,n3m) is never written (stays 0 from initia)
-        if (k <= n3m - 1) {
-            const int jmm = F.jmv[j], jpp = F.jpv[j], jp = j + 1, km = k - 1, kp = k + 1;
-            double h32 = (((Q2(jp, k) + Q2(jp, km)) * (Q3(jpp, k) + Q3(j, k))) -
-                          ((Q2(j, k) + Q2(j, km)) * (Q3(j, k) + Q3(jmm, k)))) * F.udx2m[j] * 0.25;
-            double a = Q3(j, kp) + Q3(j, k), b = Q3(j, k) + Q3(j, km);
-            double h33 = (a * a - b * b) * F.udx3c[k] * 0.25;
-            qcapv = -(h32 + h33);
-        }
-        const double udx3 = al * F.udx3c[k];
-        const double amm = F.am3ck[k], acc = F.ac3ck[k], app = F.ap3ck[k];
-        const double ammj = F.am3j[j], accj = F.ac3j[j], appj = F.ap3j[j];
-        double dq32;
-        if (j == 1) {
-            dq32 = Q3(2, k) * appj + Q3(1, k) * accj + F.q3lw * ammj;
-        } else if (j == n2m) {
-            dq32 = F.q3uw * appj + Q3(j, k) * accj + Q3(j - 1, k) * ammj;
-        } else {
-            dq32 = Q3(j + 1, k) * appj + Q3(j, k) * accj + Q3(j - 1, k) * ammj;
-        }
-        double dq33 = Q3(j, k + 1) * app + Q3(j, k) * acc + Q3(j, k - 1) * amm;
-        double dcq3 = dq32 + dq33;
-        double dpx33 = (PRV(j, k) - PRV(j, k - 1)) * udx3;
-        const size_t c = fidx(F, j, k);
-        F.rhs3[c] = (ga * qcapv + ro * F.ru3[c] + alre * dcq3 - dpx33) * dt;
-        F.ru3[c] = qcapv;
-    }
-#undef Q2
-#undef Q3
-#undef PRV
 }
 
 void launch_rhs(const FluidDev& F, hipStream_t st)
 {
-    dim3 g((F.n2m + BLK_J - 1) / BLK_J, (F.n3m + BLK_K - 1) / BLK_K);
-    hipLaunchKernelGGL(k_rhs, g, dim3(BLK_J, BLK_K), 0, st, F);
+    dim3 g((F.n2m + WJ - 1) / WJ, (F.n3m + MK - 1) / MK);
+    hipLaunchKernelGGL(k_rhs, g, dim3(64), 0, st, F);
 }
 
 // ---------------------------------------------------------------------------
 // divergence.F90:26-39
-__global__ __launch_bounds__(BLK_J* BLK_K) void k_div(const FluidDev F)
+__global__ __launch_bounds__(64) void k_div(const FluidDev F)
 {
-    const int j = blockIdx.x * BLK_J + threadIdx.x + 1;
-    const int k = blockIdx.y * BLK_K + threadIdx.y + 1;
     const DevScalars* S = F.S;
     if (S->err) return;
-    if (j > F.n2m || k > F.n3m) return;
+    const int lane = threadIdx.x;
+    const int j = WJ * blockIdx.x + lane;
+    const int k0 = MK * blockIdx.y + 1;
+    if (k0 > F.n3m) return;
+    const int k1 = min(k0 + MK - 1, F.n3m);
+    const bool inj = (j >= 1 && j <= F.n2);
+    const bool out = (lane >= 1 && lane <= WJ && j <= F.n2m);
     const double usdtal = 1.0 / (S->dt * S->al);
-    double dq = (F.q2[fidx(F, j + 1, k)] - F.q2[fidx(F, j, k)]) * F.udx2m[j] +
-                (F.q3[fidx(F, j, k + 1)] - F.q3[fidx(F, j, k)]) * F.udx3m[k];
-    F.dph[fidx(F, j, k)] = dq * usdtal;
+    const size_t P = F.P;
+    size_t c = inj ? fidx(F, j, k0) : 0;
+    const double udx2m = F.udx2m[out ? j : 1];
+    double q3c = inj ? F.q3[c] : 0.0;
+    for (int k = k0; k <= k1; ++k, c += P) {
+        const double q2c = inj ? F.q2[c] : 0.0;
+        const double q3p = inj ? F.q3[c + P] : 0.0;
+        const double q2E = dn1(q2c);
+        if (out) {
+            double dq = (q2E - q2c) * udx2m + (q3p - q3c) * F.udx3m[k];
+            F.dph[c] = dq * usdtal;
+        }
+        q3c = q3p;
+    }
 }
 
 void launch_div(const FluidDev& F, hipStream_t st)
 {
-    dim3 g((F.n2m + BLK_J - 1) / BLK_J, (F.n3m + BLK_K - 1) / BLK_K);
-    hipLaunchKernelGGL(k_div, g, dim3(BLK_J, BLK_K), 0, st, F);
+    dim3 g((F.n2m + WJ - 1) / WJ, (F.n3m + MK - 1) / MK);
+    hipLaunchKernelGGL(k_div, g, dim3(64), 0, st, F);
 }
 
 // ---------------------------------------------------------------------------
 // updvp.F90:27-56 (velocity correction) + 83-102 (prcalc) + partial sums of
 // the pressure mean (updvp.F90:127-133).  The stored pressure is "raw": the
 // previous mean (S->pend) is subtracted on read.
-__global__ __launch_bounds__(BLK_J* BLK_K) void k_upd(const FluidDev F)
+__global__ __launch_bounds__(64) void k_upd(const FluidDev F)
 {
-    __shared__ double sh[BLK_J * BLK_K];
-    const int tid = threadIdx.y * BLK_J + threadIdx.x;
-    const int j = blockIdx.x * BLK_J + threadIdx.x + 1;
-    const int k = blockIdx.y * BLK_K + threadIdx.y + 1;
     const DevScalars* S = F.S;
     if (S->err) return;
+    const int lane = threadIdx.x;
+    const int j = WJ * blockIdx.x + lane;
+    const int k0 = MK * blockIdx.y + 1;
     double part = 0.0;
-    if (j <= F.n2m && k <= F.n3m) {
-        const double al = S->al, dt = S->dt, pend = S->pend;
-        const double* dph = F.dph;
-        const size_t c = fidx(F, j, k);
-        const int jm = F.jmv[j], jp = F.jpv[j], km = F.kmv[k], kp = F.kpv[k];
-        const double dc = dph[c];
-        // q2
-        double usujm = al * dt * F.udx2c[j];
-        F.q2[c] = F.q2[c] - (dc - dph[fidx(F, jm, k)]) * usujm;
-        // q3
-        if (k >= 2) {
-            double usukm = al * dt * F.udx3c[k];
-            F.q3[c] = F.q3[c] - (dc - dph[fidx(F, j, k - 1)]) * usukm;
+    if (k0 <= F.n3m) {
+        const int k1 = min(k0 + MK - 1, F.n3m);
+        const bool inj = (j >= 1 && j <= F.n2);
+        const bool out = (lane >= 1 && lane <= WJ && j <= F.n2m);
+        const double al = S->al, dt = S->dt, pend = S->pend, be = al * S->beta;
+        const size_t P = F.P;
+        const int jj = out ? j : 1;
+        const double usujm = al * dt * F.udx2c[jj];
+        const double apj = F.apphj[jj], acj = F.acphj[jj], amj = F.amphj[jj];
+        const double dy = F.rc[jj + 1] - F.rc[jj];
+        const bool jlo = (j == 1), jhi = (j == F.n2m);
+        size_t c = inj ? fidx(F, j, k0) : 0;
+        const double* __restrict__ dph = F.dph;
+        double dm = (inj && k0 > 1) ? dph[c - P] : 0.0;
+        double dc = inj ? dph[c] : 0.0;
+        for (int k = k0; k <= k1; ++k, c += P) {
+            const double dp = (inj && k < F.n3m) ? dph[c + P] : 0.0;
+            const double dW = up1(dc), dE = dn1(dc);
+            if (out) {
+                const double dJm = jlo ? dc : dW;     // dph(jmv(j))
+                const double dJp = jhi ? dc : dE;     // dph(jpv(j))
+                const double dKm = (k == 1) ? dc : dm; // dph(kmv(k))
+                const double dKp = (k == F.n3m) ? dc : dp;
+                F.q2[c] = F.q2[c] - (dc - dJm) * usujm;
+                if (k >= 2) {
+                    const double usukm = al * dt * F.udx3c[k];
+                    F.q3[c] = F.q3[c] - (dc - dm) * usukm;
+                }
+                double lap = (dJp * apj + dc * acj + dJm * amj) +
+                             (dKp * F.apphk[k] + dc * F.acphk[k] + dKm * F.amphk[k]);
+                double prv = (F.pr[c] - pend) + dc - be * lap;
+                F.pr[c] = prv;
+                part = part + prv * (F.zz[k + 1] - F.zz[k]) * dy;
+            }
+            dm = dc;
+            dc = dp;
         }
-        // prcalc
-        const double be = al * S->beta;
-        double lap = (dph[fidx(F, jp, k)] * F.apphj[j] + dc * F.acphj[j] + dph[fidx(F, jm, k)] * F.amphj[j]) +
-                     (dph[fidx(F, j, kp)] * F.apphk[k] + dc * F.acphk[k] + dph[fidx(F, j, km)] * F.amphk[k]);
-        double prv = (F.pr[c] - pend) + dc - be * lap;
-        F.pr[c] = prv;
-        part = prv * (F.zz[k + 1] - F.zz[k]) * (F.rc[j + 1] - F.rc[j]);
     }
-    block_reduce_sum<BLK_J * BLK_K>(sh, part, tid);
-    if (tid == 0) F.pr_part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+    // wave reduction (fixed order) -> one partial per workgroup
+    for (int off = 32; off > 0; off >>= 1) part = part + __shfl_xor(part, off, 64);
+    if (lane == 0) F.pr_part[blockIdx.y * gridDim.x + blockIdx.x] = part;
 }
 
 void launch_upd(const FluidDev& F, hipStream_t st)
 {
-    dim3 g((F.n2m + BLK_J - 1) / BLK_J, (F.n3m + BLK_K - 1) / BLK_K);
-    hipLaunchKernelGGL(k_upd, g, dim3(BLK_J, BLK_K), 0, st, F);
+    dim3 g((F.n2m + WJ - 1) / WJ, (F.n3m + MK - 1) / MK);
+    hipLaunchKernelGGL(k_upd, g, dim3(64), 0, st, F);
 }
 
 // ---------------------------------------------------------------------------
