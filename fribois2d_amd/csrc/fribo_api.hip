@@ -53,7 +53,13 @@ struct fribo_ctx {
     double prof_ms[FRIBO_NPROF] = {0};
     int prof_n[FRIBO_NPROF] = {0};
     size_t lds_limit = 65536;
-    int zero_int = 0;
+    // step inputs staged in pinned memory: {dt, dt_o, ns, err=0}
+    struct StepIn { double dt, dto; int ns, err; };
+    StepIn* in = nullptr;
+    // captured step (hipGraph): rebuilt when the marker buffers change
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    int graph_nlm = -1;
 };
 
 template <class T>
@@ -380,6 +386,15 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
         return e;
     }
     memset(&c->hs, 0, sizeof(c->hs));
+    {
+        hipError_t he = hipHostMalloc((void**)&c->in, sizeof(fribo_ctx::StepIn), hipHostMallocDefault);
+        if (he != hipSuccess) {
+            for (void* q : c->allocs) hipFree(q);
+            hipStreamDestroy(c->st);
+            delete c;
+            return set_err(FRIBO_ERR_HIP, hipGetErrorString(he));
+        }
+    }
     *out = c;
     return FRIBO_OK;
 }
@@ -388,6 +403,8 @@ extern "C" int fribo_finalize(fribo_handle h)
 {
     if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
     hipStreamSynchronize(h->st);
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
+    if (h->in) hipHostFree(h->in);
     for (auto ev : h->evpool) hipEventDestroy(ev);
     for (void* q : h->mk_allocs) hipFree(q);
     for (void* q : h->allocs) hipFree(q);
@@ -582,18 +599,17 @@ static int markers_inside(fribo_ctx* c)
     return FRIBO_OK;
 }
 
-static void enqueue_cfl(fribo_ctx* c, int ns, double dt, double dt_o)
+static void enqueue_cfl(fribo_ctx* c)
 {
     FluidDev& F = c->F;
     run(c, PC_CFL, [&] { launch_cfl(F, c->st); });
-    (void)ns; (void)dt; (void)dt_o;
 }
 
-static void enqueue_substep(fribo_ctx* c, int n, int ns, double dt_in, double dt_o)
+static void enqueue_substep(fribo_ctx* c, int n)
 {
     FluidDev& F = c->F;
     hipStream_t st = c->st;
-    run(c, PC_BC, [&] { launch_bc(F, n, ns, dt_in, dt_o, st); });
+    run(c, PC_BC, [&] { launch_bc(F, n, st); });
     run(c, PC_RHS, [&] { launch_rhs(F, st); });
     if (F.ibact && F.nlm > 0) {
         if (n == 1) run(c, PC_MLS_SETUP, [&] { launch_mls_setup(F, st); });
@@ -614,10 +630,44 @@ static void enqueue_substep(fribo_ctx* c, int n, int ns, double dt_in, double dt
     run(c, PC_MEAN, [&] { launch_mean(F, st); });
 }
 
-static int reset_err(fribo_ctx* c)
+// Step inputs (dt, dt_o, ns) and a cleared error flag, one small copy.
+static int write_inputs(fribo_ctx* c, int ns, double dt, double dt_o)
 {
-    c->zero_int = 0;
-    HIPCHK(hipMemcpyAsync(&c->F.S->err, &c->zero_int, sizeof(int), hipMemcpyHostToDevice, c->st));
+    c->in->dt = dt;
+    c->in->dto = dt_o;
+    c->in->ns = ns;
+    c->in->err = 0;
+    HIPCHK(hipMemcpyAsync(&c->F.S->in_dt, c->in, sizeof(fribo_ctx::StepIn), hipMemcpyHostToDevice, c->st));
+    return FRIBO_OK;
+}
+
+static void enqueue_step(fribo_ctx* c)
+{
+    enqueue_cfl(c);
+    for (int n = 1; n <= c->F.nsst; ++n) enqueue_substep(c, n);
+}
+
+// Capture the whole step once; every kernel reads its per-step inputs from
+// device memory, so the same graph replays for every step.
+static int ensure_graph(fribo_ctx* c)
+{
+    if (c->gexec && c->graph_nlm == c->F.nlm) return FRIBO_OK;
+    if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    const bool prof = c->profiling;
+    c->profiling = false;
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    enqueue_step(c);
+    hipError_t e = hipStreamEndCapture(c->st, &g);
+    c->profiling = prof;
+    if (e != hipSuccess) return set_err(FRIBO_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return set_err(FRIBO_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    c->graph_nlm = c->F.nlm;
     return FRIBO_OK;
 }
 
@@ -643,27 +693,32 @@ extern "C" int fribo_fluid_tsch(fribo_handle h, int32_t ns, double* dt, double d
     if (!h || !dt) return set_err(FRIBO_ERR_ARG, "null argument");
     int e = markers_inside(h);
     if (e) return e;
-    if ((e = reset_err(h))) return e;
-    const double dt_in = *dt;
-    enqueue_cfl(h, ns, dt_in, dt_o);
-    for (int n = 1; n <= h->F.nsst; ++n) enqueue_substep(h, n, ns, dt_in, dt_o);
+    if ((e = write_inputs(h, ns, *dt, dt_o))) return e;
+    if (h->use_graph && !h->profiling) {
+        if ((e = ensure_graph(h))) return e;
+        HIPCHK(hipGraphLaunch(h->gexec, h->st));
+    } else {
+        enqueue_step(h);
+    }
     return finish(h, dt, cflm);
 }
 
-// The split form keeps the values fluid_tsch carries between its parts.
-static thread_local double g_split_dt = 0.0, g_split_dto = 0.0;
+extern "C" int fribo_set_graph(fribo_handle h, int32_t on)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    h->use_graph = on != 0;
+    return FRIBO_OK;
+}
 
 extern "C" int fribo_cfl_dt(fribo_handle h, int32_t ns, double* dt, double dt_o, double* cflm)
 {
     if (!h || !dt) return set_err(FRIBO_ERR_ARG, "null argument");
-    int e = reset_err(h);
+    int e = write_inputs(h, ns, *dt, dt_o);
     if (e) return e;
-    enqueue_cfl(h, ns, *dt, dt_o);
-    g_split_dt = *dt;
-    g_split_dto = dt_o;
+    enqueue_cfl(h);
     // finalize the CFL / dt control without running a substep: launch the
     // boundary kernel for substep 1 (it also prepares substep 1's factors)
-    run(h, PC_BC, [&] { launch_bc(h->F, 1, ns, *dt, dt_o, h->st); });
+    run(h, PC_BC, [&] { launch_bc(h->F, 1, h->st); });
     return finish(h, dt, cflm);
 }
 
@@ -696,7 +751,7 @@ extern "C" int fribo_substep(fribo_handle h, int32_t n, int32_t ns)
         run(h, PC_UPD, [&] { launch_upd(F, st); });
         run(h, PC_MEAN, [&] { launch_mean(F, st); });
     } else {
-        enqueue_substep(h, n, ns, g_split_dt, g_split_dto);
+        enqueue_substep(h, n);
     }
     return finish(h, nullptr, nullptr);
 }

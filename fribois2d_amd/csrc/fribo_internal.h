@@ -15,13 +15,15 @@
 #define FRIBO_NPART 512      // partial sums of the CFL reduction
 
 struct DevScalars {
+    // step inputs, written by the host before every step (one 24-byte copy)
+    double in_dt, in_dto;
+    int in_ns;
+    int err;                 // first error code raised on the device (host zeroes it)
     double dt, dt_o, beta, al, ga, ro, alre, betadx;
     double cflm, q3avg;
     double pend;             // pressure mean not yet subtracted from the interior
     double qinf, qout, area;
     double ugkks, ugkkn;
-    int err;                 // first error code raised on the device
-    int pad[5];
 };
 
 // Precomputed, line-invariant factors of one partitioned tridiagonal system
@@ -128,7 +130,7 @@ enum {
 
 // Kernel launchers (each .hip file owns its kernels).
 void launch_cfl(const FluidDev& F, hipStream_t st);
-void launch_bc(const FluidDev& F, int n, int ns, double dt_in, double dt_o, hipStream_t st);
+void launch_bc(const FluidDev& F, int n, hipStream_t st);
 void launch_rhs(const FluidDev& F, hipStream_t st);
 void launch_div(const FluidDev& F, hipStream_t st);
 void launch_upd(const FluidDev& F, hipStream_t st);
@@ -142,4 +144,3 @@ void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double*
                     double* C, int ldc, const DevScalars* S, hipStream_t st);
 size_t jsolve_lds_bytes(const FluidDev& F);
 size_t sysprep_lds_bytes(const FluidDev& F);
-void launch_bc(const FluidDev& F, int n, int ns, double dt_in, double dt_o, hipStream_t st);

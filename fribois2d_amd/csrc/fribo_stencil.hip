@@ -263,13 +263,15 @@ size_t sysprep_lds_bytes(const FluidDev& F)
 
 // Workgroup 0: finalize CFL/dt (substep 1), RK coefficients, boucdq.
 // Workgroups 1..4: factors of the four Helmholtz systems of this substep.
-__global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int ns, double dt_in, double dt_o_in)
+__global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n)
 {
     extern __shared__ double dyn[];
     __shared__ double sh[1024];
     __shared__ double sh2[1024];
     DevScalars* S = F.S;
     const int t = threadIdx.x;
+    const int ns = S->in_ns;
+    const double dt_in = S->in_dt, dt_o_in = S->in_dto;
     const StepScalars r = step_scalars(F, S, n, ns, dt_in, dt_o_in, sh, sh2);
     if (r.err) {
         if (blockIdx.x == 0 && t == 0 && n == 1) {
@@ -344,9 +346,9 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int ns, do
     }
 }
 
-void launch_bc(const FluidDev& F, int n, int ns, double dt_in, double dt_o, hipStream_t st)
+void launch_bc(const FluidDev& F, int n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_bc, dim3(5), dim3(1024), sysprep_lds_bytes(F), st, F, n, ns, dt_in, dt_o);
+    hipLaunchKernelGGL(k_bc, dim3(5), dim3(1024), sysprep_lds_bytes(F), st, F, n);
 }
 
 // ---------------------------------------------------------------------------

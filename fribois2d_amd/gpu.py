@@ -25,7 +25,7 @@ ERR_NAMES = {1: "DT_TOO_SMALL", 2: "CFL_TOO_LARGE", 3: "MARKER_OUTSIDE", 4: "PAR
 EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state", "fribo_set_markers",
            "fribo_fluid_tsch", "fribo_cfl_dt", "fribo_substep", "fribo_array_size", "fribo_get_array",
            "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_set_profiling", "fribo_get_profile",
-           "fribo_reset_profile", "fribo_profile_name"]
+           "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph"]
 
 NPROF = 12
 
@@ -83,6 +83,7 @@ def load():
     L.fribo_reset_profile.argtypes = [vp]
     L.fribo_profile_name.argtypes = [C.c_int32]
     L.fribo_profile_name.restype = C.c_char_p
+    L.fribo_set_graph.argtypes = [vp, C.c_int32]
     _lib = L
     return L
 
@@ -193,6 +194,10 @@ class FluidStep:
         if n == n2 * n3m and name in ("ru2", "ru3", "rhs2", "rhs3"):
             return out.reshape((n2, n3m), order="F")
         return out
+
+    def set_graph(self, on=True):
+        """Replay the step as one captured HIP graph (same results)."""
+        _check(load().fribo_set_graph(self.h, 1 if on else 0))
 
     # -- profiling ----------------------------------------------------------
     def set_profiling(self, on=True):

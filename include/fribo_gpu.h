@@ -110,6 +110,12 @@ long fribo_array_size(fribo_handle h, const char* name);
 int fribo_get_array(fribo_handle h, const char* name, void* out, long n);
 
 int fribo_sync(fribo_handle h);
+
+/* Execution option (no reference counterpart): with on != 0 the step's
+ * kernels are captured once into a HIP graph and replayed by every
+ * fribo_fluid_tsch call (re-captured when the marker count changes);
+ * results are identical to the default stream launches. */
+int fribo_set_graph(fribo_handle h, int32_t on);
 const char* fribo_last_error(void);
 
 /* Diagnostic entry point (no reference counterpart): C = A * B^T in FP64 on

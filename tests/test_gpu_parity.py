@@ -202,3 +202,26 @@ def test_error_codes():
     assert rc == 2
     with pytest.raises(gpu.FriboError):
         g.fluid_tsch(0, 10.0)
+
+
+def test_graph_replay_matches_stream_launches():
+    """The captured step replayed over many steps (with eager host work and
+    synchronisation between replays) equals stream launches bit for bit."""
+    c = case("cyl64")
+    dec = decomposition(c)
+    ge = gpu_for(c, dec)
+    gg = gpu_for(c, dec)
+    gg.set_graph(True)
+    t, dt = 0.0, c.params.dt0
+    for ns in range(12):
+        lm = c.markers(t)
+        ge.set_markers(*lm)
+        gg.set_markers(*lm)
+        dte, cfe = ge.fluid_tsch(ns, dt, dt)
+        dtg, cfg = gg.fluid_tsch(ns, dt, dt)
+        assert dte == dtg and cfe == cfg
+        se, sg = ge.get_state(), gg.get_state()
+        for f in ("q2", "q3", "pr", "ru2", "ru3"):
+            np.testing.assert_array_equal(se[f], sg[f], err_msg=f"{f} step {ns}")
+        t += dte
+        dt = dte
