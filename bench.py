@@ -45,11 +45,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="rect2048")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the library's plain 256 MiB copy kernel 3+1 times (HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        from fribois2d_amd.gpu import plain_copy
+        nbytes = 256 << 20
+        ms = plain_copy(nbytes, 3)
+        print(json.dumps({"kernel": "k_plain_copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "dispatches": 4, "ms_per_copy": round(ms, 4),
+                          "GB/s": round(2 * nbytes / ms / 1e6, 1)}))
+        return
 
     import numpy as np
     from fribois2d_amd import cases

@@ -887,6 +887,41 @@ extern "C" int fribo_test_gemm(int32_t M, int32_t N, int32_t K, const double* A,
     return FRIBO_OK;
 }
 
+// Streaming copy of a known byte count: the yardstick that turns the HBM
+// counters (FETCH_SIZE / WRITE_SIZE) of the step's kernels into bytes.
+__global__ void __launch_bounds__(256) k_plain_copy(const double2* __restrict__ a, double2* __restrict__ b,
+                                                    size_t n)
+{
+    const size_t s = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += s) b[i] = a[i];
+}
+
+extern "C" int fribo_test_copy(int64_t bytes, int32_t reps, double* ms_per_copy)
+{
+    if (bytes <= 0 || bytes % 16 != 0 || reps <= 0) return set_err(FRIBO_ERR_ARG, "bad copy arguments");
+    double2 *a, *b;
+    HIPCHK(hipMalloc(&a, (size_t)bytes));
+    HIPCHK(hipMalloc(&b, (size_t)bytes));
+    HIPCHK(hipMemset(a, 0, (size_t)bytes));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const size_t n = (size_t)bytes / 16;
+    hipLaunchKernelGGL(k_plain_copy, dim3(4096), dim3(256), 0, 0, a, b, n);   // warm-up
+    HIPCHK(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_plain_copy, dim3(4096), dim3(256), 0, 0, a, b, n);
+    HIPCHK(hipEventRecord(e1, 0));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return FRIBO_OK;
+}
+
 extern "C" int fribo_set_profiling(fribo_handle h, int32_t on)
 {
     if (!h) return set_err(FRIBO_ERR_ARG, "null handle");

@@ -25,7 +25,7 @@ ERR_NAMES = {1: "DT_TOO_SMALL", 2: "CFL_TOO_LARGE", 3: "MARKER_OUTSIDE", 4: "PAR
 EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state", "fribo_set_markers",
            "fribo_fluid_tsch", "fribo_cfl_dt", "fribo_substep", "fribo_array_size", "fribo_get_array",
            "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_set_profiling", "fribo_get_profile",
-           "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph"]
+           "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy"]
 
 NPROF = 12
 
@@ -84,6 +84,7 @@ def load():
     L.fribo_profile_name.argtypes = [C.c_int32]
     L.fribo_profile_name.restype = C.c_char_p
     L.fribo_set_graph.argtypes = [vp, C.c_int32]
+    L.fribo_test_copy.argtypes = [C.c_int64, C.c_int32, dp]
     _lib = L
     return L
 
@@ -224,3 +225,10 @@ def gemm_nt(A, B):
     Cm = np.zeros((M, N), order="F")
     _check(L.fribo_test_gemm(M, N, K, _dp(A), M, _dp(B), N, _dp(Cm), M))
     return Cm
+
+
+def plain_copy(nbytes, reps):
+    """Time `reps` device-to-device streaming copies of `nbytes`; ms per copy."""
+    ms = C.c_double(0.0)
+    _check(load().fribo_test_copy(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value

@@ -124,6 +124,12 @@ const char* fribo_last_error(void);
 int fribo_test_gemm(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
                     int32_t ldb, double* C, int32_t ldc);
 
+/* Diagnostic entry point (no reference counterpart): `reps` streaming copies
+ * of `bytes` (a multiple of 16) between two device buffers after one warm-up
+ * copy; returns the mean device time per copy.  It is the known-size kernel
+ * that scales the HBM traffic counters of the step's kernels. */
+int fribo_test_copy(int64_t bytes, int32_t reps, double* ms_per_copy);
+
 /* Diagnostic timing (no reference counterpart): with profiling on, every
  * kernel launch of the step is bracketed by HIP events on the library's
  * stream; fribo_get_profile returns, per kernel class, the summed device
