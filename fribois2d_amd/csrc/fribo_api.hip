@@ -8,6 +8,7 @@
 #include "../../include/fribo_gpu.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -233,6 +234,9 @@ static int setup_ksys(fribo_ctx* c, KSys& K, int n, int kofs, int src, int inter
     if ((rc = dalloc(c, &K.k2, 6 * 64))) return rc;
     if ((rc = dalloc(c, &K.ib, 64))) return rc;
     if ((rc = dalloc(c, &K.bc, 2))) return rc;
+    if ((rc = dalloc(c, &K.ra, 64))) return rc;
+    if ((rc = dalloc(c, &K.rinv, 64))) return rc;
+    if ((rc = dalloc(c, &K.rcp, 64))) return rc;
     return FRIBO_OK;
 }
 
@@ -296,6 +300,7 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     FluidDev& F = c->F;
     memset(&F, 0, sizeof(F));
     F.n2 = p->n2; F.n3 = p->n3; F.n2m = p->n2 - 1; F.n3m = p->n3 - 1;
+    F.lds_max = (int)c->lds_limit;
     F.P = (F.n2 + 15) / 16 * 16;
     F.nk = F.n3m + 6;
     F.nsst = p->nsst; F.idtv = p->idtv; F.ibact = p->ibact; F.wfun = p->wfun;
@@ -368,7 +373,9 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     if (!e) {
         int mm = std::max(std::max(F.js[0].maxm, F.js[1].maxm), F.ps.maxm);
         F.jstride = mm | 1;
-        if (jsolve_lds_bytes(F) > c->lds_limit || sysprep_lds_bytes(F) + 17000 > c->lds_limit)
+        if (F.n2m > 256 * 16)
+            e = set_err(FRIBO_ERR_LIMIT, "n2 too large for the CFL kernel of this build (n2-1 <= 4096)");
+        else if (jsolve_lds_bytes(F) > c->lds_limit || sysprep_lds_bytes(F) + 17000 > c->lds_limit)
             e = set_err(FRIBO_ERR_LIMIT, "grid too large for the LDS staging of this build");
     }
     if (!e) e = dalloc(c, &F.kYs, (size_t)2 * FRIBO_KMAXCH * F.n2m);
@@ -462,7 +469,9 @@ extern "C" int fribo_set_state(fribo_handle h, const double* q2, const double* q
         // the uploaded pressure is already mean-free: nothing pending
         if ((e = read_scalars(h))) return e;
         h->hs.pend = 0.0;
-        HIPCHK(hipMemcpyAsync(&F.S->pend, &h->hs.pend, sizeof(double), hipMemcpyHostToDevice, h->st));
+        h->hs.goff = 0.0;
+        static_assert(offsetof(DevScalars, goff) == offsetof(DevScalars, pend) + sizeof(double), "layout");
+        HIPCHK(hipMemcpyAsync(&F.S->pend, &h->hs.pend, 2 * sizeof(double), hipMemcpyHostToDevice, h->st));
     }
     HIPCHK(hipStreamSynchronize(h->st));
     return FRIBO_OK;
@@ -490,13 +499,16 @@ extern "C" int fribo_get_state(fribo_handle h, double* q2, double* q3, double* p
     if (e) return e;
     if ((e = read_scalars(h))) return e;
     if (pr) {
-        // apply the pending mean to the interior (updvp.F90:141)
-        const double pend = h->hs.pend;
-        for (int k = 1; k <= F.n3m; ++k)
-            for (int j = 1; j <= F.n2m; ++j) {
+        // apply the pending means (updvp.F90:141): the last one to the
+        // interior, their running sum to row n2 and the ghost planes
+        const double pend = h->hs.pend, goff = h->hs.goff;
+        for (int k = -2; k <= F.n3m + 3; ++k) {
+            const bool ghost = (k < 1 || k > F.n3m);
+            for (int j = 1; j <= F.n2; ++j) {
                 double& v = pr[(size_t)(j - 1) + (size_t)(k + 2) * F.n2];
-                v = v - pend;
+                v = v - ((ghost || j == F.n2) ? goff : pend);
             }
+        }
     }
     return FRIBO_OK;
 }

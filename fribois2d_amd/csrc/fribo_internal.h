@@ -22,6 +22,8 @@ struct DevScalars {
     double dt, dt_o, beta, al, ga, ro, alre, betadx;
     double cflm, q3avg;
     double pend;             // pressure mean not yet subtracted from the interior
+    double goff;             // sum of the means not yet subtracted from the pressure
+                             // entries no kernel reads (row n2, ghost planes)
     double qinf, qout, area;
     double ugkks, ugkkn;
 };
@@ -52,6 +54,9 @@ struct KSys {
     double* k2;              // [6*64] PCR multipliers (upper)
     double* ib;              // [64] 1/beta after PCR
     double* bc;              // [2] couplings of rows 0 / n-1 to the Dirichlet ends
+    double* ra;              // [64] sub-diagonal of the separator system
+    double* rinv;            // [64] 1/pivot of its Thomas factorisation
+    double* rcp;             // [64] its forward multipliers c'
 };
 
 // Fixed partition of the Poisson j-lines (n2m rows; the per-mode matrices
@@ -97,6 +102,7 @@ struct FluidDev {
     KSys js[2];
     PSys ps;
     int jstride;             // LDS chunk stride of the j-line kernels (odd)
+    int lds_max;             // LDS bytes a workgroup may use on this device
     double *kYs, *kYe, *kDe, *kX;  // [2][64][n2m] reduced-system data
     // markers
     double *lmcc, *lmvel, *lmA;    // (nlm,2) column-major

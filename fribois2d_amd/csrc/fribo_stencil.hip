@@ -44,26 +44,52 @@ __device__ inline void block_reduce_max(double* sh, double v, int t = threadIdx.
 }
 
 // ---------------------------------------------------------------------------
-// cfl.F90:19-38 -- partial max of the local CFL and partial volume sum of q3
+// cfl.F90:19-38 -- partial max of the local CFL and partial volume sum of q3.
+// Block b owns a contiguous range of planes k; thread t owns the columns
+// j = t+1, t+257, ... and carries q3 of plane k+1 into the next plane, so q2
+// and q3 are each read from HBM once.  Blocks without planes write the
+// neutral partials (max seed 1e-8, sum 0).
+#define CFL_NJ 16  // columns per thread: n2m <= 256*CFL_NJ
 __global__ __launch_bounds__(256) void k_cfl(const FluidDev F)
 {
     __shared__ double sh[256];
-    const long ncell = (long)F.n2m * F.n3m;
+    const int t = threadIdx.x;
+    const int per = (F.n3m + FRIBO_NPART - 1) / FRIBO_NPART;
+    const int k0 = blockIdx.x * per + 1;
+    const int k1 = min(k0 + per - 1, F.n3m);
+    const double* __restrict__ q2 = F.q2;
+    const double* __restrict__ q3 = F.q3;
     double m = 1.0e-8, s = 0.0;
-    for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (long)gridDim.x * 256) {
-        int j = (int)(c % F.n2m) + 1;
-        int k = (int)(c / F.n2m) + 1;
-        double q2a = F.q2[fidx(F, j, k)], q2b = F.q2[fidx(F, j + 1, k)];
-        double q3a = F.q3[fidx(F, j, k)], q3b = F.q3[fidx(F, j, k + 1)];
-        double qcf = fabs((q2a + q2b) * 0.5 * F.udx2m[j]) + fabs((q3a + q3b) * 0.5 * F.udx3m[k]);
-        m = fmax(m, qcf);
-        s = s + q3a * (F.zz[k + 1] - F.zz[k]) * (F.rc[j + 1] - F.rc[j]);
+    double u2[CFL_NJ], dy[CFL_NJ], q3c[CFL_NJ];
+#pragma unroll
+    for (int u = 0; u < CFL_NJ; ++u) {
+        const int j = t + 1 + 256 * u;
+        const bool ok = j <= F.n2m && k0 <= k1;
+        u2[u] = ok ? F.udx2m[j] : 0.0;
+        dy[u] = ok ? F.rc[j + 1] - F.rc[j] : 0.0;
+        q3c[u] = ok ? q3[fidx(F, j, k0)] : 0.0;
+    }
+    for (int k = k0; k <= k1; ++k) {
+        const double u3 = F.udx3m[k], dz = F.zz[k + 1] - F.zz[k];
+#pragma unroll
+        for (int u = 0; u < CFL_NJ; ++u) {
+            const int j = t + 1 + 256 * u;
+            if (j <= F.n2m) {
+                const size_t c = fidx(F, j, k);
+                const double q2a = q2[c], q2b = q2[c + 1];
+                const double q3a = q3c[u], q3b = q3[c + F.P];
+                const double qcf = fabs((q2a + q2b) * 0.5 * u2[u]) + fabs((q3a + q3b) * 0.5 * u3);
+                m = fmax(m, qcf);
+                s = s + q3a * dz * dy[u];
+                q3c[u] = q3b;
+            }
+        }
     }
     block_reduce_max<256>(sh, m);
     double bm = sh[0];
     __syncthreads();
     block_reduce_sum<256>(sh, s);
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         F.cfl_part[2 * blockIdx.x] = bm;
         F.cfl_part[2 * blockIdx.x + 1] = sh[0];
     }
@@ -234,6 +260,22 @@ __device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, dou
         beta = beta + ce * vs_n;
         gamma = ce * ws_n;
     }
+    if (!K.inter) {
+        // Thomas factors of the separator system, for the k-line kernels whose
+        // threads each solve one column's separators (every lane runs the
+        // same recurrence on the gathered coefficients and keeps its own row)
+        double cprev = 0.0;
+        for (int q = 0; q < 64; ++q) {
+            const double aq = __shfl(alpha, q, 64), bq = __shfl(beta, q, 64), gq = __shfl(gamma, q, 64);
+            const double iv = 1.0 / (bq - aq * cprev);
+            cprev = gq * iv;
+            if (p == q) {
+                K.ra[q] = aq;
+                K.rinv[q] = iv;
+                K.rcp[q] = cprev;
+            }
+        }
+    }
     int li = 0;
     for (int h = 1; h < 64; h <<= 1, ++li) {
         double amn = __shfl_up(alpha, h, 64), bm = __shfl_up(beta, h, 64), gm = __shfl_up(gamma, h, 64);
@@ -382,12 +424,20 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
     const double* __restrict__ q2 = F.q2;
     const double* __restrict__ q3 = F.q3;
     const double* __restrict__ pr = F.pr;
+    double* __restrict__ ru2 = F.ru2;
+    double* __restrict__ ru3 = F.ru3;
+    double* __restrict__ rhs2 = F.rhs2;
+    double* __restrict__ rhs3 = F.rhs3;
     const size_t P = F.P;
     size_t c = inj ? fidx(F, j, k0) : 0;
-    // window: planes k-1, k (and k+1 loaded each step)
+    // window: planes k-1, k; plane k+1 (and ru2/ru3/pr of plane k) are
+    // loaded one iteration ahead so two planes of loads are in flight
     double q2m = inj ? q2[c - P] : 0.0, q2c = inj ? q2[c] : 0.0;
     double q3m = inj ? q3[c - P] : 0.0, q3c = inj ? q3[c] : 0.0;
     double prm = inj ? pr[c - P] - pend : 0.0;
+    double q2n = inj ? q2[c + P] : 0.0, q3n = inj ? q3[c + P] : 0.0;
+    double prn = inj ? pr[c] - pend : 0.0;
+    double ru2n = out ? ru2[c] : 0.0, ru3n = out ? ru3[c] : 0.0;
     // per-column coefficients
     const int jj = out ? j : 1;
     const double udx2c = F.udx2c[jj], udx2m = F.udx2m[jj];
@@ -397,9 +447,14 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
     const double qb3s = F.qb3s[jj], qb3sm = F.qb3s[jj > 1 ? jj - 1 : 1];
     const double qb3n = F.qb3n[jj], qb3nm = F.qb3n[jj > 1 ? jj - 1 : 1];
     for (int k = k0; k <= k1; ++k, c += P) {
-        const double q2p = inj ? q2[c + P] : 0.0;
-        const double q3p = inj ? q3[c + P] : 0.0;
-        const double prc = inj ? pr[c] - pend : 0.0;
+        const double q2p = q2n, q3p = q3n, prc = prn, ru2c = ru2n, ru3c = ru3n;
+        if (k < k1) {
+            q2n = inj ? q2[c + 2 * P] : 0.0;
+            q3n = inj ? q3[c + 2 * P] : 0.0;
+            prn = inj ? pr[c + P] - pend : 0.0;
+            ru2n = out ? ru2[c + P] : 0.0;
+            ru3n = out ? ru3[c + P] : 0.0;
+        }
         // neighbours in j
         const double q2E = dn1(q2c), q2W = up1(q2c), q2Em = dn1(q2m), q2Ep = dn1(q2p);
         const double q3W = up1(q3c), q3Wp = up1(q3p), q3E = dn1(q3c);
@@ -438,9 +493,8 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
                     dpx22 = (prc - prW) * udx2c * al;
                 }
                 double dcq2 = d22q2 + d33q2;
-                const size_t o = fidx(F, j, k);
-                F.rhs2[o] = (ga * dphv + ro * F.ru2[o] + alre * dcq2 - dpx22) * dt;
-                F.ru2[o] = dphv;
+                rhs2[c] = (ga * dphv + ro * ru2c + alre * dcq2 - dpx22) * dt;
+                ru2[c] = dphv;
             }
             // ---- q3: hdnl3 + invtr3 (k >= 2)
             if (k >= 2) {
@@ -461,9 +515,8 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
                 double dq33 = q3p * F.ap3ck[k] + q3c * F.ac3ck[k] + q3m * F.am3ck[k];
                 double dcq3 = dq32 + dq33;
                 double dpx33 = (prc - prm) * udx3;
-                const size_t o = fidx(F, j, k);
-                F.rhs3[o] = (ga * qcapv + ro * F.ru3[o] + alre * dcq3 - dpx33) * dt;
-                F.ru3[o] = qcapv;
+                rhs3[c] = (ga * qcapv + ro * ru3c + alre * dcq3 - dpx33) * dt;
+                ru3[c] = qcapv;
             }
         }
         q2m = q2c;
@@ -497,14 +550,21 @@ __global__ __launch_bounds__(64) void k_div(const FluidDev F)
     const size_t P = F.P;
     size_t c = inj ? fidx(F, j, k0) : 0;
     const double udx2m = F.udx2m[out ? j : 1];
-    double q3c = inj ? F.q3[c] : 0.0;
+    const double* __restrict__ q2 = F.q2;
+    const double* __restrict__ q3 = F.q3;
+    double* __restrict__ dph = F.dph;
+    double q3c = inj ? q3[c] : 0.0;
+    double q2n = inj ? q2[c] : 0.0, q3n = inj ? q3[c + P] : 0.0;
     for (int k = k0; k <= k1; ++k, c += P) {
-        const double q2c = inj ? F.q2[c] : 0.0;
-        const double q3p = inj ? F.q3[c + P] : 0.0;
+        const double q2c = q2n, q3p = q3n;
+        if (k < k1) {
+            q2n = inj ? q2[c + P] : 0.0;
+            q3n = inj ? q3[c + 2 * P] : 0.0;
+        }
         const double q2E = dn1(q2c);
         if (out) {
             double dq = (q2E - q2c) * udx2m + (q3p - q3c) * F.udx3m[k];
-            F.dph[c] = dq * usdtal;
+            dph[c] = dq * usdtal;
         }
         q3c = q3p;
     }
@@ -541,25 +601,38 @@ __global__ __launch_bounds__(64) void k_upd(const FluidDev F)
         const bool jlo = (j == 1), jhi = (j == F.n2m);
         size_t c = inj ? fidx(F, j, k0) : 0;
         const double* __restrict__ dph = F.dph;
+        double* __restrict__ q2 = F.q2;
+        double* __restrict__ q3 = F.q3;
+        double* __restrict__ pr = F.pr;
+        const int n3m = F.n3m;
         double dm = (inj && k0 > 1) ? dph[c - P] : 0.0;
         double dc = inj ? dph[c] : 0.0;
+        // plane k+1 of dph and plane k of q2/q3/pr are loaded one iteration ahead
+        double dn = (inj && k0 < n3m) ? dph[c + P] : 0.0;
+        double q2n = out ? q2[c] : 0.0, q3n = out ? q3[c] : 0.0, prn = out ? pr[c] : 0.0;
         for (int k = k0; k <= k1; ++k, c += P) {
-            const double dp = (inj && k < F.n3m) ? dph[c + P] : 0.0;
+            const double dp = dn, q2v = q2n, q3v = q3n, prv0 = prn;
+            if (k < k1) {
+                dn = (inj && k + 1 < n3m) ? dph[c + 2 * P] : 0.0;
+                q2n = out ? q2[c + P] : 0.0;
+                q3n = out ? q3[c + P] : 0.0;
+                prn = out ? pr[c + P] : 0.0;
+            }
             const double dW = up1(dc), dE = dn1(dc);
             if (out) {
                 const double dJm = jlo ? dc : dW;     // dph(jmv(j))
                 const double dJp = jhi ? dc : dE;     // dph(jpv(j))
                 const double dKm = (k == 1) ? dc : dm; // dph(kmv(k))
-                const double dKp = (k == F.n3m) ? dc : dp;
-                F.q2[c] = F.q2[c] - (dc - dJm) * usujm;
+                const double dKp = (k == n3m) ? dc : dp;
+                q2[c] = q2v - (dc - dJm) * usujm;
                 if (k >= 2) {
                     const double usukm = al * dt * F.udx3c[k];
-                    F.q3[c] = F.q3[c] - (dc - dm) * usukm;
+                    q3[c] = q3v - (dc - dm) * usukm;
                 }
                 double lap = (dJp * apj + dc * acj + dJm * amj) +
                              (dKp * F.apphk[k] + dc * F.acphk[k] + dKm * F.amphk[k]);
-                double prv = (F.pr[c] - pend) + dc - be * lap;
-                F.pr[c] = prv;
+                double prv = (prv0 - pend) + dc - be * lap;
+                pr[c] = prv;
                 part = part + prv * (F.zz[k + 1] - F.zz[k]) * dy;
             }
             dm = dc;
@@ -579,8 +652,9 @@ void launch_upd(const FluidDev& F, hipStream_t st)
 
 // ---------------------------------------------------------------------------
 // pr_submean (updvp.F90:125-141) and boucqt (iobcs.F90:300-312).  The mean
-// is applied directly to the pressure entries no kernel of the step reads
-// (row j=n2 and the ghost planes) and kept pending for the interior.
+// is kept pending for the interior (S->pend, subtracted on the next read)
+// and accumulated for the entries no kernel reads (row j=n2 and the ghost
+// planes, S->goff, subtracted when the state is read back).
 __global__ __launch_bounds__(1024) void k_mean(const FluidDev F)
 {
     __shared__ double sh[1024];
@@ -591,27 +665,16 @@ __global__ __launch_bounds__(1024) void k_mean(const FluidDev F)
     for (int i = t; i < F.npr_part; i += 1024) s = s + F.pr_part[i];
     block_reduce_sum<1024>(sh, s);
     const double pravg = sh[0] / (F.rext2 * F.alx3);
-    // row j = n2, planes -2..n3m+3
-    for (int k = -2 + t; k <= F.n3m + 3; k += 1024) {
-        size_t c = fidx(F, F.n2, k);
-        F.pr[c] = F.pr[c] - pravg;
-    }
-    // ghost planes, rows 1..n2m
-    const int ng = 6 * F.n2m;
-    for (int i = t; i < ng; i += 1024) {
-        int kk = i / F.n2m, j = i % F.n2m + 1;
-        int k = (kk < 3) ? kk - 2 : F.n3m + 1 + (kk - 3);
-        size_t c = fidx(F, j, k);
-        F.pr[c] = F.pr[c] - pravg;
-    }
     for (int j = 1 + t; j <= F.n2m; j += 1024) {
         F.qb2s[j] = F.qb2s[j] + F.dqb2s[j];
         F.qb3s[j] = F.qb3s[j] + F.dqb3s[j];
         F.qb2n[j] = F.qb2n[j] + F.dqb2n[j];
         F.qb3n[j] = F.qb3n[j] + F.dqb3n[j];
     }
-    __syncthreads();
-    if (t == 0) S->pend = pravg;
+    if (t == 0) {
+        S->pend = pravg;
+        S->goff = S->goff + pravg;
+    }
 }
 
 void launch_mean(const FluidDev& F, hipStream_t st)

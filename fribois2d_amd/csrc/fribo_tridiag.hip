@@ -1,14 +1,23 @@
 // Tridiagonal line solves of the fluid step (gfx950).
 //
 // j-direction (contiguous lines): solq2j / solq3j (solve_fluid.F90:4-103)
-// and the per-mode Poisson solves of phcalc_outflow.F90:34-56.  One 64-lane
-// wave owns one line, staged in LDS with coalesced loads/stores.  The line is
-// split into up to 64 chunks (one per lane); each lane eliminates its chunk,
-// the chunk separators form a 64-unknown tridiagonal system that the wave
-// solves by parallel cyclic reduction over lane shuffles, and each lane then
-// back-substitutes its chunk.  This is an exact direct solve of the same
-// normalised system the reference hands to DGTTRF/DGTTRS (which never pivots
-// on these diagonally dominant matrices); only the rounding order differs.
+// and the per-mode Poisson solves of phcalc_outflow.F90:34-56.  A workgroup
+// of JW waves keeps the line-invariant data of one system in LDS (the
+// Helmholtz factors, or the Poisson coefficient rows) and its waves loop over
+// lines, one line per wave at a time.  A line is split into up to 64 chunks
+// (one per lane): the wave loads the line coalesced into registers (the next
+// line's loads are in flight while the current line is solved), scatters it
+// into an LDS chunk layout, each lane eliminates its chunk, the chunk
+// separators form a 64-unknown tridiagonal system that the wave solves by
+// parallel cyclic reduction over lane shuffles, each lane back-substitutes its
+// chunk, and the line is gathered back coalesced.  This is an exact direct
+// solve of the same normalised system the reference hands to DGTTRF/DGTTRS
+// (which never pivots on these diagonally dominant matrices); only the
+// rounding order differs.
+//
+// A lane's m-1 interior rows are aligned to the end of a MAXR-step loop
+// (front padding), so every step reads its LDS data unconditionally (clamped
+// index); padded steps leave the recurrences untouched.
 //
 // k-direction (strided lines): solq2k / solq3k (solve_fluid.F90:107-255).
 // Lanes run along j (coalesced), each thread owns one chunk of one k-line.
@@ -18,227 +27,403 @@
 // precomputed multipliers) -> chunk back-substitution fused with q += dq.
 #include "fribo_internal.h"
 
-size_t jsolve_lds_bytes(const FluidDev& F) { return (size_t)64 * F.jstride * sizeof(double); }
+#include <algorithm>
 
 // Row r of a j-line belongs to chunk owner(r) when chunk starts are
-// floor(p*n/nch).
-__device__ inline int chunk_owner(int r, int n, int nch) { return (int)((((long)r + 1) * nch - 1) / n); }
-
-// Coalesced global <-> LDS staging of one line in chunk layout: row t of
-// chunk p at lds[p*stride + t] (odd stride: conflict-free lane access).
-__device__ inline void line_to_lds(const double* __restrict__ src, double* lds, const int* cs, int n, int nch,
-                                   int stride, int lane)
+// floor(p*n/nch); (r+1)*nch < 2^32 for every line this build accepts.
+__device__ inline int chunk_owner(int r, int n, int nch)
 {
-    for (int r = lane; r < n; r += 64) {
-        const int p = chunk_owner(r, n, nch);
-        lds[p * stride + (r - cs[p])] = src[r];
+    return (int)(((unsigned)(r + 1) * (unsigned)nch - 1u) / (unsigned)n);
+}
+
+// Per-lane staging map of a j-line: element u of the lane (row r = lane+64u)
+// goes to LDS slot p*stride + (r - cs[p]) of the chunk layout (-1: no row).
+template <int NR>
+__device__ inline void line_map(int* slot, const int* cs, int n, int nch, int stride, int lane)
+{
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        const int r = lane + 64 * u;
+        int v = -1;
+        if (r < n) {
+            const int p = chunk_owner(r, n, nch);
+            v = p * stride + (r - cs[p]);
+        }
+        slot[u] = v;
     }
 }
 
-__device__ inline void lds_to_line(double* __restrict__ dst, const double* lds, const int* cs, int n, int nch,
-                                   int stride, int lane)
+template <int NR>
+__device__ inline void line_fetch(double* pre, const double* __restrict__ src, int n, int lane, bool act)
 {
-    for (int r = lane; r < n; r += 64) {
-        const int p = chunk_owner(r, n, nch);
-        dst[r] = lds[p * stride + (r - cs[p])];
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        const int r = lane + 64 * u;
+        pre[u] = (act && r < n) ? src[r] : 0.0;
     }
+}
+
+template <int NR>
+__device__ inline void line_put(double* stage, const double* pre, const int* slot)
+{
+#pragma unroll
+    for (int u = 0; u < NR; ++u)
+        if (slot[u] >= 0) stage[slot[u]] = pre[u];
+}
+
+template <int NR>
+__device__ inline void line_get(double* __restrict__ dst, const double* stage, const int* slot, int lane)
+{
+#pragma unroll
+    for (int u = 0; u < NR; ++u)
+        if (slot[u] >= 0) dst[lane + 64 * u] = stage[slot[u]];
+}
+
+size_t jsolve_h_lds_bytes(int maxr, int jw, int stride, bool fl)
+{
+    const size_t nr = (size_t)maxr + 1;
+    return ((fl ? 6 * nr * 64 + 2 * 384 + 2 * 64 : 0) + (size_t)jw * 64 * stride) * sizeof(double);
+}
+
+size_t jsolve_p_lds_bytes(int maxr, int jw, int stride)
+{
+    const size_t nr = (size_t)maxr + 1;
+    return (3 * nr * 64 + (size_t)jw * 64 * stride) * sizeof(double);
 }
 
 // Helmholtz j-lines (solq2j / solq3j): q2 for k = 1..n3m (rows j = 2..n2m;
 // rows 1 and n2 are the Dirichlet identity rows with zero rhs), then q3 for
 // k = 2..n3m (rows j = 1..n2m).  All factors are line-invariant (k_bc); the
-// wave only runs the rhs through them.
-template <int MAXR>
-__global__ __launch_bounds__(64) void k_jsolve_h(const FluidDev F)
+// first half of the grid takes the q2 lines, the second half the q3 lines.
+template <int MAXR, int JW, bool FL>
+__global__ __launch_bounds__(64 * JW) void k_jsolve_h(const FluidDev F)
 {
+    constexpr int NR = MAXR + 1;
     extern __shared__ double lds[];
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int lane = threadIdx.x, line = blockIdx.x;
-    const int f = line < F.n3m ? 0 : 1;
-    const int k = f == 0 ? line + 1 : line - F.n3m + 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int half = (int)gridDim.x >> 1;
+    const int f = (int)blockIdx.x < half ? 0 : 1;
+    const int wg = f == 0 ? (int)blockIdx.x : (int)blockIdx.x - half;
+    const int nwgf = f == 0 ? half : (int)gridDim.x - half;
     const KSys& K = F.js[f];
-    const double* src = (f == 0 ? F.rhs2 : F.rhs3) + fidx(F, K.kofs, k);
-    double* dst = (f == 0 ? F.sA : F.sB) + fidx(F, K.kofs, k);
+    const int nlines = f == 0 ? F.n3m : F.n3m - 1;
+    const int kfirst = f == 0 ? 1 : 2;
+    const double* srcb = f == 0 ? F.rhs2 : F.rhs3;
+    double* dstb = f == 0 ? F.sA : F.sB;
     const int n = K.n, nch = K.nch, stride = F.jstride;
-    line_to_lds(src, lds, K.cs, n, nch, stride, lane);
-    __syncthreads();
     const int p = lane;
-    double y[MAXR];
-    double yl = 0.0, acc = 0.0, de = 0.0, ae = 0.0, ce = 0.0;
-    int m = 0;
-    double* row = lds + p * stride;
-    if (p < nch) {
-        m = K.cs[p + 1] - K.cs[p];
-#pragma unroll
-        for (int t = 0; t < MAXR; ++t) {
-            if (t < m - 1) {
-                const int ix = t * 64 + p;
-                yl = (row[t] * K.S[ix] - K.a[ix] * yl) * K.inv[ix];
-                y[t] = yl;
-                acc = acc + K.g[ix] * yl;
-            }
+    const bool own = p < nch;
+    const int m = own ? K.cs[p + 1] - K.cs[p] : 1;
+    const int off = MAXR - (m - 1);
+    const int ie = (m - 1) * 64 + p;
+
+    // factors: LDS copies (FL) or the global arrays themselves
+    const double *fS, *fa, *fi, *fg, *fv, *fc, *fk1, *fk2, *fwl, *fib;
+    double ce;
+    double* stage;
+    if constexpr (FL) {
+        double* lS = lds;
+        double* la = lS + NR * 64;
+        double* li = la + NR * 64;
+        double* lg = li + NR * 64;
+        double* lv = lg + NR * 64;
+        double* lc = lv + NR * 64;
+        double* lk1 = lc + NR * 64;
+        double* lk2 = lk1 + 384;
+        double* lwl = lk2 + 384;
+        double* lib = lwl + 64;
+        stage = lib + 64 + wave * 64 * stride;
+        const int lenf = K.maxm * 64;
+        for (int i = tid; i < NR * 64; i += 64 * JW) {
+            const bool ok = i < lenf;
+            lS[i] = ok ? K.S[i] : 0.0;
+            la[i] = ok ? K.a[i] : 0.0;
+            li[i] = ok ? K.inv[i] : 0.0;
+            lg[i] = ok ? K.g[i] : 0.0;
+            lv[i] = ok ? K.vp[i] : 0.0;
+            lc[i] = ok ? K.cp[i] : 0.0;
         }
-        const int ix = (m - 1) * 64 + p;
-        de = row[m - 1] * K.S[ix];
-        ae = K.a[ix];
-        ce = K.c[ix];
+        for (int i = tid; i < 384; i += 64 * JW) {
+            lk1[i] = K.k1[i];
+            lk2[i] = K.k2[i];
+        }
+        if (tid < 64) {
+            lwl[tid] = K.wl[tid];
+            lib[tid] = K.ib[tid];
+        }
+        fS = lS; fa = la; fi = li; fg = lg; fv = lv; fc = lc; fk1 = lk1; fk2 = lk2; fwl = lwl; fib = lib;
+    } else {
+        stage = lds + wave * 64 * stride;
+        fS = K.S; fa = K.a; fi = K.inv; fg = K.g; fv = K.vp; fc = K.cp; fk1 = K.k1; fk2 = K.k2; fwl = K.wl; fib = K.ib;
     }
-    const double ysn = __shfl_down(acc, 1, 64);
-    double delta = (p < nch) ? de - ae * yl - ((p + 1 < nch) ? ce * ysn : 0.0) : 0.0;
-    int li = 0;
-    for (int h = 1; h < 64; h <<= 1, ++li) {
-        double dm = __shfl_up(delta, h, 64), dp = __shfl_down(delta, h, 64);
-        if (p - h < 0) dm = 0.0;
-        if (p + h >= 64) dp = 0.0;
-        delta = delta - K.k1[li * 64 + p] * dm - K.k2[li * 64 + p] * dp;
-    }
-    const double X = delta * K.ib[p];
-    double Xm = __shfl_up(X, 1, 64);
-    if (p == 0) Xm = 0.0;
-    if (p < nch) {
+    ce = own ? K.c[ie] : 0.0;
+    int slot[NR];
+    line_map<NR>(slot, K.cs, n, nch, stride, lane);
+    double pre[NR];
+    const int step = nwgf * JW;
+    const int iters = (nlines + step - 1) / step;
+    int line = wg * JW + wave;
+    line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + line), n, lane, line < nlines);
+    __syncthreads();
+    double* row = stage + p * stride;
+    for (int it = 0; it < iters; ++it, line += step) {
+        const bool act = line < nlines;
+        line_put<NR>(stage, pre, slot);
+        const int nxt = line + step;
+        line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (nxt < nlines ? nxt : 0)), n, lane, nxt < nlines);
+        __syncthreads();
+        double y[MAXR];
+        double yl = 0.0, acc = 0.0;
+#pragma unroll
+        for (int tt = 0; tt < MAXR; ++tt) {
+            const int t = tt - off;
+            const bool real = t >= 0;
+            const int tc = real ? t : 0;
+            const int ix = tc * 64 + p;
+            const double sv = real ? fS[ix] : 0.0;
+            const double av = real ? fa[ix] : 0.0;
+            const double gv = real ? fg[ix] : 0.0;
+            const double dv = real ? row[tc] : 0.0;
+            yl = (dv * sv - av * yl) * fi[ix];
+            y[tt] = yl;
+            acc = acc + gv * yl;
+        }
+        const double de = row[m - 1] * fS[ie];
+        const double ae = fa[ie];
+        const double ysn = __shfl_down(acc, 1, 64);
+        double delta = own ? de - ae * yl - ((p + 1 < nch) ? ce * ysn : 0.0) : 0.0;
+        int li = 0;
+        for (int h = 1; h < 64; h <<= 1, ++li) {
+            double dm = __shfl_up(delta, h, 64), dp = __shfl_down(delta, h, 64);
+            if (p - h < 0) dm = 0.0;
+            if (p + h >= 64) dp = 0.0;
+            delta = delta - fk1[li * 64 + p] * dm - fk2[li * 64 + p] * dp;
+        }
+        const double X = delta * fib[p];
+        double Xm = __shfl_up(X, 1, 64);
+        if (p == 0) Xm = 0.0;
+        const double wl = fwl[p];
         double x = 0.0;
 #pragma unroll
-        for (int t = MAXR - 1; t >= 0; --t) {
-            if (t < m - 1) {
-                const int ix = t * 64 + p;
-                if (t == m - 2) x = y[t] + K.vp[ix] * Xm + K.wl[p] * X;
-                else x = (y[t] + K.vp[ix] * Xm) - K.cp[ix] * x;
-                row[t] = x;
-            }
+        for (int tt = MAXR - 1; tt >= 0; --tt) {
+            const int t = tt - off;
+            const int ix = (t >= 0 ? t : 0) * 64 + p;
+            if (tt == MAXR - 1) x = y[tt] + fv[ix] * Xm + wl * X;
+            else x = (y[tt] + fv[ix] * Xm) - fc[ix] * x;
+            if (t >= 0 && own) row[t] = x;
         }
-        row[m - 1] = X;
+        if (own) row[m - 1] = X;
+        __syncthreads();
+        if (act) line_get<NR>(dstb + fidx(F, K.kofs, kfirst + line), stage, slot, lane);
     }
-    __syncthreads();
-    lds_to_line(dst, lds, K.cs, n, nch, stride, lane);
 }
 
 // Poisson j-lines (phcalc_outflow.F90:34-56): mode m solves
 // (A_y + wr(m)) v = hmn(:,m); each lane factors its chunk in registers
-// (unnormalised rows amphj, acphj+wr, apphj -- the same system the reference
-// normalises by 1/(acphj+wr)), the separators are solved by PCR, and the
-// singular Neumann mode is gauge-fixed by pinning its last unknown to 0.
-template <int MAXR>
-__global__ __launch_bounds__(64) void k_jsolve_p(const FluidDev F)
+// (unnormalised rows amphj, acphj+wr, apphj from LDS -- the same system the
+// reference normalises by 1/(acphj+wr)), the separators are solved by PCR,
+// and the singular Neumann mode is gauge-fixed by pinning its last unknown
+// to 0.
+template <int MAXR, int JW>
+__global__ __launch_bounds__(64 * JW) void k_jsolve_p(const FluidDev F)
 {
+    constexpr int NR = MAXR + 1;
     extern __shared__ double lds[];
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int lane = threadIdx.x, mode = blockIdx.x + 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const PSys& Q = F.ps;
-    double* col = F.hmn + (size_t)(mode - 1) * F.P;
-    const double wrm = F.wr[mode];
-    const bool pin = (mode == F.null_mode);
     const int n = Q.n, nch = Q.nch, stride = F.jstride;
-    line_to_lds(col, lds, Q.cs, n, nch, stride, lane);
-    __syncthreads();
+    double* cam = lds;
+    double* cac = cam + NR * 64;
+    double* cap = cac + NR * 64;
+    double* stage = cap + NR * 64 + wave * 64 * stride;
+    const int lenf = Q.maxm * 64;
+    for (int i = tid; i < NR * 64; i += 64 * JW) {
+        const bool ok = i < lenf;
+        cam[i] = ok ? Q.am[i] : 0.0;
+        cac[i] = ok ? Q.ac[i] : 0.0;
+        cap[i] = ok ? Q.ap[i] : 0.0;
+    }
     const int p = lane;
-    double cp[MAXR], vp[MAXR], y[MAXR];
-    double cpp = 0.0, yl = 0.0, vl = 1.0, gg = 1.0, ys = 0.0, vs = 0.0, ws = 0.0, wl = 0.0;
-    double alpha = 0.0, beta = 1.0, gamma = 0.0, delta = 0.0, ae = 0.0, ce = 0.0, de = 0.0;
-    int m = 0;
-    double* row = lds + p * stride;
-    if (p < nch) {
-        m = Q.cs[p + 1] - Q.cs[p];
+    const bool own = p < nch;
+    const int m = own ? Q.cs[p + 1] - Q.cs[p] : 1;
+    const int off = MAXR - (m - 1);
+    int slot[NR];
+    line_map<NR>(slot, Q.cs, n, nch, stride, lane);
+    double pre[NR];
+    const int nlines = F.n3m;
+    const int step = (int)gridDim.x * JW;
+    const int iters = (nlines + step - 1) / step;
+    int line = (int)blockIdx.x * JW + wave;
+    line_fetch<NR>(pre, F.hmn + (size_t)line * F.P, n, lane, line < nlines);
+    __syncthreads();
+    double* row = stage + p * stride;
+    for (int it = 0; it < iters; ++it, line += step) {
+        const bool act = line < nlines;
+        line_put<NR>(stage, pre, slot);
+        const int nxt = line + step;
+        line_fetch<NR>(pre, F.hmn + (size_t)(nxt < nlines ? nxt : 0) * F.P, n, lane, nxt < nlines);
+        __syncthreads();
+        const int mode = line + 1;
+        const double wrm = act ? F.wr[mode] : 0.0;
+        const bool pin = act && (mode == F.null_mode);
+        double cp[MAXR], vp[MAXR], y[MAXR];
+        double cpp = 0.0, yl = 0.0, vl = 1.0, gg = 1.0, ys = 0.0, vs = 0.0, ws = 0.0, wl = 0.0;
+        double alpha = 0.0, beta = 1.0, gamma = 0.0, delta = 0.0, ce = 0.0;
         double glast = 1.0;
 #pragma unroll
-        for (int t = 0; t < MAXR; ++t) {
-            if (t < m - 1) {
-                const int ix = t * 64 + p;
-                const double a = Q.am[ix], b = Q.ac[ix] + wrm, c = Q.ap[ix];
-                const double iv = 1.0 / (b - a * cpp);
-                cpp = c * iv;
-                yl = (row[t] - a * yl) * iv;
-                vl = (-a * vl) * iv;
-                cp[t] = cpp;
-                vp[t] = vl;
-                y[t] = yl;
-                ys = ys + gg * yl;
-                vs = vs + gg * vl;
-                if (t == m - 2) {
-                    wl = (-c) * iv;
-                    glast = gg;
-                }
-                gg = -cpp * gg;
+        for (int tt = 0; tt < MAXR; ++tt) {
+            const int t = tt - off;
+            const bool real = t >= 0;
+            const int ix = (real ? t : 0) * 64 + p;
+            const double a = real ? cam[ix] : 0.0;
+            const double b = real ? cac[ix] + wrm : 1.0;
+            const double c = real ? cap[ix] : 0.0;
+            const double d = real ? row[t] : 0.0;
+            const double iv = 1.0 / (b - a * cpp);
+            cpp = c * iv;
+            yl = (d - a * yl) * iv;
+            const double vn = (-a * vl) * iv;
+            vl = real ? vn : vl;
+            cp[tt] = cpp;
+            vp[tt] = vl;
+            y[tt] = yl;
+            ys = ys + gg * yl;
+            vs = real ? vs + gg * vl : vs;
+            if (tt == MAXR - 1 && real) {
+                wl = (-c) * iv;
+                glast = gg;
             }
+            gg = real ? -cpp * gg : gg;
         }
-        ws = glast * wl;
-        const int ix = (m - 1) * 64 + p;
-        ae = Q.am[ix];
-        ce = Q.ap[ix];
-        de = row[m - 1];
-        alpha = ae * vl;
-        beta = (Q.ac[ix] + wrm) + ae * wl;
-        delta = de - ae * yl;
-    }
-    const double ysn = __shfl_down(ys, 1, 64), vsn = __shfl_down(vs, 1, 64), wsn = __shfl_down(ws, 1, 64);
-    if (p + 1 < nch) {
-        beta = beta + ce * vsn;
-        gamma = ce * wsn;
-        delta = delta - ce * ysn;
-    }
-    if (pin && p == nch - 1) {
-        alpha = 0.0;
-        beta = 1.0;
-        gamma = 0.0;
-        delta = 0.0;
-    }
-    for (int h = 1; h < 64; h <<= 1) {
-        double am = __shfl_up(alpha, h, 64), bm = __shfl_up(beta, h, 64);
-        double gm = __shfl_up(gamma, h, 64), dm = __shfl_up(delta, h, 64);
-        double ap = __shfl_down(alpha, h, 64), bp = __shfl_down(beta, h, 64);
-        double gp = __shfl_down(gamma, h, 64), dp = __shfl_down(delta, h, 64);
-        if (lane - h < 0) { am = 0.0; bm = 1.0; gm = 0.0; dm = 0.0; }
-        if (lane + h >= 64) { ap = 0.0; bp = 1.0; gp = 0.0; dp = 0.0; }
-        const double m1 = alpha / bm, m2 = gamma / bp;
-        const double na = -m1 * am, ng = -m2 * gp;
-        const double nb = beta - m1 * gm - m2 * ap;
-        const double nd = delta - m1 * dm - m2 * dp;
-        alpha = na;
-        beta = nb;
-        gamma = ng;
-        delta = nd;
-    }
-    const double X = delta / beta;
-    double Xm = __shfl_up(X, 1, 64);
-    if (p == 0) Xm = 0.0;
-    if (p < nch) {
+        if (own) {
+            ws = glast * wl;
+            const int ix = (m - 1) * 64 + p;
+            const double ae = cam[ix];
+            ce = cap[ix];
+            alpha = ae * vl;
+            beta = (cac[ix] + wrm) + ae * wl;
+            delta = row[m - 1] - ae * yl;
+        }
+        const double ysn = __shfl_down(ys, 1, 64), vsn = __shfl_down(vs, 1, 64), wsn = __shfl_down(ws, 1, 64);
+        if (p + 1 < nch) {
+            beta = beta + ce * vsn;
+            gamma = ce * wsn;
+            delta = delta - ce * ysn;
+        }
+        if (pin && p == nch - 1) {
+            alpha = 0.0;
+            beta = 1.0;
+            gamma = 0.0;
+            delta = 0.0;
+        }
+        for (int h = 1; h < 64; h <<= 1) {
+            double am = __shfl_up(alpha, h, 64), bm = __shfl_up(beta, h, 64);
+            double gm = __shfl_up(gamma, h, 64), dm = __shfl_up(delta, h, 64);
+            double ap = __shfl_down(alpha, h, 64), bp = __shfl_down(beta, h, 64);
+            double gp = __shfl_down(gamma, h, 64), dp = __shfl_down(delta, h, 64);
+            if (lane - h < 0) { am = 0.0; bm = 1.0; gm = 0.0; dm = 0.0; }
+            if (lane + h >= 64) { ap = 0.0; bp = 1.0; gp = 0.0; dp = 0.0; }
+            const double m1 = alpha / bm, m2 = gamma / bp;
+            const double na = -m1 * am, ng = -m2 * gp;
+            const double nb = beta - m1 * gm - m2 * ap;
+            const double nd = delta - m1 * dm - m2 * dp;
+            alpha = na;
+            beta = nb;
+            gamma = ng;
+            delta = nd;
+        }
+        const double X = delta / beta;
+        double Xm = __shfl_up(X, 1, 64);
+        if (p == 0) Xm = 0.0;
         double x = 0.0;
 #pragma unroll
-        for (int t = MAXR - 1; t >= 0; --t) {
-            if (t < m - 1) {
-                if (t == m - 2) x = y[t] + vp[t] * Xm + wl * X;
-                else x = (y[t] + vp[t] * Xm) - cp[t] * x;
-                row[t] = x;
-            }
+        for (int tt = MAXR - 1; tt >= 0; --tt) {
+            const int t = tt - off;
+            if (tt == MAXR - 1) x = y[tt] + vp[tt] * Xm + wl * X;
+            else x = (y[tt] + vp[tt] * Xm) - cp[tt] * x;
+            if (t >= 0 && own) row[t] = x;
         }
-        row[m - 1] = X;
+        if (own) row[m - 1] = X;
+        __syncthreads();
+        if (act) line_get<NR>(F.hmn + (size_t)line * F.P, stage, slot, lane);
     }
-    __syncthreads();
-    lds_to_line(col, lds, Q.cs, n, nch, stride, lane);
+}
+
+template <int MAXR>
+static void launch_h(const FluidDev& F, hipStream_t st)
+{
+    const int lines = 2 * F.n3m - 1;
+    const int sd = F.jstride;
+    if (jsolve_h_lds_bytes(MAXR, 4, sd, true) <= (size_t)F.lds_max) {
+        const int g = std::max(2, std::min(256, (lines + 3) / 4) & ~1);
+        hipLaunchKernelGGL((k_jsolve_h<MAXR, 4, true>), dim3(g), dim3(256), jsolve_h_lds_bytes(MAXR, 4, sd, true), st,
+                           F);
+    } else if (jsolve_h_lds_bytes(MAXR, 2, sd, true) <= (size_t)F.lds_max) {
+        const int g = std::max(2, std::min(512, (lines + 1) / 2) & ~1);
+        hipLaunchKernelGGL((k_jsolve_h<MAXR, 2, true>), dim3(g), dim3(128), jsolve_h_lds_bytes(MAXR, 2, sd, true), st,
+                           F);
+    } else {
+        const int g = std::max(2, std::min(2048, lines) & ~1);
+        hipLaunchKernelGGL((k_jsolve_h<MAXR, 1, false>), dim3(g), dim3(64), jsolve_h_lds_bytes(MAXR, 1, sd, false), st,
+                           F);
+    }
+}
+
+template <int MAXR>
+static void launch_p(const FluidDev& F, hipStream_t st)
+{
+    // the per-mode factors live in registers: 8 waves only for short chunks
+    const int lines = F.n3m;
+    if (MAXR <= 8 && jsolve_p_lds_bytes(MAXR, 8, F.jstride) <= (size_t)F.lds_max) {
+        const int g = std::min(256, (lines + 7) / 8);
+        hipLaunchKernelGGL((k_jsolve_p<MAXR, 8>), dim3(g), dim3(512), jsolve_p_lds_bytes(MAXR, 8, F.jstride), st, F);
+    } else if (jsolve_p_lds_bytes(MAXR, 4, F.jstride) <= (size_t)F.lds_max) {
+        const int g = std::min(256, (lines + 3) / 4);
+        hipLaunchKernelGGL((k_jsolve_p<MAXR, 4>), dim3(g), dim3(256), jsolve_p_lds_bytes(MAXR, 4, F.jstride), st, F);
+    } else {
+        const int g = std::min(1024, lines);
+        hipLaunchKernelGGL((k_jsolve_p<MAXR, 1>), dim3(g), dim3(64), jsolve_p_lds_bytes(MAXR, 1, F.jstride), st, F);
+    }
+}
+
+static int jsolve_maxr(int maxm)
+{
+    const int mr = maxm - 1;
+    return mr <= 8 ? 8 : mr <= 16 ? 16 : mr <= 24 ? 24 : mr <= 32 ? 32 : 64;
+}
+
+// Smallest LDS the j-line kernels need for this grid (their fallback shapes).
+size_t jsolve_lds_bytes(const FluidDev& F)
+{
+    const int mh = jsolve_maxr(std::max(F.js[0].maxm, F.js[1].maxm));
+    const int mp = jsolve_maxr(F.ps.maxm);
+    return std::max(jsolve_h_lds_bytes(mh, 1, F.jstride, false), jsolve_p_lds_bytes(mp, 1, F.jstride));
 }
 
 void launch_jsolve_helm(const FluidDev& F, hipStream_t st)
 {
-    const int nlines = F.n3m + (F.n3m - 1);
-    const int mr = (F.js[0].maxm > F.js[1].maxm ? F.js[0].maxm : F.js[1].maxm) - 1;
-    const size_t lds = jsolve_lds_bytes(F);
-    if (mr <= 8) hipLaunchKernelGGL(k_jsolve_h<8>, dim3(nlines), dim3(64), lds, st, F);
-    else if (mr <= 16) hipLaunchKernelGGL(k_jsolve_h<16>, dim3(nlines), dim3(64), lds, st, F);
-    else if (mr <= 24) hipLaunchKernelGGL(k_jsolve_h<24>, dim3(nlines), dim3(64), lds, st, F);
-    else if (mr <= 32) hipLaunchKernelGGL(k_jsolve_h<32>, dim3(nlines), dim3(64), lds, st, F);
-    else hipLaunchKernelGGL(k_jsolve_h<64>, dim3(nlines), dim3(64), lds, st, F);
+    switch (jsolve_maxr(std::max(F.js[0].maxm, F.js[1].maxm))) {
+    case 8: launch_h<8>(F, st); break;
+    case 16: launch_h<16>(F, st); break;
+    case 24: launch_h<24>(F, st); break;
+    case 32: launch_h<32>(F, st); break;
+    default: launch_h<64>(F, st); break;
+    }
 }
 
 void launch_jsolve_poisson(const FluidDev& F, hipStream_t st)
 {
-    const int mr = F.ps.maxm - 1;
-    const size_t lds = jsolve_lds_bytes(F);
-    if (mr <= 8) hipLaunchKernelGGL(k_jsolve_p<8>, dim3(F.n3m), dim3(64), lds, st, F);
-    else if (mr <= 16) hipLaunchKernelGGL(k_jsolve_p<16>, dim3(F.n3m), dim3(64), lds, st, F);
-    else if (mr <= 24) hipLaunchKernelGGL(k_jsolve_p<24>, dim3(F.n3m), dim3(64), lds, st, F);
-    else if (mr <= 32) hipLaunchKernelGGL(k_jsolve_p<32>, dim3(F.n3m), dim3(64), lds, st, F);
-    else hipLaunchKernelGGL(k_jsolve_p<64>, dim3(F.n3m), dim3(64), lds, st, F);
+    switch (jsolve_maxr(F.ps.maxm)) {
+    case 8: launch_p<8>(F, st); break;
+    case 16: launch_p<16>(F, st); break;
+    case 24: launch_p<24>(F, st); break;
+    case 32: launch_p<32>(F, st); break;
+    default: launch_p<64>(F, st); break;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -259,6 +444,11 @@ __device__ inline double krhs(const FluidDev& F, const KSys& K, int f, int j, in
     return d;
 }
 
+// Pass 1: thread (j, chunk p) eliminates the interior rows of its chunk and
+// leaves the chunk's separator data (sum g*y', last y', separator rhs).  The
+// interior rows are aligned to the end of the MAXM-step loop so the loads of
+// all steps are unconditional and can be issued together.
+template <int MAXM>
 __global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
 {
     const DevScalars* S = F.S;
@@ -267,44 +457,63 @@ __global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
     const KSys& K = F.ks[f];
     const int j = blockIdx.x * 64 + threadIdx.x + 1;
     if (p >= K.nch || j > F.n2m) return;
-    const double* src = f == 0 ? F.sA : F.sB;
-    const int s = K.cs[p], e = K.cs[p + 1] - 1;
+    const double* __restrict__ src = f == 0 ? F.sA : F.sB;
+    const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;
+    const int off = MAXM - m;
     double yp = 0.0, acc = 0.0;
-    for (int i = s; i <= e - 1; ++i) {
-        double d = krhs(F, K, f, j, i, src, S);
-        yp = (d - K.a[i] * yp) * K.inv[i];
-        acc = acc + K.g[i] * yp;
+#pragma unroll
+    for (int tt = 0; tt < MAXM; ++tt) {
+        const int t = tt - off;
+        const bool real = t >= 0;
+        const int i = s + (real ? t : 0);
+        const double d = krhs(F, K, f, j, i, src, S);
+        const double yn = (d - K.a[i] * yp) * K.inv[i];
+        const double an = acc + K.g[i] * yn;
+        yp = real ? yn : yp;
+        acc = real ? an : acc;
     }
-    double de = krhs(F, K, f, j, e, src, S);
+    const double de = krhs(F, K, f, j, e, src, S);
     const size_t o = ((size_t)f * FRIBO_KMAXCH + p) * F.n2m + (j - 1);
     F.kYs[o] = acc;
     F.kYe[o] = yp;
     F.kDe[o] = de;
 }
 
+// Pass 2: thread j solves its column's separator system (nch unknowns,
+// line-invariant Thomas factors from k_bc); loads run along j (coalesced).
 __global__ __launch_bounds__(64) void k_ksolve2(const FluidDev F)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int f = blockIdx.y, j = blockIdx.x + 1, p = threadIdx.x;
+    const int f = blockIdx.y;
+    const int j = blockIdx.x * 64 + threadIdx.x + 1;
+    if (j > F.n2m) return;
     const KSys& K = F.ks[f];
+    const int nch = K.nch;
     const size_t base = (size_t)f * FRIBO_KMAXCH * F.n2m + (j - 1);
-    double delta = 0.0;
-    if (p < K.nch) {
-        int e = K.cs[p + 1] - 1;
-        double ysn = (p + 1 < K.nch) ? F.kYs[base + (size_t)(p + 1) * F.n2m] : 0.0;
-        double ye = F.kYe[base + (size_t)p * F.n2m];
-        double de = F.kDe[base + (size_t)p * F.n2m];
-        delta = de - K.a[e] * ye - K.c[e] * ysn;
+    const size_t n2m = F.n2m;
+    double z[FRIBO_KMAXCH];
+    double zp = 0.0;
+#pragma unroll
+    for (int p = 0; p < FRIBO_KMAXCH; ++p) {
+        if (p < nch) {
+            const int e = K.cs[p + 1] - 1;
+            const double ysn = (p + 1 < nch) ? F.kYs[base + (p + 1) * n2m] : 0.0;
+            const double ye = F.kYe[base + p * n2m];
+            const double de = F.kDe[base + p * n2m];
+            const double delta = de - K.a[e] * ye - K.c[e] * ysn;
+            zp = (delta - K.ra[p] * zp) * K.rinv[p];
+            z[p] = zp;
+        }
     }
-    int li = 0;
-    for (int h = 1; h < 64; h <<= 1, ++li) {
-        double dm = __shfl_up(delta, h, 64), dp = __shfl_down(delta, h, 64);
-        if (p - h < 0) dm = 0.0;
-        if (p + h >= 64) dp = 0.0;
-        delta = delta - K.k1[li * 64 + p] * dm - K.k2[li * 64 + p] * dp;
+    double X = 0.0;
+#pragma unroll
+    for (int p = FRIBO_KMAXCH - 1; p >= 0; --p) {
+        if (p < nch) {
+            X = (p == nch - 1) ? z[p] : z[p] - K.rcp[p] * X;
+            F.kX[base + p * n2m] = X;
+        }
     }
-    if (p < K.nch) F.kX[base + (size_t)p * F.n2m] = delta * K.ib[p];
 }
 
 template <int MAXM>
@@ -367,8 +576,13 @@ void launch_ksolve(const FluidDev& F, hipStream_t st)
     int nchmax = F.ks[0].nch > F.ks[1].nch ? F.ks[0].nch : F.ks[1].nch;
     int maxm = F.ks[0].maxm > F.ks[1].maxm ? F.ks[0].maxm : F.ks[1].maxm;
     dim3 g((F.n2m + 63) / 64, nchmax, 2);
-    hipLaunchKernelGGL(k_ksolve1, g, dim3(64), 0, st, F);
-    hipLaunchKernelGGL(k_ksolve2, dim3(F.n2m, 2), dim3(64), 0, st, F);
+    // maxm counts the separator row; the passes loop over the m-1 interior rows
+    if (maxm - 1 <= 8) hipLaunchKernelGGL(k_ksolve1<8>, g, dim3(64), 0, st, F);
+    else if (maxm - 1 <= 16) hipLaunchKernelGGL(k_ksolve1<16>, g, dim3(64), 0, st, F);
+    else if (maxm - 1 <= 32) hipLaunchKernelGGL(k_ksolve1<32>, g, dim3(64), 0, st, F);
+    else if (maxm - 1 <= 64) hipLaunchKernelGGL(k_ksolve1<64>, g, dim3(64), 0, st, F);
+    else hipLaunchKernelGGL(k_ksolve1<128>, g, dim3(64), 0, st, F);
+    hipLaunchKernelGGL(k_ksolve2, dim3((F.n2m + 63) / 64, 2), dim3(64), 0, st, F);
     if (maxm <= 8) hipLaunchKernelGGL(k_ksolve3<8>, g, dim3(64), 0, st, F);
     else if (maxm <= 16) hipLaunchKernelGGL(k_ksolve3<16>, g, dim3(64), 0, st, F);
     else if (maxm <= 32) hipLaunchKernelGGL(k_ksolve3<32>, g, dim3(64), 0, st, F);
