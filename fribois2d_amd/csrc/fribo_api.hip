@@ -47,6 +47,7 @@ struct fribo_ctx {
     DevScalars hs;
     // profiling
     bool profiling = false;
+    unsigned prof_mask = ~0u;   // kernel classes bracketed by events
     std::vector<hipEvent_t> evpool;
     size_t evused = 0;
     struct Rec { int cls; hipEvent_t a, b; };
@@ -59,6 +60,7 @@ struct fribo_ctx {
     StepIn* in = nullptr;
     // captured step (hipGraph): rebuilt when the marker buffers change
     bool use_graph = false;
+    bool yavail = false;     // transform along y set up
     hipGraphExec_t gexec = nullptr;
     int graph_nlm = -1;
 };
@@ -234,9 +236,6 @@ static int setup_ksys(fribo_ctx* c, KSys& K, int n, int kofs, int src, int inter
     if ((rc = dalloc(c, &K.k2, 6 * 64))) return rc;
     if ((rc = dalloc(c, &K.ib, 64))) return rc;
     if ((rc = dalloc(c, &K.bc, 2))) return rc;
-    if ((rc = dalloc(c, &K.ra, 64))) return rc;
-    if ((rc = dalloc(c, &K.rinv, 64))) return rc;
-    if ((rc = dalloc(c, &K.rcp, 64))) return rc;
     return FRIBO_OK;
 }
 
@@ -279,6 +278,51 @@ static int upload1i(fribo_ctx* c, int** dst, const std::vector<int>& h)
     int rc = dalloc(c, dst, h.size());
     if (rc) return rc;
     HIPCHK(hipMemcpy(*dst, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+    return FRIBO_OK;
+}
+
+// Eigen-system of the y pressure operator (fribo_eig.hip) and the gauge
+// weights zg = zmmt(null_mode, :); leaves taxis = 1 when it succeeds and the
+// y operator has a single null mode, otherwise keeps the transform along z.
+static int setup_ytransform(fribo_ctx* c, const double* zmmt, int ldz)
+{
+    FluidDev& F = c->F;
+    const int n = F.n2m;
+    F.ldy = (n + 15) / 16 * 16;
+    std::vector<double> Y((size_t)F.ldy * n, 0.0), Yi((size_t)F.ldy * n, 0.0), wy(n);
+    if (ytransform_build(n, c->h1[A_AMPHJ].data() + 1, c->h1[A_ACPHJ].data() + 1, c->h1[A_APPHJ].data() + 1, F.ldy,
+                         Y.data(), Yi.data(), wy.data()) != 0)
+        return FRIBO_OK;
+    double wmax = 0.0, wmin = 1e300;
+    int imin = 0;
+    for (int m = 0; m < n; ++m) {
+        const double a = fabs(wy[m]);
+        wmax = std::max(wmax, a);
+        if (a < wmin) { wmin = a; imin = m; }
+    }
+    if (!(wmin <= 1e-9 * wmax)) return FRIBO_OK;
+    std::vector<double> zg(F.n3m);
+    double zs = 0.0;
+    for (int k = 0; k < F.n3m; ++k) {
+        zg[k] = zmmt[(size_t)(F.null_mode - 1) + (size_t)k * ldz];
+        zs = zs + zg[k];
+    }
+    if (!(fabs(zs) > 0.0)) return FRIBO_OK;
+    int e = 0;
+    if (!e) e = dalloc(c, &F.ymm, (size_t)F.ldy * n);
+    if (!e) e = dalloc(c, &F.ymmi, (size_t)F.ldy * n);
+    if (!e) e = dalloc(c, &F.wy, n);
+    if (!e) e = dalloc(c, &F.zg, F.n3m);
+    if (!e) e = dalloc(c, &F.ysum, (size_t)7 * n * FRIBO_KMAXCH);
+    if (e) return e;
+    HIPCHK(hipMemcpy(F.ymm, Y.data(), Y.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(F.ymmi, Yi.data(), Yi.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(F.wy, wy.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(F.zg, zg.data(), F.n3m * sizeof(double), hipMemcpyHostToDevice));
+    F.ynull = imin + 1;
+    F.zginv = 1.0 / zs;
+    F.taxis = 1;
+    c->yavail = true;
     return FRIBO_OK;
 }
 
@@ -383,6 +427,9 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     if (!e) e = dalloc(c, &F.kDe, (size_t)2 * FRIBO_KMAXCH * F.n2m);
     if (!e) e = dalloc(c, &F.kX, (size_t)2 * FRIBO_KMAXCH * F.n2m);
     if (!e) e = dalloc(c, &F.S, 1);
+    // the Poisson transform along y costs 2*n3m*n2m^2 flops against
+    // 2*n2m*n3m^2 along z: set it up when y is the shorter axis
+    if (!e && F.n2m < F.n3m && F.null_mode > 0) e = setup_ytransform(c, zmmt, ldz);
     if (!e) {
         hipError_t he = hipStreamCreate(&c->st);
         if (he != hipSuccess) e = set_err(FRIBO_ERR_HIP, hipGetErrorString(he));
@@ -570,7 +617,7 @@ static hipEvent_t next_event(fribo_ctx* c)
 template <class Fn>
 static void run(fribo_ctx* c, int cls, Fn fn)
 {
-    if (!c->profiling) {
+    if (!c->profiling || !((c->prof_mask >> cls) & 1u)) {
         fn();
         return;
     }
@@ -617,6 +664,34 @@ static void enqueue_cfl(fribo_ctx* c)
     run(c, PC_CFL, [&] { launch_cfl(F, c->st); });
 }
 
+// phcalc_outflow (phcalc_outflow.F90:4-71).  Along z, the reference's order:
+// hmn = dph * zmmt^T, one j-line per z-mode, dph = V * zmm^T.  Along y:
+// hmn = Yinv * dph, one k-line per y-mode, dph = Y * V, and the constant that
+// restores the z-order gauge.
+static void enqueue_poisson(fribo_ctx* c)
+{
+    FluidDev& F = c->F;
+    hipStream_t st = c->st;
+    if (F.taxis == 1) {
+        run(c, PC_GEMM, [&] {
+            launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ymmi, F.ldy, F.dph + fidx(F, 1, 1), F.P, F.hmn, F.P, F.S, st);
+        });
+        run(c, PC_JP, [&] { launch_ysolve(F, st); });
+        run(c, PC_GEMM, [&] {
+            launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ymm, F.ldy, F.hmn, F.P, F.dph + fidx(F, 1, 1), F.P, F.S, st);
+        });
+        run(c, PC_JP, [&] { launch_ygauge(F, st); });
+        return;
+    }
+    run(c, PC_GEMM, [&] {
+        launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.dph + fidx(F, 1, 1), F.P, F.zmmt, F.ldz, F.hmn, F.P, F.S, st);
+    });
+    run(c, PC_JP, [&] { launch_jsolve_poisson(F, st); });
+    run(c, PC_GEMM, [&] {
+        launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.hmn, F.P, F.zmm, F.ldz, F.dph + fidx(F, 1, 1), F.P, F.S, st);
+    });
+}
+
 static void enqueue_substep(fribo_ctx* c, int n)
 {
     FluidDev& F = c->F;
@@ -630,14 +705,7 @@ static void enqueue_substep(fribo_ctx* c, int n)
     run(c, PC_JH, [&] { launch_jsolve_helm(F, st); });
     run(c, PC_K, [&] { launch_ksolve(F, st); });
     run(c, PC_DIV, [&] { launch_div(F, st); });
-    // phcalc_outflow: hmn = dph * zmmt^T ; per-mode j solves ; dph = V * zmm^T
-    run(c, PC_GEMM, [&] {
-        launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.dph + fidx(F, 1, 1), F.P, F.zmmt, F.ldz, F.hmn, F.P, F.S, st);
-    });
-    run(c, PC_JP, [&] { launch_jsolve_poisson(F, st); });
-    run(c, PC_GEMM, [&] {
-        launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.hmn, F.P, F.zmm, F.ldz, F.dph + fidx(F, 1, 1), F.P, F.S, st);
-    });
+    enqueue_poisson(c);
     run(c, PC_UPD, [&] { launch_upd(F, st); });
     run(c, PC_MEAN, [&] { launch_mean(F, st); });
 }
@@ -753,13 +821,7 @@ extern "C" int fribo_substep(fribo_handle h, int32_t n, int32_t ns)
         run(h, PC_JH, [&] { launch_jsolve_helm(F, st); });
         run(h, PC_K, [&] { launch_ksolve(F, st); });
         run(h, PC_DIV, [&] { launch_div(F, st); });
-        run(h, PC_GEMM, [&] {
-            launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.dph + fidx(F, 1, 1), F.P, F.zmmt, F.ldz, F.hmn, F.P, F.S, st);
-        });
-        run(h, PC_JP, [&] { launch_jsolve_poisson(F, st); });
-        run(h, PC_GEMM, [&] {
-            launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.hmn, F.P, F.zmm, F.ldz, F.dph + fidx(F, 1, 1), F.P, F.S, st);
-        });
+        enqueue_poisson(h);
         run(h, PC_UPD, [&] { launch_upd(F, st); });
         run(h, PC_MEAN, [&] { launch_mean(F, st); });
     } else {
@@ -813,6 +875,16 @@ extern "C" int fribo_get_array(fribo_handle h, const char* name, void* out, long
     if (s == "q2" || s == "q3" || s == "dph" || s == "sA" || s == "sB") {
         double* src = s == "q2" ? F.q2 : (s == "q3" ? F.q3 : (s == "dph" ? F.dph : (s == "sA" ? F.sA : F.sB)));
         e = get_field(h, (double*)out, src, -2, F.nk);
+        if (!e && s == "dph" && F.taxis == 1) {
+            if ((e = read_scalars(h))) return e;
+            const double off = h->hs.dphoff;
+            double* o = (double*)out;
+            for (int k = 1; k <= F.n3m; ++k)
+                for (int j = 1; j <= F.n2m; ++j) {
+                    double& v = o[(size_t)(j - 1) + (size_t)(k + 2) * F.n2];
+                    v = v - off;
+                }
+        }
     } else if (s == "hmn") {
         HIPCHK(hipMemcpy2DAsync(out, F.n2m * sizeof(double), F.hmn, F.P * sizeof(double), F.n2m * sizeof(double),
                                 F.n3m, hipMemcpyDeviceToHost, h->st));
@@ -934,10 +1006,36 @@ extern "C" int fribo_test_copy(int64_t bytes, int32_t reps, double* ms_per_copy)
     return FRIBO_OK;
 }
 
+extern "C" int fribo_set_transform(fribo_handle h, int32_t axis)
+{
+    if (!h || axis < -1 || axis > 1) return set_err(FRIBO_ERR_ARG, "bad transform axis");
+    if (axis == 1 && !h->yavail)
+        return set_err(FRIBO_ERR_LIMIT, "no transform along y for this grid (needs n2-1 < n3-1 and Neumann null modes)");
+    const int want = axis == -1 ? (h->yavail ? 1 : 0) : axis;
+    if (want != h->F.taxis) {
+        HIPCHK(hipStreamSynchronize(h->st));
+        h->F.taxis = want;
+        const double zero = 0.0;
+        HIPCHK(hipMemcpy(&h->F.S->dphoff, &zero, sizeof(double), hipMemcpyHostToDevice));
+        if (h->gexec) {
+            hipGraphExecDestroy(h->gexec);
+            h->gexec = nullptr;
+            h->graph_nlm = -1;
+        }
+    }
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_get_transform(fribo_handle h)
+{
+    return h ? h->F.taxis : -1;
+}
+
 extern "C" int fribo_set_profiling(fribo_handle h, int32_t on)
 {
     if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
     h->profiling = on != 0;
+    h->prof_mask = (on == 2) ? (1u << PC_GEMM) : ~0u;
     return FRIBO_OK;
 }
 

@@ -1,0 +1,148 @@
+// Host-side eigen-decomposition of the y pressure operator (set-up only).
+//
+// The Poisson solve of phcalc_outflow.F90:4-71 diagonalises the 1-D Neumann
+// operator along z (zmm, zmmt, wr from phini.F90:35-99) and solves one
+// tridiagonal system along y per z-mode.  The same discrete problem can be
+// diagonalised along y instead (one tridiagonal system along z per y-mode);
+// the dense transform then costs 2*n3m*n2m^2 instead of 2*n2m*n3m^2 flops,
+// which is cheaper whenever n2m < n3m.  This file builds the y eigen-system.
+//
+// A_y (rows amphj, acphj, apphj; Neumann at both ends) is similar to a
+// symmetric tridiagonal matrix: with D = diag(d), d_1 = 1,
+// (d_{j+1}/d_j)^2 = A(j+1,j)/A(j,j+1), S = D^-1 A_y D has off-diagonal
+// sqrt(A(j,j+1) A(j+1,j)).  S = Q diag(w) Q^T by the implicit QL method with
+// Wilkinson shifts (Bowdler, Martin, Reinsch and Wilkinson, "The QR and QL
+// algorithms for symmetric matrices", Numer. Math. 11 (1968); EISPACK tql2),
+// so A_y = (D Q) diag(w) (Q^T D^-1): Y = D Q, Yinv = Q^T D^-1.
+#include "fribo_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <thread>
+#include <vector>
+
+// Symmetric tridiagonal eigen-problem by implicit QL.  d[0..n-1] diagonal
+// (overwritten by the eigenvalues), e[0..n-2] off-diagonal (destroyed),
+// z (n x n, column-major, initialised to I) receives the orthonormal
+// eigenvectors as columns.  Returns 0, or the index of an eigenvalue that
+// did not converge in 60 iterations.  The iteration on (d, e) does not
+// depend on z, so the plane rotations are recorded first and then replayed
+// on z by several threads, each owning a block of rows (rotations act on
+// column pairs, so rows are independent).
+struct Rot {
+    int i;
+    double s, c;
+};
+
+int tridiag_ql(int n, double* d, double* e, double* z)
+{
+    std::vector<double> ee(n, 0.0);
+    for (int i = 0; i + 1 < n; ++i) ee[i] = e[i];
+    std::vector<Rot> rots;
+    rots.reserve((size_t)n * n);
+    for (int l = 0; l < n; ++l) {
+        int iter = 0;
+        for (;;) {
+            int m = l;
+            for (; m < n - 1; ++m) {
+                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(ee[m]) + dd == dd) break;   // negligible in FP64
+            }
+            if (m == l) break;
+            if (++iter > 60) return l + 1;
+            // Wilkinson shift from the leading 2x2 block
+            double g = (d[l + 1] - d[l]) / (2.0 * ee[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + ee[l] / (g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool deflated = false;
+            for (int i = m - 1; i >= l; --i) {
+                const double f = s * ee[i];
+                const double b = c * ee[i];
+                r = std::hypot(f, g);
+                ee[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    ee[m] = 0.0;
+                    deflated = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                rots.push_back({i, s, c});
+            }
+            if (deflated) continue;
+            d[l] -= p;
+            ee[l] = g;
+            ee[m] = 0.0;
+        }
+    }
+    // replay the rotations on z (columns i, i+1), rows split over threads
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt < 1 ? 1 : (nt > 8 ? 8 : nt);
+    if ((size_t)n < 64 * (size_t)nt) nt = 1;
+    auto work = [&](int k0, int k1) {
+        for (const Rot& q : rots) {
+            double* zi = z + (size_t)q.i * n;
+            double* zj = zi + n;
+            const double s = q.s, c = q.c;
+            for (int k = k0; k < k1; ++k) {
+                const double f = zj[k];
+                zj[k] = s * zi[k] + c * f;
+                zi[k] = c * zi[k] - s * f;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    const int blk = (n + (int)nt - 1) / (int)nt;
+    for (unsigned t = 1; t < nt; ++t) {
+        const int k0 = (int)t * blk, k1 = std::min(n, k0 + blk);
+        if (k0 < k1) th.emplace_back(work, k0, k1);
+    }
+    work(0, std::min(n, blk));
+    for (auto& x : th) x.join();
+    return 0;
+}
+
+// Y (eigenvectors of A_y as columns), Yinv = Y^-1 and the eigenvalues wy of
+// the n x n operator with rows (am, ac, ap) (0-based, am[0] = ap[n-1] = 0).
+// Y and Yinv are column-major with leading dimension ld.
+int ytransform_build(int n, const double* am, const double* ac, const double* ap, int ld, double* Y, double* Yinv,
+                     double* wy)
+{
+    std::vector<double> d(n), e(n > 1 ? n - 1 : 1), dg(n), q((size_t)n * n, 0.0);
+    dg[0] = 1.0;
+    for (int j = 0; j + 1 < n; ++j) {
+        const double up = ap[j], lo = am[j + 1];
+        if (!(up > 0.0) || !(lo > 0.0)) return -1;   // not similarity-symmetrisable
+        e[j] = std::sqrt(up * lo);
+        dg[j + 1] = dg[j] * std::sqrt(lo / up);
+    }
+    for (int j = 0; j < n; ++j) {
+        d[j] = ac[j];
+        q[(size_t)j * n + j] = 1.0;
+    }
+    const int rc = tridiag_ql(n, d.data(), e.data(), q.data());
+    if (rc) return rc;
+    for (int m = 0; m < n; ++m) {
+        wy[m] = d[m];
+        for (int j = 0; j < n; ++j) {
+            const double qv = q[(size_t)m * n + j];   // Q(j, m)
+            Y[(size_t)m * ld + j] = dg[j] * qv;       // Y(j, m) = d_j Q(j, m)
+            Yinv[(size_t)j * ld + m] = qv / dg[j];    // Yinv(m, j) = Q(j, m) / d_j
+        }
+    }
+    return 0;
+}
+
+extern "C" int fribo_test_ytransform(int32_t n, const double* am, const double* ac, const double* ap, double* Y,
+                                     double* Yinv, double* wy)
+{
+    if (n <= 0 || !am || !ac || !ap || !Y || !Yinv || !wy) return 5;
+    return ytransform_build(n, am, ac, ap, n, Y, Yinv, wy) == 0 ? 0 : 7;
+}

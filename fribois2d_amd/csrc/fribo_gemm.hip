@@ -1,6 +1,7 @@
 // FP64 MFMA GEMM for the Poisson eigen-transforms (phcalc_outflow.F90:31,61).
 //
 //   C[i + n*ldc] = sum_kk A[i + kk*lda] * B[n + kk*ldb]     (C = A * B^T)
+//   C[i + n*ldc] = sum_kk A[i + kk*lda] * B[kk + n*ldb]     (C = A * B, "NN")
 //
 // Both transforms have this form on the device: the forward transform
 // hmn = dph * zmmt^T takes A = dph (j contiguous) and B = zmmt; the backward
@@ -17,7 +18,10 @@
 // buffers (one barrier per K-slice: the next slice is fetched into registers
 // while the current one is multiplied, then stored into the other buffer).
 // LDS rows are padded to 16 (mod 32) doubles so the four k-rows a fragment
-// read touches fall in disjoint bank halves.  The large-problem shape
+// read touches fall in disjoint bank halves.  In the NN form (the transforms
+// along y, whose B operand is the field itself, k-contiguous) the B slice is
+// read as 16-double runs of each column and stored with a row pitch of BN+1,
+// which spreads the transposing LDS stores over the banks.  The large-problem shape
 // 96 x 128 gives exactly 256 workgroups (one per CU) for the 1536 x 2048
 // transforms of the 2048x1536-cell grid.
 #include "fribo_internal.h"
@@ -26,7 +30,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 #define GBK 16
 
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, bool NN>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, const double* __restrict__ A, int lda,
                                                           const double* __restrict__ B, int ldb,
                                                           double* __restrict__ C, int ldc, const DevScalars* S)
@@ -35,7 +39,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
     constexpr int TM = BM / WM, TN = BN / WN;     // wave tile
     constexpr int RM = TM / 16, RN = TN / 16;     // MFMA blocks per wave
     constexpr int LA = BM + ((16 - BM % 32) + 32) % 32;
-    constexpr int LB = BN + ((16 - BN % 32) + 32) % 32;
+    constexpr int LB = NN ? BN + 1 : BN + ((16 - BN % 32) + 32) % 32;
     constexpr int NA = (BM * GBK + NT - 1) / NT;  // A elements per thread per slice
     constexpr int NB = (BN * GBK + NT - 1) / NT;
     static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile");
@@ -59,9 +63,9 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             int e = t + u * NT;
-            int kk = e / BN, r = e % BN;
+            int kk = NN ? e % GBK : e / BN, r = NN ? e / GBK : e % BN;
             bool ok = (e < BN * GBK) && (k0 + kk < K) && (n0 + r < N);
-            rb[u] = ok ? B[(size_t)(k0 + kk) * ldb + n0 + r] : 0.0;
+            rb[u] = ok ? (NN ? B[(size_t)(n0 + r) * ldb + k0 + kk] : B[(size_t)(k0 + kk) * ldb + n0 + r]) : 0.0;
         }
     };
     auto store = [&](int buf) {
@@ -73,7 +77,10 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             int e = t + u * NT;
-            if (e < BN * GBK) Bs[buf][e / BN][e % BN] = rb[u];
+            if (e < BN * GBK) {
+                if (NN) Bs[buf][e % GBK][e / GBK] = rb[u];
+                else Bs[buf][e / BN][e % BN] = rb[u];
+            }
         }
     };
 
@@ -123,17 +130,30 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+template <bool NN>
+static void launch_gemm(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+                        const DevScalars* S, hipStream_t st)
+{
+    if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
+        hipLaunchKernelGGL((k_gemm_nt<96, 128, 2, 4, NN>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M, N,
+                           K, A, lda, B, ldb, C, ldc, S);
+    } else if ((long)cdiv(M, 64) * cdiv(N, 64) >= 96) {
+        hipLaunchKernelGGL((k_gemm_nt<64, 64, 2, 2, NN>), dim3(cdiv(M, 64), cdiv(N, 64)), dim3(256), 0, st, M, N, K,
+                           A, lda, B, ldb, C, ldc, S);
+    } else {
+        hipLaunchKernelGGL((k_gemm_nt<32, 32, 2, 2, NN>), dim3(cdiv(M, 32), cdiv(N, 32)), dim3(256), 0, st, M, N, K,
+                           A, lda, B, ldb, C, ldc, S);
+    }
+}
+
 void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                     const DevScalars* S, hipStream_t st)
 {
-    if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
-        hipLaunchKernelGGL((k_gemm_nt<96, 128, 2, 4>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M, N, K,
-                           A, lda, B, ldb, C, ldc, S);
-    } else if ((long)cdiv(M, 64) * cdiv(N, 64) >= 96) {
-        hipLaunchKernelGGL((k_gemm_nt<64, 64, 2, 2>), dim3(cdiv(M, 64), cdiv(N, 64)), dim3(256), 0, st, M, N, K, A,
-                           lda, B, ldb, C, ldc, S);
-    } else {
-        hipLaunchKernelGGL((k_gemm_nt<32, 32, 2, 2>), dim3(cdiv(M, 32), cdiv(N, 32)), dim3(256), 0, st, M, N, K, A,
-                           lda, B, ldb, C, ldc, S);
-    }
+    launch_gemm<false>(M, N, K, A, lda, B, ldb, C, ldc, S, st);
+}
+
+void launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+                    const DevScalars* S, hipStream_t st)
+{
+    launch_gemm<true>(M, N, K, A, lda, B, ldb, C, ldc, S, st);
 }

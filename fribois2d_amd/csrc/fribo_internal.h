@@ -24,6 +24,8 @@ struct DevScalars {
     double pend;             // pressure mean not yet subtracted from the interior
     double goff;             // sum of the means not yet subtracted from the pressure
                              // entries no kernel reads (row n2, ghost planes)
+    double dphoff;           // constant to subtract from the interior of dph (gauge
+                             // of the transform along y; 0 along z)
     double qinf, qout, area;
     double ugkks, ugkkn;
 };
@@ -54,9 +56,6 @@ struct KSys {
     double* k2;              // [6*64] PCR multipliers (upper)
     double* ib;              // [64] 1/beta after PCR
     double* bc;              // [2] couplings of rows 0 / n-1 to the Dirichlet ends
-    double* ra;              // [64] sub-diagonal of the separator system
-    double* rinv;            // [64] 1/pivot of its Thomas factorisation
-    double* rcp;             // [64] its forward multipliers c'
 };
 
 // Fixed partition of the Poisson j-lines (n2m rows; the per-mode matrices
@@ -75,6 +74,7 @@ struct FluidDev {
     int nsst, idtv, ibact, wfun, inslws, inslwn;
     int nlm;
     int null_mode;           // 1-based index of the singular Poisson mode, 0 if none
+    int taxis;               // eigen-transform axis of the Poisson solve: 0 z (zmm), 1 y (ymm)
     double alx3, rext2, ren, cou, q3uw, q3lw, dtmax, cflmax, cfllim, radiusSD;
     double dx2, dx3, dx2q, dx3q;
     double gam[4], rom[4], alm[4];
@@ -84,6 +84,12 @@ struct FluidDev {
     double *hmn;             // n2m x n3m, leading dimension P
     double *zmm, *zmmt;      // n3m x n3m, leading dimension ldz
     int ldz;
+    // transform along y: Y (eigenvectors of A_y), Yinv, eigenvalues wy (0-based),
+    // the y null mode (1-based), gauge weights zg = zmmt(null_mode, :) and 1/sum(zg),
+    // and the per-mode z-line workspace (7 x n2m x 64)
+    double *ymm, *ymmi, *wy, *zg, *ysum;
+    int ldy, ynull;
+    double zginv;
     // outflow arrays (1-based, n2+2)
     double *qb2s, *qb3s, *qb2n, *qb3n, *dqb2s, *dqb3s, *dqb2n, *dqb3n, *dq2x2o, *dq3x2o;
     // 1-D grid (1-based)
@@ -103,7 +109,7 @@ struct FluidDev {
     PSys ps;
     int jstride;             // LDS chunk stride of the j-line kernels (odd)
     int lds_max;             // LDS bytes a workgroup may use on this device
-    double *kYs, *kYe, *kDe, *kX;  // [2][64][n2m] reduced-system data
+    double *kYs, *kYe, *kDe, *kX;  // [2][n2m][64] reduced-system data (chunk fastest)
     // markers
     double *lmcc, *lmvel, *lmA;    // (nlm,2) column-major
     int *lmind;                    // (nlm,2,2)
@@ -148,5 +154,11 @@ void launch_jsolve_poisson(const FluidDev& F, hipStream_t st);
 void launch_ksolve(const FluidDev& F, hipStream_t st);
 void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc, const DevScalars* S, hipStream_t st);
+void launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
+                    double* C, int ldc, const DevScalars* S, hipStream_t st);
+void launch_ysolve(const FluidDev& F, hipStream_t st);
+void launch_ygauge(const FluidDev& F, hipStream_t st);
+int ytransform_build(int n, const double* am, const double* ac, const double* ap, int ld, double* Y, double* Yinv,
+                     double* wy);
 size_t jsolve_lds_bytes(const FluidDev& F);
 size_t sysprep_lds_bytes(const FluidDev& F);

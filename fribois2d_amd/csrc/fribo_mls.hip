@@ -34,13 +34,24 @@ static const unsigned long long KEY_NONE = ~0ull;
 #define ENT_BITS 20
 #define ENT_MASK ((1ull << ENT_BITS) - 1)
 
-// Fortran MINLOC(|x(1:len) - v|) over a 1-based array, one wave.
+// Fortran MINLOC(|x(1:len) - v|) over a 1-based, strictly increasing array,
+// one wave.  |x(i) - v| computed in FP64 falls then rises along i (rounding is
+// monotone), so the first minimum lies in the stretch that brackets v: the
+// wave samples every step-th entry, keeps the stretch [s, s+step] with
+// x(s) <= v (s = 1 if v < x(1)) and takes the first minimum there, with the
+// same comparisons as a scan of the whole array.
 __device__ int wave_minloc(const double* x, int len, double v)
 {
     const int lane = threadIdx.x;
+    const int step = (len + 63) / 64;
+    const int si = 1 + lane * step;
+    const bool le = si <= len && x[si] <= v;
+    const unsigned long long bal = __ballot(le);
+    const int s = bal ? 1 + (63 - __clzll(bal)) * step : 1;
+    const int hi = min(len, s + step);
     double best = 0.0;
     int bi = 0x7fffffff;
-    for (int i = lane + 1; i <= len; i += 64) {
+    for (int i = s + lane; i <= hi; i += 64) {
         double a = fabs(x[i] - v);
         if (bi == 0x7fffffff || a < best) {
             best = a;
@@ -241,11 +252,44 @@ __global__ __launch_bounds__(64) void k_mls_sup(const FluidDev F)
     }
 }
 
-// Bitonic sort of the (cell, entry) keys of each direction, ascending.
-// Blocks of SORT_BLK keys are sorted (or merged) in LDS; the strides of the
-// larger merge stages run as global compare-exchange passes, one thread per
-// pair.  Key order is unique, so the result is deterministic.
+// Bitonic sort of the (cell, entry) keys of each direction, ascending.  Up
+// to SORT_ONE keys a single workgroup per direction sorts them in LDS in one
+// launch; larger key sets sort blocks of SORT_BLK keys in LDS and run the
+// strides of the larger merge stages as global compare-exchange passes.  Key
+// order is unique, so the result is deterministic.
 #define SORT_BLK 2048
+#define SORT_ONE 16384
+
+// index of the first element of compare-exchange pair q at a power-of-two stride
+__device__ inline int pair_lo(int q, int stride) { return ((q & ~(stride - 1)) << 1) | (q & (stride - 1)); }
+
+__global__ __launch_bounds__(1024) void k_mls_sort_one(const FluidDev F)
+{
+    __shared__ unsigned long long sk[SORT_ONE];
+    if (F.S->err) return;
+    const int d = blockIdx.x, t = threadIdx.x;
+    const int nk = F.maxkeys;
+    const int nreal = F.nlm * FRIBO_DIMSD;
+    unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
+    for (int i = t; i < nk; i += 1024) sk[i] = i < nreal ? gk[i] : KEY_NONE;
+    __syncthreads();
+    for (int size = 2; size <= nk; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int q = t; q < nk / 2; q += 1024) {
+                const int i = pair_lo(q, stride);
+                const int pj = i + stride;
+                const bool up = (i & size) == 0;
+                const unsigned long long x = sk[i], y = sk[pj];
+                if ((x > y) == up) {
+                    sk[i] = y;
+                    sk[pj] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = t; i < nk; i += 1024) gk[i] = sk[i];
+}
 
 __global__ __launch_bounds__(1024) void k_mls_sort_local(const FluidDev F, int size0, int first)
 {
@@ -266,7 +310,7 @@ __global__ __launch_bounds__(1024) void k_mls_sort_local(const FluidDev F, int s
     for (int size = smin; size <= smax; size <<= 1) {
         for (int stride = (size < SORT_BLK ? size : SORT_BLK) >> 1; stride > 0; stride >>= 1) {
             for (int q = t; q < SORT_BLK / 2; q += 1024) {
-                const int i = (q / stride) * 2 * stride + (q % stride);
+                const int i = pair_lo(q, stride);
                 const int pj = i + stride;
                 const bool up = (((base + i) & size) == 0);
                 const unsigned long long x = sk[i], y = sk[pj];
@@ -288,7 +332,7 @@ __global__ __launch_bounds__(256) void k_mls_sort_global(const FluidDev F, int s
     const int q = blockIdx.x * 256 + threadIdx.x;
     if (q >= F.maxkeys / 2) return;
     unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
-    const int i = (q / stride) * 2 * stride + (q % stride);
+    const int i = pair_lo(q, stride);
     const int pj = i + stride;
     const bool up = ((i & size) == 0);
     const unsigned long long x = gk[i], y = gk[pj];
@@ -303,6 +347,10 @@ void launch_mls_setup(const FluidDev& F, hipStream_t st)
     if (F.nlm <= 0) return;
     hipLaunchKernelGGL(k_mls_idx, dim3(F.nlm, 4), dim3(64), 0, st, F);
     hipLaunchKernelGGL(k_mls_sup, dim3((2 * F.nlm + 63) / 64), dim3(64), 0, st, F);
+    if (F.maxkeys <= SORT_ONE) {
+        hipLaunchKernelGGL(k_mls_sort_one, dim3(2), dim3(1024), 0, st, F);
+        return;
+    }
     const int nblk = F.maxkeys / SORT_BLK;
     hipLaunchKernelGGL(k_mls_sort_local, dim3(nblk, 2), dim3(1024), 0, st, F, 0, 1);
     for (int size = 2 * SORT_BLK; size <= F.maxkeys; size <<= 1) {
@@ -324,24 +372,27 @@ __global__ __launch_bounds__(64) void k_mls_interp(const FluidDev F)
     if (!F.active[n - 1]) return;
     const double* q = (d == 1) ? F.q2 : F.q3;
     const double* r = (d == 1) ? F.rhs2 : F.rhs3;
-    double u = 0.0;
+    // gather the nine supports first (independent loads), then sum in order
+    double qf[FRIBO_DIMSD], ph[FRIBO_DIMSD];
+#pragma unroll
     for (int m = 1; m <= FRIBO_DIMSD; ++m) {
-        int j = EULINS(n, d, 1, m), k = EULINS(n, d, 2, m);
+        const int j = EULINS(n, d, 1, m), k = EULINS(n, d, 2, m);
         // q2f is written for 2<=j<=n2m (INVTR.F90:145-149), q3f for
         // 2<=k<=n3m (INVTR.F90:273-277); elsewhere it keeps its zero start
-        bool def = (d == 1) ? (j >= 2 && j <= F.n2m && k >= 1 && k <= F.n3m)
-                            : (j >= 1 && j <= F.n2m && k >= 2 && k <= F.n3m);
-        double qf = 0.0;
-        if (def) {
-            size_t c = fidx(F, j, k);
-            qf = q[c] + r[c];
-        }
-        u = u + PHI(n, d, m) * qf;
+        const bool def = (d == 1) ? (j >= 2 && j <= F.n2m && k >= 1 && k <= F.n3m)
+                                  : (j >= 1 && j <= F.n2m && k >= 2 && k <= F.n3m);
+        const size_t c = def ? fidx(F, j, k) : 0;
+        qf[m - 1] = def ? q[c] + r[c] : 0.0;
+        ph[m - 1] = PHI(n, d, m);
     }
+    double u = 0.0;
+#pragma unroll
+    for (int m = 0; m < FRIBO_DIMSD; ++m) u = u + ph[m] * qf[m];
     const double f = (LMVEL(n, d) - u) / S->dt;
     double* con = F.contrib + (size_t)(d - 1) * F.nlm * FRIBO_DIMSD;
     const double cl = CL(n, d);
-    for (int m = 1; m <= FRIBO_DIMSD; ++m) con[(size_t)(n - 1) * FRIBO_DIMSD + (m - 1)] = cl * PHI(n, d, m) * f;
+#pragma unroll
+    for (int m = 0; m < FRIBO_DIMSD; ++m) con[(size_t)(n - 1) * FRIBO_DIMSD + m] = cl * ph[m] * f;
 }
 
 // One thread per sorted entry; the first entry of each cell (segment head)
@@ -360,13 +411,28 @@ __global__ __launch_bounds__(64) void k_mls_spread(const FluidDev F)
     if (i > 0 && (gk[i - 1] >> ENT_BITS) == cell) return;
     const double* con = F.contrib + (size_t)d * F.nlm * FRIBO_DIMSD;
     const int nreal = F.nlm * FRIBO_DIMSD;
+    // the segment is read eight keys at a time (independent loads), summed in order
     double acc = 0.0;
-    for (int e = i; e < nreal; ++e) {
-        const unsigned long long ke = gk[e];
-        if (ke == KEY_NONE || (ke >> ENT_BITS) != cell) break;
-        acc = acc + con[ke & ENT_MASK];
+    bool open = true;
+    for (int e0 = i; open && e0 < nreal; e0 += 8) {
+        unsigned long long kb[8];
+        double cv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) kb[u] = (e0 + u < nreal) ? gk[e0 + u] : KEY_NONE;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool in = kb[u] != KEY_NONE && (kb[u] >> ENT_BITS) == cell;
+            cv[u] = in ? con[kb[u] & ENT_MASK] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool in = open && kb[u] != KEY_NONE && (kb[u] >> ENT_BITS) == cell;
+            if (in) acc = acc + cv[u];
+            open = in;
+        }
     }
-    const int j = (int)(cell % F.n2) + 1, k = (int)(cell / F.n2) + 1;
+    const unsigned cl32 = (unsigned)cell, n2u = (unsigned)F.n2;
+    const int j = (int)(cl32 % n2u) + 1, k = (int)(cl32 / n2u) + 1;
     const double fval = (k == F.n3m) ? 0.0 : acc;
     F.segval[(size_t)d * F.maxkeys + i] = fval;
     if (k == F.n3m) return;

@@ -260,22 +260,6 @@ __device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, dou
         beta = beta + ce * vs_n;
         gamma = ce * ws_n;
     }
-    if (!K.inter) {
-        // Thomas factors of the separator system, for the k-line kernels whose
-        // threads each solve one column's separators (every lane runs the
-        // same recurrence on the gathered coefficients and keeps its own row)
-        double cprev = 0.0;
-        for (int q = 0; q < 64; ++q) {
-            const double aq = __shfl(alpha, q, 64), bq = __shfl(beta, q, 64), gq = __shfl(gamma, q, 64);
-            const double iv = 1.0 / (bq - aq * cprev);
-            cprev = gq * iv;
-            if (p == q) {
-                K.ra[q] = aq;
-                K.rinv[q] = iv;
-                K.rcp[q] = cprev;
-            }
-        }
-    }
     int li = 0;
     for (int h = 1; h < 64; h <<= 1, ++li) {
         double amn = __shfl_up(alpha, h, 64), bm = __shfl_up(beta, h, 64), gm = __shfl_up(gamma, h, 64);
@@ -605,15 +589,16 @@ __global__ __launch_bounds__(64) void k_upd(const FluidDev F)
         double* __restrict__ q3 = F.q3;
         double* __restrict__ pr = F.pr;
         const int n3m = F.n3m;
-        double dm = (inj && k0 > 1) ? dph[c - P] : 0.0;
-        double dc = inj ? dph[c] : 0.0;
+        const double doff = S->dphoff;   // gauge constant of the transform along y (0 along z)
+        double dm = (inj && k0 > 1) ? dph[c - P] - doff : 0.0;
+        double dc = inj ? dph[c] - doff : 0.0;
         // plane k+1 of dph and plane k of q2/q3/pr are loaded one iteration ahead
-        double dn = (inj && k0 < n3m) ? dph[c + P] : 0.0;
+        double dn = (inj && k0 < n3m) ? dph[c + P] - doff : 0.0;
         double q2n = out ? q2[c] : 0.0, q3n = out ? q3[c] : 0.0, prn = out ? pr[c] : 0.0;
         for (int k = k0; k <= k1; ++k, c += P) {
             const double dp = dn, q2v = q2n, q3v = q3n, prv0 = prn;
             if (k < k1) {
-                dn = (inj && k + 1 < n3m) ? dph[c + 2 * P] : 0.0;
+                dn = (inj && k + 1 < n3m) ? dph[c + 2 * P] - doff : 0.0;
                 q2n = out ? q2[c + P] : 0.0;
                 q3n = out ? q3[c + P] : 0.0;
                 prn = out ? pr[c + P] : 0.0;

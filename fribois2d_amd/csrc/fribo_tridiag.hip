@@ -429,25 +429,33 @@ void launch_jsolve_poisson(const FluidDev& F, hipStream_t st)
 // ---------------------------------------------------------------------------
 // k-direction solves.  Field f = 0: q2 (rows kc=1..n3m), f = 1: q3 (interior
 // rows kc=2..n3m of the n3 system; the Dirichlet rows dqb3s/dqb3n enter the
-// first/last rhs).
-__device__ inline double krhs(const FluidDev& F, const KSys& K, int f, int j, int r, const double* src,
-                              const DevScalars* S)
+// first/last rhs).  The chunk summaries of column j live at
+// ((f*n2m + j-1)*64 + p): chunk-fastest, so the separator pass reads a
+// column's 64 chunks with one coalesced access.
+
+__device__ inline size_t kidx(const FluidDev& F, int f, int j, int p)
 {
-    double d = src[fidx(F, j, r + K.kofs)] * K.S[r];
+    return ((size_t)f * F.n2m + (size_t)(j - 1)) * FRIBO_KMAXCH + p;
+}
+
+// Boundary increments of the first (b0) and last (bn) row of a k-line
+// (solve_fluid.F90:130-131,152-153 for q2; :215,223 for q3).
+__device__ inline void kbound(const FluidDev& F, const KSys& K, int f, int j, const DevScalars* S, double& b0,
+                              double& bn)
+{
     if (f == 0) {
-        if (r == 0) d = d + F.dqb2s[j] * S->ugkks;
-        if (r == K.n - 1) d = d + F.dqb2n[j] * S->ugkkn;
+        b0 = F.dqb2s[j] * S->ugkks;
+        bn = F.dqb2n[j] * S->ugkkn;
     } else {
-        if (r == 0) d = d - K.bc[0] * F.dqb3s[j];
-        if (r == K.n - 1) d = d - K.bc[1] * F.dqb3n[j];
+        b0 = -(K.bc[0] * F.dqb3s[j]);
+        bn = -(K.bc[1] * F.dqb3n[j]);
     }
-    return d;
 }
 
 // Pass 1: thread (j, chunk p) eliminates the interior rows of its chunk and
 // leaves the chunk's separator data (sum g*y', last y', separator rhs).  The
-// interior rows are aligned to the end of the MAXM-step loop so the loads of
-// all steps are unconditional and can be issued together.
+// interior rows are aligned to the end of the MAXM-step loop, so every step
+// loads unconditionally and the loads of all steps can be issued together.
 template <int MAXM>
 __global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
 {
@@ -457,65 +465,68 @@ __global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
     const KSys& K = F.ks[f];
     const int j = blockIdx.x * 64 + threadIdx.x + 1;
     if (p >= K.nch || j > F.n2m) return;
-    const double* __restrict__ src = f == 0 ? F.sA : F.sB;
+    const double* __restrict__ src = (f == 0 ? F.sA : F.sB) + fidx(F, j, K.kofs);
+    const size_t P = F.P;
     const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;
     const int off = MAXM - m;
+    double b0, bn;
+    kbound(F, K, f, j, S, b0, bn);
+    // all loads of the chunk first (one memory round trip), then the recurrence
+    double dv[MAXM];
+#pragma unroll
+    for (int tt = 0; tt < MAXM; ++tt) {
+        const int t = tt - off;
+        dv[tt] = src[(size_t)(s + (t >= 0 ? t : 0)) * P];
+    }
+    const double dsep = src[(size_t)e * P];
     double yp = 0.0, acc = 0.0;
 #pragma unroll
     for (int tt = 0; tt < MAXM; ++tt) {
         const int t = tt - off;
         const bool real = t >= 0;
         const int i = s + (real ? t : 0);
-        const double d = krhs(F, K, f, j, i, src, S);
+        double d = dv[tt] * K.S[i];
+        d = (i == 0) ? d + b0 : d;
         const double yn = (d - K.a[i] * yp) * K.inv[i];
         const double an = acc + K.g[i] * yn;
         yp = real ? yn : yp;
         acc = real ? an : acc;
     }
-    const double de = krhs(F, K, f, j, e, src, S);
-    const size_t o = ((size_t)f * FRIBO_KMAXCH + p) * F.n2m + (j - 1);
+    double de = dsep * K.S[e];
+    de = (e == 0) ? de + b0 : de;
+    de = (e == K.n - 1) ? de + bn : de;
+    const size_t o = kidx(F, f, j, p);
     F.kYs[o] = acc;
     F.kYe[o] = yp;
     F.kDe[o] = de;
 }
 
-// Pass 2: thread j solves its column's separator system (nch unknowns,
-// line-invariant Thomas factors from k_bc); loads run along j (coalesced).
+// Pass 2: one wave per column solves the nch separators (PCR over lanes with
+// the precomputed line-invariant multipliers).
 __global__ __launch_bounds__(64) void k_ksolve2(const FluidDev F)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int f = blockIdx.y;
-    const int j = blockIdx.x * 64 + threadIdx.x + 1;
-    if (j > F.n2m) return;
+    const int f = blockIdx.y, j = blockIdx.x + 1, p = threadIdx.x;
     const KSys& K = F.ks[f];
-    const int nch = K.nch;
-    const size_t base = (size_t)f * FRIBO_KMAXCH * F.n2m + (j - 1);
-    const size_t n2m = F.n2m;
-    double z[FRIBO_KMAXCH];
-    double zp = 0.0;
-#pragma unroll
-    for (int p = 0; p < FRIBO_KMAXCH; ++p) {
-        if (p < nch) {
-            const int e = K.cs[p + 1] - 1;
-            const double ysn = (p + 1 < nch) ? F.kYs[base + (p + 1) * n2m] : 0.0;
-            const double ye = F.kYe[base + p * n2m];
-            const double de = F.kDe[base + p * n2m];
-            const double delta = de - K.a[e] * ye - K.c[e] * ysn;
-            zp = (delta - K.ra[p] * zp) * K.rinv[p];
-            z[p] = zp;
-        }
+    const size_t o = kidx(F, f, j, p);
+    double delta = 0.0;
+    if (p < K.nch) {
+        const int e = K.cs[p + 1] - 1;
+        const double ysn = (p + 1 < K.nch) ? F.kYs[o + 1] : 0.0;
+        delta = F.kDe[o] - K.a[e] * F.kYe[o] - K.c[e] * ysn;
     }
-    double X = 0.0;
-#pragma unroll
-    for (int p = FRIBO_KMAXCH - 1; p >= 0; --p) {
-        if (p < nch) {
-            X = (p == nch - 1) ? z[p] : z[p] - K.rcp[p] * X;
-            F.kX[base + p * n2m] = X;
-        }
+    int li = 0;
+    for (int h = 1; h < 64; h <<= 1, ++li) {
+        double dm = __shfl_up(delta, h, 64), dp = __shfl_down(delta, h, 64);
+        if (p - h < 0) dm = 0.0;
+        if (p + h >= 64) dp = 0.0;
+        delta = delta - K.k1[li * 64 + p] * dm - K.k2[li * 64 + p] * dp;
     }
+    if (p < K.nch) F.kX[o] = delta * K.ib[p];
 }
 
+// Pass 3: chunk back-substitution fused with q += dq (solve_fluid.F90:246-250).
 template <int MAXM>
 __global__ __launch_bounds__(64) void k_ksolve3(const FluidDev F)
 {
@@ -525,48 +536,59 @@ __global__ __launch_bounds__(64) void k_ksolve3(const FluidDev F)
     const KSys& K = F.ks[f];
     const int j = blockIdx.x * 64 + threadIdx.x + 1;
     if (p >= K.nch || j > F.n2m) return;
-    const double* __restrict__ src = f == 0 ? F.sA : F.sB;
-    double* __restrict__ q = f == 0 ? F.q2 : F.q3;
+    const size_t P = F.P;
+    const double* __restrict__ src = (f == 0 ? F.sA : F.sB) + fidx(F, j, K.kofs);
+    double* __restrict__ q = (f == 0 ? F.q2 : F.q3) + fidx(F, j, K.kofs);
     const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;   // m interior rows
-    const size_t base = (size_t)f * FRIBO_KMAXCH * F.n2m + (j - 1);
-    const double X = F.kX[base + (size_t)p * F.n2m];
-    const double Xm = (p > 0) ? F.kX[base + (size_t)(p - 1) * F.n2m] : 0.0;
+    const size_t o = kidx(F, f, j, p);
+    const double X = F.kX[o];
+    const double Xm = (p > 0) ? F.kX[o - 1] : 0.0;
     const bool zero_row = (f == 0 && j == 1);
+    double b0, bn;
+    kbound(F, K, f, j, S, b0, bn);
+    // all loads of the chunk first (one memory round trip), then the recurrence
     double y[MAXM], qv[MAXM];
+#pragma unroll
+    for (int t = 0; t < MAXM; ++t) {
+        const int i = s + (t < m ? t : 0);
+        y[t] = src[(size_t)i * P];
+        qv[t] = q[(size_t)i * P];
+    }
+    const double qe = q[(size_t)e * P];
     double yp = 0.0;
 #pragma unroll
     for (int t = 0; t < MAXM; ++t) {
         if (t < m) {
             const int i = s + t;
-            const double d = krhs(F, K, f, j, i, src, S);
-            qv[t] = q[fidx(F, j, i + K.kofs)];
+            double d = y[t] * K.S[i];
+            d = (i == 0) ? d + b0 : d;
             yp = (d - K.a[i] * yp) * K.inv[i];
             y[t] = yp;
         }
     }
-    const size_t ce = fidx(F, j, e + K.kofs);
-    const double qe = q[ce];
     double x = 0.0;
+    double* qn2 = (f == 0 ? F.q2 : F.q3) + fidx(F, F.n2, K.kofs);
 #pragma unroll
     for (int t = MAXM - 1; t >= 0; --t) {
         if (t < m) {
             const int i = s + t;
             if (t == m - 1) x = y[t] + K.vp[i] * Xm + K.wl[p] * X;
             else x = (y[t] + K.vp[i] * Xm) - K.cp[i] * x;
-            q[fidx(F, j, i + K.kofs)] = zero_row ? 0.0 : qv[t] + x;
-            if (zero_row) q[fidx(F, F.n2, i + K.kofs)] = 0.0;
+            q[(size_t)i * P] = zero_row ? 0.0 : qv[t] + x;
+            if (zero_row) qn2[(size_t)i * P] = 0.0;
         }
     }
-    q[ce] = zero_row ? 0.0 : qe + X;
-    if (zero_row) q[fidx(F, F.n2, e + K.kofs)] = 0.0;
+    q[(size_t)e * P] = zero_row ? 0.0 : qe + X;
+    if (zero_row) qn2[(size_t)e * P] = 0.0;
     if (f == 1) {
+        double* qf = F.q3;
         if (p == 0) {
             const size_t c = fidx(F, j, 1);
-            q[c] = q[c] + F.dqb3s[j];
+            qf[c] = qf[c] + F.dqb3s[j];
         }
         if (p == K.nch - 1) {
             const size_t c = fidx(F, j, F.n3);
-            q[c] = q[c] + F.dqb3n[j];
+            qf[c] = qf[c] + F.dqb3n[j];
         }
     }
 }
@@ -582,7 +604,7 @@ void launch_ksolve(const FluidDev& F, hipStream_t st)
     else if (maxm - 1 <= 32) hipLaunchKernelGGL(k_ksolve1<32>, g, dim3(64), 0, st, F);
     else if (maxm - 1 <= 64) hipLaunchKernelGGL(k_ksolve1<64>, g, dim3(64), 0, st, F);
     else hipLaunchKernelGGL(k_ksolve1<128>, g, dim3(64), 0, st, F);
-    hipLaunchKernelGGL(k_ksolve2, dim3((F.n2m + 63) / 64, 2), dim3(64), 0, st, F);
+    hipLaunchKernelGGL(k_ksolve2, dim3(F.n2m, 2), dim3(64), 0, st, F);
     if (maxm <= 8) hipLaunchKernelGGL(k_ksolve3<8>, g, dim3(64), 0, st, F);
     else if (maxm <= 16) hipLaunchKernelGGL(k_ksolve3<16>, g, dim3(64), 0, st, F);
     else if (maxm <= 32) hipLaunchKernelGGL(k_ksolve3<32>, g, dim3(64), 0, st, F);

@@ -1,0 +1,216 @@
+// Poisson solve with the eigen-transform along y (used when n2m < n3m).
+//
+// phcalc_outflow.F90:4-71 solves (A_y (x) I + I (x) A_z) dph = rhs by
+// diagonalising A_z (zmm, zmmt, wr) and solving one tridiagonal system along
+// y per z-mode.  Diagonalising A_y instead (Y, Yinv, wy; fribo_eig.hip)
+// gives the same discrete solution with one tridiagonal system along z per
+// y-mode and a dense transform of 2*n3m*n2m^2 flops instead of
+// 2*n2m*n3m^2:
+//
+//   hmn = Yinv * dph           (GEMM, B operand k-contiguous)
+//   (A_z + wy(j)) v(j,:) = hmn(j,:)   for every y-mode j   (this file)
+//   dph = Y * v                (GEMM)
+//   dph -= c                   (gauge, k_ygauge)
+//
+// The Neumann problem is singular (constants); the reference's order pins
+// the z null mode's last unknown, which fixes the constant through
+// sum_k zmmt(k0,k) dph(n2m,k) = 0.  Here the y null mode is pinned instead
+// and k_ygauge computes the constant c that restores the reference's gauge;
+// the kernels that read dph subtract it (S->dphoff).
+//
+// The per-mode z-lines use the partition of the q2 k-system (ks[0]: n3m rows,
+// up to 64 chunks).  Lanes run along the y-mode index (coalesced), each
+// thread owns one chunk of one line; the matrix rows amphk, acphk + wy(j),
+// apphk are read as wave-uniform scalars, the shift per lane.  Three passes
+// as in the Helmholtz k-solves: chunk elimination -> separator system (one
+// wave per line, PCR) -> chunk back-substitution.
+#include "fribo_internal.h"
+
+__device__ inline size_t yidx(const FluidDev& F, int q, int j, int p)
+{
+    // summary q of chunk p of y-mode j: chunk fastest
+    return ((size_t)q * F.n2m + (size_t)(j - 1)) * FRIBO_KMAXCH + p;
+}
+
+// Chunk elimination of line j, chunk p (rows s..e-1 interior, e separator).
+// Returns the recurrence state; cp/vp/y are kept when STORE.
+template <int MAXM, bool STORE>
+__device__ inline void ychunk(const FluidDev& F, const KSys& K, int j, int p, double w, const double* col,
+                              double* cp, double* vp, double* y, double& yl, double& vl, double& ys, double& vs,
+                              double& ws, double& wl)
+{
+    const size_t P = F.P;
+    const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;
+    double dv[MAXM];
+#pragma unroll
+    for (int t = 0; t < MAXM; ++t) dv[t] = col[(size_t)(s + (t < m ? t : 0)) * P];
+    double cpp = 0.0, gg = 1.0, glast = 1.0;
+    yl = 0.0;
+    vl = 1.0;
+    ys = 0.0;
+    vs = 0.0;
+    wl = 0.0;
+#pragma unroll
+    for (int t = 0; t < MAXM; ++t) {
+        if (t < m) {
+            const int k = s + t + 1;
+            const double a = F.amphk[k], b = F.acphk[k] + w, c = F.apphk[k];
+            const double iv = 1.0 / (b - a * cpp);
+            cpp = c * iv;
+            yl = (dv[t] - a * yl) * iv;
+            vl = (-a * vl) * iv;
+            if (STORE) {
+                cp[t] = cpp;
+                vp[t] = vl;
+                y[t] = yl;
+            }
+            ys = ys + gg * yl;
+            vs = vs + gg * vl;
+            if (t == m - 1) {
+                wl = (-c) * iv;
+                glast = gg;
+            }
+            gg = -cpp * gg;
+        }
+    }
+    ws = glast * wl;
+}
+
+template <int MAXM>
+__global__ __launch_bounds__(64) void k_ysolve1(const FluidDev F)
+{
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    const KSys& K = F.ks[0];
+    const int p = blockIdx.y;
+    const int j = blockIdx.x * 64 + threadIdx.x + 1;
+    if (p >= K.nch || j > F.n2m) return;
+    const double* col = F.hmn + (j - 1);
+    const double w = F.wy[j - 1];
+    double yl, vl, ys, vs, ws, wl;
+    ychunk<MAXM, false>(F, K, j, p, w, col, nullptr, nullptr, nullptr, yl, vl, ys, vs, ws, wl);
+    const int e = K.cs[p + 1] - 1, k = e + 1;
+    const double ae = F.amphk[k];
+    const double de = col[(size_t)e * F.P];
+    double* Y = F.ysum;
+    Y[yidx(F, 0, j, p)] = ae * vl;                        // alpha
+    Y[yidx(F, 1, j, p)] = (F.acphk[k] + w) + ae * wl;     // beta before the coupling to chunk p+1
+    Y[yidx(F, 2, j, p)] = de - ae * yl;                   // delta before the coupling
+    Y[yidx(F, 3, j, p)] = ys;
+    Y[yidx(F, 4, j, p)] = vs;
+    Y[yidx(F, 5, j, p)] = ws;
+}
+
+__global__ __launch_bounds__(64) void k_ysolve2(const FluidDev F)
+{
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    const KSys& K = F.ks[0];
+    const int j = blockIdx.x + 1, p = threadIdx.x, lane = p;
+    const int nch = K.nch;
+    const double* Y = F.ysum;
+    double alpha = 0.0, beta = 1.0, gamma = 0.0, delta = 0.0;
+    if (p < nch) {
+        alpha = Y[yidx(F, 0, j, p)];
+        beta = Y[yidx(F, 1, j, p)];
+        delta = Y[yidx(F, 2, j, p)];
+        if (p + 1 < nch) {
+            const double ce = F.apphk[K.cs[p + 1]];   // row e = cs[p+1]-1, k = e+1
+            beta = beta + ce * Y[yidx(F, 4, j, p + 1)];
+            gamma = ce * Y[yidx(F, 5, j, p + 1)];
+            delta = delta - ce * Y[yidx(F, 3, j, p + 1)];
+        }
+        if (j == F.ynull && p == nch - 1) {
+            alpha = 0.0;
+            beta = 1.0;
+            gamma = 0.0;
+            delta = 0.0;
+        }
+    }
+    for (int h = 1; h < 64; h <<= 1) {
+        double am = __shfl_up(alpha, h, 64), bm = __shfl_up(beta, h, 64);
+        double gm = __shfl_up(gamma, h, 64), dm = __shfl_up(delta, h, 64);
+        double ap = __shfl_down(alpha, h, 64), bp = __shfl_down(beta, h, 64);
+        double gp = __shfl_down(gamma, h, 64), dp = __shfl_down(delta, h, 64);
+        if (lane - h < 0) { am = 0.0; bm = 1.0; gm = 0.0; dm = 0.0; }
+        if (lane + h >= 64) { ap = 0.0; bp = 1.0; gp = 0.0; dp = 0.0; }
+        const double m1 = alpha / bm, m2 = gamma / bp;
+        const double na = -m1 * am, ng = -m2 * gp;
+        const double nb = beta - m1 * gm - m2 * ap;
+        const double nd = delta - m1 * dm - m2 * dp;
+        alpha = na;
+        beta = nb;
+        gamma = ng;
+        delta = nd;
+    }
+    if (p < nch) F.ysum[yidx(F, 6, j, p)] = delta / beta;
+}
+
+template <int MAXM>
+__global__ __launch_bounds__(64) void k_ysolve3(const FluidDev F)
+{
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    const KSys& K = F.ks[0];
+    const int p = blockIdx.y;
+    const int j = blockIdx.x * 64 + threadIdx.x + 1;
+    if (p >= K.nch || j > F.n2m) return;
+    double* col = F.hmn + (j - 1);
+    const double w = F.wy[j - 1];
+    double cp[MAXM], vp[MAXM], y[MAXM];
+    double yl, vl, ys, vs, ws, wl;
+    ychunk<MAXM, true>(F, K, j, p, w, col, cp, vp, y, yl, vl, ys, vs, ws, wl);
+    const double X = F.ysum[yidx(F, 6, j, p)];
+    const double Xm = p > 0 ? F.ysum[yidx(F, 6, j, p - 1)] : 0.0;
+    const size_t P = F.P;
+    const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;
+    double x = 0.0;
+#pragma unroll
+    for (int t = MAXM - 1; t >= 0; --t) {
+        if (t < m) {
+            if (t == m - 1) x = y[t] + vp[t] * Xm + wl * X;
+            else x = (y[t] + vp[t] * Xm) - cp[t] * x;
+            col[(size_t)(s + t) * P] = x;
+        }
+    }
+    col[(size_t)e * P] = X;
+}
+
+// c = sum_k zg(k) dph(n2m,k) / sum_k zg(k): the constant that moves the
+// y-pinned solution to the reference's z-pinned gauge (fixed-order sum).
+__global__ __launch_bounds__(1024) void k_ygauge(const FluidDev F)
+{
+    __shared__ double sh[1024];
+    DevScalars* S = F.S;
+    if (S->err) return;
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int k = 1 + t; k <= F.n3m; k += 1024) s = s + F.zg[k - 1] * F.dph[fidx(F, F.n2m, k)];
+    sh[t] = s;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h) sh[t] = sh[t] + sh[t + h];
+        __syncthreads();
+    }
+    if (t == 0) S->dphoff = sh[0] * F.zginv;
+}
+
+void launch_ysolve(const FluidDev& F, hipStream_t st)
+{
+    const int mm = F.ks[0].maxm - 1;   // interior rows per chunk
+    dim3 g((F.n2m + 63) / 64, F.ks[0].nch);
+    if (mm <= 16) hipLaunchKernelGGL(k_ysolve1<16>, g, dim3(64), 0, st, F);
+    else if (mm <= 32) hipLaunchKernelGGL(k_ysolve1<32>, g, dim3(64), 0, st, F);
+    else if (mm <= 64) hipLaunchKernelGGL(k_ysolve1<64>, g, dim3(64), 0, st, F);
+    else hipLaunchKernelGGL(k_ysolve1<128>, g, dim3(64), 0, st, F);
+    hipLaunchKernelGGL(k_ysolve2, dim3(F.n2m), dim3(64), 0, st, F);
+    if (mm <= 16) hipLaunchKernelGGL(k_ysolve3<16>, g, dim3(64), 0, st, F);
+    else if (mm <= 32) hipLaunchKernelGGL(k_ysolve3<32>, g, dim3(64), 0, st, F);
+    else if (mm <= 64) hipLaunchKernelGGL(k_ysolve3<64>, g, dim3(64), 0, st, F);
+    else hipLaunchKernelGGL(k_ysolve3<128>, g, dim3(64), 0, st, F);
+}
+
+void launch_ygauge(const FluidDev& F, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ygauge, dim3(1), dim3(1024), 0, st, F);
+}

@@ -25,7 +25,8 @@ ERR_NAMES = {1: "DT_TOO_SMALL", 2: "CFL_TOO_LARGE", 3: "MARKER_OUTSIDE", 4: "PAR
 EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state", "fribo_set_markers",
            "fribo_fluid_tsch", "fribo_cfl_dt", "fribo_substep", "fribo_array_size", "fribo_get_array",
            "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_set_profiling", "fribo_get_profile",
-           "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy"]
+           "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
+           "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform"]
 
 NPROF = 12
 
@@ -85,6 +86,9 @@ def load():
     L.fribo_profile_name.restype = C.c_char_p
     L.fribo_set_graph.argtypes = [vp, C.c_int32]
     L.fribo_test_copy.argtypes = [C.c_int64, C.c_int32, dp]
+    L.fribo_set_transform.argtypes = [vp, C.c_int32]
+    L.fribo_get_transform.argtypes = [vp]
+    L.fribo_test_ytransform.argtypes = [C.c_int32, dp, dp, dp, dp, dp, dp]
     _lib = L
     return L
 
@@ -196,13 +200,23 @@ class FluidStep:
             return out.reshape((n2, n3m), order="F")
         return out
 
+    def set_transform(self, axis):
+        """Poisson eigen-transform axis: "z" (reference order), "y" or "auto"."""
+        a = {"z": 0, "y": 1, "auto": -1}[axis]
+        _check(load().fribo_set_transform(self.h, a))
+
+    @property
+    def transform(self):
+        return "y" if load().fribo_get_transform(self.h) == 1 else "z"
+
     def set_graph(self, on=True):
         """Replay the step as one captured HIP graph (same results)."""
         _check(load().fribo_set_graph(self.h, 1 if on else 0))
 
     # -- profiling ----------------------------------------------------------
-    def set_profiling(self, on=True):
-        _check(load().fribo_set_profiling(self.h, 1 if on else 0))
+    def set_profiling(self, on=True, gemm_only=False):
+        """Event timing of every kernel class (or of the transform GEMMs only)."""
+        _check(load().fribo_set_profiling(self.h, (2 if gemm_only else 1) if on else 0))
 
     def reset_profile(self):
         _check(load().fribo_reset_profile(self.h))
@@ -232,3 +246,15 @@ def plain_copy(nbytes, reps):
     ms = C.c_double(0.0)
     _check(load().fribo_test_copy(int(nbytes), int(reps), C.byref(ms)))
     return ms.value
+
+
+def ytransform(am, ac, ap):
+    """Host eigen-decomposition of a tridiagonal operator (rows am, ac, ap):
+    (Y, Yinv, wy) with operator @ Y = Y * wy."""
+    n = len(ac)
+    Y = np.zeros((n, n), order="F")
+    Yi = np.zeros((n, n), order="F")
+    w = np.zeros(n)
+    arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (am, ac, ap)]
+    _check(load().fribo_test_ytransform(n, *[_dp(x) for x in arr], _dp(Y), _dp(Yi), _dp(w)))
+    return Y, Yi, w

@@ -124,16 +124,39 @@ const char* fribo_last_error(void);
 int fribo_test_gemm(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
                     int32_t ldb, double* C, int32_t ldc);
 
+/* Execution option (no reference counterpart): the axis of the Poisson
+ * eigen-transform.  axis 0: along z with the caller's zmm/zmmt/wr, the
+ * reference's order (phcalc_outflow.F90:4-71); axis 1: along y with the
+ * eigen-system of the y pressure operator built by fribo_init (one
+ * tridiagonal system along z per y-mode), available when n2-1 < n3-1 and
+ * both 1-D operators have their Neumann null mode; axis -1: y when
+ * available, else z (the default).  Both orders solve the same discrete
+ * system with the same gauge (the constant fixed by the z-order's pinned
+ * null mode); the y order costs 2*(n3-1)*(n2-1)^2 flops per transform
+ * instead of 2*(n2-1)*(n3-1)^2.  fribo_get_transform returns the axis in
+ * use. */
+int fribo_set_transform(fribo_handle h, int32_t axis);
+int fribo_get_transform(fribo_handle h);
+
+/* Diagnostic entry point (no reference counterpart): eigen-decomposition of
+ * an n x n tridiagonal operator with rows (am, ac, ap), 0-based, that is
+ * similar to a symmetric one (am[j+1]*ap[j] > 0): Y (eigenvectors as
+ * columns), Yinv = Y^-1 (both n x n, column-major) and the eigenvalues wy. */
+int fribo_test_ytransform(int32_t n, const double* am, const double* ac, const double* ap, double* Y, double* Yinv,
+                          double* wy);
+
 /* Diagnostic entry point (no reference counterpart): `reps` streaming copies
  * of `bytes` (a multiple of 16) between two device buffers after one warm-up
  * copy; returns the mean device time per copy.  It is the known-size kernel
  * that scales the HBM traffic counters of the step's kernels. */
 int fribo_test_copy(int64_t bytes, int32_t reps, double* ms_per_copy);
 
-/* Diagnostic timing (no reference counterpart): with profiling on, every
- * kernel launch of the step is bracketed by HIP events on the library's
- * stream; fribo_get_profile returns, per kernel class, the summed device
- * time (ms) and the number of launches since the last reset. */
+/* Diagnostic timing (no reference counterpart): with profiling on (on = 1),
+ * every kernel launch of the step is bracketed by HIP events on the
+ * library's stream; on = 2 brackets only the transform GEMMs (the least
+ * intrusive timing of the dominant kernel).  fribo_get_profile returns, per
+ * kernel class, the summed device time (ms) and the number of launches since
+ * the last reset. */
 #define FRIBO_NPROF 12
 int fribo_set_profiling(fribo_handle h, int32_t on);
 int fribo_get_profile(fribo_handle h, double* ms, int32_t* launches, int32_t nclass);

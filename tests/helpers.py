@@ -28,11 +28,15 @@ def oracle_for(c, dec=None, state=None, params=None):
     return o
 
 
-def gpu_for(c, dec=None, state=None, params=None):
+def gpu_for(c, dec=None, state=None, params=None, axis=None):
+    """GPU step for case c; axis "z" / "y" selects the Poisson transform
+    (default: the library's choice, y when it is the shorter axis)."""
     from fribois2d_amd.gpu import FluidStep
     p = params or c.params
     zmm, zmmt, wr = dec or decomposition(c)
     g = FluidStep(p, c.rc, c.zz, zmm, zmmt, wr)
+    if axis is not None:
+        g.set_transform(axis)
     g.set_state(state if state is not None else cases.initial_state(p, c.seed))
     return g
 

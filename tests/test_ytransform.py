@@ -1,0 +1,46 @@
+"""Host eigen-decomposition behind the Poisson transform along y (CPU only:
+the set-up code of libfribo_gpu, no kernel runs)."""
+import numpy as np
+import pytest
+
+from fribois2d_amd import cases
+
+
+def _ops(c):
+    return cases.tridiag_z(c.rc, c.params.rext2)
+
+
+def test_uniform_grid_matches_the_closed_form():
+    """Uniform spacing: the Neumann operator dx2q*tridiag(1,-2,1) (ends -1)
+    has eigenvalues -4*dx2q*sin^2(pi*m/(2n)), m = 0..n-1."""
+    from fribois2d_amd import gpu
+    n, L = 40, 3.0
+    rc = np.linspace(0.0, L, n + 1)
+    am, ac, ap = cases.tridiag_z(rc, L)
+    Y, Yi, w = gpu.ytransform(am, ac, ap)
+    dx2q = (n / L) ** 2
+    ref = np.sort(-4.0 * dx2q * np.sin(np.pi * np.arange(n) / (2 * n)) ** 2)
+    np.testing.assert_allclose(np.sort(w), ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+    # eigenvectors: cos(pi*m*(j-1/2)/n) up to scale
+    m = int(np.argsort(w)[-2])            # second smallest |w|: m = 1
+    v = Y[:, m] / Y[0, m]
+    j = np.arange(1, n + 1)
+    np.testing.assert_allclose(v, np.cos(np.pi * (j - 0.5) / n) / np.cos(np.pi * 0.5 / n), atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["cyl64", "cyl384"])
+def test_stretched_grid_decomposition(name):
+    from fribois2d_amd import gpu
+    c = cases.make_case(name)
+    am, ac, ap = _ops(c)
+    n = ac.size
+    A = np.diag(ac) + np.diag(ap[:-1], 1) + np.diag(am[1:], -1)
+    Y, Yi, w = gpu.ytransform(am, ac, ap)
+    scale = np.abs(A).max()
+    assert np.abs(A @ Y - Y * w).max() <= 1e-13 * scale
+    assert np.abs(Yi @ Y - np.eye(n)).max() <= 1e-13
+    # exactly one Neumann null mode, the rest strictly negative
+    ws = np.sort(w)
+    assert abs(ws[-1]) <= 1e-9 * scale and ws[-2] < -1e-9 * scale
+    # the same spectrum as a dense eigen-solver
+    np.testing.assert_allclose(ws, np.sort(np.linalg.eigvals(A).real), rtol=0, atol=1e-11 * scale)
