@@ -28,7 +28,7 @@ import numpy as np
 
 __all__ = [
     "FluidParams", "Case", "cubicstr", "read_grid_file", "read_lmdata", "markers_from_vertices",
-    "cylinder_vertices", "rectangle_vertices", "tridiag_z", "phini", "initial_state",
+    "cylinder_vertices", "rectangle_vertices", "createnorm", "tridiag_z", "phini", "initial_state",
     "splitmix64_uniform", "make_case", "CASES",
 ]
 
@@ -289,12 +289,53 @@ class Case:
         lmvel = np.stack([vy - om * rzr, vz + om * ryr], axis=1)
         return lmcc, lmvel, lmA
 
+    def surface(self, t=0.0):
+        """lmnor (nlm,2) and lmacc (nlm,2) as rigid() leaves them at time t:
+        normals of the moved polygon by createnorm (rigid.F90:468-495, called
+        with or=-1 at rigid.F90:123) and the rigid-body acceleration of every
+        marker (rigid.F90:84-104)."""
+        lmcc0, _ = markers_from_vertices(self.verts)
+        yc, zc = self.center
+        if self.motion is None:
+            dy = dz = th = om = ay = az = al = 0.0
+        else:
+            dy, dz, th, _, _, om = self.motion(t)[:6]
+            ay, az, al = self.motion.accel(t) if hasattr(self.motion, "accel") else (0.0, 0.0, 0.0)
+        c, s_ = math.cos(th), math.sin(th)
+        vr = self.verts - np.array([yc, zc])
+        verts = np.stack([yc + dy + c * vr[:, 0] - s_ * vr[:, 1], zc + dz + s_ * vr[:, 0] + c * vr[:, 1]], axis=1)
+        lmnor = createnorm(verts, -1.0)
+        ry, rz = lmcc0[:, 0] - yc, lmcc0[:, 1] - zc
+        ryr, rzr = c * ry - s_ * rz, s_ * ry + c * rz
+        # accP = a_c + alpha x rho + omega x (omega x rho)
+        c1y, c1z = -om * rzr, om * ryr
+        lmacc = np.stack([ay - al * rzr - om * c1z, az + al * ryr + om * c1y], axis=1)
+        return lmnor, lmacc
+
+
+def createnorm(verts, orient=-1.0):
+    """rigid.F90:468-495: unit normal of every segment from the cross product
+    of the segment (x=0 plane) with the vector to its midpoint lifted to x=or."""
+    v = np.asarray(verts, dtype=np.float64)
+    nlm = v.shape[0] - 1
+    out = np.zeros((nlm, 2))
+    for i in range(nlm):
+        t1 = np.array([0.0, v[i, 0], v[i, 1]])
+        t2 = np.array([0.0, v[i + 1, 0], v[i + 1, 1]])
+        t3 = np.array([orient, (v[i, 0] + v[i + 1, 0]) * 0.5, (v[i, 1] + v[i + 1, 1]) * 0.5])
+        a, b = t2 - t1, t3 - t1
+        cr = np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]])
+        cr = cr / math.sqrt(float(np.sum(cr ** 2)))
+        out[i] = cr[1:]
+    return out
+
 
 def _heave(amp, freq):
     w = 2.0 * math.pi * freq
 
     def motion(t):
         return (amp * math.sin(w * t), 0.0, 0.0, amp * w * math.cos(w * t), 0.0, 0.0)
+    motion.accel = lambda t: (-amp * w * w * math.sin(w * t), 0.0, 0.0)
     return motion
 
 
@@ -303,6 +344,7 @@ def _pitch(amp, freq):
 
     def motion(t):
         return (0.0, 0.0, amp * math.sin(w * t), 0.0, 0.0, amp * w * math.cos(w * t))
+    motion.accel = lambda t: (0.0, 0.0, -amp * w * w * math.sin(w * t))
     return motion
 
 

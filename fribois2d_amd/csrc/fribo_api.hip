@@ -61,6 +61,8 @@ struct fribo_ctx {
     // captured step (hipGraph): rebuilt when the marker buffers change
     bool use_graph = false;
     bool yavail = false;     // transform along y set up
+    double dmin = 0.0;       // smallest node spacing of the grid (cordin.F90:217-231)
+    double* nb = nullptr;    // device: the two checkbcs norms
     hipGraphExec_t gexec = nullptr;
     int graph_nlm = -1;
 };
@@ -357,6 +359,7 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     F.dx2q = F.dx2 * F.dx2;
     F.dx3q = F.dx3 * F.dx3;
     F.ren = 1.0 / p->visc;
+    F.visc = p->visc;
     if (p->nsst > 1) {
         F.gam[1] = 8.0 / 15.0; F.gam[2] = 5.0 / 12.0; F.gam[3] = 3.0 / 4.0;
         F.rom[1] = 0.0; F.rom[2] = -17.0 / 60.0; F.rom[3] = -5.0 / 12.0;
@@ -427,6 +430,13 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     if (!e) e = dalloc(c, &F.kDe, (size_t)2 * FRIBO_KMAXCH * F.n2m);
     if (!e) e = dalloc(c, &F.kX, (size_t)2 * FRIBO_KMAXCH * F.n2m);
     if (!e) e = dalloc(c, &F.S, 1);
+    if (!e) e = dalloc(c, &c->nb, 2);
+    {
+        double dymin = 1.0e+6, dzmin = 1.0e+6;
+        for (int i = 2; i <= F.n2; ++i) dymin = std::min(dymin, rc[i - 1] - rc[i - 2]);
+        for (int i = 2; i <= F.n3; ++i) dzmin = std::min(dzmin, zz[i - 1] - zz[i - 2]);
+        c->dmin = std::min(dzmin, dymin);
+    }
     // the Poisson transform along y costs 2*n3m*n2m^2 flops against
     // 2*n2m*n3m^2 along z: set it up when y is the shorter axis
     if (!e && F.n2m < F.n3m && F.null_mode > 0) e = setup_ytransform(c, zmmt, ldz);
@@ -588,6 +598,12 @@ extern "C" int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc
         if (!e) e = dalloc(h, &F.contrib, 2 * FRIBO_DIMSD * (size_t)nlm, L);
         if (!e) e = dalloc(h, &F.keys, 2 * (size_t)maxkeys, L);
         if (!e) e = dalloc(h, &F.segval, 2 * (size_t)maxkeys, L);
+        if (!e) e = dalloc(h, &F.lmnor, 2 * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.lmacc, 2 * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.press, (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.tau, 2 * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.lwork, FRIBO_LWORK * (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.bcerr, 2 * (size_t)nlm, L);
         if (e) return e;
         h->nlm_alloc = nlm;
         F.nlm = nlm;
@@ -1003,6 +1019,48 @@ extern "C" int fribo_test_copy(int64_t bytes, int32_t reps, double* ms_per_copy)
     hipEventDestroy(e1);
     hipFree(a);
     hipFree(b);
+    return FRIBO_OK;
+}
+
+// loads_transfer_c (loads_transfer.F90:4-162) on the state the last step
+// left: probes at hpr = ddmin*kprobe along lmnor, where cordin already scaled
+// ddmin by kprobe (cordin.F90:231-233, loads_transfer.F90:41).
+extern "C" int fribo_loads_transfer(fribo_handle h, double kprobe, const double* lmnor, const double* lmacc,
+                                    double* press, double* tau)
+{
+    if (!h || !lmnor || !lmacc || !press || !tau) return set_err(FRIBO_ERR_ARG, "null argument");
+    FluidDev& F = h->F;
+    if (F.nlm <= 0) return set_err(FRIBO_ERR_ARG, "no markers set");
+    const size_t nb = (size_t)F.nlm * sizeof(double);
+    HIPCHK(hipMemcpyAsync(F.lmnor, lmnor, 2 * nb, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(F.lmacc, lmacc, 2 * nb, hipMemcpyHostToDevice, h->st));
+    const double hpr = (h->dmin * kprobe) * kprobe;
+    launch_loads(F, hpr, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(press, F.press, nb, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(tau, F.tau, 2 * nb, hipMemcpyDeviceToHost, h->st));
+    int e = read_scalars(h);
+    if (e) return e;
+    if (h->hs.err) return set_err(h->hs.err, "MLS partition of unity violated at a load probe");
+    return FRIBO_OK;
+}
+
+// checkbcs (ckbcs.F90:5-86) on the state the last step left.
+extern "C" int fribo_checkbcs(fribo_handle h, double* bcerrv, double* bcerrw, double* nbcv, double* nbcw)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    FluidDev& F = h->F;
+    if (F.nlm <= 0) return set_err(FRIBO_ERR_ARG, "no markers set");
+    launch_checkbcs(F, h->nb, h->st);
+    HIPCHK(hipGetLastError());
+    const size_t nb = (size_t)F.nlm * sizeof(double);
+    if (bcerrv) HIPCHK(hipMemcpyAsync(bcerrv, F.bcerr, nb, hipMemcpyDeviceToHost, h->st));
+    if (bcerrw) HIPCHK(hipMemcpyAsync(bcerrw, F.bcerr + F.nlm, nb, hipMemcpyDeviceToHost, h->st));
+    double v[2];
+    HIPCHK(hipMemcpyAsync(v, h->nb, sizeof(v), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (nbcv) *nbcv = v[0];
+    if (nbcw) *nbcw = v[1];
     return FRIBO_OK;
 }
 

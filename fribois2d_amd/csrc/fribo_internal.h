@@ -13,6 +13,7 @@
 #define FRIBO_DIMSD 9
 #define FRIBO_KMAXCH 64      // max chunks of a partitioned k-line (one wave)
 #define FRIBO_NPART 512      // partial sums of the CFL reduction
+#define FRIBO_LWORK 20       // per-marker record of the load computation
 
 struct DevScalars {
     // step inputs, written by the host before every step (one 24-byte copy)
@@ -75,7 +76,7 @@ struct FluidDev {
     int nlm;
     int null_mode;           // 1-based index of the singular Poisson mode, 0 if none
     int taxis;               // eigen-transform axis of the Poisson solve: 0 z (zmm), 1 y (ymm)
-    double alx3, rext2, ren, cou, q3uw, q3lw, dtmax, cflmax, cfllim, radiusSD;
+    double alx3, rext2, ren, visc, cou, q3uw, q3lw, dtmax, cflmax, cfllim, radiusSD;
     double dx2, dx3, dx2q, dx3q;
     double gam[4], rom[4], alm[4];
     // fields (ghosted layout)
@@ -120,6 +121,11 @@ struct FluidDev {
     unsigned long long *keys;      // [2][maxkeys]
     double *segval;                // [2][maxkeys] cell value at segment heads
     int maxkeys;
+    // loads on the body (loads_transfer_c) and the forcing check (checkbcs)
+    double *lmnor, *lmacc;         // (nlm,2) column-major, from the host
+    double *press, *tau;           // (nlm), (nlm,2)
+    double *lwork;                 // [nlm][FRIBO_LWORK] per-marker probe data
+    double *bcerr;                 // (nlm,2)
     DevScalars* S;
 };
 
@@ -157,6 +163,8 @@ void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double*
 void launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc, const DevScalars* S, hipStream_t st);
 void launch_ysolve(const FluidDev& F, hipStream_t st);
+void launch_loads(const FluidDev& F, double hpr, hipStream_t st);
+void launch_checkbcs(const FluidDev& F, double* nb, hipStream_t st);
 void launch_ygauge(const FluidDev& F, hipStream_t st);
 int ytransform_build(int n, const double* am, const double* ac, const double* ap, int ld, double* Y, double* Yinv,
                      double* wy);

@@ -11,6 +11,12 @@
 //   k_mls_sort  orders the 9 contributions of every marker by (cell, marker)
 //               (bitonic sort: LDS blocks plus global merge passes); equal
 //               cells form contiguous segments
+// After the step, on request (fluid_tsch.F90:167-177):
+//   k_loads      loads_transfer_c (loads_transfer.F90:4-162): probes at
+//                ddmin*kprobe along the normal, MLS_SF_d shape functions and
+//                derivatives (MLSint.F90:173-258,342-402,492-585), getpbup
+//                gathers, pressure and friction stress per marker
+//   k_checkbcs   checkbcs (ckbcs.F90:5-86)
 // Every substep (interp_spread, forcing_MLS.F90:135-182):
 //   k_mls_interp  u_m = sum phi q*f, f = (V - u_m)/dt, contribution cl*phi*f
 //   k_mls_spread  one thread per touched cell sums its contributions in
@@ -454,4 +460,349 @@ void launch_mls_force(const FluidDev& F, hipStream_t st)
     if (F.nlm <= 0) return;
     hipLaunchKernelGGL(k_mls_interp, dim3((2 * F.nlm + 63) / 64), dim3(64), 0, st, F);
     hipLaunchKernelGGL(k_mls_spread, dim3((F.nlm * FRIBO_DIMSD + 63) / 64, 2), dim3(64), 0, st, F);
+}
+
+// ---------------------------------------------------------------------------
+// Loads on the body (loads_transfer.F90:4-162) and the forcing check.
+
+#define LMNOR(n, c) F.lmnor[(size_t)((n) - 1) + (size_t)((c) - 1) * F.nlm]
+#define LMACC(n, c) F.lmacc[(size_t)((n) - 1) + (size_t)((c) - 1) * F.nlm]
+#define LW(n, q) F.lwork[(size_t)((n) - 1) * FRIBO_LWORK + (q)]
+
+// MLSint.F90:492-585 (weight_function_d), one direction: value and derivative
+__device__ inline void weight1d_d(double difc, double sdd, int wfun, double radiusSD, double& wc, double& dw)
+{
+    const double wcon = 0.3, eps = 1.0e-20;
+    const double sdds = radiusSD * sdd;
+    const double drdc = (fabs(difc) <= eps) ? 0.0 : (difc / fabs(difc)) / sdds;
+    const double rcv = fabs(difc) / sdds;
+    const double rc2 = rcv * rcv, rc3 = rcv * rcv * rcv, rc4 = rcv * rcv * rcv * rcv;
+    wc = 0.0;
+    dw = 0.0;
+    if (wfun == 1) {
+        if (rcv > 1.0) {
+        } else if (rcv <= 0.5) {
+            wc = (2.0 / 3.0) - 4.0 * rc2 + 4.0 * rc3;
+            dw = (-8.0 * rcv + 12.0 * rc2) * drdc;
+        } else {
+            wc = (4.0 / 3.0) - 4.0 * rcv + 4.0 * rc2 - (4.0 / 3.0) * rc3;
+            dw = (-4.0 + 8.0 * rcv - 4.0 * rc2) * drdc;
+        }
+    } else if (wfun == 2) {
+        if (rcv <= 1.0) {
+            const double t = rcv / wcon;
+            wc = exp(-(t * t));
+            dw = (-1.0 / (wcon * wcon) * 2.0 * rcv * exp(-(t * t))) * drdc;
+        }
+    } else if (wfun == 3) {
+        if (rcv <= 1.0) {
+            wc = 1.0 - 6.0 * rc2 + 8.0 * rc3 - 3.0 * rc4;
+            dw = -12.0 * rcv + 24.0 * rc2 - 12.0 * rc3;
+        }
+    } else if (wfun == 4) {
+        if (rcv > 1.0) {
+        } else if (rcv < 0.5) {
+            wc = 1.0 / 3.0 * (1.0 + sqrt(-3.0 * rc2 + 1.0));
+            dw = -rcv * pow(-3.0 * rc2 - 1.0, -0.5);
+        } else {
+            const double om = 1.0 - rcv;
+            wc = 1.0 / 6.0 * (5.0 - 3.0 * rcv - sqrt(-3.0 * (om * om) + 1.0));
+            dw = -(1.0 - rcv) / (2.0 * sqrt(1.0 - 3.0 * (om * om))) - 0.5;
+        }
+    }
+}
+
+// first minimum of |x(i) - v| over i = c-5..c+5 inside 1..n (probeind)
+__device__ inline int window_minloc(const double* x, int n, int c, double v)
+{
+    int best = 0;
+    double bv = 0.0;
+    for (int i = c - 5; i <= c + 5; ++i) {
+        if (i < 1 || i > n) continue;
+        const double a = fabs(x[i] - v);
+        if (best == 0 || a < bv) {
+            bv = a;
+            best = i;
+        }
+    }
+    return best;
+}
+
+// Probe values: ucy/ucz/pc (loads_transfer.F90:210-267) need f(j-1,k) at j=1,
+// which the reference's column-major storage resolves to f(n2,k-1).
+__device__ inline double fat(const FluidDev& F, const double* f, int j, int k)
+{
+    if (j == 0) {
+        j = F.n2;
+        k = k - 1;
+    }
+    return f[fidx(F, j, k)];
+}
+
+__device__ inline double prat(const FluidDev& F, const DevScalars* S, int j, int k)
+{
+    if (j == 0) {
+        j = F.n2;
+        k = k - 1;
+    }
+    const bool edge = (j == F.n2) || k < 1 || k > F.n3m;
+    return F.pr[fidx(F, j, k)] - (edge ? S->goff : S->pend);
+}
+
+// One thread per marker: probes, MLS_SF_d, getpbup and the pressure; the
+// velocity-derivative combinations are left in lwork for the friction stress,
+// which needs the tangent of the previous marker when this one has none.
+__global__ __launch_bounds__(64) void k_loads(const FluidDev F, double hpr)
+{
+    DevScalars* S = F.S;
+    if (S->err) return;
+    const int n = blockIdx.x * 64 + threadIdx.x + 1;
+    if (n > F.nlm) return;
+    const double pos[2] = {LMCC(n, 1), LMCC(n, 2)};
+    const double nor[2] = {LMNOR(n, 1), LMNOR(n, 2)};
+    const double probe[2] = {pos[0] + nor[0] * hpr, pos[1] + nor[1] * hpr};
+    F.press[n - 1] = 0.0;
+    LW(n, 0) = 0.0;   // inside
+    LW(n, 1) = 0.0;   // has a tangent
+    if (!(F.zz[1] < pos[1] && pos[1] < F.zz[F.n3])) return;
+    double u[3][7];
+    for (int idir = 1; idir <= 3; ++idir) {
+        const int mi1 = (idir == 3) ? LMIND(n, 2, 1) : LMIND(n, idir, 1);
+        const int mi2 = (idir == 3) ? LMIND(n, 1, 2) : LMIND(n, idir, 2);
+        const double* ya = (idir == 1) ? F.rc : F.rm;
+        const double* za = (idir == 2) ? F.zz : F.zm;
+        const int i1 = window_minloc(ya, F.n2, mi1, probe[0]);
+        const int i2 = window_minloc(za, F.n3, mi2, probe[1]);
+        // MLS_SD_ind
+        double sdc[2][FRIBO_DIMSD], sdd[2][FRIBO_DIMSD];
+        int eul[2][FRIBO_DIMSD];
+        int m = 0;
+        for (int j = 1; j <= 3; ++j)
+            for (int k = 1; k <= 3; ++k) {
+                const int jj = (i1 == 1) ? j + 1 : ((i1 == F.n2m) ? j - 1 : j);
+                const int kk = (i2 == 1) ? k + 1 : ((i2 == F.n3m) ? k - 1 : k);
+                const int ej = i1 + jj - 2, ek = i2 + kk - 2;
+                sdc[0][m] = ya[ej];
+                sdd[0][m] = ya[ej + 1] - ya[ej];
+                sdc[1][m] = za[ek];
+                sdd[1][m] = za[ek + 1] - za[ek];
+                eul[0][m] = ej;
+                eul[1][m] = ek;
+                ++m;
+            }
+        // MLS_SF_d with compute_AB_d
+        double w[FRIBO_DIMSD][3], pk[3][FRIBO_DIMSD], A[3][3][3];
+        for (int i = 0; i < FRIBO_DIMSD; ++i) {
+            pk[0][i] = 1.0;
+            pk[1][i] = sdc[0][i];
+            pk[2][i] = sdc[1][i];
+            double w1, d1, w2, d2;
+            weight1d_d(probe[0] - sdc[0][i], sdd[0][i], F.wfun, F.radiusSD, w1, d1);
+            weight1d_d(probe[1] - sdc[1][i], sdd[1][i], F.wfun, F.radiusSD, w2, d2);
+            w[i][0] = w1 * w2;
+            w[i][1] = w2 * d1;
+            w[i][2] = w1 * d2;
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int l = 0; l < 3; ++l) A[a][b][l] = 0.0;
+        for (int i = 0; i < FRIBO_DIMSD; ++i)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double pp = pk[a][i] * pk[b][i];
+                    for (int l = 0; l < 3; ++l) A[a][b][l] = A[a][b][l] + w[i][l] * pp;
+                }
+        double Aa[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) Aa[a][b] = A[a][b][0];
+        double gam[3][3];
+        const double g0[3] = {1.0, probe[0], probe[1]};
+        double x[3] = {0.0, 0.0, 0.0};
+        gesolve3(Aa, g0, x);
+        for (int a = 0; a < 3; ++a) gam[a][0] = x[a];
+        for (int l = 1; l <= 2; ++l) {
+            double C[3];
+            for (int a = 0; a < 3; ++a) {
+                double mv = 0.0;
+                for (int b = 0; b < 3; ++b) mv = mv + A[a][b][l] * gam[b][0];
+                C[a] = ((a == l) ? 1.0 : 0.0) - mv;
+            }
+            x[0] = x[1] = x[2] = 0.0;
+            gesolve3(Aa, C, x);
+            for (int a = 0; a < 3; ++a) gam[a][l] = x[a];
+        }
+        double ph[3][FRIBO_DIMSD], sumphi = 0.0;
+        for (int i = 0; i < FRIBO_DIMSD; ++i) {
+            ph[0][i] = 0.0;
+            ph[1][i] = 0.0;
+            ph[2][i] = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                const double b0 = pk[a][i] * w[i][0], b1 = pk[a][i] * w[i][1], b2 = pk[a][i] * w[i][2];
+                ph[0][i] = ph[0][i] + gam[a][0] * b0;
+                ph[1][i] = ph[1][i] + gam[a][1] * b0 + gam[a][0] * b1;
+                ph[2][i] = ph[2][i] + gam[a][2] * b0 + gam[a][0] * b2;
+            }
+        }
+        for (int i = 0; i < FRIBO_DIMSD; ++i) sumphi = sumphi + ph[0][i];
+        if (fabs(1.0 - fabs(sumphi)) > 1.0e-10) atomicCAS(&S->err, 0, FRIBO_E_PARTITION_UNITY);
+        // getpbup
+        double* uu = u[idir - 1];
+        for (int i = 0; i < 7; ++i) uu[i] = 0.0;
+        const double* f = (idir == 1) ? F.q2 : F.q3;
+        for (int i = 0; i < FRIBO_DIMSD; ++i) {
+            const int cy = eul[0][i], cz = eul[1][i];
+            double fc, fjm, fkm;
+            if (idir == 3) {
+                fc = prat(F, S, cy, cz);
+                fjm = prat(F, S, cy - 1, cz);
+                fkm = prat(F, S, cy, cz - 1);
+            } else {
+                fc = fat(F, f, cy, cz);
+                fjm = fat(F, f, cy - 1, cz);
+                fkm = fat(F, f, cy, cz - 1);
+            }
+            const double uk0 = fc;
+            const double uk1 = (fc - fjm) * F.dx2 / F.g2rc[cy];
+            const double uk2 = (fc - fkm) * F.dx3 / F.g3rc[cz];
+            for (int q = 0; q < 3; ++q) uu[q] = uu[q] + ph[q][i] * uk0;
+            uu[3] = uu[3] + ph[1][i] * uk1;
+            uu[4] = uu[4] + ph[1][i] * uk2;
+            uu[5] = uu[5] + ph[2][i] * uk1;
+            uu[6] = uu[6] + ph[2][i] * uk2;
+        }
+    }
+    const double* ucy = u[0];
+    const double* ucz = u[1];
+    const double* pc = u[2];
+    const double veln_md = ucy[0] * nor[0] + ucz[0] * nor[1];
+    const double vt0 = ucy[0] - veln_md * nor[0], vt1 = ucz[0] - veln_md * nor[1];
+    const double velt_md = sqrt(vt0 * vt0 + vt1 * vt1);
+    LW(n, 0) = 1.0;
+    if (velt_md != 0.0) {
+        const double nrm = sqrt(vt0 * vt0 + vt1 * vt1);
+        LW(n, 1) = 1.0;
+        LW(n, 2) = vt0 / nrm;
+        LW(n, 3) = vt1 / nrm;
+    }
+    double dpdnw = LMACC(n, 1) * nor[0] + LMACC(n, 2) * nor[1];
+    dpdnw = -1.0 * dpdnw;
+    const double dpdne = pc[1] * nor[0] + pc[2] * nor[1];
+    F.press[n - 1] = pc[0] - 0.5 * (dpdnw + dpdne) * hpr;
+    for (int q = 1; q < 7; ++q) {
+        LW(n, 3 + q) = ucy[q];
+        LW(n, 9 + q) = ucz[q];
+    }
+    const double nory = nor[0], norz = nor[1];
+    LW(n, 16) = (pc[3] * nory * nory + pc[4] * nory * norz + pc[5] * norz * nory + pc[6] * norz * norz) * 1.0 / F.visc;
+}
+
+// Tangent of every marker: its own, else the last one set before it
+// (loads_transfer.F90:94-96), by one wave with a "last set" scan over lane
+// chunks; then the friction stress (loads_transfer.F90:111-128).
+__global__ __launch_bounds__(64) void k_loads_tau(const FluidDev F, double hpr)
+{
+    DevScalars* S = F.S;
+    if (S->err) return;
+    const int lane = threadIdx.x, nlm = F.nlm;
+    const int per = (nlm + 63) / 64;
+    const int n0 = lane * per + 1, n1 = min(nlm, n0 + per - 1);
+    bool has = false;
+    double ty = 0.0, tz = 0.0;
+    for (int n = n0; n <= n1; ++n)
+        if (LW(n, 0) != 0.0 && LW(n, 1) != 0.0) {
+            has = true;
+            ty = LW(n, 2);
+            tz = LW(n, 3);
+        }
+    // inclusive scan of (has, t) with "later set wins", then shift by one lane
+    for (int h = 1; h < 64; h <<= 1) {
+        const int oh = __shfl_up((int)has, h, 64);
+        const double oy = __shfl_up(ty, h, 64), oz = __shfl_up(tz, h, 64);
+        if (lane >= h && !has && oh) {
+            has = true;
+            ty = oy;
+            tz = oz;
+        }
+    }
+    int ih = __shfl_up((int)has, 1, 64);
+    double cy = __shfl_up(ty, 1, 64), cz = __shfl_up(tz, 1, 64);
+    if (lane == 0 || !ih) {
+        cy = 0.0;
+        cz = 0.0;
+    }
+    const double visc = F.visc;
+    for (int n = n0; n <= n1; ++n) {
+        F.tau[(size_t)(n - 1) + nlm] = 0.0;
+        if (LW(n, 0) == 0.0) {
+            F.tau[n - 1] = 0.0;
+            continue;
+        }
+        if (LW(n, 1) != 0.0) {
+            cy = LW(n, 2);
+            cz = LW(n, 3);
+        }
+        const double nory = LMNOR(n, 1), norz = LMNOR(n, 2), tany = cy, tanz = cz;
+        const double u1 = LW(n, 4), u2 = LW(n, 5), u3 = LW(n, 6), u4 = LW(n, 7), u5 = LW(n, 8), u6 = LW(n, 9);
+        const double v1 = LW(n, 10), v2 = LW(n, 11), v3 = LW(n, 12), v4 = LW(n, 13), v5 = LW(n, 14), v6 = LW(n, 15);
+        const double d3ucdn3 = LW(n, 16);
+        const double ducdn = (u1 * tany * nory + u2 * tany * norz + v1 * tanz * nory + v2 * tanz * norz);
+        const double d2ucdn2 = (u3 * tany * nory * nory + u4 * tany * nory * norz + u5 * tany * norz * nory +
+                                u6 * tany * norz * norz + v3 * tanz * nory * nory + v4 * tanz * nory * norz +
+                                v5 * tanz * norz * nory + v6 * tanz * norz * norz);
+        const double ducdnm = ducdn - d2ucdn2 * hpr - 0.5 * d3ucdn3 * (hpr * hpr);
+        F.tau[n - 1] = visc * ducdnm * tany;
+        F.tau[(size_t)(n - 1) + nlm] = visc * ducdnm * tanz;
+    }
+}
+
+void launch_loads(const FluidDev& F, double hpr, hipStream_t st)
+{
+    if (F.nlm <= 0) return;
+    hipLaunchKernelGGL(k_loads, dim3((F.nlm + 63) / 64), dim3(64), 0, st, F, hpr);
+    hipLaunchKernelGGL(k_loads_tau, dim3(1), dim3(64), 0, st, F, hpr);
+}
+
+// ckbcs.F90:5-86: velocity error at the markers after the step and the norms
+// sqrt(sum err^2)/nlm (fixed-order tree sum); nb[0..1] receive the norms.
+__global__ __launch_bounds__(1024) void k_checkbcs(const FluidDev F, double* nb)
+{
+    __shared__ double sv[1024], sw[1024];
+    const int t = threadIdx.x;
+    double av = 0.0, aw = 0.0;
+    for (int np = 1 + t; np <= F.nlm; np += 1024) {
+        double ev = 0.0, ew = 0.0;
+        const double z = LMCC(np, 2);
+        if (F.zz[1] < z && z < F.zz[F.n3]) {
+            double ucy = 0.0, ucz = 0.0;
+            for (int m = 1; m <= FRIBO_DIMSD; ++m)
+                ucy = ucy + PHIPRED(np, 1, m) * F.q2[fidx(F, EULINS(np, 1, 1, m), EULINS(np, 1, 2, m))];
+            ev = LMVEL(np, 1) - ucy;
+            for (int m = 1; m <= FRIBO_DIMSD; ++m)
+                ucz = ucz + PHIPRED(np, 2, m) * F.q3[fidx(F, EULINS(np, 2, 1, m), EULINS(np, 2, 2, m))];
+            ew = LMVEL(np, 2) - ucz;
+        }
+        F.bcerr[np - 1] = ev;
+        F.bcerr[(size_t)(np - 1) + F.nlm] = ew;
+        av = av + ev * ev;
+        aw = aw + ew * ew;
+    }
+    sv[t] = av;
+    sw[t] = aw;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h) {
+            sv[t] = sv[t] + sv[t + h];
+            sw[t] = sw[t] + sw[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        nb[0] = sqrt(sv[0]) / (double)F.nlm;
+        nb[1] = sqrt(sw[0]) / (double)F.nlm;
+    }
+}
+
+void launch_checkbcs(const FluidDev& F, double* nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_checkbcs, dim3(1), dim3(1024), 0, st, F, nb);
 }

@@ -115,6 +115,31 @@
           type(c_ptr), value :: h
         end function fribo_sync
 
+        ! loads_transfer_c (loads_transfer.F90:4-162) on the device
+        integer(c_int) function fribo_loads_transfer(h, kprobe, lmnor, lmacc, press, tau) &
+            bind(c, name="fribo_loads_transfer")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), value :: kprobe
+          real(c_double), intent(in) :: lmnor(*), lmacc(*)
+          real(c_double), intent(out) :: press(*), tau(*)
+        end function fribo_loads_transfer
+
+        ! checkbcs (ckbcs.F90:5-86) on the device
+        integer(c_int) function fribo_checkbcs(h, bcerrv, bcerrw, nbcv, nbcw) bind(c, name="fribo_checkbcs")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: h
+          real(c_double), intent(out) :: bcerrv(*), bcerrw(*)
+          real(c_double), intent(out) :: nbcv, nbcw
+        end function fribo_checkbcs
+
+        ! axis of the Poisson eigen-transform: 0 z (reference order), 1 y, -1 auto
+        integer(c_int) function fribo_set_transform(h, axis) bind(c, name="fribo_set_transform")
+          import :: c_int, c_ptr, c_int32_t
+          type(c_ptr), value :: h
+          integer(c_int32_t), value :: axis
+        end function fribo_set_transform
+
         type(c_ptr) function fribo_last_error() bind(c, name="fribo_last_error")
           import :: c_ptr
         end function fribo_last_error

@@ -26,7 +26,8 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_fluid_tsch", "fribo_cfl_dt", "fribo_substep", "fribo_array_size", "fribo_get_array",
            "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_set_profiling", "fribo_get_profile",
            "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
-           "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform"]
+           "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform", "fribo_loads_transfer",
+           "fribo_checkbcs"]
 
 NPROF = 12
 
@@ -89,6 +90,8 @@ def load():
     L.fribo_set_transform.argtypes = [vp, C.c_int32]
     L.fribo_get_transform.argtypes = [vp]
     L.fribo_test_ytransform.argtypes = [C.c_int32, dp, dp, dp, dp, dp, dp]
+    L.fribo_loads_transfer.argtypes = [vp, C.c_double, dp, dp, dp, dp]
+    L.fribo_checkbcs.argtypes = [vp, dp, dp, dp, dp]
     _lib = L
     return L
 
@@ -199,6 +202,22 @@ class FluidStep:
         if n == n2 * n3m and name in ("ru2", "ru3", "rhs2", "rhs3"):
             return out.reshape((n2, n3m), order="F")
         return out
+
+    def loads_transfer(self, kprobe, lmnor, lmacc):
+        """loads_transfer_c after the last step: (press (nlm,), tau (nlm, 2))."""
+        lmnor = np.asfortranarray(lmnor, dtype=np.float64)
+        lmacc = np.asfortranarray(lmacc, dtype=np.float64)
+        press = np.zeros(self.nlm)
+        tau = np.zeros((self.nlm, 2), order="F")
+        _check(load().fribo_loads_transfer(self.h, float(kprobe), _dp(lmnor), _dp(lmacc), _dp(press), _dp(tau)))
+        return press, tau
+
+    def checkbcs(self):
+        """checkbcs after the last step: (bcerrv, bcerrw, nbcv, nbcw)."""
+        bv, bw = np.zeros(self.nlm), np.zeros(self.nlm)
+        nv, nw = C.c_double(0.0), C.c_double(0.0)
+        _check(load().fribo_checkbcs(self.h, _dp(bv), _dp(bw), C.byref(nv), C.byref(nw)))
+        return bv, bw, nv.value, nw.value
 
     def set_transform(self, axis):
         """Poisson eigen-transform axis: "z" (reference order), "y" or "auto"."""

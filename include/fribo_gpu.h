@@ -111,6 +111,24 @@ int fribo_get_array(fribo_handle h, const char* name, void* out, long n);
 
 int fribo_sync(fribo_handle h);
 
+/* Loads on the body after a step: SUBROUTINE loads_transfer_c(ns)
+ * (loads_transfer.F90:4-162, called at fluid_tsch.F90:175-177).  Probes at
+ * ddmin*kprobe^2 along the markers' normals (ddmin = smallest node spacing,
+ * cordin.F90:217-233; kprobe from global_par.in), MLS shape functions with
+ * derivatives (MLS_SF_d, MLSint.F90:173-258), pressure press(nlm) and
+ * friction stress tau(nlm,2) at every marker -- the press_r / tau_r that
+ * rigid() integrates.  lmnor, lmacc (nlm,2) as rigid() leaves them
+ * (rigid.F90:104,123).  Uses the markers of the last step.  Returns
+ * FRIBO_ERR_PARTITION_UNITY when a probe's MLS fit fails
+ * (MLSint.F90:249-255). */
+int fribo_loads_transfer(fribo_handle h, double kprobe, const double* lmnor, const double* lmacc,
+                         double* press, double* tau);
+
+/* SUBROUTINE checkbcs (ckbcs.F90:5-86, called at fluid_tsch.F90:169): the
+ * marker velocity error after the step, bcerrv/bcerrw (nlm) (NULL to skip),
+ * and nbcv/nbcw = norm2(err)/nlm. */
+int fribo_checkbcs(fribo_handle h, double* bcerrv, double* bcerrw, double* nbcv, double* nbcw);
+
 /* Execution option (no reference counterpart): with on != 0 the step's
  * kernels are captured once into a HIP graph and replayed by every
  * fribo_fluid_tsch call (re-captured when the marker count changes);

@@ -1386,6 +1386,299 @@ int orc_set(orc_state *s, const char *name, const void *in)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Loads on the body and the forcing check (called after the step by
+ * fluid_tsch.F90:167-177 when ibact/cbody are set).                         */
+
+/* MLSint.F90:492-585 (weight_function_d): W and its derivatives w.r.t. the
+ * probe position, w[i][0] = W, w[i][1] = dW/dy, w[i][2] = dW/dz. */
+static void weight_function_d(const double dif[2][DIMSD], const double sdd[2][DIMSD], double w[DIMSD][3],
+                              int wfun, double radiusSD)
+{
+    const double wcon = 0.3, eps = 1.0e-20;
+    for (int i = 0; i < DIMSD; ++i) {
+        double wc[2] = {0.0, 0.0}, dwcdc[2] = {0.0, 0.0};
+        for (int k = 0; k < 2; ++k) {
+            const double difc = dif[k][i], sdds = radiusSD * sdd[k][i];
+            const double drdc = (fabs(difc) <= eps) ? 0.0 : (difc / fabs(difc)) / sdds;
+            const double rcv = fabs(difc) / sdds;
+            const double rc2 = rcv * rcv, rc3 = rcv * rcv * rcv, rc4 = rcv * rcv * rcv * rcv;
+            if (wfun == 1) {
+                if (rcv > 1.0) { wc[k] = 0.0; dwcdc[k] = 0.0; }
+                else if (rcv <= 0.5) {
+                    wc[k] = (2.0 / 3.0) - 4.0 * rc2 + 4.0 * rc3;
+                    dwcdc[k] = (-8.0 * rcv + 12.0 * rc2) * drdc;
+                } else {
+                    wc[k] = (4.0 / 3.0) - 4.0 * rcv + 4.0 * rc2 - (4.0 / 3.0) * rc3;
+                    dwcdc[k] = (-4.0 + 8.0 * rcv - 4.0 * rc2) * drdc;
+                }
+            } else if (wfun == 2) {
+                if (rcv <= 1.0) {
+                    const double t = rcv / wcon;
+                    wc[k] = exp(-(t * t));
+                    dwcdc[k] = (-1.0 / (wcon * wcon) * 2.0 * rcv * exp(-(t * t))) * drdc;
+                }
+            } else if (wfun == 3) {
+                if (rcv <= 1.0) {
+                    wc[k] = 1.0 - 6.0 * rc2 + 8.0 * rc3 - 3.0 * rc4;
+                    dwcdc[k] = -12.0 * rcv + 24.0 * rc2 - 12.0 * rc3;   /* no drdc, as the reference */
+                }
+            } else if (wfun == 4) {
+                if (rcv > 1.0) { wc[k] = 0.0; dwcdc[k] = 0.0; }
+                else if (rcv < 0.5) {
+                    wc[k] = 1.0 / 3.0 * (1.0 + sqrt(-3.0 * rc2 + 1.0));
+                    dwcdc[k] = -rcv * pow(-3.0 * rc2 - 1.0, -0.5);    /* NaN, as the reference */
+                } else {
+                    const double om = 1.0 - rcv;
+                    wc[k] = 1.0 / 6.0 * (5.0 - 3.0 * rcv - sqrt(-3.0 * (om * om) + 1.0));
+                    dwcdc[k] = -(1.0 - rcv) / (2.0 * sqrt(1.0 - 3.0 * (om * om))) - 0.5;
+                }
+            }
+        }
+        w[i][0] = wc[0] * wc[1];
+        w[i][1] = wc[1] * dwcdc[0];
+        w[i][2] = wc[0] * dwcdc[1];
+    }
+}
+
+/* MLSint.F90:173-258 (MLS_SF_d) with mls_basis_func (264-280) and
+ * compute_AB_d (342-402): phid[0][i] = phi_i, phid[1][i] = dphi_i/dy,
+ * phid[2][i] = dphi_i/dz at the probe.  Returns ORC_ERR_PARTITION_UNITY when
+ * |1-|sum phi|| > 1e-10 (MLSint.F90:249-255). */
+static int mls_sf_d(orc_state *s, const double pos[2], const double sdc[2][DIMSD], const double sdd[2][DIMSD],
+                    double phid[3][DIMSD])
+{
+    double dif[2][DIMSD], w[DIMSD][3], pk[3][DIMSD], B[3][DIMSD][3], A[3][3][3], Aa[3][3];
+    double gp[3][3], gam[3][3] = {{0.0}}, C[3], x[3];
+    int i, j, k, l;
+    for (i = 0; i < DIMSD; ++i) {
+        pk[0][i] = 1.0;
+        pk[1][i] = sdc[0][i];
+        pk[2][i] = sdc[1][i];
+        dif[0][i] = pos[0] - sdc[0][i];
+        dif[1][i] = pos[1] - sdc[1][i];
+    }
+    weight_function_d(dif, sdd, w, s->p.wfun, s->p.radiusSD);
+    for (i = 0; i < 3; ++i)
+        for (j = 0; j < DIMSD; ++j)
+            for (k = 0; k < 3; ++k) B[i][j][k] = pk[i][j] * w[j][k];
+    for (j = 0; j < 3; ++j)
+        for (k = 0; k < 3; ++k)
+            for (l = 0; l < 3; ++l) A[j][k][l] = 0.0;
+    for (i = 0; i < DIMSD; ++i)
+        for (j = 0; j < 3; ++j)
+            for (k = 0; k < 3; ++k) {
+                const double pp = pk[j][i] * pk[k][i];
+                for (l = 0; l < 3; ++l) A[j][k][l] = A[j][k][l] + w[i][l] * pp;
+            }
+    /* gp: identity with gp(1,2:3) = pos */
+    for (i = 0; i < 3; ++i)
+        for (j = 0; j < 3; ++j) gp[i][j] = (i == j) ? 1.0 : 0.0;
+    gp[0][1] = pos[0];
+    gp[0][2] = pos[1];
+    for (j = 0; j < 3; ++j)
+        for (k = 0; k < 3; ++k) Aa[j][k] = A[j][k][0];
+    x[0] = x[1] = x[2] = 0.0;
+    gesolve3((const double(*)[3])Aa, gp[0], x);
+    for (j = 0; j < 3; ++j) gam[j][0] = x[j];
+    for (l = 1; l <= 2; ++l) {
+        /* C = gp(l+1,:) - matmul(A(:,:,l+1), gam(:,1)) */
+        for (i = 0; i < 3; ++i) {
+            double m = 0.0;
+            for (k = 0; k < 3; ++k) m = m + A[i][k][l] * gam[k][0];
+            C[i] = gp[l][i] - m;
+        }
+        x[0] = x[1] = x[2] = 0.0;
+        gesolve3((const double(*)[3])Aa, C, x);
+        for (j = 0; j < 3; ++j) gam[j][l] = x[j];
+    }
+    double sumphi = 0.0;
+    for (i = 0; i < DIMSD; ++i) {
+        phid[0][i] = 0.0;
+        phid[1][i] = 0.0;
+        phid[2][i] = 0.0;
+        for (j = 0; j < 3; ++j) {
+            phid[0][i] = phid[0][i] + gam[j][0] * B[j][i][0];
+            phid[1][i] = phid[1][i] + gam[j][1] * B[j][i][0] + gam[j][0] * B[j][i][1];
+            phid[2][i] = phid[2][i] + gam[j][2] * B[j][i][0] + gam[j][0] * B[j][i][2];
+        }
+    }
+    for (i = 0; i < DIMSD; ++i) sumphi = sumphi + phid[0][i];
+    if (fabs(1.0 - fabs(sumphi)) > 1.0e-10) return ORC_ERR_PARTITION_UNITY;
+    return ORC_OK;
+}
+
+/* loads_transfer.F90:168-206 (probeind): first minimum of |x - p| over the
+ * 11 nodes around the marker's own node; nodes outside the grid array are
+ * never selected (the reference reads past its array there). */
+static int window_minloc(const double *x, int n, int c, double v)
+{
+    int best = 0;
+    double bv = 0.0;
+    for (int i = c - 5; i <= c + 5; ++i) {
+        if (i < 1 || i > n) continue;
+        const double a = fabs(x[i] - v);
+        if (best == 0 || a < bv) { bv = a; best = i; }
+    }
+    return best;
+}
+
+static void probeind(orc_state *s, const int markind[2], const double probe[2], int ind[2], int idir)
+{
+    const double *y = (idir == 1) ? s->rc : s->rm;
+    const double *z = (idir == 2) ? s->zz : s->zm;
+    ind[0] = window_minloc(y, s->n2, markind[0], probe[0]);
+    ind[1] = window_minloc(z, s->n3, markind[1], probe[1]);
+}
+
+/* MLS_SD_ind (MLSint.F90:4-104) for all three grids: idir 3 is the pressure
+ * grid (rm, zm). */
+static void mls_sd_ind3(orc_state *s, int idir, const int indse[2], double posk[2][DIMSD], double dsk[2][DIMSD],
+                        int eul[2][DIMSD])
+{
+    if (idir != 3) {
+        mls_sd_ind(s, idir, indse, posk, dsk, eul);
+        return;
+    }
+    int j, k, jj, kk, m = 0;
+    for (j = 1; j <= 3; ++j)
+        for (k = 1; k <= 3; ++k) {
+            jj = (indse[0] == 1) ? j + 1 : (indse[0] == s->n2m ? j - 1 : j);
+            kk = (indse[1] == 1) ? k + 1 : (indse[1] == s->n3m ? k - 1 : k);
+            const int ej = indse[0] + jj - 2, ek = indse[1] + kk - 2;
+            posk[0][m] = s->rm[ej];
+            dsk[0][m] = s->rm[ej + 1] - s->rm[ej];
+            posk[1][m] = s->zm[ek];
+            dsk[1][m] = s->zm[ek + 1] - s->zm[ek];
+            eul[0][m] = ej;
+            eul[1][m] = ek;
+            ++m;
+        }
+}
+
+/* loads_transfer.F90:210-267 (getpbup): value and first/second derivative
+ * combinations of q2, q3, pr at the probes.  The backward differences at
+ * j = 1 read f(0,k), which in the reference's column-major storage is
+ * f(n2,k-1); GX reproduces that address. */
+static void getpbup(orc_state *s, const int eul[3][2][DIMSD], const double phi[3][3][DIMSD], double ucy[7],
+                    double ucz[7], double pc[7])
+{
+    const double *fld[3] = {s->q2, s->q3, s->pr};
+    double *out[3] = {ucy, ucz, pc};
+    for (int d = 0; d < 3; ++d) {
+        double *u = out[d];
+        for (int i = 0; i < 7; ++i) u[i] = 0.0;
+        for (int n = 0; n < DIMSD; ++n) {
+            const int cy = eul[d][0][n], cz = eul[d][1][n];
+            const double *f = fld[d];
+            double uk[3];
+            uk[0] = f[GX(s, cy, cz)];
+            uk[1] = (f[GX(s, cy, cz)] - f[GX(s, cy - 1, cz)]) * s->dx2 / s->g2rc[cy];
+            uk[2] = (f[GX(s, cy, cz)] - f[GX(s, cy, cz - 1)]) * s->dx3 / s->g3rc[cz];
+            for (int i = 0; i < 3; ++i) u[i] = u[i] + phi[d][i][n] * uk[0];
+            for (int i = 0; i < 2; ++i) u[i + 3] = u[i + 3] + phi[d][1][n] * uk[1 + i];
+            for (int i = 0; i < 2; ++i) u[i + 5] = u[i + 5] + phi[d][2][n] * uk[1 + i];
+        }
+    }
+}
+
+/* loads_transfer.F90:4-162 (loads_transfer_c) on one rank.  lmnor, lmacc are
+ * (nlm,2) as rigid() leaves them; press (nlm) and tau (nlm,2) out.  hpr is
+ * ddmin*kprobe with ddmin already scaled by kprobe in cordin.F90:217-233.
+ * lmtan keeps its value from the previous marker when the probe velocity
+ * has no tangential part (loads_transfer.F90:94-96). */
+int orc_loads_transfer(orc_state *s, double kprobe, const double *lmnor, const double *lmacc, double *press,
+                       double *tau)
+{
+    int n, nlm = s->nlm, rc = ORC_OK;
+    double dymin = 1.0e+6, dzmin = 1.0e+6, lmtan[2] = {0.0, 0.0};
+    for (int i = 2; i <= s->n2; ++i) dymin = fmin(dymin, s->rc[i] - s->rc[i - 1]);
+    for (int i = 2; i <= s->n3; ++i) dzmin = fmin(dzmin, s->zz[i] - s->zz[i - 1]);
+    const double ddmin = fmin(dzmin, dymin) * kprobe;
+    const double hpr = ddmin * kprobe, visc = s->p.visc;
+#define NOR(n, c) lmnor[(size_t)((n) - 1) + (size_t)((c) - 1) * nlm]
+#define ACC(n, c) lmacc[(size_t)((n) - 1) + (size_t)((c) - 1) * nlm]
+    for (n = 1; n <= nlm; ++n) {
+        press[n - 1] = 0.0;
+        tau[n - 1] = 0.0;
+        tau[n - 1 + nlm] = 0.0;
+    }
+    for (n = 1; n <= nlm; ++n) {
+        const double pos[2] = {LMCC(n, 1), LMCC(n, 2)};
+        const double probe[2] = {pos[0] + NOR(n, 1) * hpr, pos[1] + NOR(n, 2) * hpr};
+        if (!(s->zz[1] < pos[1] && pos[1] < s->zz[s->n3m + 1])) continue;
+        int eul[3][2][DIMSD];
+        double phi[3][3][DIMSD];
+        for (int idir = 1; idir <= 3; ++idir) {
+            int tmp[2], indse[2];
+            double sdc[2][DIMSD], sdd[2][DIMSD];
+            if (idir == 3) { tmp[0] = LMIND(n, 2, 1); tmp[1] = LMIND(n, 1, 2); }
+            else { tmp[0] = LMIND(n, idir, 1); tmp[1] = LMIND(n, idir, 2); }
+            probeind(s, tmp, probe, indse, idir);
+            mls_sd_ind3(s, idir, indse, sdc, sdd, eul[idir - 1]);
+            const int e = mls_sf_d(s, probe, (const double(*)[DIMSD])sdc, (const double(*)[DIMSD])sdd,
+                                   phi[idir - 1]);
+            if (e && rc == ORC_OK) rc = e;
+        }
+        double ucy[7], ucz[7], pc[7];
+        getpbup(s, (const int(*)[2][DIMSD])eul, (const double(*)[3][DIMSD])phi, ucy, ucz, pc);
+        const double vel[2] = {ucy[0], ucz[0]};
+        const double veln_md = vel[0] * NOR(n, 1) + vel[1] * NOR(n, 2);
+        const double velt[2] = {vel[0] - veln_md * NOR(n, 1), vel[1] - veln_md * NOR(n, 2)};
+        const double velt_md = sqrt(velt[0] * velt[0] + velt[1] * velt[1]);
+        if (velt_md != 0.0) {
+            const double nrm = sqrt(velt[0] * velt[0] + velt[1] * velt[1]);
+            lmtan[0] = velt[0] / nrm;
+            lmtan[1] = velt[1] / nrm;
+        }
+        const double nory = NOR(n, 1), norz = NOR(n, 2), tany = lmtan[0], tanz = lmtan[1];
+        double dpdnw = ACC(n, 1) * NOR(n, 1) + ACC(n, 2) * NOR(n, 2);
+        dpdnw = -1.0 * dpdnw;
+        const double dpdne = pc[1] * NOR(n, 1) + pc[2] * NOR(n, 2);
+        press[n - 1] = pc[0] - 0.5 * (dpdnw + dpdne) * hpr;
+        const double ducdn = (ucy[1] * tany * nory + ucy[2] * tany * norz + ucz[1] * tanz * nory + ucz[2] * tanz * norz);
+        const double d3ucdn3 = (pc[3] * nory * nory + pc[4] * nory * norz + pc[5] * norz * nory + pc[6] * norz * norz) *
+                               1.0 / visc;
+        const double d2ucdn2 = (ucy[3] * tany * nory * nory + ucy[4] * tany * nory * norz +
+                                ucy[5] * tany * norz * nory + ucy[6] * tany * norz * norz +
+                                ucz[3] * tanz * nory * nory + ucz[4] * tanz * nory * norz +
+                                ucz[5] * tanz * norz * nory + ucz[6] * tanz * norz * norz);
+        const double ducdnm = ducdn - d2ucdn2 * hpr - 0.5 * d3ucdn3 * (hpr * hpr);
+        tau[n - 1] = visc * ducdnm * lmtan[0];
+        tau[n - 1 + nlm] = visc * ducdnm * lmtan[1];
+    }
+#undef NOR
+#undef ACC
+    return rc;
+}
+
+/* ckbcs.F90:5-86 (checkbcs): velocity error at the markers after the step,
+ * interpolated with phipred over the forcing supports; nbcv/nbcw are the
+ * Euclidean norms divided by nlm. */
+int orc_checkbcs(orc_state *s, double *bcerrv, double *bcerrw, double *nbcv, double *nbcw)
+{
+    int np, n, nlm = s->nlm;
+    double sv = 0.0, sw = 0.0;
+    for (np = 1; np <= nlm; ++np) {
+        bcerrv[np - 1] = 0.0;
+        bcerrw[np - 1] = 0.0;
+        const double z = LMCC(np, 2);
+        if (!(s->zz[1] < z && z < s->zz[s->n3m + 1])) continue;
+        double ucy = 0.0, ucz = 0.0;
+        for (n = 1; n <= DIMSD; ++n) ucy = ucy + PHIPRED(np, 1, n) * Q2(EULINS(np, 1, 1, n), EULINS(np, 1, 2, n));
+        bcerrv[np - 1] = LMVEL(np, 1) - ucy;
+        for (n = 1; n <= DIMSD; ++n) ucz = ucz + PHIPRED(np, 2, n) * Q3(EULINS(np, 2, 1, n), EULINS(np, 2, 2, n));
+        bcerrw[np - 1] = LMVEL(np, 2) - ucz;
+    }
+    for (np = 0; np < nlm; ++np) {
+        sv = sv + bcerrv[np] * bcerrv[np];
+        sw = sw + bcerrw[np] * bcerrw[np];
+    }
+    *nbcv = sqrt(sv) / (double)nlm;
+    *nbcw = sqrt(sw) / (double)nlm;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Test hooks: the restated LAPACK / GEsolve / Poisson pieces in isolation.  */
 int orc_test_dgtsv(int n, const double *dl0, const double *d0, const double *du0, double *b0)
 {

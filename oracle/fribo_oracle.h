@@ -74,6 +74,15 @@ int orc_fluid_tsch(orc_state *s, int ns, double *dt, double dt_o, double *cflm);
 int orc_cfl_dt(orc_state *s, int ns, double *dt, double dt_o, double *cflm);
 int orc_substep(orc_state *s, int n, int ns);
 
+/* loads_transfer_c (loads_transfer.F90:4-162) after a step: pressure
+ * press(nlm) and friction stress tau(nlm,2) at the markers from the probe
+ * values; lmnor, lmacc (nlm,2) as rigid() leaves them, kprobe from
+ * global_par.in.  checkbcs (ckbcs.F90:5-86): marker velocity errors and
+ * their norms.  Both use the marker data of the last step. */
+int orc_loads_transfer(orc_state *s, double kprobe, const double *lmnor, const double *lmacc, double *press,
+                       double *tau);
+int orc_checkbcs(orc_state *s, double *bcerrv, double *bcerrw, double *nbcv, double *nbcw);
+
 /* Test hooks: DGTTRF+DGTTRS on one system, GEsolve on a 3x3 (column-major),
  * and phcalc_outflow applied to a (n2m x n3m) column-major right-hand side. */
 int orc_test_dgtsv(int n, const double *dl, const double *d, const double *du, double *b);

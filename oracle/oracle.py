@@ -53,6 +53,8 @@ def _lib():
         L.orc_test_dgtsv.argtypes = [C.c_int, dp, dp, dp, dp]
         L.orc_test_gesolve.argtypes = [dp, dp, dp]
         L.orc_test_phcalc.argtypes = [vp, dp]
+        L.orc_loads_transfer.argtypes = [vp, C.c_double, dp, dp, dp, dp]
+        L.orc_checkbcs.argtypes = [vp, dp, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -156,6 +158,24 @@ class Oracle:
 
     def substep(self, n, ns):
         return _lib().orc_substep(self.h, n, ns)
+
+    def loads_transfer(self, kprobe, lmnor, lmacc):
+        """loads_transfer_c after a step: (rc, press (nlm,), tau (nlm, 2))."""
+        lmnor = np.asfortranarray(lmnor, dtype=np.float64)
+        lmacc = np.asfortranarray(lmacc, dtype=np.float64)
+        nlm = lmnor.shape[0]
+        press = np.zeros(nlm)
+        tau = np.zeros((nlm, 2), order="F")
+        rc = _lib().orc_loads_transfer(self.h, float(kprobe), _dp(lmnor), _dp(lmacc), _dp(press), _dp(tau))
+        return rc, press, tau
+
+    def checkbcs(self):
+        """checkbcs after a step: (bcerrv, bcerrw, nbcv, nbcw)."""
+        nlm = self._mk[2].size
+        bv, bw = np.zeros(nlm), np.zeros(nlm)
+        nv, nw = C.c_double(0.0), C.c_double(0.0)
+        _lib().orc_checkbcs(self.h, _dp(bv), _dp(bw), C.byref(nv), C.byref(nw))
+        return bv, bw, nv.value, nw.value
 
 
 def dgtsv(dl, d, du, b):

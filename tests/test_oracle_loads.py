@@ -1,0 +1,114 @@
+"""Pins of the oracle's loads_transfer_c / checkbcs restatement (CPU).
+
+On a uniform grid the reference's one-sided differences of getpbup are exact
+for linear fields and the MLS fit reproduces linear functions and their first
+derivatives, so for q2 = a.(1,y,z), q3 = b.(1,y,z), pr = p.(1,y,z) every probe
+value has a closed form: pc = (p(probe), p_y, p_z, 0, 0, 0, 0), ucy and ucz
+likewise, hence
+    press = p(probe) - hpr/2 (-(acc.n) + p_y n_y + p_z n_z)
+    tau   = visc (a_y t_y n_y + a_z t_y n_z + b_y t_z n_y + b_z t_z n_z) t
+with t the unit tangential part of the probe velocity.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fribois2d_amd import cases
+from helpers import oracle_for
+
+A = (1.0, 0.30, -0.20)     # q2 = A0 + A1 y + A2 z
+B = (0.8, -0.15, 0.25)     # q3
+P = (0.5, 0.70, -0.40)     # pr
+
+
+def uniform_case():
+    n2, n3 = 41, 57
+    rc = np.linspace(0.0, 4.0, n2)
+    zz = np.linspace(0.0, 5.6, n3)
+    p = cases.FluidParams(n2=n2, n3=n3, alx3=5.6, rext2=4.0, visc=0.01, idtv=True)
+    verts = cases.cylinder_vertices(2.0, 2.4, 0.5, 40)
+    return cases.Case("uniform", p, rc, zz, verts, (2.0, 2.4), cases._heave(0.05, 0.5), 20261202)
+
+
+def linear_fields(c):
+    p = c.params
+    n2, n3m = p.n2, p.n3 - 1
+    rc, zz = c.rc, c.zz
+    rm = np.append((rc[:-1] + rc[1:]) * 0.5, rc[-1])
+    zm = np.append((zz[:-1] + zz[1:]) * 0.5, zz[-1])
+    ks = np.arange(-2, n3m + 4)
+    kc = np.clip(ks, 1, n3m + 1) - 1
+    Y2, Z2 = np.meshgrid(rc, zm[np.clip(ks, 1, n3m) - 1], indexing="ij")
+    Y3, Z3 = np.meshgrid(rm, zz[kc], indexing="ij")
+    Yp, Zp = np.meshgrid(rm, zm[np.clip(ks, 1, n3m) - 1], indexing="ij")
+    f = lambda C, y, z: np.asfortranarray(C[0] + C[1] * y + C[2] * z)
+    return f(A, Y2, Z2), f(B, Y3, Z3), f(P, Yp, Zp)
+
+
+def test_loads_closed_form_linear_fields():
+    c = uniform_case()
+    o = oracle_for(c)
+    lm = c.markers(0.0)
+    o.set_markers(*lm)
+    rc_, _, _ = o.fluid_tsch(0, 1e-3)
+    assert rc_ == 0
+    q2, q3, pr = linear_fields(c)
+    o.set("q2", q2)
+    o.set("q3", q3)
+    o.set("pr", pr)
+    nor, acc = c.surface(0.0)
+    kprobe = 1.3
+    rc_, press, tau = o.loads_transfer(kprobe, nor, acc)
+    assert rc_ == 0
+    h = min(np.diff(c.rc).min(), np.diff(c.zz).min())
+    hpr = h * kprobe * kprobe
+    lmcc = lm[0]
+    prb = lmcc + nor * hpr
+    pp = P[0] + P[1] * prb[:, 0] + P[2] * prb[:, 1]
+    dpdnw = -(acc[:, 0] * nor[:, 0] + acc[:, 1] * nor[:, 1])
+    dpdne = P[1] * nor[:, 0] + P[2] * nor[:, 1]
+    press_ref = pp - 0.5 * (dpdnw + dpdne) * hpr
+    np.testing.assert_allclose(press, press_ref, rtol=0, atol=1e-11)
+    uy = A[0] + A[1] * prb[:, 0] + A[2] * prb[:, 1]
+    uz = B[0] + B[1] * prb[:, 0] + B[2] * prb[:, 1]
+    vn = uy * nor[:, 0] + uz * nor[:, 1]
+    ty, tz = uy - vn * nor[:, 0], uz - vn * nor[:, 1]
+    tn = np.hypot(ty, tz)
+    ty, tz = ty / tn, tz / tn
+    ducdn = A[1] * ty * nor[:, 0] + A[2] * ty * nor[:, 1] + B[1] * tz * nor[:, 0] + B[2] * tz * nor[:, 1]
+    np.testing.assert_allclose(tau[:, 0], c.params.visc * ducdn * ty, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(tau[:, 1], c.params.visc * ducdn * tz, rtol=0, atol=1e-12)
+
+
+def test_checkbcs_vanishes_for_a_reproduced_linear_field():
+    c = uniform_case()
+    o = oracle_for(c)
+    lmcc, lmvel, lmA = c.markers(0.0)
+    q2, q3, _ = linear_fields(c)
+    vel = np.stack([A[0] + A[1] * lmcc[:, 0] + A[2] * lmcc[:, 1],
+                    B[0] + B[1] * lmcc[:, 0] + B[2] * lmcc[:, 1]], axis=1)
+    o.set_markers(lmcc, vel, lmA)
+    assert o.fluid_tsch(0, 1e-3)[0] == 0
+    o.set("q2", q2)
+    o.set("q3", q3)
+    bv, bw, nv, nw = o.checkbcs()
+    assert np.abs(bv).max() < 1e-12 and np.abs(bw).max() < 1e-12
+    assert nv < 1e-13 and nw < 1e-13
+    # a known error is reported in the norm: norm2(err)/nlm
+    o.set_markers(lmcc, vel + 0.01, lmA)
+    bv, bw, nv, nw = o.checkbcs()
+    np.testing.assert_allclose(bv, 0.01, atol=1e-12)
+    assert abs(nv - math.sqrt(lmA.size * 1e-4) / lmA.size) < 1e-12
+
+
+def test_createnorm_points_out_of_the_body():
+    c = cases.make_case("cyl64")
+    nor, acc = c.surface(0.0)
+    lmcc = c.markers(0.0)[0]
+    r = lmcc - np.array(c.center)
+    r /= np.linalg.norm(r, axis=1)[:, None]
+    assert np.abs(np.linalg.norm(nor, axis=1) - 1).max() < 1e-14
+    assert np.min(np.sum(nor * r, axis=1)) > 0.99
+    # heave: every marker shares the body acceleration
+    assert np.allclose(acc, acc[0])
