@@ -63,6 +63,8 @@ struct fribo_ctx {
     bool yavail = false;     // transform along y set up
     double dmin = 0.0;       // smallest node spacing of the grid (cordin.F90:217-231)
     double* nb = nullptr;    // device: the two checkbcs norms
+    double* dpart = nullptr; // device: diagnostics partials [8][256] and results [9]
+    int vort_k = 0;          // plane of getintvort's z = 18.5 line (0: none)
     hipGraphExec_t gexec = nullptr;
     int graph_nlm = -1;
 };
@@ -431,6 +433,15 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     if (!e) e = dalloc(c, &F.kX, (size_t)2 * FRIBO_KMAXCH * F.n2m);
     if (!e) e = dalloc(c, &F.S, 1);
     if (!e) e = dalloc(c, &c->nb, 2);
+    if (!e) e = dalloc(c, &c->dpart, 8 * 256 + 16);
+    {
+        // utils_proc.F90:600-602: first node nearest to z = 18.5
+        int indz = 1;
+        double best = fabs(zz[0] - 18.5);
+        for (int k = 2; k <= F.n3; ++k)
+            if (fabs(zz[k - 1] - 18.5) < best) { best = fabs(zz[k - 1] - 18.5); indz = k; }
+        c->vort_k = (indz <= F.n3m) ? indz : 0;
+    }
     {
         double dymin = 1.0e+6, dzmin = 1.0e+6;
         for (int i = 2; i <= F.n2; ++i) dymin = std::min(dymin, rc[i - 1] - rc[i - 2]);
@@ -1061,6 +1072,18 @@ extern "C" int fribo_checkbcs(fribo_handle h, double* bcerrv, double* bcerrw, do
     HIPCHK(hipStreamSynchronize(h->st));
     if (nbcv) *nbcv = v[0];
     if (nbcw) *nbcw = v[1];
+    return FRIBO_OK;
+}
+
+// ttime's diagnostics (ttime.F90:29-69,136) of the current state.
+extern "C" int fribo_diagnostics(fribo_handle h, double* out)
+{
+    if (!h || !out) return set_err(FRIBO_ERR_ARG, "null argument");
+    double* res = h->dpart + 8 * 256;
+    launch_diag(h->F, h->dpart, h->vort_k, res, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, res, 9 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
     return FRIBO_OK;
 }
 

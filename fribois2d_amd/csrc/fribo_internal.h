@@ -165,6 +165,7 @@ void launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double*
 void launch_ysolve(const FluidDev& F, hipStream_t st);
 void launch_loads(const FluidDev& F, double hpr, hipStream_t st);
 void launch_checkbcs(const FluidDev& F, double* nb, hipStream_t st);
+void launch_diag(const FluidDev& F, double* part, int kv, double* out, hipStream_t st);
 void launch_ygauge(const FluidDev& F, hipStream_t st);
 int ytransform_build(int n, const double* am, const double* ac, const double* ap, int ld, double* Y, double* Yinv,
                      double* wy);

@@ -666,3 +666,107 @@ void launch_mean(const FluidDev& F, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mean, dim3(1), dim3(1024), 0, st, F);
 }
+
+// ---------------------------------------------------------------------------
+// ttime's per-step diagnostics (ttime.F90:29-69,136) on the device: one pass
+// over q2/q3 gives vmaxv/vminv (vcheck.F90:3-73), divgck
+// (divergence.F90:49-78), kinen (utils_proc.F90:474-515) and cfl
+// (cfl.F90:19-38) as per-block partials; k_diag_final reduces them in a
+// fixed order and adds getintvort (utils_proc.F90:582-629) on the plane kv
+// nearest to z = 18.5 (0: outside the grid).  out[9] = {vmax2, vmax3, vmin2,
+// vmin3, max|div|, ektot, vorticity integral, cflm, q3avg}.
+#define DIAG_NB 256
+__global__ __launch_bounds__(256) void k_diag(const FluidDev F, double* part)
+{
+    __shared__ double sh[256];
+    const int t = threadIdx.x;
+    const int per = (F.n3m + DIAG_NB - 1) / DIAG_NB;
+    const int k0 = blockIdx.x * per + 1, k1 = min(k0 + per - 1, F.n3m);
+    double v2 = -100.0, v3 = -100.0, m2 = 100.0, m3 = 100.0, qm = -100.0, ek = 0.0, cf = 1.0e-8, qs = 0.0;
+    for (int k = k0; k <= k1; ++k) {
+        const double u3 = F.udx3m[k], dz = F.zz[k + 1] - F.zz[k];
+        for (int j = 1 + t; j <= F.n2m; j += 256) {
+            const size_t c = fidx(F, j, k);
+            const double a2 = F.q2[c], b2 = F.q2[c + 1], a3 = F.q3[c], b3 = F.q3[c + F.P];
+            v2 = fmax(v2, fabs(a2));
+            v3 = fmax(v3, fabs(a3));
+            m2 = fmin(m2, a2);
+            m3 = fmin(m3, a3);
+            const double dq = (b2 - a2) * F.udx2m[j] + (b3 - a3) * u3;
+            qm = fmax(fabs(dq), qm);
+            const double dy = F.rc[j + 1] - F.rc[j];
+            const double vol = dz * dy;
+            const double vccv = (a2 + b2) * 0.5, vccw = (a3 + b3) * 0.5;
+            ek = ek + (vccv * vccv + vccw * vccw) * vol;
+            cf = fmax(cf, fabs((a2 + b2) * 0.5 * F.udx2m[j]) + fabs((a3 + b3) * 0.5 * u3));
+            qs = qs + a3 * dz * dy;
+        }
+    }
+    double vals[8] = {v2, v3, m2, m3, qm, ek, cf, qs};
+    for (int q = 0; q < 8; ++q) {
+        sh[t] = vals[q];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (t < h) {
+                const double a = sh[t], b = sh[t + h];
+                sh[t] = (q == 2 || q == 3) ? fmin(a, b) : ((q == 5 || q == 7) ? a + b : fmax(a, b));
+            }
+            __syncthreads();
+        }
+        if (t == 0) part[q * DIAG_NB + blockIdx.x] = sh[0];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_diag_final(const FluidDev F, const double* part, int kv, double* out)
+{
+    __shared__ double sh[256];
+    const int t = threadIdx.x;
+    double r[8];
+    for (int q = 0; q < 8; ++q) {
+        sh[t] = part[q * DIAG_NB + t];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (t < h) {
+                const double a = sh[t], b = sh[t + h];
+                sh[t] = (q == 2 || q == 3) ? fmin(a, b) : ((q == 5 || q == 7) ? a + b : fmax(a, b));
+            }
+            __syncthreads();
+        }
+        r[q] = sh[0];
+        __syncthreads();
+    }
+    double vi = 0.0;
+    if (kv > 0) {
+        const double dz = F.zz[kv + 1] - F.zz[kv];
+        for (int j = 1 + t; j <= F.n2m; j += 256) {
+            const double dyj = F.rc[j + 1] - F.rc[j];
+            const double dwdy = (F.q3[fidx(F, j + 1, kv)] - F.q3[fidx(F, j, kv)]) / dyj;
+            const double dvdz = (F.q2[fidx(F, j, kv + 1)] - F.q2[fidx(F, j, kv)]) / dz;
+            vi = vi + (dwdy - dvdz) * dyj;
+        }
+    }
+    sh[t] = vi;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) sh[t] = sh[t] + sh[t + h];
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = r[0];
+        out[1] = r[1];
+        out[2] = r[2];
+        out[3] = r[3];
+        out[4] = r[4];
+        out[5] = 0.5 * r[5] / (F.rext2 * F.alx3);
+        out[6] = sh[0];
+        out[7] = r[6];
+        out[8] = r[7] / (F.rext2 * F.alx3);
+    }
+}
+
+void launch_diag(const FluidDev& F, double* part, int kv, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_diag, dim3(DIAG_NB), dim3(256), 0, st, F, part);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, F, (const double*)part, kv, out);
+}

@@ -27,9 +27,11 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_set_profiling", "fribo_get_profile",
            "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
            "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform", "fribo_loads_transfer",
-           "fribo_checkbcs"]
+           "fribo_checkbcs", "fribo_diagnostics"]
 
 NPROF = 12
+
+DIAG_KEYS = ("vmax2", "vmax3", "vmin2", "vmin3", "divmax", "ektot", "vorti", "cflm", "q3avg")
 
 
 class FriboError(RuntimeError):
@@ -92,6 +94,7 @@ def load():
     L.fribo_test_ytransform.argtypes = [C.c_int32, dp, dp, dp, dp, dp, dp]
     L.fribo_loads_transfer.argtypes = [vp, C.c_double, dp, dp, dp, dp]
     L.fribo_checkbcs.argtypes = [vp, dp, dp, dp, dp]
+    L.fribo_diagnostics.argtypes = [vp, dp]
     _lib = L
     return L
 
@@ -211,6 +214,12 @@ class FluidStep:
         tau = np.zeros((self.nlm, 2), order="F")
         _check(load().fribo_loads_transfer(self.h, float(kprobe), _dp(lmnor), _dp(lmacc), _dp(press), _dp(tau)))
         return press, tau
+
+    def diagnostics(self):
+        """ttime's per-step diagnostics (dict, same keys as the oracle's)."""
+        out = np.zeros(9)
+        _check(load().fribo_diagnostics(self.h, _dp(out)))
+        return dict(zip(DIAG_KEYS, out))
 
     def checkbcs(self):
         """checkbcs after the last step: (bcerrv, bcerrw, nbcv, nbcw)."""

@@ -124,6 +124,15 @@ int fribo_sync(fribo_handle h);
 int fribo_loads_transfer(fribo_handle h, double kprobe, const double* lmnor, const double* lmacc,
                          double* press, double* tau);
 
+/* The per-step diagnostics of SUBROUTINE ttime (ttime.F90:29-69,136) on the
+ * device, so that the fields need not cross PCIe for them: out[9] =
+ * {vmaxv(1), vmaxv(2) (vcheck.F90:3-36), vminv(1), vminv(2)
+ * (vcheck.F90:42-73), divgck (divergence.F90:49-78), kinen
+ * (utils_proc.F90:474-515), getintvort (utils_proc.F90:582-629), cflm and
+ * q3avg of cfl (cfl.F90:4-51)}.  Maxima and minima are exact; the sums are
+ * fixed-order tree sums. */
+int fribo_diagnostics(fribo_handle h, double* out);
+
 /* SUBROUTINE checkbcs (ckbcs.F90:5-86, called at fluid_tsch.F90:169): the
  * marker velocity error after the step, bcerrv/bcerrw (nlm) (NULL to skip),
  * and nbcv/nbcw = norm2(err)/nlm. */

@@ -1679,6 +1679,63 @@ int orc_checkbcs(orc_state *s, double *bcerrv, double *bcerrw, double *nbcv, dou
 }
 
 /* ------------------------------------------------------------------------ */
+/* Diagnostics of ttime (ttime.F90:29-69,136) on the current state:
+ * out[0..1] vmaxv (vcheck.F90:3-36), out[2..3] vminv (vcheck.F90:42-73),
+ * out[4] divgck (divergence.F90:49-78), out[5] kinen (utils_proc.F90:474-515),
+ * out[6] getintvort (utils_proc.F90:582-629), out[7] cflm and out[8] q3avg
+ * of cfl (cfl.F90:4-51).                                                    */
+int orc_diagnostics(orc_state *s, double *out)
+{
+    int j, k, n2m = s->n2m, n3m = s->n3m;
+    double vmax2 = -100.0, vmax3 = -100.0, vmin2 = 100.0, vmin3 = 100.0, qmax = -100.0, ek = 0.0;
+    for (k = 1; k <= n3m; ++k)
+        for (j = 1; j <= n2m; ++j) {
+            vmax2 = fmax(vmax2, fabs(Q2(j, k)));
+            vmax3 = fmax(vmax3, fabs(Q3(j, k)));
+            vmin2 = fmin(vmin2, Q2(j, k));
+            vmin3 = fmin(vmin3, Q3(j, k));
+        }
+    for (k = 1; k <= n3m; ++k)
+        for (j = 1; j <= n2m; ++j) {
+            const double dq = (Q2(j + 1, k) - Q2(j, k)) * s->udx2m[j] + (Q3(j, k + 1) - Q3(j, k)) * s->udx3m[k];
+            qmax = fmax(fabs(dq), qmax);
+        }
+    for (k = 1; k <= n3m; ++k)
+        for (j = 1; j <= n2m; ++j) {
+            const double vol = (s->zz[k + 1] - s->zz[k]) * (s->rc[j + 1] - s->rc[j]);
+            const double vccv = (Q2(j, k) + Q2(j + 1, k)) * 0.5;
+            const double vccw = (Q3(j, k) + Q3(j, k + 1)) * 0.5;
+            const double vel = vccv * vccv + vccw * vccw;
+            ek = ek + vel * vol;
+        }
+    ek = 0.5 * ek / (s->p.rext2 * s->p.alx3);
+    /* vorticity integral along the z = 18.5 line (first node nearest to it) */
+    double vi = 0.0;
+    {
+        const double locz = 18.5;
+        int indz = 1;
+        double best = fabs(s->zz[1] - locz);
+        for (k = 2; k <= s->n3; ++k)
+            if (fabs(s->zz[k] - locz) < best) { best = fabs(s->zz[k] - locz); indz = k; }
+        if (indz <= n3m) {
+            const double dz = s->zz[indz + 1] - s->zz[indz];
+            for (j = 1; j <= n2m; ++j) {
+                const double dwdy = (Q3(j + 1, indz) - Q3(j, indz)) / (s->rc[j + 1] - s->rc[j]);
+                const double dvdz = (Q2(j, indz + 1) - Q2(j, indz)) / dz;
+                vi = vi + (dwdy - dvdz) * (s->rc[j + 1] - s->rc[j]);
+            }
+        }
+    }
+    const double cflm0 = s->cflm, q3avg0 = s->q3avg;
+    cfl(s);
+    out[0] = vmax2; out[1] = vmax3; out[2] = vmin2; out[3] = vmin3;
+    out[4] = qmax; out[5] = ek; out[6] = vi; out[7] = s->cflm; out[8] = s->q3avg;
+    s->cflm = cflm0;
+    s->q3avg = q3avg0;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Test hooks: the restated LAPACK / GEsolve / Poisson pieces in isolation.  */
 int orc_test_dgtsv(int n, const double *dl0, const double *d0, const double *du0, double *b0)
 {

@@ -83,6 +83,11 @@ int orc_loads_transfer(orc_state *s, double kprobe, const double *lmnor, const d
                        double *tau);
 int orc_checkbcs(orc_state *s, double *bcerrv, double *bcerrw, double *nbcv, double *nbcw);
 
+/* The diagnostics ttime prints every step (ttime.F90:29-69,136): out[9] =
+ * {vmax q2, vmax q3, vmin q2, vmin q3, max|div|, kinetic energy, vorticity
+ * integral on z=18.5, cflm, q3avg}. */
+int orc_diagnostics(orc_state *s, double *out);
+
 /* Test hooks: DGTTRF+DGTTRS on one system, GEsolve on a 3x3 (column-major),
  * and phcalc_outflow applied to a (n2m x n3m) column-major right-hand side. */
 int orc_test_dgtsv(int n, const double *dl, const double *d, const double *du, double *b);

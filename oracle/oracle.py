@@ -55,9 +55,12 @@ def _lib():
         L.orc_test_phcalc.argtypes = [vp, dp]
         L.orc_loads_transfer.argtypes = [vp, C.c_double, dp, dp, dp, dp]
         L.orc_checkbcs.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_diagnostics.argtypes = [vp, dp]
         _LIB = L
     return _LIB
 
+
+DIAG_KEYS = ("vmax2", "vmax3", "vmin2", "vmin3", "divmax", "ektot", "vorti", "cflm", "q3avg")
 
 _INT_ARRAYS = {"lmind", "eulins", "jmv", "jpv", "kmv", "kpv"}
 
@@ -168,6 +171,12 @@ class Oracle:
         tau = np.zeros((nlm, 2), order="F")
         rc = _lib().orc_loads_transfer(self.h, float(kprobe), _dp(lmnor), _dp(lmacc), _dp(press), _dp(tau))
         return rc, press, tau
+
+    def diagnostics(self):
+        """ttime's per-step diagnostics as a dict."""
+        out = np.zeros(9)
+        _lib().orc_diagnostics(self.h, _dp(out))
+        return dict(zip(DIAG_KEYS, out))
 
     def checkbcs(self):
         """checkbcs after a step: (bcerrv, bcerrw, nbcv, nbcw)."""

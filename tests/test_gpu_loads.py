@@ -87,3 +87,24 @@ def test_tangent_carries_over_at_rest():
     press, tau = g.loads_transfer(1.0, nor, acc)
     assert np.all(tau == 0.0)
     assert np.all(np.isfinite(press))
+
+
+@pytest.mark.parametrize("name", ["cyl64", "cyl384"])
+def test_diagnostics_match_the_oracle(name):
+    """ttime's diagnostics: extrema exact, tree sums to 1e-13."""
+    c = case(name)
+    dec = decomposition(c)
+    o = oracle_for(c, dec)
+    g = gpu_for(c, dec)
+    st = cases.initial_state(c.params, c.seed)
+    # the same state on both sides: the GPU's own state after steps
+    run_steps(g, c, 3)
+    sg = g.get_state()
+    o.set_state({k: sg[k] for k in ("q2", "q3", "pr")})
+    do, dg = o.diagnostics(), g.diagnostics()
+    for k in ("vmax2", "vmax3", "vmin2", "vmin3", "divmax", "cflm"):
+        assert dg[k] == do[k], k
+    for k in ("ektot", "q3avg", "vorti"):
+        assert abs(dg[k] - do[k]) <= 1e-13 * max(abs(do[k]), 1e-30) + 1e-16, k
+    if name == "cyl384":
+        assert do["vorti"] != 0.0

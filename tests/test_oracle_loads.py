@@ -112,3 +112,31 @@ def test_createnorm_points_out_of_the_body():
     assert np.min(np.sum(nor * r, axis=1)) > 0.99
     # heave: every marker shares the body acceleration
     assert np.allclose(acc, acc[0])
+
+
+def test_diagnostics_closed_forms():
+    """ttime's diagnostics on fields with known values: uniform flow q3 = 1
+    (kinetic energy 1/2, no divergence, cflm = max udx3m, q3 mean 1) and a
+    shear q3 = s*y whose vorticity integral telescopes to q3(n2) - q3(1)."""
+    c = cases.make_case("cyl384")
+    o = oracle_for(c)
+    p = c.params
+    n2, n3m = p.n2, p.n3 - 1
+    q2 = np.zeros((n2, n3m + 6), order="F")
+    q3 = np.ones((n2, n3m + 6), order="F")
+    o.set("q2", q2)
+    o.set("q3", q3)
+    d = o.diagnostics()
+    assert d["vmax2"] == 0.0 and d["vmax3"] == 1.0 and d["vmin2"] == 0.0 and d["vmin3"] == 1.0
+    assert d["divmax"] == 0.0
+    # the cell volumes sum to rext2*alx3 up to the rounding of 110k terms
+    assert abs(d["ektot"] - 0.5) < 1e-12 and abs(d["q3avg"] - 1.0) < 1e-12
+    dx3 = n3m / p.alx3
+    assert abs(d["cflm"] - np.max(dx3 / (np.diff(c.zz) * dx3))) <= 1e-14 * d["cflm"]
+    assert d["vorti"] == 0.0
+    rm = np.append((c.rc[:-1] + c.rc[1:]) * 0.5, c.rc[-1])
+    s = 0.37
+    q3s = np.asfortranarray(np.repeat((s * rm)[:, None], n3m + 6, axis=1))
+    o.set("q3", q3s)
+    d = o.diagnostics()
+    assert abs(d["vorti"] - (s * rm[-1] - s * rm[0])) < 1e-12
