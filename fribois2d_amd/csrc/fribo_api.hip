@@ -64,6 +64,9 @@ struct fribo_ctx {
     double dmin = 0.0;       // smallest node spacing of the grid (cordin.F90:217-231)
     double* nb = nullptr;    // device: the two checkbcs norms
     double* dpart = nullptr; // device: diagnostics partials [8][256] and results [9]
+    // rollback copy of the prognostic state for iterated coupling
+    double* snap = nullptr;
+    bool snap_valid = false;
     int vort_k = 0;          // plane of getintvort's z = 18.5 line (0: none)
     hipGraphExec_t gexec = nullptr;
     int graph_nlm = -1;
@@ -1085,6 +1088,55 @@ extern "C" int fribo_diagnostics(fribo_handle h, double* out)
     HIPCHK(hipMemcpyAsync(out, res, 9 * sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return FRIBO_OK;
+}
+
+// Rollback copy for iterated (strong) coupling: q2, q3, pr, ru2, ru3 (the
+// whole device arrays) -- the q2_o, q3_o, pr_o, ru2_o, ru3_o that the
+// reference declares "for strong coupling" and never uses
+// (declarations.F90:155-159) -- plus the ten outflow arrays and the step
+// scalars a repeated step also needs.  Device to device, on the library's
+// stream; allocated on first use.
+static size_t snap_doubles(const FluidDev& F)
+{
+    return 5 * (size_t)F.P * F.nk + 10 * (size_t)(F.n2 + 2) + (sizeof(DevScalars) + 7) / 8;
+}
+
+static int snap_copy(fribo_ctx* h, bool save)
+{
+    FluidDev& F = h->F;
+    const size_t fsz = (size_t)F.P * F.nk * sizeof(double), bsz = (size_t)(F.n2 + 2) * sizeof(double);
+    double* fl[5] = {F.q2, F.q3, F.pr, F.ru2, F.ru3};
+    double* bc[10] = {F.qb2s, F.qb3s, F.qb2n, F.qb3n, F.dqb2s, F.dqb3s, F.dqb2n, F.dqb3n, F.dq2x2o, F.dq3x2o};
+    char* p = (char*)h->snap;
+    for (int i = 0; i < 5; ++i, p += fsz)
+        HIPCHK(hipMemcpyAsync(save ? (void*)p : (void*)fl[i], save ? (void*)fl[i] : (void*)p, fsz,
+                              hipMemcpyDeviceToDevice, h->st));
+    for (int i = 0; i < 10; ++i, p += bsz)
+        HIPCHK(hipMemcpyAsync(save ? (void*)p : (void*)bc[i], save ? (void*)bc[i] : (void*)p, bsz,
+                              hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(save ? (void*)p : (void*)F.S, save ? (void*)F.S : (void*)p, sizeof(DevScalars),
+                          hipMemcpyDeviceToDevice, h->st));
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_save_state(fribo_handle h)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    if (!h->snap) {
+        int e = dalloc(h, &h->snap, snap_doubles(h->F));
+        if (e) return e;
+    }
+    int e = snap_copy(h, true);
+    if (e) return e;
+    h->snap_valid = true;
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_restore_state(fribo_handle h)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    if (!h->snap_valid) return set_err(FRIBO_ERR_ARG, "no saved state");
+    return snap_copy(h, false);
 }
 
 extern "C" int fribo_set_transform(fribo_handle h, int32_t axis)

@@ -27,7 +27,7 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_set_profiling", "fribo_get_profile",
            "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
            "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform", "fribo_loads_transfer",
-           "fribo_checkbcs", "fribo_diagnostics"]
+           "fribo_checkbcs", "fribo_diagnostics", "fribo_save_state", "fribo_restore_state"]
 
 NPROF = 12
 
@@ -95,6 +95,8 @@ def load():
     L.fribo_loads_transfer.argtypes = [vp, C.c_double, dp, dp, dp, dp]
     L.fribo_checkbcs.argtypes = [vp, dp, dp, dp, dp]
     L.fribo_diagnostics.argtypes = [vp, dp]
+    L.fribo_save_state.argtypes = [vp]
+    L.fribo_restore_state.argtypes = [vp]
     _lib = L
     return L
 
@@ -214,6 +216,13 @@ class FluidStep:
         tau = np.zeros((self.nlm, 2), order="F")
         _check(load().fribo_loads_transfer(self.h, float(kprobe), _dp(lmnor), _dp(lmacc), _dp(press), _dp(tau)))
         return press, tau
+
+    def save_state(self):
+        """Device rollback copy of the prognostic state (iterated coupling)."""
+        _check(load().fribo_save_state(self.h))
+
+    def restore_state(self):
+        _check(load().fribo_restore_state(self.h))
 
     def diagnostics(self):
         """ttime's per-step diagnostics (dict, same keys as the oracle's)."""

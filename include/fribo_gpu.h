@@ -133,6 +133,16 @@ int fribo_loads_transfer(fribo_handle h, double kprobe, const double* lmnor, con
  * fixed-order tree sums. */
 int fribo_diagnostics(fribo_handle h, double* out);
 
+/* Rollback copy for iterated (strong) fluid-structure coupling, the role of
+ * the q2_o, q3_o, pr_o, ru2_o, ru3_o arrays the reference declares "for
+ * strong coupling" but never uses (declarations.F90:155-159):
+ * fribo_save_state copies the prognostic state (fields, outflow arrays, step
+ * scalars) on the device;
+ * fribo_restore_state puts it back, so the next fribo_fluid_tsch repeats the
+ * saved step with new marker kinematics.  One slot per handle. */
+int fribo_save_state(fribo_handle h);
+int fribo_restore_state(fribo_handle h);
+
 /* SUBROUTINE checkbcs (ckbcs.F90:5-86, called at fluid_tsch.F90:169): the
  * marker velocity error after the step, bcerrv/bcerrw (nlm) (NULL to skip),
  * and nbcv/nbcw = norm2(err)/nlm. */

@@ -162,6 +162,16 @@ class Oracle:
     def substep(self, n, ns):
         return _lib().orc_substep(self.h, n, ns)
 
+    _SNAP = ("q2", "q3", "pr", "ru2", "ru3", "qb2s", "qb3s", "qb2n", "qb3n", "dq2x2o", "dq3x2o")
+
+    def save_state(self):
+        """Rollback copy of the prognostic state (iterated coupling)."""
+        self._snap = {k: self.get(k).copy() for k in self._SNAP}
+
+    def restore_state(self):
+        for k, v in self._snap.items():
+            self.set(k, v)
+
     def loads_transfer(self, kprobe, lmnor, lmacc):
         """loads_transfer_c after a step: (rc, press (nlm,), tau (nlm, 2))."""
         lmnor = np.asfortranarray(lmnor, dtype=np.float64)

@@ -81,3 +81,32 @@ def run_steps(solver, c, nsteps, dt=None, t0=0.0, ns0=0, hist=None):
         dto = dtn
         dt = dtn
     return t, dt
+
+
+class OracleFluid:
+    """The oracle behind the GPU step's method signatures (errors raise),
+    so the coupling drivers run unchanged on either."""
+
+    def __init__(self, o):
+        self.o = o
+
+    def set_markers(self, *a):
+        self.o.set_markers(*a)
+
+    def fluid_tsch(self, ns, dt, dt_o=None):
+        rc, dtn, cflm = self.o.fluid_tsch(ns, dt, dt_o)
+        if rc:
+            raise RuntimeError(f"oracle step failed ({rc})")
+        return dtn, cflm
+
+    def loads_transfer(self, kprobe, lmnor, lmacc):
+        rc, press, tau = self.o.loads_transfer(kprobe, lmnor, lmacc)
+        if rc:
+            raise RuntimeError(f"oracle loads failed ({rc})")
+        return press, tau
+
+    def save_state(self):
+        self.o.save_state()
+
+    def restore_state(self):
+        self.o.restore_state()

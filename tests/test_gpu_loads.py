@@ -108,3 +108,26 @@ def test_diagnostics_match_the_oracle(name):
         assert abs(dg[k] - do[k]) <= 1e-13 * max(abs(do[k]), 1e-30) + 1e-16, k
     if name == "cyl384":
         assert do["vorti"] != 0.0
+
+
+def test_strong_coupling_on_the_gpu_matches_the_oracle():
+    """The strongly coupled step with the GPU fluid (device rollback between
+    sub-iterations) converges like the oracle's and to the same loads."""
+    from fribois2d_amd import coupling
+    from helpers import OracleFluid
+    from test_coupling import cylinder_body
+    c = case("cyl64")
+    p = cases.FluidParams(**{**c.params.__dict__, "idtv": True})
+    dec = decomposition(c)
+    g = gpu_for(c, dec, params=p)
+    o = OracleFluid(oracle_for(c, dec, params=p))
+    bg, bo = cylinder_body(c), cylinder_body(c)
+    lg = lo = (np.zeros(2), 0.0)
+    dt = 1e-3
+    for ns in range(3):
+        lg, bg, itg, _, _ = coupling.strong_step(g, bg, lg, ns, dt)
+        lo, bo, ito, _, _ = coupling.strong_step(o, bo, lo, ns, dt)
+        print(f"step {ns}: sub-iterations gpu {itg} oracle {ito}; lift {lg[0][0]:.6e} / {lo[0][0]:.6e}", flush=True)
+        assert abs(itg - ito) <= 1
+        assert abs(lg[0][0] - lo[0][0]) <= 1e-8 * max(abs(lo[0][0]), 1e-3)
+    assert abs(bg.cm[0] - bo.cm[0]) <= 1e-12

@@ -96,8 +96,10 @@ def test_substep_stages(name, axis):
             e = relmax(g.get(f), o.get(f))
             print(f"  {f}: {e:.2e}", flush=True)
             assert e <= 1e-11, f
+        # outflow arrays are computed from q2/q3 on the last plane, so they
+        # carry the fields' solver rounding (same bar)
         for f in ("qb2n", "qb3n", "dq2x2o", "dq3x2o"):
-            assert relmax(g.get(f), o.get(f)) <= 1e-12, f
+            assert relmax(g.get(f), o.get(f)) <= 1e-11, f
 
 
 @pytest.mark.parametrize("axis", ["z", "y"])
