@@ -336,7 +336,24 @@ static int setup_ytransform(fribo_ctx* c, const double* zmmt, int ldz)
 extern "C" int fribo_init(const fribo_params* p, const double* rc, const double* zz, const double* zmm,
                           const double* zmmt, int ldz, const double* wr, fribo_handle* out)
 {
-    if (!p || !rc || !zz || !zmm || !zmmt || !wr || !out) return set_err(FRIBO_ERR_ARG, "null argument");
+    if (!p || !rc || !zz || !out) return set_err(FRIBO_ERR_ARG, "null argument");
+    if (!(zmm && zmmt && wr) && (zmm || zmmt || wr))
+        return set_err(FRIBO_ERR_ARG, "zmm, zmmt and wr are given together or not at all");
+    // no phini inputs: the library decomposes the z operator itself
+    std::vector<double> own_zmm, own_zmmt, own_wr;
+    if (!zmm) {
+        const int n3m = p->n3 - 1;
+        if (n3m < 2) return set_err(FRIBO_ERR_ARG, "n3 too small");
+        own_zmm.assign((size_t)n3m * n3m, 0.0);
+        own_zmmt.assign((size_t)n3m * n3m, 0.0);
+        own_wr.assign(n3m, 0.0);
+        if (phini_sym(p->n3, zz, p->alx3, n3m, own_zmm.data(), own_zmmt.data(), own_wr.data()) != 0)
+            return set_err(FRIBO_ERR_LIMIT, "eigen-decomposition of the z pressure operator failed");
+        zmm = own_zmm.data();
+        zmmt = own_zmmt.data();
+        wr = own_wr.data();
+        ldz = n3m;
+    }
     if (p->n2 < 5 || p->n3 < 5) return set_err(FRIBO_ERR_ARG, "grid too small (n2, n3 >= 5)");
     if (p->nsst != 1 && p->nsst != 3) return set_err(FRIBO_ERR_ARG, "nsst must be 1 or 3");
     if (ldz < p->n3 - 1) return set_err(FRIBO_ERR_ARG, "ldz < n3m");

@@ -146,3 +146,38 @@ extern "C" int fribo_test_ytransform(int32_t n, const double* am, const double* 
     if (n <= 0 || !am || !ac || !ap || !Y || !Yinv || !wy) return 5;
     return ytransform_build(n, am, ac, ap, n, Y, Yinv, wy) == 0 ? 0 : 7;
 }
+
+// phini (phini.F90:4-99) without DGEEV/DGESV: the z pressure operator of
+// tridiag_matrices (phini.F90:121-142, Neumann at both ends) from the node
+// coordinates zz(1:n3), decomposed like the y operator above.  zmm
+// (eigenvectors as columns) and zmmt = zmm^-1 are n3m x n3m with leading
+// dimension ld; wr(n3m) the eigenvalues.  The eigen-pairs are those of
+// DGEEV up to order and scaling, which the Poisson solve does not see.
+int phini_sym(int n3, const double* zz, double alx3, int ld, double* zmm, double* zmmt, double* wr)
+{
+    const int n3m = n3 - 1;
+    if (n3m < 2 || ld < n3m) return -1;
+    const double dx3 = 1.0 / (alx3 / (double)n3m), dx3q = dx3 * dx3;
+    std::vector<double> g3rm(n3m + 2), g3rc(n3 + 2), am(n3m), ac(n3m), ap(n3m);
+    // cordin.F90:197-215 (1-based)
+    for (int k = 1; k <= n3m; ++k) g3rm[k] = (zz[k] - zz[k - 1]) * dx3;
+    for (int k = 2; k <= n3m; ++k) g3rc[k] = (zz[k] - zz[k - 2]) * dx3 * 0.5;
+    g3rc[1] = (zz[1] - zz[0]) * dx3;
+    g3rc[n3] = (zz[n3 - 1] - zz[n3m - 1]) * dx3;
+    for (int kc = 1; kc <= n3m; ++kc) {
+        const int kmc = (kc == 1) ? 0 : 1, kpc = (kc == n3m) ? 0 : 1;
+        const int kp = (kc == n3m) ? kc : kc + 1;
+        const double a33icc = kmc * dx3q / g3rc[kc], a33icp = kpc * dx3q / g3rc[kp], ug = 1.0 / g3rm[kc];
+        am[kc - 1] = a33icc * ug;
+        ap[kc - 1] = a33icp * ug;
+        ac[kc - 1] = -(am[kc - 1] + ap[kc - 1]);
+    }
+    return ytransform_build(n3m, am.data(), ac.data(), ap.data(), ld, zmm, zmmt, wr);
+}
+
+extern "C" int fribo_phini(int32_t n3, const double* zz, double alx3, int32_t ld, double* zmm, double* zmmt,
+                           double* wr)
+{
+    if (n3 < 3 || !zz || !zmm || !zmmt || !wr || ld < n3 - 1 || !(alx3 > 0.0)) return 5;
+    return phini_sym(n3, zz, alx3, ld, zmm, zmmt, wr) == 0 ? 0 : 7;
+}

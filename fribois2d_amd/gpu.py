@@ -27,7 +27,8 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_set_profiling", "fribo_get_profile",
            "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
            "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform", "fribo_loads_transfer",
-           "fribo_checkbcs", "fribo_diagnostics", "fribo_save_state", "fribo_restore_state"]
+           "fribo_checkbcs", "fribo_diagnostics", "fribo_save_state", "fribo_restore_state",
+           "fribo_phini"]
 
 NPROF = 12
 
@@ -96,6 +97,7 @@ def load():
     L.fribo_checkbcs.argtypes = [vp, dp, dp, dp, dp]
     L.fribo_diagnostics.argtypes = [vp, dp]
     L.fribo_save_state.argtypes = [vp]
+    L.fribo_phini.argtypes = [C.c_int32, dp, C.c_double, C.c_int32, dp, dp, dp]
     L.fribo_restore_state.argtypes = [vp]
     _lib = L
     return L
@@ -116,14 +118,22 @@ _INT = {"lmind", "eulins"}
 class FluidStep:
     """One fluid instance on the GPU (reference-layout host interface)."""
 
-    def __init__(self, params, rc, zz, zmm, zmmt, wr):
+    def __init__(self, params, rc, zz, zmm=None, zmmt=None, wr=None):
         L = load()
         self.params = params
         self.n2, self.n3 = params.n2, params.n3
         p = Params.from_fluid(params)
+        h = C.c_void_p()
+        if zmm is None:
+            # the library computes phini's decomposition itself
+            zz = np.ascontiguousarray(zz, dtype=np.float64)
+            rc = np.ascontiguousarray(rc, dtype=np.float64)
+            _check(L.fribo_init(C.byref(p), _dp(rc), _dp(zz), None, None, 0, None, C.byref(h)))
+            self.h = h
+            self.nlm = 0
+            return
         zmm = np.asfortranarray(zmm, dtype=np.float64)
         zmmt = np.asfortranarray(zmmt, dtype=np.float64)
-        h = C.c_void_p()
         rc = np.ascontiguousarray(rc, dtype=np.float64)
         zz = np.ascontiguousarray(zz, dtype=np.float64)
         wr = np.ascontiguousarray(wr, dtype=np.float64)
@@ -295,3 +305,14 @@ def ytransform(am, ac, ap):
     arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (am, ac, ap)]
     _check(load().fribo_test_ytransform(n, *[_dp(x) for x in arr], _dp(Y), _dp(Yi), _dp(w)))
     return Y, Yi, w
+
+
+def phini(zz, alx3):
+    """phini's (zmm, zmmt, wr) by the library's symmetrised QL decomposition."""
+    zz = np.ascontiguousarray(zz, dtype=np.float64)
+    n = zz.size - 1
+    zmm = np.zeros((n, n), order="F")
+    zmmt = np.zeros((n, n), order="F")
+    wr = np.zeros(n)
+    _check(load().fribo_phini(zz.size, _dp(zz), float(alx3), n, _dp(zmm), _dp(zmmt), _dp(wr)))
+    return zmm, zmmt, wr

@@ -67,7 +67,8 @@ typedef struct fribo_ctx* fribo_handle;
  * metrics (indic.F90:4-44, cordin.F90:105-215, coetar.F90:5-144,
  * phini.F90:109-145) from the node coordinates rc(1:n2) and zz(1:n3), and
  * the phini eigen-decomposition zmm, zmmt, wr (phini.F90:35-92) computed by
- * the host.  Allocates all device memory. */
+ * the host; pass zmm = zmmt = wr = NULL to have fribo_phini's decomposition
+ * computed here instead.  Allocates all device memory. */
 int fribo_init(const fribo_params* p, const double* rc, const double* zz, const double* zmm,
                const double* zmmt, int ldz, const double* wr, fribo_handle* out);
 int fribo_finalize(fribo_handle h);
@@ -174,6 +175,15 @@ int fribo_test_gemm(int32_t M, int32_t N, int32_t K, const double* A, int32_t ld
  * use. */
 int fribo_set_transform(fribo_handle h, int32_t axis);
 int fribo_get_transform(fribo_handle h);
+
+/* phini (phini.F90:4-99) without LAPACK: the z pressure operator of
+ * tridiag_matrices (phini.F90:121-142) built from zz(1:n3) and alx3 is
+ * similar to a symmetric tridiagonal matrix, decomposed by implicit QL;
+ * zmm (eigenvectors as columns), zmmt = zmm^-1 (n3-1 x n3-1, leading
+ * dimension ld) and wr(n3-1).  Same eigen-pairs as DGEEV up to order and
+ * scaling, which the Poisson solve does not depend on; O(n^2) + a threaded
+ * O(n^3) eigenvector accumulation instead of DGEEV's dense O(n^3). */
+int fribo_phini(int32_t n3, const double* zz, double alx3, int32_t ld, double* zmm, double* zmmt, double* wr);
 
 /* Diagnostic entry point (no reference counterpart): eigen-decomposition of
  * an n x n tridiagonal operator with rows (am, ac, ap), 0-based, that is

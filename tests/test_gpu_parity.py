@@ -252,3 +252,23 @@ def test_transform_axes_agree():
     e = relmax(gy.get("dph"), gz.get("dph"))
     print(f"y vs z dph (same gauge): {e:.2e}", flush=True)
     assert e <= 1e-10
+
+
+@pytest.mark.parametrize("axis", ["z", "y"])
+def test_library_phini_gives_the_same_step(axis):
+    """fribo_init without phini inputs (the library's own symmetrised QL
+    decomposition of the z operator) steps like the host DGEEV inputs."""
+    from fribois2d_amd.gpu import FluidStep
+    c = case("cyl64")
+    dec = decomposition(c, "dgeev")
+    ga = gpu_for(c, dec, axis=axis)
+    gb = FluidStep(c.params, c.rc, c.zz)
+    gb.set_transform(axis)
+    gb.set_state(cases.initial_state(c.params, c.seed))
+    run_steps(ga, c, 5)
+    run_steps(gb, c, 5)
+    sa, sb = ga.get_state(), gb.get_state()
+    for f in ("q2", "q3", "pr"):
+        e = relmax(sb[f], sa[f])
+        print(f"library vs host phini ({axis}) {f}: {e:.2e}", flush=True)
+        assert e <= 1e-11, f

@@ -44,3 +44,18 @@ def test_stretched_grid_decomposition(name):
     assert abs(ws[-1]) <= 1e-9 * scale and ws[-2] < -1e-9 * scale
     # the same spectrum as a dense eigen-solver
     np.testing.assert_allclose(ws, np.sort(np.linalg.eigvals(A).real), rtol=0, atol=1e-11 * scale)
+
+
+def test_library_phini_matches_the_operator():
+    """fribo_phini: the z operator of tridiag_matrices decomposed without
+    LAPACK; same spectrum as DGEEV (scipy) and a valid inverse pair."""
+    from fribois2d_amd import gpu
+    c = cases.make_case("cyl384")
+    am, ac, ap = cases.tridiag_z(c.zz, c.params.alx3)
+    A = np.diag(ac) + np.diag(ap[:-1], 1) + np.diag(am[1:], -1)
+    zmm, zmmt, wr = gpu.phini(c.zz, c.params.alx3)
+    scale = np.abs(A).max()
+    assert np.abs(A @ zmm - zmm * wr).max() <= 1e-13 * scale
+    assert np.abs(zmmt @ zmm - np.eye(ac.size)).max() <= 1e-13
+    _, _, wref = cases.phini(c.zz, c.params.alx3, method="dgeev")
+    np.testing.assert_allclose(np.sort(wr), np.sort(wref), rtol=0, atol=1e-10 * scale)
