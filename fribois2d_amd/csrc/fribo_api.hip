@@ -996,19 +996,21 @@ extern "C" int fribo_sync(fribo_handle h)
 
 extern "C" const char* fribo_last_error(void) { return g_err.c_str(); }
 
-extern "C" int fribo_test_gemm(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
-                               int32_t ldb, double* C, int32_t ldc)
+static int test_gemm(bool nn, int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
+                     int32_t ldb, double* C, int32_t ldc)
 {
-    if (M <= 0 || N <= 0 || K <= 0 || lda < M || ldb < N || ldc < M || !A || !B || !C)
+    if (M <= 0 || N <= 0 || K <= 0 || lda < M || ldb < (nn ? K : N) || ldc < M || !A || !B || !C)
         return set_err(FRIBO_ERR_ARG, "bad gemm arguments");
+    const size_t nb = (size_t)ldb * (nn ? N : K);
     double *dA, *dB, *dC;
     HIPCHK(hipMalloc(&dA, (size_t)lda * K * sizeof(double)));
-    HIPCHK(hipMalloc(&dB, (size_t)ldb * K * sizeof(double)));
+    HIPCHK(hipMalloc(&dB, nb * sizeof(double)));
     HIPCHK(hipMalloc(&dC, (size_t)ldc * N * sizeof(double)));
     HIPCHK(hipMemcpy(dA, A, (size_t)lda * K * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dB, B, (size_t)ldb * K * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dB, B, nb * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dC, C, (size_t)ldc * N * sizeof(double), hipMemcpyHostToDevice));
-    launch_gemm_nt(M, N, K, dA, lda, dB, ldb, dC, ldc, nullptr, 0);
+    if (nn) launch_gemm_nn(M, N, K, dA, lda, dB, ldb, dC, ldc, nullptr, 0);
+    else launch_gemm_nt(M, N, K, dA, lda, dB, ldb, dC, ldc, nullptr, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(C, dC, (size_t)ldc * N * sizeof(double), hipMemcpyDeviceToHost));
@@ -1016,6 +1018,18 @@ extern "C" int fribo_test_gemm(int32_t M, int32_t N, int32_t K, const double* A,
     hipFree(dB);
     hipFree(dC);
     return FRIBO_OK;
+}
+
+extern "C" int fribo_test_gemm(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
+                               int32_t ldb, double* C, int32_t ldc)
+{
+    return test_gemm(false, M, N, K, A, lda, B, ldb, C, ldc);
+}
+
+extern "C" int fribo_test_gemm_nn(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
+                                  int32_t ldb, double* C, int32_t ldc)
+{
+    return test_gemm(true, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
 // Streaming copy of a known byte count: the yardstick that turns the HBM

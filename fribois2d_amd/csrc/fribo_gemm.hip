@@ -21,14 +21,28 @@
 // read touches fall in disjoint bank halves.  In the NN form (the transforms
 // along y, whose B operand is the field itself, k-contiguous) the B slice is
 // read as 16-double runs of each column and stored with a row pitch of BN+1,
-// which spreads the transposing LDS stores over the banks.  The large-problem shape
-// 96 x 128 gives exactly 256 workgroups (one per CU) for the 1536 x 2048
-// transforms of the 2048x1536-cell grid.
+// which spreads the transposing LDS stores over the banks.
+//
+// Large problems (k_gemm_dma): 96 x 128 tiles, 8 waves, and the operand
+// slices move from global memory straight into LDS (global_load_lds_dwordx4,
+// 16 bytes per lane, no register staging), double-buffered with one
+// workgroup barrier per K-slice.  A direct LDS write lands lane-contiguous,
+// so the LDS layout follows the global one: A and the NT-form B as k-rows
+// (pitch 112 / 144 doubles: 48 / 64 active 16-byte slots of 56 / 72), the
+// NN-form B as n-rows of BK k-values with pitch BK+2 (the fragment read then
+// hits 16 distinct bank groups).  Lanes whose pair lies outside the matrix
+// do not load; the rows and columns they leave stale only feed outputs that
+// are not stored, and the k-rows past K of the last slice are cleared before
+// use.  The 96 x 128 shape gives exactly 256 workgroups (one per CU) for the
+// 1536 x 2048 transforms of the 2048x1536-cell grid.
 #include "fribo_internal.h"
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 #define GBK 16
+#ifndef GEMM_DMA_BK
+#define GEMM_DMA_BK 32
+#endif
 
 template <int BM, int BN, int WM, int WN, bool NN>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, const double* __restrict__ A, int lda,
@@ -128,13 +142,131 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
         }
 }
 
+template <int BK, bool NN>
+__global__ __launch_bounds__(512) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A, int lda,
+                                                  const double* __restrict__ B, int ldb, double* __restrict__ C,
+                                                  int ldc, const DevScalars* S)
+{
+    constexpr int BM = 96, BN = 128, WM = 2, WN = 4;
+    constexpr int TM = BM / WM, TN = BN / WN, RM = TM / 16, RN = TN / 16;
+    constexpr int LA = 112, SLA = LA / 2;                   // A: k-rows, 56 slots (48 used)
+    constexpr int LB = NN ? BK + 2 : 144, SLB = LB / 2;     // B: n-rows (NN) or k-rows (NT)
+    constexpr int SA = BK * LA, SB = NN ? BN * LB : BK * LB;   // doubles per stage
+    constexpr int IA = SA / 128, IB = SB / 128;             // 1 KB instructions per stage
+    constexpr int NI = (IA + IB + 7) / 8;                   // per wave
+    static_assert(SA % 128 == 0 && SB % 128 == 0, "stage size");
+    if (S && S->err) return;
+    __shared__ __attribute__((aligned(1024))) double As[2 * SA];
+    __shared__ __attribute__((aligned(1024))) double Bs[2 * SB];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int i0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+
+    auto issue = [&](int k0, int st) {
+#pragma unroll
+        for (int r = 0; r < NI; ++r) {
+            const int ii = wave + 8 * r;
+            if (ii < IA) {
+                const int slot = ii * 64 + lane, kk = slot / SLA, q = slot % SLA;
+                if (q < BM / 2 && k0 + kk < K && i0 + 2 * q < M)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void*)(A + (size_t)(k0 + kk) * lda + i0 + 2 * q),
+                        (__attribute__((address_space(3))) void*)(As + st * SA + ii * 128), 16, 0, 0);
+            } else if (ii < IA + IB) {
+                const int jj = ii - IA, slot = jj * 64 + lane, row = slot / SLB, q = slot % SLB;
+                const double* src;
+                bool ok;
+                if (NN) {
+                    ok = q < BK / 2 && n0 + row < N && k0 + 2 * q < K;
+                    src = B + (size_t)(n0 + row) * ldb + k0 + 2 * q;
+                } else {
+                    ok = q < BN / 2 && k0 + row < K && n0 + 2 * q < N;
+                    src = B + (size_t)(k0 + row) * ldb + n0 + 2 * q;
+                }
+                if (ok)
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                                     (__attribute__((address_space(3))) void*)(Bs + st * SB + jj * 128),
+                                                     16, 0, 0);
+            }
+        }
+    };
+
+    f64x4 acc[RM][RN];
+#pragma unroll
+    for (int r = 0; r < RM; ++r)
+#pragma unroll
+        for (int c = 0; c < RN; ++c) acc[r][c] = (f64x4){0.0, 0.0, 0.0, 0.0};
+
+    const int nk = (K + BK - 1) / BK;
+    issue(0, 0);
+    for (int s = 0; s < nk; ++s) {
+        const int st = s & 1;
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's slice has landed in LDS
+        __syncthreads();                      // ... and every other wave's
+        const int kv = K - s * BK;            // valid k-rows of this slice
+        if (kv < BK) {
+            for (int e = t; e < (BK - kv) * LA; e += 512) As[st * SA + kv * LA + e] = 0.0;
+            if (NN) {
+                for (int e = t; e < BN * (BK - kv); e += 512)
+                    Bs[st * SB + (e / (BK - kv)) * LB + kv + e % (BK - kv)] = 0.0;
+            } else {
+                for (int e = t; e < (BK - kv) * LB; e += 512) Bs[st * SB + kv * LB + e] = 0.0;
+            }
+            __syncthreads();
+        }
+        if (s + 1 < nk) issue((s + 1) * BK, st ^ 1);
+        const double* as = As + st * SA;
+        const double* bs = Bs + st * SB;
+#pragma unroll
+        for (int k4 = 0; k4 < BK; k4 += 4) {
+            const int kr = k4 + (lane >> 4);
+            double a[RM], b[RN];
+#pragma unroll
+            for (int r = 0; r < RM; ++r) a[r] = as[kr * LA + wm * TM + r * 16 + (lane & 15)];
+#pragma unroll
+            for (int c = 0; c < RN; ++c)
+                b[c] = NN ? bs[(wn * TN + c * 16 + (lane & 15)) * LB + kr] : bs[kr * LB + wn * TN + c * 16 + (lane & 15)];
+#pragma unroll
+            for (int r = 0; r < RM; ++r)
+#pragma unroll
+                for (int c = 0; c < RN; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RM; ++r)
+#pragma unroll
+        for (int c = 0; c < RN; ++c) {
+            const int col = n0 + wn * TN + c * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = i0 + wm * TM + r * 16 + (lane >> 4) + 4 * v;
+                if (row < M && col < N) C[(size_t)col * ldc + row] = acc[r][c][v];
+            }
+        }
+}
+
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// The direct-to-LDS path reads operand pairs (16 bytes): both operands must be
+// 16-byte aligned with even leading dimensions, and a pair that starts on the
+// last row of an odd extent reads one padding element, which must exist.
+template <bool NN>
+static bool dma_ok(int M, int N, int K, const double* A, int lda, const double* B, int ldb)
+{
+    const int bext = NN ? K : N;
+    return ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0 && lda % 2 == 0 &&
+           ldb % 2 == 0 && lda >= M + (M & 1) && ldb >= bext + (bext & 1);
+}
 
 template <bool NN>
 static void launch_gemm(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                         const DevScalars* S, hipStream_t st)
 {
-    if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
+    if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192 && dma_ok<NN>(M, N, K, A, lda, B, ldb)) {
+        hipLaunchKernelGGL((k_gemm_dma<GEMM_DMA_BK, NN>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M, N, K,
+                           A, lda, B, ldb, C, ldc, S);
+    } else if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
         hipLaunchKernelGGL((k_gemm_nt<96, 128, 2, 4, NN>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M, N,
                            K, A, lda, B, ldb, C, ldc, S);
     } else if ((long)cdiv(M, 64) * cdiv(N, 64) >= 96) {

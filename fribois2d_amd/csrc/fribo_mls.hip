@@ -8,9 +8,10 @@
 //               compute_AB + GEsolve (MLSint.F90:115-166,284-486,
 //               utils_math.f90:637-725) and the momentum coefficient cl
 //               (forcing_MLS.F90:72-81); one thread per (marker, direction)
-//   k_mls_sort  orders the 9 contributions of every marker by (cell, marker)
-//               (bitonic sort: LDS blocks plus global merge passes); equal
-//               cells form contiguous segments
+//   k_mls_radix orders the 9 contributions of every marker by (cell, marker)
+//               (stable LDS radix sort on the cell; bitonic LDS blocks plus
+//               global merge passes for very large marker sets); equal cells
+//               form contiguous segments
 // After the step, on request (fluid_tsch.F90:167-177):
 //   k_loads      loads_transfer_c (loads_transfer.F90:4-162): probes at
 //                ddmin*kprobe along the normal, MLS_SF_d shape functions and
@@ -258,43 +259,129 @@ __global__ __launch_bounds__(64) void k_mls_sup(const FluidDev F)
     }
 }
 
-// Bitonic sort of the (cell, entry) keys of each direction, ascending.  Up
-// to SORT_ONE keys a single workgroup per direction sorts them in LDS in one
-// launch; larger key sets sort blocks of SORT_BLK keys in LDS and run the
-// strides of the larger merge stages as global compare-exchange passes.  Key
-// order is unique, so the result is deterministic.
+// Ordering of the (cell, entry) keys of each direction, ascending.
+//
+// Up to RADIX_MAX keys (the usual case: nine supports per marker) one
+// workgroup per direction runs a stable LSD radix sort on the cell field
+// (8-bit digits, as many passes as the grid's cell count needs).  The keys are
+// generated in entry order, so stability yields (cell, entry) order; an unused
+// support (KEY_NONE) takes the largest digit value and lands at the end.  Each
+// pass holds the keys in registers (thread-contiguous runs of 64-key rounds per
+// wave), builds per-wave digit histograms, scans them digit-major and
+// scatters into one LDS buffer; equal digits within a round are ranked with a
+// peer mask built from eight ballots, and only the lowest lane of each peer
+// group touches the wave's histogram counter.
+//
+// Larger key sets sort blocks of SORT_BLK keys in LDS (bitonic) and run the
+// strides of the larger merge stages as global compare-exchange passes.
+// Both orders are unique, so the result is deterministic.
 #define SORT_BLK 2048
-#define SORT_ONE 16384
+#define RADIX_MAX 16384
+#define RADIX_WAVES 16
+#define RADIX_R (RADIX_MAX / (64 * RADIX_WAVES))   // rounds of 64 keys per wave
 
 // index of the first element of compare-exchange pair q at a power-of-two stride
 __device__ inline int pair_lo(int q, int stride) { return ((q & ~(stride - 1)) << 1) | (q & (stride - 1)); }
 
-__global__ __launch_bounds__(1024) void k_mls_sort_one(const FluidDev F)
+__global__ __launch_bounds__(1024) void k_mls_radix(const FluidDev F, int nbits)
 {
-    __shared__ unsigned long long sk[SORT_ONE];
+    __shared__ unsigned long long sk[RADIX_MAX];
+    __shared__ unsigned int hist[256 * RADIX_WAVES];   // [digit][wave]
+    __shared__ unsigned int wsum[RADIX_WAVES];
     if (F.S->err) return;
-    const int d = blockIdx.x, t = threadIdx.x;
-    const int nk = F.maxkeys;
-    const int nreal = F.nlm * FRIBO_DIMSD;
+    const int d = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int n = F.nlm * FRIBO_DIMSD;
     unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
-    for (int i = t; i < nk; i += 1024) sk[i] = i < nreal ? gk[i] : KEY_NONE;
-    __syncthreads();
-    for (int size = 2; size <= nk; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int q = t; q < nk / 2; q += 1024) {
-                const int i = pair_lo(q, stride);
-                const int pj = i + stride;
-                const bool up = (i & size) == 0;
-                const unsigned long long x = sk[i], y = sk[pj];
-                if ((x > y) == up) {
-                    sk[i] = y;
-                    sk[pj] = x;
-                }
-            }
-            __syncthreads();
+    const int per = ((n + 64 * RADIX_WAVES - 1) / (64 * RADIX_WAVES)) * 64;   // keys per wave
+    const unsigned int none = (nbits >= 32) ? 0xffffffffu : ((1u << nbits) - 1u);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int shift = 0; shift < nbits; shift += 8) {
+        unsigned long long v[RADIX_R];
+#pragma unroll
+        for (int r = 0; r < RADIX_R; ++r) {
+            const int e = w * per + r * 64 + lane;
+            const bool ok = (r * 64 < per) && e < n;
+            v[r] = ok ? (shift == 0 ? gk[e] : sk[e]) : KEY_NONE;
         }
+        // digit of key r; 256 marks a lane without a key
+        auto digit = [&](int r) -> unsigned int {
+            const int e = w * per + r * 64 + lane;
+            if (!((r * 64 < per) && e < n)) return 256u;
+            const unsigned int cell = (v[r] == KEY_NONE) ? none : (unsigned int)(v[r] >> ENT_BITS);
+            return (cell >> shift) & 255u;
+        };
+        // lanes of this round holding the same digit
+        auto peers = [&](unsigned int dg) -> unsigned long long {
+            unsigned long long m = __ballot(dg < 256u);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const unsigned long long bb = __ballot((dg >> b) & 1u);
+                m &= ((dg >> b) & 1u) ? bb : ~bb;
+            }
+            return m;
+        };
+        for (int i = t; i < 256 * RADIX_WAVES; i += 1024) hist[i] = 0u;
+        __syncthreads();
+        // per-wave histogram: the lowest lane of each peer group adds the group size
+#pragma unroll
+        for (int r = 0; r < RADIX_R; ++r) {
+            if (r * 64 >= per) break;
+            const unsigned int dg = digit(r);
+            const unsigned long long m = peers(dg);
+            if (dg < 256u && (m & below) == 0ull)
+                __hip_atomic_fetch_add(&hist[dg * RADIX_WAVES + w], (unsigned int)__popcll(m), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        __syncthreads();
+        // exclusive scan of the 4096 counters in [digit][wave] order: four per thread
+        unsigned int c0 = hist[4 * t], c1 = hist[4 * t + 1], c2 = hist[4 * t + 2], c3 = hist[4 * t + 3];
+        const unsigned int tot = c0 + c1 + c2 + c3;
+        unsigned int inc = tot;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned int y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        unsigned int base = 0u;
+        for (int i = 0; i < w; ++i) base += wsum[i];
+        const unsigned int ex = base + inc - tot;
+        hist[4 * t] = ex;
+        hist[4 * t + 1] = ex + c0;
+        hist[4 * t + 2] = ex + c0 + c1;
+        hist[4 * t + 3] = ex + c0 + c1 + c2;
+        __syncthreads();
+        // stable scatter: rounds in order, lanes in order within a round.  The
+        // group leaders claim their ranges with returning LDS adds, issued for
+        // all rounds back to back (LDS applies one wave's operations in
+        // order, so a digit's claims follow the round order); the members
+        // then take the leader's base by a shuffle.
+        unsigned long long pm[RADIX_R];
+        unsigned int pb[RADIX_R];
+#pragma unroll
+        for (int r = 0; r < RADIX_R; ++r) {
+            pm[r] = 0ull;
+            pb[r] = 0u;
+            if (r * 64 < per) {
+                const unsigned int dg = digit(r);
+                pm[r] = peers(dg);
+                if (dg < 256u && (pm[r] & below) == 0ull)
+                    pb[r] = __hip_atomic_fetch_add(&hist[dg * RADIX_WAVES + w], (unsigned int)__popcll(pm[r]),
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RADIX_R; ++r) {
+            if (r * 64 >= per) break;
+            const unsigned int dg = digit(r);
+            const int lead = pm[r] ? __ffsll((long long)pm[r]) - 1 : lane;
+            const unsigned int b = (unsigned int)__shfl((int)pb[r], lead, 64);
+            if (dg < 256u) sk[b + (unsigned int)__popcll(pm[r] & below)] = v[r];
+        }
+        __syncthreads();
     }
-    for (int i = t; i < nk; i += 1024) gk[i] = sk[i];
+    for (int i = t; i < n; i += 1024) gk[i] = sk[i];
 }
 
 __global__ __launch_bounds__(1024) void k_mls_sort_local(const FluidDev F, int size0, int first)
@@ -353,8 +440,11 @@ void launch_mls_setup(const FluidDev& F, hipStream_t st)
     if (F.nlm <= 0) return;
     hipLaunchKernelGGL(k_mls_idx, dim3(F.nlm, 4), dim3(64), 0, st, F);
     hipLaunchKernelGGL(k_mls_sup, dim3((2 * F.nlm + 63) / 64), dim3(64), 0, st, F);
-    if (F.maxkeys <= SORT_ONE) {
-        hipLaunchKernelGGL(k_mls_sort_one, dim3(2), dim3(1024), 0, st, F);
+    const unsigned long long ncell = (unsigned long long)F.n2 * (unsigned long long)F.n3;
+    if (F.nlm * FRIBO_DIMSD <= RADIX_MAX && ncell < 0xffffffffull) {
+        int nbits = 1;
+        while (nbits < 32 && (1ull << nbits) <= ncell) ++nbits;   // 2^nbits > every cell + 1
+        hipLaunchKernelGGL(k_mls_radix, dim3(2), dim3(1024), 0, st, F, nbits);
         return;
     }
     const int nblk = F.maxkeys / SORT_BLK;

@@ -38,28 +38,30 @@ __device__ inline int chunk_owner(int r, int n, int nch)
 
 // Per-lane staging map of a j-line: element u of the lane (row r = lane+64u)
 // goes to LDS slot p*stride + (r - cs[p]) of the chunk layout (-1: no row).
+// Chunk starts come from the lanes (lane p holds cs[p]) by shuffles.
 template <int NR>
 __device__ inline void line_map(int* slot, const int* cs, int n, int nch, int stride, int lane)
 {
+    const int csl = lane < nch ? cs[lane] : 0;
 #pragma unroll
     for (int u = 0; u < NR; ++u) {
         const int r = lane + 64 * u;
-        int v = -1;
-        if (r < n) {
-            const int p = chunk_owner(r, n, nch);
-            v = p * stride + (r - cs[p]);
-        }
-        slot[u] = v;
+        const int p = chunk_owner(r < n ? r : n - 1, n, nch);
+        const int c0 = __shfl(csl, p, 64);
+        slot[u] = r < n ? p * stride + (r - c0) : -1;
     }
 }
 
+// Loads of a line's rows (lane + 64u), clamped to the last row so every load
+// is unconditional; rows past the line are never staged, and the caller
+// passes a valid line for an inactive wave.
 template <int NR>
-__device__ inline void line_fetch(double* pre, const double* __restrict__ src, int n, int lane, bool act)
+__device__ inline void line_fetch(double* pre, const double* __restrict__ src, int n, int lane)
 {
 #pragma unroll
     for (int u = 0; u < NR; ++u) {
         const int r = lane + 64 * u;
-        pre[u] = (act && r < n) ? src[r] : 0.0;
+        pre[u] = src[r < n ? r : n - 1];
     }
 }
 
@@ -77,6 +79,17 @@ __device__ inline void line_get(double* __restrict__ dst, const double* stage, c
 #pragma unroll
     for (int u = 0; u < NR; ++u)
         if (slot[u] >= 0) dst[lane + 64 * u] = stage[slot[u]];
+}
+
+// The staging rows of a j-line are private to its wave, so the line loop
+// only has to order the wave's own LDS accesses across its lanes (LDS
+// executes a wave's accesses in order).  A workgroup barrier here would also
+// wait for the next line's global loads, serialising load and solve.
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 size_t jsolve_h_lds_bytes(int maxr, int jw, int stride, bool fl)
@@ -165,15 +178,15 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_h(const FluidDev F)
     const int step = nwgf * JW;
     const int iters = (nlines + step - 1) / step;
     int line = wg * JW + wave;
-    line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + line), n, lane, line < nlines);
+    line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (line < nlines ? line : 0)), n, lane);
     __syncthreads();
     double* row = stage + p * stride;
     for (int it = 0; it < iters; ++it, line += step) {
         const bool act = line < nlines;
         line_put<NR>(stage, pre, slot);
         const int nxt = line + step;
-        line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (nxt < nlines ? nxt : 0)), n, lane, nxt < nlines);
-        __syncthreads();
+        line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (nxt < nlines ? nxt : 0)), n, lane);
+        wave_lds_sync();
         double y[MAXR];
         double yl = 0.0, acc = 0.0;
 #pragma unroll
@@ -215,8 +228,9 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_h(const FluidDev F)
             if (t >= 0 && own) row[t] = x;
         }
         if (own) row[m - 1] = X;
-        __syncthreads();
+        wave_lds_sync();
         if (act) line_get<NR>(dstb + fidx(F, K.kofs, kfirst + line), stage, slot, lane);
+        wave_lds_sync();
     }
 }
 
@@ -258,15 +272,15 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_p(const FluidDev F)
     const int step = (int)gridDim.x * JW;
     const int iters = (nlines + step - 1) / step;
     int line = (int)blockIdx.x * JW + wave;
-    line_fetch<NR>(pre, F.hmn + (size_t)line * F.P, n, lane, line < nlines);
+    line_fetch<NR>(pre, F.hmn + (size_t)(line < nlines ? line : 0) * F.P, n, lane);
     __syncthreads();
     double* row = stage + p * stride;
     for (int it = 0; it < iters; ++it, line += step) {
         const bool act = line < nlines;
         line_put<NR>(stage, pre, slot);
         const int nxt = line + step;
-        line_fetch<NR>(pre, F.hmn + (size_t)(nxt < nlines ? nxt : 0) * F.P, n, lane, nxt < nlines);
-        __syncthreads();
+        line_fetch<NR>(pre, F.hmn + (size_t)(nxt < nlines ? nxt : 0) * F.P, n, lane);
+        wave_lds_sync();
         const int mode = line + 1;
         const double wrm = act ? F.wr[mode] : 0.0;
         const bool pin = act && (mode == F.null_mode);
@@ -348,8 +362,9 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_p(const FluidDev F)
             if (t >= 0 && own) row[t] = x;
         }
         if (own) row[m - 1] = X;
-        __syncthreads();
+        wave_lds_sync();
         if (act) line_get<NR>(F.hmn + (size_t)line * F.P, stage, slot, lane);
+        wave_lds_sync();
     }
 }
 

@@ -24,7 +24,7 @@ ERR_NAMES = {1: "DT_TOO_SMALL", 2: "CFL_TOO_LARGE", 3: "MARKER_OUTSIDE", 4: "PAR
 
 EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state", "fribo_set_markers",
            "fribo_fluid_tsch", "fribo_cfl_dt", "fribo_substep", "fribo_array_size", "fribo_get_array",
-           "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_set_profiling", "fribo_get_profile",
+           "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_test_gemm_nn", "fribo_set_profiling", "fribo_get_profile",
            "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
            "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform", "fribo_loads_transfer",
            "fribo_checkbcs", "fribo_diagnostics", "fribo_save_state", "fribo_restore_state",
@@ -83,6 +83,7 @@ def load():
     L.fribo_sync.argtypes = [vp]
     L.fribo_last_error.restype = C.c_char_p
     L.fribo_test_gemm.argtypes = [C.c_int32, C.c_int32, C.c_int32, dp, C.c_int32, dp, C.c_int32, dp, C.c_int32]
+    L.fribo_test_gemm_nn.argtypes = [C.c_int32, C.c_int32, C.c_int32, dp, C.c_int32, dp, C.c_int32, dp, C.c_int32]
     L.fribo_set_profiling.argtypes = [vp, C.c_int32]
     L.fribo_get_profile.argtypes = [vp, dp, C.POINTER(C.c_int32), C.c_int32]
     L.fribo_reset_profile.argtypes = [vp]
@@ -276,16 +277,41 @@ class FluidStep:
         return {L.fribo_profile_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(NPROF)}
 
 
-def gemm_nt(A, B):
-    """C = A @ B.T on the GPU through the library's MFMA kernel (test hook)."""
+def _padded(X, ld):
+    """Column-major copy of X with leading dimension ld (zero padding)."""
+    P = np.zeros((ld, X.shape[1]), order="F")
+    P[: X.shape[0]] = X
+    return P
+
+
+def gemm_nt(A, B, pad=0):
+    """C = A @ B.T on the GPU through the library's MFMA kernels (test hook).
+    ``pad`` rounds the leading dimensions up to a multiple of it (the layout
+    of the library's own operands, which admits the direct-to-LDS kernel)."""
     L = load()
-    A = np.asfortranarray(A, dtype=np.float64)
-    B = np.asfortranarray(B, dtype=np.float64)
     M, K = A.shape
     N = B.shape[0]
-    Cm = np.zeros((M, N), order="F")
-    _check(L.fribo_test_gemm(M, N, K, _dp(A), M, _dp(B), N, _dp(Cm), M))
-    return Cm
+    la = -(-M // pad) * pad if pad else M
+    lb = -(-N // pad) * pad if pad else N
+    Ap = _padded(np.asarray(A, dtype=np.float64), la)
+    Bp = _padded(np.asarray(B, dtype=np.float64), lb)
+    Cm = np.zeros((la, N), order="F")
+    _check(L.fribo_test_gemm(M, N, K, _dp(Ap), la, _dp(Bp), lb, _dp(Cm), la))
+    return Cm[:M]
+
+
+def gemm_nn(A, B, pad=0):
+    """C = A @ B on the GPU (the form of the transforms along y; test hook)."""
+    L = load()
+    M, K = A.shape
+    N = B.shape[1]
+    la = -(-M // pad) * pad if pad else M
+    lb = -(-K // pad) * pad if pad else K
+    Ap = _padded(np.asarray(A, dtype=np.float64), la)
+    Bp = _padded(np.asarray(B, dtype=np.float64), lb)
+    Cm = np.zeros((la, N), order="F")
+    _check(L.fribo_test_gemm_nn(M, N, K, _dp(Ap), la, _dp(Bp), lb, _dp(Cm), la))
+    return Cm[:M]
 
 
 def plain_copy(nbytes, reps):

@@ -162,6 +162,12 @@ const char* fribo_last_error(void);
 int fribo_test_gemm(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
                     int32_t ldb, double* C, int32_t ldc);
 
+/* Diagnostic entry point (no reference counterpart): C = A * B in FP64, the
+ * form of the transforms along y.  A is MxK (lda), B is KxN (ldb >= K), C is
+ * MxN (ldc), all column-major host arrays. */
+int fribo_test_gemm_nn(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
+                       int32_t ldb, double* C, int32_t ldc);
+
 /* Execution option (no reference counterpart): the axis of the Poisson
  * eigen-transform.  axis 0: along z with the caller's zmm/zmmt/wr, the
  * reference's order (phcalc_outflow.F90:4-71); axis 1: along y with the

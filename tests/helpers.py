@@ -47,6 +47,15 @@ def relmax(a, b):
     return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
 
 
+def poisson_kappa(c):
+    """Condition-number estimate of the discrete pressure Poisson operator:
+    largest eigenvalue 4/hy^2 + 4/hz^2 (smallest spacings) over the smallest
+    non-zero one, pi^2/L^2 of the longer side."""
+    hy, hz = np.diff(c.rc).min(), np.diff(c.zz).min()
+    ly, lz = c.rc[-1] - c.rc[0], c.zz[-1] - c.zz[0]
+    return (4.0 / hy**2 + 4.0 / hz**2) / (np.pi**2 / max(ly, lz) ** 2)
+
+
 def divergence(st, c):
     """max |div u| over cells (divergence.F90:28-39 without the 1/(dt al))."""
     n2m, n3m = c.params.n2 - 1, c.params.n3 - 1

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from fribois2d_amd import cases
-from helpers import case, decomposition, divergence, gpu_for, oracle_for, relmax, run_steps
+from helpers import case, decomposition, divergence, gpu_for, oracle_for, poisson_kappa, relmax, run_steps
 
 pytestmark = pytest.mark.gpu
 
@@ -34,6 +34,30 @@ def test_gemm_matches_numpy():
         err = np.max(np.abs(C - ref) / scale)
         print(f"gemm {M}x{N}x{K}: max err/scale = {err:.3e}", flush=True)
         assert err < 1e-14
+
+
+@pytest.mark.parametrize("form", ["nt", "nn"])
+def test_gemm_large_tiles_match_numpy(form):
+    """Shapes of at least 192 96x128 tiles take the direct-to-LDS kernel when
+    the operands are laid out like the library's own (leading dimensions
+    padded to 16); odd extents and K tails exercise its masked loads and the
+    cleared k-rows.  Unpadded operands take the register-staged kernel; both
+    must agree with numpy."""
+    from fribois2d_amd import gpu
+    rng = np.random.default_rng(11)
+    for (M, N, K) in [(1100, 2000, 77), (1153, 1665, 333), (1536, 2048, 1536)]:
+        A = rng.standard_normal((M, K))
+        B = rng.standard_normal((N, K)) if form == "nt" else rng.standard_normal((K, N))
+        f = gpu.gemm_nt if form == "nt" else gpu.gemm_nn
+        ref = A @ (B.T if form == "nt" else B)
+        scale = np.abs(A) @ np.abs(B.T if form == "nt" else B)
+        C16 = f(A, B, pad=16)
+        err = np.max(np.abs(C16 - ref) / scale)
+        print(f"gemm {form} {M}x{N}x{K} padded: max err/scale = {err:.3e}", flush=True)
+        assert err < 1e-14
+        if K < 400:
+            C1 = f(A, B)
+            np.testing.assert_array_equal(C1, C16)
 
 
 def test_metrics_bit_exact():
@@ -142,10 +166,17 @@ def test_reference_examples(name, axis):
     np.testing.assert_array_equal(g.get("lmind"), o.get("lmind"))
     np.testing.assert_array_equal(g.get("eulins"), o.get("eulins"))
     sg = g.get_state()
+    # With the transform along z the device applies the oracle's own
+    # eigenvectors and the pressure agrees to 1e-10.  The transform along y
+    # diagonalises the other operator: the same discrete solution, reached
+    # through a different rounding, which the Poisson solve amplifies by up
+    # to its condition number (about 7e6 on the 561x731 grid).  Its pressure
+    # bar is the first-order bound kappa * eps.
+    pbar = 1e-10 if g.transform == "z" else max(1e-10, poisson_kappa(c) * np.finfo(np.float64).eps)
     for f in ("q2", "q3", "pr"):
         e = relmax(sg[f], o.get(f))
         print(f"{name} {f}: {e:.2e}", flush=True)
-        assert e <= 1e-10, f
+        assert e <= (pbar if f == "pr" else 1e-10), f
 
 
 def test_large_properties():
