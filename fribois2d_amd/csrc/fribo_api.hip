@@ -432,7 +432,7 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
                            F.n3m, hipMemcpyHostToDevice));
     }
     if (!e) e = dalloc(c, &F.cfl_part, 2 * FRIBO_NPART);
-    F.npr_part = ((F.n2m + 61) / 62) * ((F.n3m + 15) / 16);
+    F.npr_part = fribo_npr_part(F.n2m, F.n3m);
     if (!e) e = dalloc(c, &F.pr_part, F.npr_part);
     if (!e) e = setup_ksys(c, F.ks[0], F.n3m, 1, 0, 0);
     if (!e) e = setup_ksys(c, F.ks[1], F.n3m - 1, 2, 0, 0);

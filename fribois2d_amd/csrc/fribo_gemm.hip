@@ -41,7 +41,13 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 #define GBK 16
 #ifndef GEMM_DMA_BK
-#define GEMM_DMA_BK 32
+#define GEMM_DMA_BK 16
+#endif
+#ifndef GEMM_DMA_WM
+#define GEMM_DMA_WM 2
+#endif
+#ifndef GEMM_DMA_WN
+#define GEMM_DMA_WN 4
 #endif
 
 template <int BM, int BN, int WM, int WN, bool NN>
@@ -142,19 +148,20 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
         }
 }
 
-template <int BK, bool NN>
-__global__ __launch_bounds__(512) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A, int lda,
-                                                  const double* __restrict__ B, int ldb, double* __restrict__ C,
-                                                  int ldc, const DevScalars* S)
+template <int BK, bool NN, int WM = 2, int WN = 4>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A,
+                                                           int lda, const double* __restrict__ B, int ldb,
+                                                           double* __restrict__ C, int ldc, const DevScalars* S)
 {
-    constexpr int BM = 96, BN = 128, WM = 2, WN = 4;
+    constexpr int BM = 96, BN = 128, NW = WM * WN, NT = 64 * NW;
     constexpr int TM = BM / WM, TN = BN / WN, RM = TM / 16, RN = TN / 16;
     constexpr int LA = 112, SLA = LA / 2;                   // A: k-rows, 56 slots (48 used)
     constexpr int LB = NN ? BK + 2 : 144, SLB = LB / 2;     // B: n-rows (NN) or k-rows (NT)
     constexpr int SA = BK * LA, SB = NN ? BN * LB : BK * LB;   // doubles per stage
     constexpr int IA = SA / 128, IB = SB / 128;             // 1 KB instructions per stage
-    constexpr int NI = (IA + IB + 7) / 8;                   // per wave
+    constexpr int NI = (IA + IB + NW - 1) / NW;             // per wave
     static_assert(SA % 128 == 0 && SB % 128 == 0, "stage size");
+    static_assert(TM % 16 == 0 && TN % 16 == 0 && BM % WM == 0 && BN % WN == 0, "wave tile");
     if (S && S->err) return;
     __shared__ __attribute__((aligned(1024))) double As[2 * SA];
     __shared__ __attribute__((aligned(1024))) double Bs[2 * SB];
@@ -165,7 +172,7 @@ __global__ __launch_bounds__(512) void k_gemm_dma(int M, int N, int K, const dou
     auto issue = [&](int k0, int st) {
 #pragma unroll
         for (int r = 0; r < NI; ++r) {
-            const int ii = wave + 8 * r;
+            const int ii = wave + NW * r;
             if (ii < IA) {
                 const int slot = ii * 64 + lane, kk = slot / SLA, q = slot % SLA;
                 if (q < BM / 2 && k0 + kk < K && i0 + 2 * q < M)
@@ -205,12 +212,12 @@ __global__ __launch_bounds__(512) void k_gemm_dma(int M, int N, int K, const dou
         __syncthreads();                      // ... and every other wave's
         const int kv = K - s * BK;            // valid k-rows of this slice
         if (kv < BK) {
-            for (int e = t; e < (BK - kv) * LA; e += 512) As[st * SA + kv * LA + e] = 0.0;
+            for (int e = t; e < (BK - kv) * LA; e += NT) As[st * SA + kv * LA + e] = 0.0;
             if (NN) {
-                for (int e = t; e < BN * (BK - kv); e += 512)
+                for (int e = t; e < BN * (BK - kv); e += NT)
                     Bs[st * SB + (e / (BK - kv)) * LB + kv + e % (BK - kv)] = 0.0;
             } else {
-                for (int e = t; e < (BK - kv) * LB; e += 512) Bs[st * SB + kv * LB + e] = 0.0;
+                for (int e = t; e < (BK - kv) * LB; e += NT) Bs[st * SB + kv * LB + e] = 0.0;
             }
             __syncthreads();
         }
@@ -264,8 +271,8 @@ static void launch_gemm(int M, int N, int K, const double* A, int lda, const dou
                         const DevScalars* S, hipStream_t st)
 {
     if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192 && dma_ok<NN>(M, N, K, A, lda, B, ldb)) {
-        hipLaunchKernelGGL((k_gemm_dma<GEMM_DMA_BK, NN>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M, N, K,
-                           A, lda, B, ldb, C, ldc, S);
+        hipLaunchKernelGGL((k_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN>), dim3(cdiv(M, 96), cdiv(N, 128)),
+                           dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), 0, st, M, N, K, A, lda, B, ldb, C, ldc, S);
     } else if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
         hipLaunchKernelGGL((k_gemm_nt<96, 128, 2, 4, NN>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M, N,
                            K, A, lda, B, ldb, C, ldc, S);

@@ -13,7 +13,24 @@
 #define FRIBO_DIMSD 9
 #define FRIBO_KMAXCH 64      // max chunks of a partitioned k-line (one wave)
 #define FRIBO_NPART 512      // partial sums of the CFL reduction
-#define FRIBO_LWORK 20       // per-marker record of the load computation
+#define FRIBO_LWORK 20
+
+// Marching tiles of the stencil kernels (k_rhs, k_div, k_upd): a wave owns
+// FRIBO_WJ output columns (plus a halo lane each side) and MK planes
+// (FRIBO_MK_RHS for k_rhs, whose register window limits its occupancy;
+// FRIBO_MK for k_div and k_upd, whose shorter tiles put more waves in
+// flight).  k_upd writes one pressure partial per tile.
+#ifndef FRIBO_MK
+#define FRIBO_MK 8
+#endif
+#ifndef FRIBO_MK_RHS
+#define FRIBO_MK_RHS 16
+#endif
+#define FRIBO_WJ 62
+inline int fribo_npr_part(int n2m, int n3m)
+{
+    return ((n2m + FRIBO_WJ - 1) / FRIBO_WJ) * ((n3m + FRIBO_MK - 1) / FRIBO_MK);
+}       // per-marker record of the load computation
 
 struct DevScalars {
     // step inputs, written by the host before every step (one 24-byte copy)

@@ -8,8 +8,9 @@
 //               compute_AB + GEsolve (MLSint.F90:115-166,284-486,
 //               utils_math.f90:637-725) and the momentum coefficient cl
 //               (forcing_MLS.F90:72-81); one thread per (marker, direction)
-//   k_mls_radix orders the 9 contributions of every marker by (cell, marker)
-//               (stable LDS radix sort on the cell; bitonic LDS blocks plus
+//   k_mls_order orders the 9 contributions of every marker by (cell, marker)
+//               (counting sort over the supports' bounding box of cells, or a
+//               stable LDS radix sort on the cell; bitonic LDS blocks plus
 //               global merge passes for very large marker sets); equal cells
 //               form contiguous segments
 // After the step, on request (fluid_tsch.F90:167-177):
@@ -262,8 +263,10 @@ __global__ __launch_bounds__(64) void k_mls_sup(const FluidDev F)
 // Ordering of the (cell, entry) keys of each direction, ascending.
 //
 // Up to RADIX_MAX keys (the usual case: nine supports per marker) one
-// workgroup per direction runs a stable LSD radix sort on the cell field
-// (8-bit digits, as many passes as the grid's cell count needs).  The keys are
+// workgroup per direction orders them (k_mls_order): a counting sort over the
+// bounding box of the supports' cells when the box is small, else a stable
+// LSD radix sort on the cell field (8-bit digits, as many passes as the grid's
+// cell count needs).  For the radix sort: the keys are
 // generated in entry order, so stability yields (cell, entry) order; an unused
 // support (KEY_NONE) takes the largest digit value and lands at the end.  Each
 // pass holds the keys in registers (thread-contiguous runs of 64-key rounds per
@@ -283,15 +286,12 @@ __global__ __launch_bounds__(64) void k_mls_sup(const FluidDev F)
 // index of the first element of compare-exchange pair q at a power-of-two stride
 __device__ inline int pair_lo(int q, int stride) { return ((q & ~(stride - 1)) << 1) | (q & (stride - 1)); }
 
-__global__ __launch_bounds__(1024) void k_mls_radix(const FluidDev F, int nbits)
+// Radix path of k_mls_order: sk (RADIX_MAX keys), hist (256 x RADIX_WAVES)
+// and wsum (RADIX_WAVES) are the workgroup's LDS.
+__device__ void radix_sort_block(unsigned long long* __restrict__ gk, int n, int nbits, unsigned long long* sk,
+                                 unsigned int* hist, unsigned int* wsum)
 {
-    __shared__ unsigned long long sk[RADIX_MAX];
-    __shared__ unsigned int hist[256 * RADIX_WAVES];   // [digit][wave]
-    __shared__ unsigned int wsum[RADIX_WAVES];
-    if (F.S->err) return;
-    const int d = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int n = F.nlm * FRIBO_DIMSD;
-    unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = ((n + 64 * RADIX_WAVES - 1) / (64 * RADIX_WAVES)) * 64;   // keys per wave
     const unsigned int none = (nbits >= 32) ? 0xffffffffu : ((1u << nbits) - 1u);
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -384,6 +384,139 @@ __global__ __launch_bounds__(1024) void k_mls_radix(const FluidDev F, int nbits)
     for (int i = t; i < n; i += 1024) gk[i] = sk[i];
 }
 
+// Block-wide exclusive scan of one value per thread (1024 threads); returns
+// the thread's prefix, *total gets the sum.  wsum: RADIX_WAVES words of LDS.
+__device__ inline unsigned int block_exscan(unsigned int v, unsigned int* wsum, unsigned int* total)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    unsigned int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    unsigned int base = 0u, all = 0u;
+    for (int i = 0; i < RADIX_WAVES; ++i) {
+        if (i < w) base += wsum[i];
+        all += wsum[i];
+    }
+    *total = all;
+    return base + inc - v;
+}
+
+// Ordering of one direction's keys (one workgroup of 1024 threads).  The
+// supports of a body span a small box of cells: when its W x H cells fit
+// CELL_MAX, every key is counted into its box cell (LDS atomics), the counts
+// are scanned into cell ranges, each key claims a slot of its range and
+// records its entry there, and finally each key's place within its cell is
+// its rank among the cell's entries -- the (cell, entry) order, whatever the
+// order of the claims.  Larger boxes take the radix sort.
+#define CELL_MAX 16384
+__global__ __launch_bounds__(1024) void k_mls_order(const FluidDev F, int nbits)
+{
+    __shared__ unsigned long long sk[RADIX_MAX];         // radix keys, or cnt[CELL_MAX] + ent[RADIX_MAX] (u32)
+    __shared__ unsigned int hist[256 * RADIX_WAVES];
+    __shared__ unsigned int wsum[RADIX_WAVES];
+    __shared__ int box[4];
+    if (F.S->err) return;
+    const int d = blockIdx.x, t = threadIdx.x, lane = t & 63;
+    const int n = F.nlm * FRIBO_DIMSD;
+    unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
+    const unsigned int n2u = (unsigned int)F.n2;
+    constexpr int KPT = RADIX_MAX / 1024;                // keys per thread
+    unsigned long long key[KPT];
+    int jmin = 0x7fffffff, jmax = -1, kmin = 0x7fffffff, kmax = -1;
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+        const int i = t + 1024 * u;
+        key[u] = i < n ? gk[i] : KEY_NONE;
+    }
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+        if (key[u] != KEY_NONE) {
+            const unsigned int cell = (unsigned int)(key[u] >> ENT_BITS);
+            const int j = (int)(cell % n2u), k = (int)(cell / n2u);
+            jmin = min(jmin, j);
+            jmax = max(jmax, j);
+            kmin = min(kmin, k);
+            kmax = max(kmax, k);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        jmin = min(jmin, __shfl_xor(jmin, o, 64));
+        jmax = max(jmax, __shfl_xor(jmax, o, 64));
+        kmin = min(kmin, __shfl_xor(kmin, o, 64));
+        kmax = max(kmax, __shfl_xor(kmax, o, 64));
+    }
+    if (t < 4) box[t] = t & 1 ? -1 : 0x7fffffff;
+    __syncthreads();
+    if (lane == 0) {
+        atomicMin(&box[0], jmin);
+        atomicMax(&box[1], jmax);
+        atomicMin(&box[2], kmin);
+        atomicMax(&box[3], kmax);
+    }
+    __syncthreads();
+    const int j0 = box[0], k0 = box[2];
+    const int W = box[1] >= box[0] ? box[1] - box[0] + 1 : 0;
+    const int H = box[3] >= box[2] ? box[3] - box[2] + 1 : 0;
+    if ((long)W * H > CELL_MAX) {
+        radix_sort_block(gk, n, nbits, sk, hist, wsum);
+        return;
+    }
+    const int nb = W * H;
+    unsigned int* cnt = reinterpret_cast<unsigned int*>(sk);
+    unsigned int* ent = cnt + CELL_MAX;
+    for (int i = t; i < nb; i += 1024) cnt[i] = 0u;
+    __syncthreads();
+    int lc[KPT];
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+        lc[u] = -1;
+        if (key[u] != KEY_NONE) {
+            const unsigned int cell = (unsigned int)(key[u] >> ENT_BITS);
+            lc[u] = ((int)(cell % n2u) - j0) + ((int)(cell / n2u) - k0) * W;
+            __hip_atomic_fetch_add(&cnt[lc[u]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    __syncthreads();
+    // cell ranges: thread t scans cells [t*per, (t+1)*per)
+    const int per = (nb + 1023) / 1024;
+    unsigned int loc = 0u;
+    for (int i = t * per; i < min(nb, (t + 1) * per); ++i) loc += cnt[i];
+    unsigned int nvalid;
+    unsigned int run = block_exscan(loc, wsum, &nvalid);
+    for (int i = t * per; i < min(nb, (t + 1) * per); ++i) {
+        const unsigned int c = cnt[i];
+        cnt[i] = run;
+        run += c;
+    }
+    __syncthreads();
+    // claims: afterwards cnt[c] is the end of cell c's range
+#pragma unroll
+    for (int u = 0; u < KPT; ++u)
+        if (lc[u] >= 0) {
+            const unsigned int pos =
+                __hip_atomic_fetch_add(&cnt[lc[u]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            ent[pos] = (unsigned int)(key[u] & ENT_MASK);
+        }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KPT; ++u)
+        if (lc[u] >= 0) {
+            const unsigned int e = cnt[lc[u]], s0 = lc[u] > 0 ? cnt[lc[u] - 1] : 0u;
+            const unsigned int me = (unsigned int)(key[u] & ENT_MASK);
+            unsigned int rank = 0u;
+            for (unsigned int q = s0; q < e; ++q) rank += ent[q] < me ? 1u : 0u;
+            gk[s0 + rank] = key[u];
+        }
+    for (int i = (int)nvalid + t; i < n; i += 1024) gk[i] = KEY_NONE;
+}
+
 __global__ __launch_bounds__(1024) void k_mls_sort_local(const FluidDev F, int size0, int first)
 {
     __shared__ unsigned long long sk[SORT_BLK];
@@ -444,7 +577,7 @@ void launch_mls_setup(const FluidDev& F, hipStream_t st)
     if (F.nlm * FRIBO_DIMSD <= RADIX_MAX && ncell < 0xffffffffull) {
         int nbits = 1;
         while (nbits < 32 && (1ull << nbits) <= ncell) ++nbits;   // 2^nbits > every cell + 1
-        hipLaunchKernelGGL(k_mls_radix, dim3(2), dim3(1024), 0, st, F, nbits);
+        hipLaunchKernelGGL(k_mls_order, dim3(2), dim3(1024), 0, st, F, nbits);
         return;
     }
     const int nblk = F.maxkeys / SORT_BLK;

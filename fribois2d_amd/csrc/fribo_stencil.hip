@@ -380,11 +380,12 @@ void launch_bc(const FluidDev& F, int n, hipStream_t st)
 // ---------------------------------------------------------------------------
 // Marching layout shared by k_rhs, k_div and k_upd: one 64-lane wave owns
 // columns j = 62*bx + lane (lanes 1..62 produce output, lanes 0 and 63 are
-// the j-1 / j+1 halo) and marches MK planes k = MK*by+1 .. MK*by+MK, keeping
-// a register window of the planes it needs; j-neighbours come from the
-// adjacent lanes, so every field value is read from HBM once.
-#define MK 16
-#define WJ 62
+// the j-1 / j+1 halo) and marches MK planes k = MK*by+1 .. MK*by+MK (MKR for
+// k_rhs), keeping a register window of the planes it needs; j-neighbours come
+// from the adjacent lanes, so every field value is read from HBM once.
+#define MK FRIBO_MK
+#define MKR FRIBO_MK_RHS
+#define WJ FRIBO_WJ
 
 __device__ inline double up1(double v) { return __shfl_up(v, 1, 64); }
 __device__ inline double dn1(double v) { return __shfl_down(v, 1, 64); }
@@ -398,10 +399,10 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
     if (S->err) return;
     const int lane = threadIdx.x;
     const int j = WJ * blockIdx.x + lane;
-    const int k0 = MK * blockIdx.y + 1;
+    const int k0 = MKR * blockIdx.y + 1;
     const int n2m = F.n2m, n3m = F.n3m;
     if (k0 > n3m) return;
-    const int k1 = min(k0 + MK - 1, n3m);
+    const int k1 = min(k0 + MKR - 1, n3m);
     const bool inj = (j >= 1 && j <= F.n2);
     const bool out = (lane >= 1 && lane <= WJ && j <= n2m);
     const double dt = S->dt, ga = S->ga, ro = S->ro, al = S->al, alre = S->alre, pend = S->pend;
@@ -513,7 +514,7 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
 
 void launch_rhs(const FluidDev& F, hipStream_t st)
 {
-    dim3 g((F.n2m + WJ - 1) / WJ, (F.n3m + MK - 1) / MK);
+    dim3 g((F.n2m + WJ - 1) / WJ, (F.n3m + MKR - 1) / MKR);
     hipLaunchKernelGGL(k_rhs, g, dim3(64), 0, st, F);
 }
 

@@ -148,23 +148,53 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_h(const FluidDev F)
         double* lwl = lk2 + 384;
         double* lib = lwl + 64;
         stage = lib + 64 + wave * 64 * stride;
+        // every load of the copy is issued before the first LDS store
         const int lenf = K.maxm * 64;
-        for (int i = tid; i < NR * 64; i += 64 * JW) {
-            const bool ok = i < lenf;
-            lS[i] = ok ? K.S[i] : 0.0;
-            la[i] = ok ? K.a[i] : 0.0;
-            li[i] = ok ? K.inv[i] : 0.0;
-            lg[i] = ok ? K.g[i] : 0.0;
-            lv[i] = ok ? K.vp[i] : 0.0;
-            lc[i] = ok ? K.cp[i] : 0.0;
+        constexpr int NCP = (NR * 64 + 64 * JW - 1) / (64 * JW);
+        constexpr int NCK = (384 + 64 * JW - 1) / (64 * JW);
+        double tS[NCP], ta[NCP], ti[NCP], tg[NCP], tv[NCP], tc[NCP], t1[NCK], t2[NCK];
+#pragma unroll
+        for (int u = 0; u < NCP; ++u) {
+            const int i = tid + u * 64 * JW;
+            const int ic = i < lenf ? i : 0;
+            tS[u] = K.S[ic];
+            ta[u] = K.a[ic];
+            ti[u] = K.inv[ic];
+            tg[u] = K.g[ic];
+            tv[u] = K.vp[ic];
+            tc[u] = K.cp[ic];
         }
-        for (int i = tid; i < 384; i += 64 * JW) {
-            lk1[i] = K.k1[i];
-            lk2[i] = K.k2[i];
+#pragma unroll
+        for (int u = 0; u < NCK; ++u) {
+            const int i = tid + u * 64 * JW;
+            t1[u] = K.k1[i < 384 ? i : 0];
+            t2[u] = K.k2[i < 384 ? i : 0];
+        }
+        const double twl = K.wl[lane], tib = K.ib[lane];
+#pragma unroll
+        for (int u = 0; u < NCP; ++u) {
+            const int i = tid + u * 64 * JW;
+            if (i < NR * 64) {
+                const bool ok = i < lenf;
+                lS[i] = ok ? tS[u] : 0.0;
+                la[i] = ok ? ta[u] : 0.0;
+                li[i] = ok ? ti[u] : 0.0;
+                lg[i] = ok ? tg[u] : 0.0;
+                lv[i] = ok ? tv[u] : 0.0;
+                lc[i] = ok ? tc[u] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NCK; ++u) {
+            const int i = tid + u * 64 * JW;
+            if (i < 384) {
+                lk1[i] = t1[u];
+                lk2[i] = t2[u];
+            }
         }
         if (tid < 64) {
-            lwl[tid] = K.wl[tid];
-            lib[tid] = K.ib[tid];
+            lwl[tid] = twl;
+            lib[tid] = tib;
         }
         fS = lS; fa = la; fi = li; fg = lg; fv = lv; fc = lc; fk1 = lk1; fk2 = lk2; fwl = lwl; fib = lib;
     } else {
@@ -255,11 +285,28 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_p(const FluidDev F)
     double* cap = cac + NR * 64;
     double* stage = cap + NR * 64 + wave * 64 * stride;
     const int lenf = Q.maxm * 64;
-    for (int i = tid; i < NR * 64; i += 64 * JW) {
-        const bool ok = i < lenf;
-        cam[i] = ok ? Q.am[i] : 0.0;
-        cac[i] = ok ? Q.ac[i] : 0.0;
-        cap[i] = ok ? Q.ap[i] : 0.0;
+    {
+        // every load of the copy is issued before the first LDS store
+        constexpr int NCP = (NR * 64 + 64 * JW - 1) / (64 * JW);
+        double tm[NCP], tc[NCP], tp[NCP];
+#pragma unroll
+        for (int u = 0; u < NCP; ++u) {
+            const int i = tid + u * 64 * JW;
+            const int ic = i < lenf ? i : 0;
+            tm[u] = Q.am[ic];
+            tc[u] = Q.ac[ic];
+            tp[u] = Q.ap[ic];
+        }
+#pragma unroll
+        for (int u = 0; u < NCP; ++u) {
+            const int i = tid + u * 64 * JW;
+            if (i < NR * 64) {
+                const bool ok = i < lenf;
+                cam[i] = ok ? tm[u] : 0.0;
+                cac[i] = ok ? tc[u] : 0.0;
+                cap[i] = ok ? tp[u] : 0.0;
+            }
+        }
     }
     const int p = lane;
     const bool own = p < nch;
