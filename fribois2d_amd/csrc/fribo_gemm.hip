@@ -47,7 +47,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 #define GEMM_DMA_WM 2
 #endif
 #ifndef GEMM_DMA_WN
-#define GEMM_DMA_WN 4
+#define GEMM_DMA_WN 8
 #endif
 
 template <int BM, int BN, int WM, int WN, bool NN>

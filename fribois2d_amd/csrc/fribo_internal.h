@@ -146,6 +146,17 @@ struct FluidDev {
     DevScalars* S;
 };
 
+// Value of v in lane l of the wave (l wave-uniform): a matrix row that one
+// lane loaded and every lane of the wave uses, moved through scalar registers
+// instead of a scalar memory load per use.
+__device__ inline double lane_bcast(double v, int l)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __host__ __device__ inline size_t fidx(const FluidDev& F, int j, int k)
 {
     return (size_t)(j - 1) + (size_t)(k + 2) * (size_t)F.P;

@@ -386,6 +386,7 @@ void launch_bc(const FluidDev& F, int n, hipStream_t st)
 #define MK FRIBO_MK
 #define MKR FRIBO_MK_RHS
 #define WJ FRIBO_WJ
+static_assert(MK <= 64 && MKR <= 64, "one plane of a tile per lane");
 
 __device__ inline double up1(double v) { return __shfl_up(v, 1, 64); }
 __device__ inline double dn1(double v) { return __shfl_down(v, 1, 64); }
@@ -431,6 +432,11 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
     const double qb2s = F.qb2s[jj], qb2n = F.qb2n[jj];
     const double qb3s = F.qb3s[jj], qb3sm = F.qb3s[jj > 1 ? jj - 1 : 1];
     const double qb3n = F.qb3n[jj], qb3nm = F.qb3n[jj > 1 ? jj - 1 : 1];
+    // per-plane coefficients of the tile: lane l holds plane k0 + l
+    const int kl = min(k0 + lane, n3m);
+    const double l_udx3m = F.udx3m[kl], l_udx3c = F.udx3c[kl];
+    const double l_ap3sk = F.ap3sk[kl], l_ac3sk = F.ac3sk[kl], l_am3sk = F.am3sk[kl];
+    const double l_ap3ck = F.ap3ck[kl], l_ac3ck = F.ac3ck[kl], l_am3ck = F.am3ck[kl];
     for (int k = k0; k <= k1; ++k, c += P) {
         const double q2p = q2n, q3p = q3n, prc = prn, ru2c = ru2n, ru3c = ru3n;
         if (k < k1) {
@@ -440,6 +446,12 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
             ru2n = out ? ru2[c + P] : 0.0;
             ru3n = out ? ru3[c + P] : 0.0;
         }
+        const int kk = k - k0;
+        const double udx3m_k = lane_bcast(l_udx3m, kk), udx3c_k = lane_bcast(l_udx3c, kk);
+        const double ap3sk_k = lane_bcast(l_ap3sk, kk), ac3sk_k = lane_bcast(l_ac3sk, kk);
+        const double am3sk_k = lane_bcast(l_am3sk, kk);
+        const double ap3ck_k = lane_bcast(l_ap3ck, kk), ac3ck_k = lane_bcast(l_ac3ck, kk);
+        const double am3ck_k = lane_bcast(l_am3ck, kk);
         // neighbours in j
         const double q2E = dn1(q2c), q2W = up1(q2c), q2Em = dn1(q2m), q2Ep = dn1(q2p);
         const double q3W = up1(q3c), q3Wp = up1(q3p), q3E = dn1(q3c);
@@ -453,17 +465,17 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
                 double dphv = -h22;
                 double h23;
                 if (k >= 2 && k <= n3m - 1) {
-                    h23 = ((q3p + q3Wp) * (q2p + q2c) - (q3c + q3W) * (q2c + q2m)) * F.udx3m[k] * 0.25;
+                    h23 = ((q3p + q3Wp) * (q2p + q2c) - (q3c + q3W) * (q2c + q2m)) * udx3m_k * 0.25;
                 } else if (k == n3m) {
-                    h23 = (qb2n * (qb3n + qb3nm) - (q3c + q3W) * (q2c + q2m) * 0.5) * F.udx3m[k] * 0.5;
+                    h23 = (qb2n * (qb3n + qb3nm) - (q3c + q3W) * (q2c + q2m) * 0.5) * udx3m_k * 0.5;
                 } else {
-                    h23 = ((q3p + q3Wp) * (q2c + q2p) * 0.5 - qb2s * (qb3s + qb3sm)) * F.udx3m[1] * 0.5;
+                    h23 = ((q3p + q3Wp) * (q2c + q2p) * 0.5 - qb2s * (qb3s + qb3sm)) * udx3m_k * 0.5;
                 }
                 dphv = dphv - h23;
                 double d22q2 = q2E * ap2 + q2c * ac2 + q2W * am2;
                 double d33q2, dpx22;
                 if (k >= 2 && k <= n3m - 1) {
-                    d33q2 = q2p * F.ap3sk[k] + q2c * F.ac3sk[k] + q2m * F.am3sk[k];
+                    d33q2 = q2p * ap3sk_k + q2c * ac3sk_k + q2m * am3sk_k;
                     double udx2 = al * udx2c;
                     dpx22 = (prc - prW) * udx2;
                 } else if (k == 1) {
@@ -489,15 +501,15 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
                     const double q3mm = (j == 1) ? q3c : q3W;     // jmv clamps at 1
                     double h32 = (((q2E + q2Em) * (q3pp + q3c)) - ((q2c + q2m) * (q3c + q3mm))) * udx2m * 0.25;
                     double a = q3p + q3c, b = q3c + q3m;
-                    double h33 = (a * a - b * b) * F.udx3c[k] * 0.25;
+                    double h33 = (a * a - b * b) * udx3c_k * 0.25;
                     qcapv = -(h32 + h33);
                 }
-                const double udx3 = al * F.udx3c[k];
+                const double udx3 = al * udx3c_k;
                 double dq32;
                 if (j == 1) dq32 = q3E * ap3 + q3c * ac3 + F.q3lw * am3;
                 else if (j == n2m) dq32 = F.q3uw * ap3 + q3c * ac3 + q3W * am3;
                 else dq32 = q3E * ap3 + q3c * ac3 + q3W * am3;
-                double dq33 = q3p * F.ap3ck[k] + q3c * F.ac3ck[k] + q3m * F.am3ck[k];
+                double dq33 = q3p * ap3ck_k + q3c * ac3ck_k + q3m * am3ck_k;
                 double dcq3 = dq32 + dq33;
                 double dpx33 = (prc - prm) * udx3;
                 rhs3[c] = (ga * qcapv + ro * ru3c + alre * dcq3 - dpx33) * dt;
@@ -540,6 +552,7 @@ __global__ __launch_bounds__(64) void k_div(const FluidDev F)
     double* __restrict__ dph = F.dph;
     double q3c = inj ? q3[c] : 0.0;
     double q2n = inj ? q2[c] : 0.0, q3n = inj ? q3[c + P] : 0.0;
+    const double l_udx3m = F.udx3m[min(k0 + lane, F.n3m)];   // lane l: plane k0 + l
     for (int k = k0; k <= k1; ++k, c += P) {
         const double q2c = q2n, q3p = q3n;
         if (k < k1) {
@@ -547,8 +560,9 @@ __global__ __launch_bounds__(64) void k_div(const FluidDev F)
             q3n = inj ? q3[c + 2 * P] : 0.0;
         }
         const double q2E = dn1(q2c);
+        const double udx3m_k = lane_bcast(l_udx3m, k - k0);
         if (out) {
-            double dq = (q2E - q2c) * udx2m + (q3p - q3c) * F.udx3m[k];
+            double dq = (q2E - q2c) * udx2m + (q3p - q3c) * udx3m_k;
             dph[c] = dq * usdtal;
         }
         q3c = q3p;
@@ -596,7 +610,15 @@ __global__ __launch_bounds__(64) void k_upd(const FluidDev F)
         // plane k+1 of dph and plane k of q2/q3/pr are loaded one iteration ahead
         double dn = (inj && k0 < n3m) ? dph[c + P] - doff : 0.0;
         double q2n = out ? q2[c] : 0.0, q3n = out ? q3[c] : 0.0, prn = out ? pr[c] : 0.0;
+        // per-plane coefficients of the tile: lane l holds plane k0 + l
+        const int kl = min(k0 + lane, n3m);
+        const double l_udx3c = F.udx3c[kl], l_apphk = F.apphk[kl], l_acphk = F.acphk[kl];
+        const double l_amphk = F.amphk[kl], l_dz = F.zz[kl + 1] - F.zz[kl];
         for (int k = k0; k <= k1; ++k, c += P) {
+            const int kk = k - k0;
+            const double udx3c_k = lane_bcast(l_udx3c, kk), apphk_k = lane_bcast(l_apphk, kk);
+            const double acphk_k = lane_bcast(l_acphk, kk), amphk_k = lane_bcast(l_amphk, kk);
+            const double dz_k = lane_bcast(l_dz, kk);
             const double dp = dn, q2v = q2n, q3v = q3n, prv0 = prn;
             if (k < k1) {
                 dn = (inj && k + 1 < n3m) ? dph[c + 2 * P] - doff : 0.0;
@@ -612,14 +634,14 @@ __global__ __launch_bounds__(64) void k_upd(const FluidDev F)
                 const double dKp = (k == n3m) ? dc : dp;
                 q2[c] = q2v - (dc - dJm) * usujm;
                 if (k >= 2) {
-                    const double usukm = al * dt * F.udx3c[k];
+                    const double usukm = al * dt * udx3c_k;
                     q3[c] = q3v - (dc - dm) * usukm;
                 }
                 double lap = (dJp * apj + dc * acj + dJm * amj) +
-                             (dKp * F.apphk[k] + dc * F.acphk[k] + dKm * F.amphk[k]);
+                             (dKp * apphk_k + dc * acphk_k + dKm * amphk_k);
                 double prv = (prv0 - pend) + dc - be * lap;
                 pr[c] = prv;
-                part = part + prv * (F.zz[k + 1] - F.zz[k]) * dy;
+                part = part + prv * dz_k * dy;
             }
             dm = dc;
             dc = dp;

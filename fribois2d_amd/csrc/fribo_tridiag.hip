@@ -514,25 +514,56 @@ __device__ inline void kbound(const FluidDev& F, const KSys& K, int f, int j, co
     }
 }
 
+// Row coefficients of the interior rows s..s+m-1 of a chunk, one row per
+// lane (lane l holds row s + l + 64g in register g; clamped past the end):
+// every lane of a workgroup works on the same chunk, so each step takes its
+// row from a lane instead of issuing a scalar load it must wait for.
+template <int NG>
+__device__ inline void chunk_rows(const double* __restrict__ a, int s, int m, double (&v)[NG])
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int r = lane + 64 * g;
+        v[g] = a[s + (r < m ? r : (m > 0 ? m - 1 : 0))];
+    }
+}
+
+template <int NG>
+__device__ inline double chunk_row(const double (&v)[NG], int t)
+{
+    return lane_bcast(NG == 1 || t < 64 ? v[0] : v[NG - 1], t & 63);
+}
+
 // Pass 1: thread (j, chunk p) eliminates the interior rows of its chunk and
 // leaves the chunk's separator data (sum g*y', last y', separator rhs).  The
 // interior rows are aligned to the end of the MAXM-step loop, so every step
 // loads unconditionally and the loads of all steps can be issued together.
+// Lanes past the last column load column n2m and store nothing.
 template <int MAXM>
 __global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
 {
+    constexpr int NG = (MAXM + 63) / 64;
+    static_assert(NG <= 2, "chunk rows");
     const DevScalars* S = F.S;
     if (S->err) return;
     const int f = blockIdx.z, p = blockIdx.y;
     const KSys& K = F.ks[f];
+    if (p >= K.nch) return;
     const int j = blockIdx.x * 64 + threadIdx.x + 1;
-    if (p >= K.nch || j > F.n2m) return;
-    const double* __restrict__ src = (f == 0 ? F.sA : F.sB) + fidx(F, j, K.kofs);
+    const bool act = j <= F.n2m;
+    const int jc = act ? j : F.n2m;
+    const double* __restrict__ src = (f == 0 ? F.sA : F.sB) + fidx(F, jc, K.kofs);
     const size_t P = F.P;
     const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;
     const int off = MAXM - m;
+    double cS[NG], ca[NG], ci[NG], cg[NG];
+    chunk_rows<NG>(K.S, s, m, cS);
+    chunk_rows<NG>(K.a, s, m, ca);
+    chunk_rows<NG>(K.inv, s, m, ci);
+    chunk_rows<NG>(K.g, s, m, cg);
     double b0, bn;
-    kbound(F, K, f, j, S, b0, bn);
+    kbound(F, K, f, jc, S, b0, bn);
     // all loads of the chunk first (one memory round trip), then the recurrence
     double dv[MAXM];
 #pragma unroll
@@ -546,17 +577,19 @@ __global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
     for (int tt = 0; tt < MAXM; ++tt) {
         const int t = tt - off;
         const bool real = t >= 0;
-        const int i = s + (real ? t : 0);
-        double d = dv[tt] * K.S[i];
+        const int tc = real ? t : 0;
+        const int i = s + tc;
+        double d = dv[tt] * chunk_row<NG>(cS, tc);
         d = (i == 0) ? d + b0 : d;
-        const double yn = (d - K.a[i] * yp) * K.inv[i];
-        const double an = acc + K.g[i] * yn;
+        const double yn = (d - chunk_row<NG>(ca, tc) * yp) * chunk_row<NG>(ci, tc);
+        const double an = acc + chunk_row<NG>(cg, tc) * yn;
         yp = real ? yn : yp;
         acc = real ? an : acc;
     }
     double de = dsep * K.S[e];
     de = (e == 0) ? de + b0 : de;
     de = (e == K.n - 1) ? de + bn : de;
+    if (!act) return;
     const size_t o = kidx(F, f, j, p);
     F.kYs[o] = acc;
     F.kYe[o] = yp;
@@ -592,22 +625,33 @@ __global__ __launch_bounds__(64) void k_ksolve2(const FluidDev F)
 template <int MAXM>
 __global__ __launch_bounds__(64) void k_ksolve3(const FluidDev F)
 {
+    constexpr int NG = (MAXM + 63) / 64;
+    static_assert(NG <= 2, "chunk rows");
     const DevScalars* S = F.S;
     if (S->err) return;
     const int f = blockIdx.z, p = blockIdx.y;
     const KSys& K = F.ks[f];
+    if (p >= K.nch) return;
     const int j = blockIdx.x * 64 + threadIdx.x + 1;
-    if (p >= K.nch || j > F.n2m) return;
+    const bool act = j <= F.n2m;
+    const int jc = act ? j : F.n2m;
     const size_t P = F.P;
-    const double* __restrict__ src = (f == 0 ? F.sA : F.sB) + fidx(F, j, K.kofs);
-    double* __restrict__ q = (f == 0 ? F.q2 : F.q3) + fidx(F, j, K.kofs);
+    const double* __restrict__ src = (f == 0 ? F.sA : F.sB) + fidx(F, jc, K.kofs);
+    double* __restrict__ q = (f == 0 ? F.q2 : F.q3) + fidx(F, jc, K.kofs);
     const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;   // m interior rows
-    const size_t o = kidx(F, f, j, p);
+    double cS[NG], ca[NG], ci[NG], cv[NG], cc[NG];
+    chunk_rows<NG>(K.S, s, m, cS);
+    chunk_rows<NG>(K.a, s, m, ca);
+    chunk_rows<NG>(K.inv, s, m, ci);
+    chunk_rows<NG>(K.vp, s, m, cv);
+    chunk_rows<NG>(K.cp, s, m, cc);
+    const size_t o = kidx(F, f, jc, p);
     const double X = F.kX[o];
     const double Xm = (p > 0) ? F.kX[o - 1] : 0.0;
+    const double wlp = K.wl[p];
     const bool zero_row = (f == 0 && j == 1);
     double b0, bn;
-    kbound(F, K, f, j, S, b0, bn);
+    kbound(F, K, f, jc, S, b0, bn);
     // all loads of the chunk first (one memory round trip), then the recurrence
     double y[MAXM], qv[MAXM];
 #pragma unroll
@@ -622,24 +666,26 @@ __global__ __launch_bounds__(64) void k_ksolve3(const FluidDev F)
     for (int t = 0; t < MAXM; ++t) {
         if (t < m) {
             const int i = s + t;
-            double d = y[t] * K.S[i];
+            double d = y[t] * chunk_row<NG>(cS, t);
             d = (i == 0) ? d + b0 : d;
-            yp = (d - K.a[i] * yp) * K.inv[i];
+            yp = (d - chunk_row<NG>(ca, t) * yp) * chunk_row<NG>(ci, t);
             y[t] = yp;
         }
     }
+    // every lane stays active through the broadcasts; only stores are masked
     double x = 0.0;
     double* qn2 = (f == 0 ? F.q2 : F.q3) + fidx(F, F.n2, K.kofs);
 #pragma unroll
     for (int t = MAXM - 1; t >= 0; --t) {
         if (t < m) {
             const int i = s + t;
-            if (t == m - 1) x = y[t] + K.vp[i] * Xm + K.wl[p] * X;
-            else x = (y[t] + K.vp[i] * Xm) - K.cp[i] * x;
-            q[(size_t)i * P] = zero_row ? 0.0 : qv[t] + x;
-            if (zero_row) qn2[(size_t)i * P] = 0.0;
+            if (t == m - 1) x = y[t] + chunk_row<NG>(cv, t) * Xm + wlp * X;
+            else x = (y[t] + chunk_row<NG>(cv, t) * Xm) - chunk_row<NG>(cc, t) * x;
+            if (act) q[(size_t)i * P] = zero_row ? 0.0 : qv[t] + x;
+            if (act && zero_row) qn2[(size_t)i * P] = 0.0;
         }
     }
+    if (!act) return;
     q[(size_t)e * P] = zero_row ? 0.0 : qe + X;
     if (zero_row) qn2[(size_t)e * P] = 0.0;
     if (f == 1) {

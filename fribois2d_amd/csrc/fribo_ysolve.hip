@@ -32,15 +32,41 @@ __device__ inline size_t yidx(const FluidDev& F, int q, int j, int p)
     return ((size_t)q * F.n2m + (size_t)(j - 1)) * FRIBO_KMAXCH + p;
 }
 
+// Rows k = s+1 .. s+m of the z operator for the interior rows of chunk p,
+// one row per lane (every lane of a workgroup works on the same chunk), taken
+// by each step from its lane.
+template <int NG>
+__device__ inline void yrows(const double* __restrict__ a, int s, int m, double (&v)[NG])
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int r = lane + 64 * g;
+        v[g] = a[s + 1 + (r < m ? r : (m > 0 ? m - 1 : 0))];
+    }
+}
+
+template <int NG>
+__device__ inline double yrow(const double (&v)[NG], int t)
+{
+    return lane_bcast(NG == 1 || t < 64 ? v[0] : v[NG - 1], t & 63);
+}
+
 // Chunk elimination of line j, chunk p (rows s..e-1 interior, e separator).
 // Returns the recurrence state; cp/vp/y are kept when STORE.
 template <int MAXM, bool STORE>
-__device__ inline void ychunk(const FluidDev& F, const KSys& K, int j, int p, double w, const double* col,
-                              double* cp, double* vp, double* y, double& yl, double& vl, double& ys, double& vs,
-                              double& ws, double& wl)
+__device__ inline void ychunk(const FluidDev& F, const KSys& K, int p, double w, const double* col, double* cp,
+                              double* vp, double* y, double& yl, double& vl, double& ys, double& vs, double& ws,
+                              double& wl)
 {
+    constexpr int NG = (MAXM + 63) / 64;
+    static_assert(NG <= 2, "chunk rows");
     const size_t P = F.P;
     const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;
+    double ra[NG], rb[NG], rc[NG];
+    yrows<NG>(F.amphk, s, m, ra);
+    yrows<NG>(F.acphk, s, m, rb);
+    yrows<NG>(F.apphk, s, m, rc);
     double dv[MAXM];
 #pragma unroll
     for (int t = 0; t < MAXM; ++t) dv[t] = col[(size_t)(s + (t < m ? t : 0)) * P];
@@ -53,8 +79,7 @@ __device__ inline void ychunk(const FluidDev& F, const KSys& K, int j, int p, do
 #pragma unroll
     for (int t = 0; t < MAXM; ++t) {
         if (t < m) {
-            const int k = s + t + 1;
-            const double a = F.amphk[k], b = F.acphk[k] + w, c = F.apphk[k];
+            const double a = yrow<NG>(ra, t), b = yrow<NG>(rb, t) + w, c = yrow<NG>(rc, t);
             const double iv = 1.0 / (b - a * cpp);
             cpp = c * iv;
             yl = (dv[t] - a * yl) * iv;
@@ -76,6 +101,8 @@ __device__ inline void ychunk(const FluidDev& F, const KSys& K, int j, int p, do
     ws = glast * wl;
 }
 
+// Lanes past the last y-mode work on mode n2m and store nothing (every lane
+// stays active through the row broadcasts).
 template <int MAXM>
 __global__ __launch_bounds__(64) void k_ysolve1(const FluidDev F)
 {
@@ -83,12 +110,15 @@ __global__ __launch_bounds__(64) void k_ysolve1(const FluidDev F)
     if (S->err) return;
     const KSys& K = F.ks[0];
     const int p = blockIdx.y;
+    if (p >= K.nch) return;
     const int j = blockIdx.x * 64 + threadIdx.x + 1;
-    if (p >= K.nch || j > F.n2m) return;
-    const double* col = F.hmn + (j - 1);
-    const double w = F.wy[j - 1];
+    const bool act = j <= F.n2m;
+    const int jc = act ? j : F.n2m;
+    const double* col = F.hmn + (jc - 1);
+    const double w = F.wy[jc - 1];
     double yl, vl, ys, vs, ws, wl;
-    ychunk<MAXM, false>(F, K, j, p, w, col, nullptr, nullptr, nullptr, yl, vl, ys, vs, ws, wl);
+    ychunk<MAXM, false>(F, K, p, w, col, nullptr, nullptr, nullptr, yl, vl, ys, vs, ws, wl);
+    if (!act) return;
     const int e = K.cs[p + 1] - 1, k = e + 1;
     const double ae = F.amphk[k];
     const double de = col[(size_t)e * F.P];
@@ -153,13 +183,16 @@ __global__ __launch_bounds__(64) void k_ysolve3(const FluidDev F)
     if (S->err) return;
     const KSys& K = F.ks[0];
     const int p = blockIdx.y;
+    if (p >= K.nch) return;
     const int j = blockIdx.x * 64 + threadIdx.x + 1;
-    if (p >= K.nch || j > F.n2m) return;
-    double* col = F.hmn + (j - 1);
-    const double w = F.wy[j - 1];
+    const bool act = j <= F.n2m;
+    const int jc = act ? j : F.n2m;
+    double* col = F.hmn + (jc - 1);
+    const double w = F.wy[jc - 1];
     double cp[MAXM], vp[MAXM], y[MAXM];
     double yl, vl, ys, vs, ws, wl;
-    ychunk<MAXM, true>(F, K, j, p, w, col, cp, vp, y, yl, vl, ys, vs, ws, wl);
+    ychunk<MAXM, true>(F, K, p, w, col, cp, vp, y, yl, vl, ys, vs, ws, wl);
+    if (!act) return;
     const double X = F.ysum[yidx(F, 6, j, p)];
     const double Xm = p > 0 ? F.ysum[yidx(F, 6, j, p - 1)] : 0.0;
     const size_t P = F.P;
