@@ -122,15 +122,21 @@ def main():
     axis = g.transform
     gemm_flops = 2.0 * n3m * n2m * n2m if axis == "y" else 2.0 * n2m * n3m * n3m
     achieved = gemm_flops / gemm_avg_s / 1e12
+    # the transforms of this workload take the direct-to-LDS kernel (96x128
+    # tiles, >= 192 workgroups, operands with padded leading dimensions)
+    gemm_kernel = "k_gemm_dma" if -(-n2m // 96) * -(-n3m // 128) >= 192 else "k_gemm_nt"
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_MFMA_TFLOPS, 4), "traffic": pmc_traffic("k_gemm_nt"),
-            "kernel": "k_gemm_nt", "avg_ms": round(gemm_avg_s * 1e3, 4), "transform_axis": axis,
+            "frac": round(achieved / PEAK_FP64_MFMA_TFLOPS, 4), "traffic": pmc_traffic(gemm_kernel),
+            "kernel": gemm_kernel, "avg_ms": round(gemm_avg_s * 1e3, 4), "transform_axis": axis,
             "flops_per_launch": gemm_flops}
     kernels = {k: {"ms_per_step": round(v[0] / nb, 4), "launches_per_step": v[1] / nb}
                for k, v in prof_all.items() if v[1]}
     # HBM fraction of the memory-bound classes: compulsory FP64 field values
     # per cell per launch (reads + writes), at the 8 TB/s MI355X peak
-    values = {"rhs": 9, "jsolve_helm": 4, "ksolve": 6, "div": 3, "jsolve_poisson": 2, "upd": 7, "cfl": 2}
+    # (the per-mode Poisson lines: along z one read and one write of hmn; along
+    # y the two chunk passes read it twice and write it once)
+    values = {"rhs": 9, "jsolve_helm": 4, "ksolve": 6, "div": 3, "jsolve_poisson": 3 if axis == "y" else 2,
+              "upd": 7, "cfl": 2}
     for k, nv in values.items():
         if k in kernels and kernels[k]["ms_per_step"] > 0:
             byts = nv * 8.0 * cells * kernels[k]["launches_per_step"]

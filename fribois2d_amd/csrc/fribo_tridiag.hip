@@ -536,10 +536,9 @@ __device__ inline double chunk_row(const double (&v)[NG], int t)
 }
 
 // Pass 1: thread (j, chunk p) eliminates the interior rows of its chunk and
-// leaves the chunk's separator data (sum g*y', last y', separator rhs).  The
-// interior rows are aligned to the end of the MAXM-step loop, so every step
-// loads unconditionally and the loads of all steps can be issued together.
-// Lanes past the last column load column n2m and store nothing.
+// leaves the chunk's separator data (sum g*y', last y', separator rhs).  All
+// loads of the chunk are issued together (clamped rows past the chunk end);
+// lanes past the last column load column n2m and store nothing.
 template <int MAXM>
 __global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
 {
@@ -556,7 +555,6 @@ __global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
     const double* __restrict__ src = (f == 0 ? F.sA : F.sB) + fidx(F, jc, K.kofs);
     const size_t P = F.P;
     const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;
-    const int off = MAXM - m;
     double cS[NG], ca[NG], ci[NG], cg[NG];
     chunk_rows<NG>(K.S, s, m, cS);
     chunk_rows<NG>(K.a, s, m, ca);
@@ -567,24 +565,18 @@ __global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
     // all loads of the chunk first (one memory round trip), then the recurrence
     double dv[MAXM];
 #pragma unroll
-    for (int tt = 0; tt < MAXM; ++tt) {
-        const int t = tt - off;
-        dv[tt] = src[(size_t)(s + (t >= 0 ? t : 0)) * P];
-    }
+    for (int t = 0; t < MAXM; ++t) dv[t] = src[(size_t)(s + (t < m ? t : 0)) * P];
     const double dsep = src[(size_t)e * P];
     double yp = 0.0, acc = 0.0;
 #pragma unroll
-    for (int tt = 0; tt < MAXM; ++tt) {
-        const int t = tt - off;
-        const bool real = t >= 0;
-        const int tc = real ? t : 0;
-        const int i = s + tc;
-        double d = dv[tt] * chunk_row<NG>(cS, tc);
-        d = (i == 0) ? d + b0 : d;
-        const double yn = (d - chunk_row<NG>(ca, tc) * yp) * chunk_row<NG>(ci, tc);
-        const double an = acc + chunk_row<NG>(cg, tc) * yn;
-        yp = real ? yn : yp;
-        acc = real ? an : acc;
+    for (int t = 0; t < MAXM; ++t) {
+        if (t < m) {
+            const int i = s + t;
+            double d = dv[t] * chunk_row<NG>(cS, t);
+            d = (i == 0) ? d + b0 : d;
+            yp = (d - chunk_row<NG>(ca, t) * yp) * chunk_row<NG>(ci, t);
+            acc = acc + chunk_row<NG>(cg, t) * yp;
+        }
     }
     double de = dsep * K.S[e];
     de = (e == 0) ? de + b0 : de;
