@@ -440,8 +440,7 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     if (!e) e = setup_ksys(c, F.js[1], F.n2m, 1, 2, 1);
     if (!e) e = setup_psys(c, F.ps, F.n2m, c->h1[A_AMPHJ], c->h1[A_ACPHJ], c->h1[A_APPHJ]);
     if (!e) {
-        int mm = std::max(std::max(F.js[0].maxm, F.js[1].maxm), F.ps.maxm);
-        F.jstride = mm | 1;
+        F.jstride = jsolve_stride(F);
         if (F.n2m > 256 * 16)
             e = set_err(FRIBO_ERR_LIMIT, "n2 too large for the CFL kernel of this build (n2-1 <= 4096)");
         else if (jsolve_lds_bytes(F) > c->lds_limit || sysprep_lds_bytes(F) + 17000 > c->lds_limit)

@@ -198,5 +198,6 @@ void launch_ygauge(const FluidDev& F, hipStream_t st);
 int ytransform_build(int n, const double* am, const double* ac, const double* ap, int ld, double* Y, double* Yinv,
                      double* wy);
 int phini_sym(int n3, const double* zz, double alx3, int ld, double* zmm, double* zmmt, double* wr);
+int jsolve_stride(const FluidDev& F);
 size_t jsolve_lds_bytes(const FluidDev& F);
 size_t sysprep_lds_bytes(const FluidDev& F);

@@ -52,6 +52,25 @@ __device__ inline void line_map(int* slot, const int* cs, int n, int nch, int st
     }
 }
 
+// Staging map with every chunk front-padded to a common length: the m-1
+// interior rows of chunk p sit at offsets off_p .. MAXR-1 of its staging row
+// (off_p = MAXR - (m_p - 1)) and the separator at offset MAXR, so a solve step
+// tt reads offset tt of every lane with no per-lane clamping.
+template <int NR>
+__device__ inline void line_map_pad(int* slot, const int* cs, int n, int nch, int stride, int lane, int maxr)
+{
+    const int csl = lane < nch ? cs[lane] : 0;
+    const int csn = lane < nch ? cs[lane + 1] : 0;
+    const int offl = maxr - (csn - csl - 1);
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        const int r = lane + 64 * u;
+        const int p = chunk_owner(r < n ? r : n - 1, n, nch);
+        const int c0 = __shfl(csl, p, 64), o = __shfl(offl, p, 64);
+        slot[u] = r < n ? p * stride + (r - c0) + o : -1;
+    }
+}
+
 // Loads of a line's rows (lane + 64u), clamped to the last row so every load
 // is unconditional; rows past the line are never staged, and the caller
 // passes a valid line for an inactive wave.
@@ -108,8 +127,13 @@ size_t jsolve_p_lds_bytes(int maxr, int jw, int stride)
 // rows 1 and n2 are the Dirichlet identity rows with zero rhs), then q3 for
 // k = 2..n3m (rows j = 1..n2m).  All factors are line-invariant (k_bc); the
 // first half of the grid takes the q2 lines, the second half the q3 lines.
-template <int MAXR, int JW, bool FL>
-__global__ __launch_bounds__(64 * JW) void k_jsolve_h(const FluidDev F)
+//
+// The workgroup copies the factors into LDS front-padded per chunk like the
+// staged line (padding rows: all factors zero, which keeps the recurrences at
+// their zero start), so every elimination step tt reads row tt of every lane
+// without clamps or selects.
+template <int MAXR, int JW>
+__global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
 {
     constexpr int NR = MAXR + 1;
     extern __shared__ double lds[];
@@ -129,34 +153,34 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_h(const FluidDev F)
     const int p = lane;
     const bool own = p < nch;
     const int m = own ? K.cs[p + 1] - K.cs[p] : 1;
-    const int off = MAXR - (m - 1);
     const int ie = (m - 1) * 64 + p;
 
-    // factors: LDS copies (FL) or the global arrays themselves
-    const double *fS, *fa, *fi, *fg, *fv, *fc, *fk1, *fk2, *fwl, *fib;
-    double ce;
-    double* stage;
-    if constexpr (FL) {
-        double* lS = lds;
-        double* la = lS + NR * 64;
-        double* li = la + NR * 64;
-        double* lg = li + NR * 64;
-        double* lv = lg + NR * 64;
-        double* lc = lv + NR * 64;
-        double* lk1 = lc + NR * 64;
-        double* lk2 = lk1 + 384;
-        double* lwl = lk2 + 384;
-        double* lib = lwl + 64;
-        stage = lib + 64 + wave * 64 * stride;
-        // every load of the copy is issued before the first LDS store
-        const int lenf = K.maxm * 64;
+    double* fS = lds;
+    double* fa = fS + NR * 64;
+    double* fi = fa + NR * 64;
+    double* fg = fi + NR * 64;
+    double* fv = fg + NR * 64;
+    double* fc = fv + NR * 64;
+    double* fk1 = fc + NR * 64;
+    double* fk2 = fk1 + 384;
+    double* fwl = fk2 + 384;
+    double* fib = fwl + 64;
+    double* stage = fib + 64 + wave * 64 * stride;
+    {
+        // padded copy: LDS row tt of lane q holds factor row tt - off_q of
+        // chunk q (zero where that is negative); all loads before the stores
         constexpr int NCP = (NR * 64 + 64 * JW - 1) / (64 * JW);
         constexpr int NCK = (384 + 64 * JW - 1) / (64 * JW);
         double tS[NCP], ta[NCP], ti[NCP], tg[NCP], tv[NCP], tc[NCP], t1[NCK], t2[NCK];
+        bool ok[NCP];
 #pragma unroll
         for (int u = 0; u < NCP; ++u) {
             const int i = tid + u * 64 * JW;
-            const int ic = i < lenf ? i : 0;
+            const int q = i & 63, tt = i >> 6;
+            const int mq = q < nch ? K.cs[q + 1] - K.cs[q] : 0;
+            const int t = tt - MAXR + mq - 1;
+            ok[u] = i < NR * 64 && mq > 0 && t >= 0;
+            const int ic = ok[u] ? t * 64 + q : 0;
             tS[u] = K.S[ic];
             ta[u] = K.a[ic];
             ti[u] = K.inv[ic];
@@ -175,35 +199,32 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_h(const FluidDev F)
         for (int u = 0; u < NCP; ++u) {
             const int i = tid + u * 64 * JW;
             if (i < NR * 64) {
-                const bool ok = i < lenf;
-                lS[i] = ok ? tS[u] : 0.0;
-                la[i] = ok ? ta[u] : 0.0;
-                li[i] = ok ? ti[u] : 0.0;
-                lg[i] = ok ? tg[u] : 0.0;
-                lv[i] = ok ? tv[u] : 0.0;
-                lc[i] = ok ? tc[u] : 0.0;
+                fS[i] = ok[u] ? tS[u] : 0.0;
+                fa[i] = ok[u] ? ta[u] : 0.0;
+                fi[i] = ok[u] ? ti[u] : 0.0;
+                fg[i] = ok[u] ? tg[u] : 0.0;
+                fv[i] = ok[u] ? tv[u] : 0.0;
+                fc[i] = ok[u] ? tc[u] : 0.0;
             }
         }
 #pragma unroll
         for (int u = 0; u < NCK; ++u) {
             const int i = tid + u * 64 * JW;
             if (i < 384) {
-                lk1[i] = t1[u];
-                lk2[i] = t2[u];
+                fk1[i] = t1[u];
+                fk2[i] = t2[u];
             }
         }
         if (tid < 64) {
-            lwl[tid] = twl;
-            lib[tid] = tib;
+            fwl[tid] = twl;
+            fib[tid] = tib;
         }
-        fS = lS; fa = la; fi = li; fg = lg; fv = lv; fc = lc; fk1 = lk1; fk2 = lk2; fwl = lwl; fib = lib;
-    } else {
-        stage = lds + wave * 64 * stride;
-        fS = K.S; fa = K.a; fi = K.inv; fg = K.g; fv = K.vp; fc = K.cp; fk1 = K.k1; fk2 = K.k2; fwl = K.wl; fib = K.ib;
+        // the padding rows of the staging area stay zero
+        for (int i = lane; i < 64 * stride; i += 64) stage[i] = 0.0;
     }
-    ce = own ? K.c[ie] : 0.0;
+    const double ce = own ? K.c[ie] : 0.0;
     int slot[NR];
-    line_map<NR>(slot, K.cs, n, nch, stride, lane);
+    line_map_pad<NR>(slot, K.cs, n, nch, stride, lane, MAXR);
     double pre[NR];
     const int step = nwgf * JW;
     const int iters = (nlines + step - 1) / step;
@@ -215,6 +236,86 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_h(const FluidDev F)
         const bool act = line < nlines;
         line_put<NR>(stage, pre, slot);
         const int nxt = line + step;
+        line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (nxt < nlines ? nxt : 0)), n, lane);
+        wave_lds_sync();
+        double y[MAXR];
+        double yl = 0.0, acc = 0.0;
+#pragma unroll
+        for (int tt = 0; tt < MAXR; ++tt) {
+            const int ix = tt * 64 + p;
+            yl = (row[tt] * fS[ix] - fa[ix] * yl) * fi[ix];
+            y[tt] = yl;
+            acc = acc + fg[ix] * yl;
+        }
+        const double de = row[MAXR] * fS[MAXR * 64 + p];
+        const double ae = fa[MAXR * 64 + p];
+        const double ysn = __shfl_down(acc, 1, 64);
+        double delta = own ? de - ae * yl - ((p + 1 < nch) ? ce * ysn : 0.0) : 0.0;
+        int li = 0;
+        for (int h = 1; h < 64; h <<= 1, ++li) {
+            double dm = __shfl_up(delta, h, 64), dp = __shfl_down(delta, h, 64);
+            if (p - h < 0) dm = 0.0;
+            if (p + h >= 64) dp = 0.0;
+            delta = delta - fk1[li * 64 + p] * dm - fk2[li * 64 + p] * dp;
+        }
+        const double X = delta * fib[p];
+        double Xm = __shfl_up(X, 1, 64);
+        if (p == 0) Xm = 0.0;
+        const double wl = fwl[p];
+        double x = 0.0;
+#pragma unroll
+        for (int tt = MAXR - 1; tt >= 0; --tt) {
+            const int ix = tt * 64 + p;
+            if (tt == MAXR - 1) x = y[tt] + fv[ix] * Xm + wl * X;
+            else x = (y[tt] + fv[ix] * Xm) - fc[ix] * x;
+            row[tt] = x;
+        }
+        row[MAXR] = X;
+        wave_lds_sync();
+        if (act) line_get<NR>(dstb + fidx(F, K.kofs, kfirst + line), stage, slot, lane);
+        wave_lds_sync();
+    }
+}
+
+// The same solve with the factors read from global memory (grids whose chunk
+// factors do not fit LDS).
+template <int MAXR>
+__global__ __launch_bounds__(64) void k_jsolve_h(const FluidDev F)
+{
+    constexpr int NR = MAXR + 1;
+    extern __shared__ double lds[];
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    const int lane = threadIdx.x & 63;
+    const int half = (int)gridDim.x >> 1;
+    const int f = (int)blockIdx.x < half ? 0 : 1;
+    const int wg = f == 0 ? (int)blockIdx.x : (int)blockIdx.x - half;
+    const int nwgf = f == 0 ? half : (int)gridDim.x - half;
+    const KSys& K = F.js[f];
+    const int nlines = f == 0 ? F.n3m : F.n3m - 1;
+    const int kfirst = f == 0 ? 1 : 2;
+    const double* srcb = f == 0 ? F.rhs2 : F.rhs3;
+    double* dstb = f == 0 ? F.sA : F.sB;
+    const int n = K.n, nch = K.nch, stride = F.jstride;
+    const int p = lane;
+    const bool own = p < nch;
+    const int m = own ? K.cs[p + 1] - K.cs[p] : 1;
+    const int off = MAXR - (m - 1);
+    const int ie = (m - 1) * 64 + p;
+    const double *fS = K.S, *fa = K.a, *fi = K.inv, *fg = K.g, *fv = K.vp, *fc = K.cp;
+    double* stage = lds;
+    const double ce = own ? K.c[ie] : 0.0;
+    int slot[NR];
+    line_map<NR>(slot, K.cs, n, nch, stride, lane);
+    double pre[NR];
+    const int iters = (nlines + nwgf - 1) / nwgf;
+    int line = wg;
+    line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (line < nlines ? line : 0)), n, lane);
+    double* row = stage + p * stride;
+    for (int it = 0; it < iters; ++it, line += nwgf) {
+        const bool act = line < nlines;
+        line_put<NR>(stage, pre, slot);
+        const int nxt = line + nwgf;
         line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (nxt < nlines ? nxt : 0)), n, lane);
         wave_lds_sync();
         double y[MAXR];
@@ -242,12 +343,12 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_h(const FluidDev F)
             double dm = __shfl_up(delta, h, 64), dp = __shfl_down(delta, h, 64);
             if (p - h < 0) dm = 0.0;
             if (p + h >= 64) dp = 0.0;
-            delta = delta - fk1[li * 64 + p] * dm - fk2[li * 64 + p] * dp;
+            delta = delta - K.k1[li * 64 + p] * dm - K.k2[li * 64 + p] * dp;
         }
-        const double X = delta * fib[p];
+        const double X = delta * K.ib[p];
         double Xm = __shfl_up(X, 1, 64);
         if (p == 0) Xm = 0.0;
-        const double wl = fwl[p];
+        const double wl = K.wl[p];
         double x = 0.0;
 #pragma unroll
         for (int tt = MAXR - 1; tt >= 0; --tt) {
@@ -422,16 +523,13 @@ static void launch_h(const FluidDev& F, hipStream_t st)
     const int sd = F.jstride;
     if (jsolve_h_lds_bytes(MAXR, 4, sd, true) <= (size_t)F.lds_max) {
         const int g = std::max(2, std::min(256, (lines + 3) / 4) & ~1);
-        hipLaunchKernelGGL((k_jsolve_h<MAXR, 4, true>), dim3(g), dim3(256), jsolve_h_lds_bytes(MAXR, 4, sd, true), st,
-                           F);
+        hipLaunchKernelGGL((k_jsolve_hp<MAXR, 4>), dim3(g), dim3(256), jsolve_h_lds_bytes(MAXR, 4, sd, true), st, F);
     } else if (jsolve_h_lds_bytes(MAXR, 2, sd, true) <= (size_t)F.lds_max) {
         const int g = std::max(2, std::min(512, (lines + 1) / 2) & ~1);
-        hipLaunchKernelGGL((k_jsolve_h<MAXR, 2, true>), dim3(g), dim3(128), jsolve_h_lds_bytes(MAXR, 2, sd, true), st,
-                           F);
+        hipLaunchKernelGGL((k_jsolve_hp<MAXR, 2>), dim3(g), dim3(128), jsolve_h_lds_bytes(MAXR, 2, sd, true), st, F);
     } else {
         const int g = std::max(2, std::min(2048, lines) & ~1);
-        hipLaunchKernelGGL((k_jsolve_h<MAXR, 1, false>), dim3(g), dim3(64), jsolve_h_lds_bytes(MAXR, 1, sd, false), st,
-                           F);
+        hipLaunchKernelGGL((k_jsolve_h<MAXR>), dim3(g), dim3(64), jsolve_h_lds_bytes(MAXR, 1, sd, false), st, F);
     }
 }
 
@@ -456,6 +554,15 @@ static int jsolve_maxr(int maxm)
 {
     const int mr = maxm - 1;
     return mr <= 8 ? 8 : mr <= 16 ? 16 : mr <= 24 ? 24 : mr <= 32 ? 32 : 64;
+}
+
+// Staging stride of the j-line kernels: every chunk (and, for the Helmholtz
+// lines, the front padding up to MAXR plus the separator) fits; odd, so the
+// lanes' rows fall in distinct banks.
+int jsolve_stride(const FluidDev& F)
+{
+    const int mm = std::max(std::max(F.js[0].maxm, F.js[1].maxm), F.ps.maxm);
+    return std::max(mm, jsolve_maxr(std::max(F.js[0].maxm, F.js[1].maxm)) + 1) | 1;
 }
 
 // Smallest LDS the j-line kernels need for this grid (their fallback shapes).
