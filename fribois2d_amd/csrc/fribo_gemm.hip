@@ -23,18 +23,28 @@
 // read as 16-double runs of each column and stored with a row pitch of BN+1,
 // which spreads the transposing LDS stores over the banks.
 //
-// Large problems (k_gemm_dma): 96 x 128 tiles, 8 waves, and the operand
-// slices move from global memory straight into LDS (global_load_lds_dwordx4,
-// 16 bytes per lane, no register staging), double-buffered with one
-// workgroup barrier per K-slice.  A direct LDS write lands lane-contiguous,
-// so the LDS layout follows the global one: A and the NT-form B as k-rows
-// (pitch 112 / 144 doubles: 48 / 64 active 16-byte slots of 56 / 72), the
-// NN-form B as n-rows of BK k-values with pitch BK+2 (the fragment read then
-// hits 16 distinct bank groups).  Lanes whose pair lies outside the matrix
-// do not load; the rows and columns they leave stale only feed outputs that
-// are not stored, and the k-rows past K of the last slice are cleared before
-// use.  The 96 x 128 shape gives exactly 256 workgroups (one per CU) for the
-// 1536 x 2048 transforms of the 2048x1536-cell grid.
+// Large problems (k_gemm_dma): 96 x 128 tiles, 16 waves of 48 x 16, and the
+// operand slices move from global memory straight into LDS
+// (global_load_lds_dwordx4, 16 bytes per lane, no register staging) through
+// three stages, so two K-slices of loads are in flight while one is
+// multiplied.  Each stage is its own LDS array and the slice loop is unrolled
+// by three, which lets the compiler see that the fragment reads of one stage
+// do not alias the loads still landing in the others (it would otherwise
+// wait for every outstanding load before each read).  Every wave issues the
+// same number of loads per slice (addresses clamped into the matrix), so
+// "slice s has landed" is vmcnt(loads per slice): slice s+1 may stay in
+// flight; the workgroup barrier is a plain s_barrier for the same reason.
+// A direct LDS write lands lane-contiguous, so the LDS layout follows the
+// global one: A and the NT-form B as k-rows (pitch 112 / 144 doubles: 48 / 64
+// used 16-byte slots of 56 / 72), the NN-form B as n-rows of BK k-values with
+// pitch BK+2 (the fragment read then hits 16 distinct bank groups).  Clamped
+// loads past the matrix only feed rows and columns that are not stored, or
+// the k-rows past K of the last slice, which are cleared before use.  The
+// 96 x 128 shape gives exactly 256 workgroups (one per CU) for the 1536 x 2048
+// transforms of the 2048x1536-cell grid; workgroups are dispatched to the 8
+// XCDs round-robin, so when the grid is a multiple of 256 tiles in 4 x 8
+// patches the tile indices are permuted to give each XCD compact patches
+// (its L2 then serves 4 A- and 8 B-tiles to 32 workgroups).
 #include "fribo_internal.h"
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -148,53 +158,61 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
         }
 }
 
-template <int BK, bool NN, int WM = 2, int WN = 4>
+// s_barrier without the memory fence of __syncthreads (whose release would
+// wait for the LDS loads still in flight for later stages); the asm memory
+// clobber keeps the compiler from moving memory accesses across it.
+__device__ inline void raw_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+template <int BK, bool NN, int WM = 2, int WN = 8>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A,
                                                            int lda, const double* __restrict__ B, int ldb,
                                                            double* __restrict__ C, int ldc, const DevScalars* S)
 {
-    constexpr int BM = 96, BN = 128, NW = WM * WN, NT = 64 * NW;
+    constexpr int BM = 96, BN = 128, NW = WM * WN;
     constexpr int TM = BM / WM, TN = BN / WN, RM = TM / 16, RN = TN / 16;
     constexpr int LA = 112, SLA = LA / 2;                   // A: k-rows, 56 slots (48 used)
     constexpr int LB = NN ? BK + 2 : 144, SLB = LB / 2;     // B: n-rows (NN) or k-rows (NT)
     constexpr int SA = BK * LA, SB = NN ? BN * LB : BK * LB;   // doubles per stage
-    constexpr int IA = SA / 128, IB = SB / 128;             // 1 KB instructions per stage
-    constexpr int NI = (IA + IB + NW - 1) / NW;             // per wave
+    constexpr int IA = SA / 128, IB = SB / 128;             // 1 KB loads per stage
+    constexpr int NI = (IA + IB) / NW;                      // loads per wave per stage
     static_assert(SA % 128 == 0 && SB % 128 == 0, "stage size");
+    static_assert((IA + IB) % NW == 0 && NI < 16, "same load count in every wave");
     static_assert(TM % 16 == 0 && TN % 16 == 0 && BM % WM == 0 && BN % WN == 0, "wave tile");
     if (S && S->err) return;
-    __shared__ __attribute__((aligned(1024))) double As[2 * SA];
-    __shared__ __attribute__((aligned(1024))) double Bs[2 * SB];
+    __shared__ __attribute__((aligned(1024))) double S0[SA + SB], S1[SA + SB], S2[SA + SB];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave % WM, wn = wave / WM;
-    const int i0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (gridDim.x % 4 == 0 && gridDim.y % 8 == 0 && (gridDim.x * gridDim.y) % 256 == 0) {
+        // dispatch id -> XCD (id % 8); 32 consecutive ids of one XCD form a 4 x 8
+        // patch (a bijection when every XCD receives whole patches)
+        const int id = blockIdx.x + gridDim.x * blockIdx.y, x = id % 8, l = id / 8;
+        const int npm = gridDim.x / 4;
+        const int patch = x + 8 * (l / 32), ll = l % 32;
+        bx = (patch % npm) * 4 + ll % 4;
+        by = (patch / npm) * 8 + ll / 4;
+    }
+    const int i0 = bx * BM, n0 = by * BN;
+    const int mlast = M - 2 + (M & 1), klast = K - 2 + (K & 1), nlast = N - 2 + (N & 1);
 
-    auto issue = [&](int k0, int st) {
+    auto issue = [&](int k0, double* As, double* Bs) {
 #pragma unroll
         for (int r = 0; r < NI; ++r) {
             const int ii = wave + NW * r;
+            const double* src;
+            double* dst;
             if (ii < IA) {
                 const int slot = ii * 64 + lane, kk = slot / SLA, q = slot % SLA;
-                if (q < BM / 2 && k0 + kk < K && i0 + 2 * q < M)
-                    __builtin_amdgcn_global_load_lds(
-                        (const __attribute__((address_space(1))) void*)(A + (size_t)(k0 + kk) * lda + i0 + 2 * q),
-                        (__attribute__((address_space(3))) void*)(As + st * SA + ii * 128), 16, 0, 0);
-            } else if (ii < IA + IB) {
+                src = A + (size_t)min(k0 + kk, K - 1) * lda + min(i0 + 2 * q, mlast);
+                dst = As + ii * 128;
+            } else {
                 const int jj = ii - IA, slot = jj * 64 + lane, row = slot / SLB, q = slot % SLB;
-                const double* src;
-                bool ok;
-                if (NN) {
-                    ok = q < BK / 2 && n0 + row < N && k0 + 2 * q < K;
-                    src = B + (size_t)(n0 + row) * ldb + k0 + 2 * q;
-                } else {
-                    ok = q < BN / 2 && k0 + row < K && n0 + 2 * q < N;
-                    src = B + (size_t)(k0 + row) * ldb + n0 + 2 * q;
-                }
-                if (ok)
-                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                                     (__attribute__((address_space(3))) void*)(Bs + st * SB + jj * 128),
-                                                     16, 0, 0);
+                if (NN) src = B + (size_t)min(n0 + row, N - 1) * ldb + min(k0 + 2 * q, klast);
+                else src = B + (size_t)min(k0 + row, K - 1) * ldb + min(n0 + 2 * q, nlast);
+                dst = Bs + jj * 128;
             }
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
         }
     };
 
@@ -205,40 +223,45 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, 
         for (int c = 0; c < RN; ++c) acc[r][c] = (f64x4){0.0, 0.0, 0.0, 0.0};
 
     const int nk = (K + BK - 1) / BK;
-    issue(0, 0);
-    for (int s = 0; s < nk; ++s) {
-        const int st = s & 1;
-        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's slice has landed in LDS
-        __syncthreads();                      // ... and every other wave's
+    // slice s is in stage (As, Bs); slice s+2 goes into (An, Bn), the stage of slice s-1
+    auto step = [&](int s, double* As, double* Bs, double* An, double* Bn) {
+        if (s + 1 < nk) __builtin_amdgcn_s_waitcnt(0x0F70 | NI);   // vmcnt(NI): only slice s+1 in flight
+        else __builtin_amdgcn_s_waitcnt(0x0F70);                    // vmcnt(0)
+        raw_barrier();
         const int kv = K - s * BK;            // valid k-rows of this slice
         if (kv < BK) {
-            for (int e = t; e < (BK - kv) * LA; e += NT) As[st * SA + kv * LA + e] = 0.0;
+            for (int e = t; e < (BK - kv) * LA; e += 64 * NW) As[kv * LA + e] = 0.0;
             if (NN) {
-                for (int e = t; e < BN * (BK - kv); e += NT)
-                    Bs[st * SB + (e / (BK - kv)) * LB + kv + e % (BK - kv)] = 0.0;
+                for (int e = t; e < BN * (BK - kv); e += 64 * NW) Bs[(e / (BK - kv)) * LB + kv + e % (BK - kv)] = 0.0;
             } else {
-                for (int e = t; e < (BK - kv) * LB; e += NT) Bs[st * SB + kv * LB + e] = 0.0;
+                for (int e = t; e < (BK - kv) * LB; e += 64 * NW) Bs[kv * LB + e] = 0.0;
             }
-            __syncthreads();
+            __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+            raw_barrier();
         }
-        if (s + 1 < nk) issue((s + 1) * BK, st ^ 1);
-        const double* as = As + st * SA;
-        const double* bs = Bs + st * SB;
+        if (s + 2 < nk) issue((s + 2) * BK, An, Bn);
 #pragma unroll
         for (int k4 = 0; k4 < BK; k4 += 4) {
             const int kr = k4 + (lane >> 4);
             double a[RM], b[RN];
 #pragma unroll
-            for (int r = 0; r < RM; ++r) a[r] = as[kr * LA + wm * TM + r * 16 + (lane & 15)];
+            for (int r = 0; r < RM; ++r) a[r] = As[kr * LA + wm * TM + r * 16 + (lane & 15)];
 #pragma unroll
             for (int c = 0; c < RN; ++c)
-                b[c] = NN ? bs[(wn * TN + c * 16 + (lane & 15)) * LB + kr] : bs[kr * LB + wn * TN + c * 16 + (lane & 15)];
+                b[c] = NN ? Bs[(wn * TN + c * 16 + (lane & 15)) * LB + kr] : Bs[kr * LB + wn * TN + c * 16 + (lane & 15)];
 #pragma unroll
             for (int r = 0; r < RM; ++r)
 #pragma unroll
                 for (int c = 0; c < RN; ++c)
                     acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
         }
+    };
+    issue(0, S0, S0 + SA);
+    if (nk > 1) issue(BK, S1, S1 + SA);
+    for (int s = 0; s < nk; s += 3) {
+        step(s, S0, S0 + SA, S2, S2 + SA);
+        if (s + 1 < nk) step(s + 1, S1, S1 + SA, S0, S0 + SA);
+        if (s + 2 < nk) step(s + 2, S2, S2 + SA, S1, S1 + SA);
     }
 #pragma unroll
     for (int r = 0; r < RM; ++r)

@@ -19,28 +19,38 @@
 #include "fribo_internal.h"
 
 
-// Fixed-order tree reductions over the NT threads of a block; t is the flat
-// thread index (threadIdx.x + blockDim.x * threadIdx.y).
+// Fixed-order reductions over the NT threads of a block: a butterfly inside
+// each wave (every lane ends with the same value: the pairwise sums are
+// symmetric), then the wave results in wave order.  Every thread returns the
+// result.  sh holds NT/64 values and may be reused by consecutive calls.
 template <int NT>
-__device__ inline void block_reduce_sum(double* sh, double v, int t = threadIdx.x)
+__device__ inline double block_sum(double* sh, double v)
 {
-    sh[t] = v;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
     __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-        if (t < s) sh[t] = sh[t] + sh[t + s];
-        __syncthreads();
-    }
+    if ((t & 63) == 0) sh[t >> 6] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) r = r + sh[w];
+    return r;
 }
 
 template <int NT>
-__device__ inline void block_reduce_max(double* sh, double v, int t = threadIdx.x)
+__device__ inline double block_max(double* sh, double v)
 {
-    sh[t] = v;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     __syncthreads();
-    for (int s = NT / 2; s > 0; s >>= 1) {
-        if (t < s) sh[t] = fmax(sh[t], sh[t + s]);
-        __syncthreads();
-    }
+    if ((t & 63) == 0) sh[t >> 6] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) r = fmax(r, sh[w]);
+    return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -85,13 +95,11 @@ __global__ __launch_bounds__(256) void k_cfl(const FluidDev F)
             }
         }
     }
-    block_reduce_max<256>(sh, m);
-    double bm = sh[0];
-    __syncthreads();
-    block_reduce_sum<256>(sh, s);
+    const double bm = block_max<256>(sh, m);
+    const double bs = block_sum<256>(sh, s);
     if (t == 0) {
         F.cfl_part[2 * blockIdx.x] = bm;
-        F.cfl_part[2 * blockIdx.x + 1] = sh[0];
+        F.cfl_part[2 * blockIdx.x + 1] = bs;
     }
 }
 
@@ -114,21 +122,22 @@ __device__ StepScalars step_scalars(const FluidDev& F, const DevScalars* S, int 
 {
     __shared__ StepScalars out;
     const int t = threadIdx.x;
+    double cmax = 0.0, q3s = 0.0;
     if (n == 1) {
         double m = 1.0e-8, s = 0.0;
         for (int i = t; i < FRIBO_NPART; i += 1024) {
             m = fmax(m, F.cfl_part[2 * i]);
             s = s + F.cfl_part[2 * i + 1];
         }
-        block_reduce_max<1024>(sh, m);
-        block_reduce_sum<1024>(sh2, s);
+        cmax = block_max<1024>(sh, m);
+        q3s = block_sum<1024>(sh2, s);
     }
     if (t == 0) {
         StepScalars r;
         r.err = 0;
         if (n == 1) {
-            double cflm = sh[0], dt = dt_in;
-            r.q3sum = sh2[0];
+            double cflm = cmax, dt = dt_in;
+            r.q3sum = q3s;
             if (!F.idtv) {
                 if (ns == 0) {
                     cflm = cflm * dt;
@@ -353,9 +362,8 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n)
         area = area + darea;
         qout = qout + F.dqb3n[j] * darea;
     }
-    block_reduce_sum<1024>(sh, area);
-    block_reduce_sum<1024>(sh2, qout);
-    const double tarea = sh[0], tqout = sh2[0];
+    const double tarea = block_sum<1024>(sh, area);
+    const double tqout = block_sum<1024>(sh2, qout);
     const double qinf = 0.0 * tarea;   // dqb3s == 0 (steady inflow)
     const double cor = (qinf - tqout) / tarea;
     for (int j = 1 + t; j <= F.n2m; j += 1024) F.dqb3n[j] = F.dqb3n[j] + cor;
@@ -671,8 +679,7 @@ __global__ __launch_bounds__(1024) void k_mean(const FluidDev F)
     const int t = threadIdx.x;
     double s = 0.0;
     for (int i = t; i < F.npr_part; i += 1024) s = s + F.pr_part[i];
-    block_reduce_sum<1024>(sh, s);
-    const double pravg = sh[0] / (F.rext2 * F.alx3);
+    const double pravg = block_sum<1024>(sh, s) / (F.rext2 * F.alx3);
     for (int j = 1 + t; j <= F.n2m; j += 1024) {
         F.qb2s[j] = F.qb2s[j] + F.dqb2s[j];
         F.qb3s[j] = F.qb3s[j] + F.dqb3s[j];

@@ -40,12 +40,14 @@ def test_gemm_matches_numpy():
 def test_gemm_large_tiles_match_numpy(form):
     """Shapes of at least 192 96x128 tiles take the direct-to-LDS kernel when
     the operands are laid out like the library's own (leading dimensions
-    padded to 16); odd extents and K tails exercise its masked loads and the
-    cleared k-rows.  Unpadded operands take the register-staged kernel; both
-    must agree with numpy."""
+    padded to 16); odd extents and K tails exercise its clamped loads and the
+    cleared k-rows, grids of 256 and 512 tiles its XCD patch order.  Odd
+    unpadded operands take the register-staged kernel; both must agree with
+    numpy and with each other."""
     from fribois2d_amd import gpu
     rng = np.random.default_rng(11)
-    for (M, N, K) in [(1100, 2000, 77), (1153, 1665, 333), (1536, 2048, 1536)]:
+    # (760, 4000, 45) and (1536, 4096, 64): 256 and 512 tiles, the XCD patch order
+    for (M, N, K) in [(1100, 2000, 77), (1153, 1665, 333), (760, 4000, 45), (1536, 4096, 64), (1536, 2048, 1536)]:
         A = rng.standard_normal((M, K))
         B = rng.standard_normal((N, K)) if form == "nt" else rng.standard_normal((K, N))
         f = gpu.gemm_nt if form == "nt" else gpu.gemm_nn
