@@ -59,6 +59,9 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 #ifndef GEMM_DMA_WN
 #define GEMM_DMA_WN 8
 #endif
+#ifndef GEMM_DMA_BN
+#define GEMM_DMA_BN 128
+#endif
 
 template <int BM, int BN, int WM, int WN, bool NN>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, const double* __restrict__ A, int lda,
@@ -163,23 +166,25 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
 // clobber keeps the compiler from moving memory accesses across it.
 __device__ inline void raw_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
-template <int BK, bool NN, int WM = 2, int WN = 8>
+template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A,
                                                            int lda, const double* __restrict__ B, int ldb,
                                                            double* __restrict__ C, int ldc, const DevScalars* S)
 {
-    constexpr int BM = 96, BN = 128, NW = WM * WN;
+    constexpr int BM = 96, NW = WM * WN;
     constexpr int TM = BM / WM, TN = BN / WN, RM = TM / 16, RN = TN / 16;
     constexpr int LA = 112, SLA = LA / 2;                   // A: k-rows, 56 slots (48 used)
-    constexpr int LB = NN ? BK + 2 : 144, SLB = LB / 2;     // B: n-rows (NN) or k-rows (NT)
+    constexpr int LB = NN ? BK + 2 : BN + 16, SLB = LB / 2; // B: n-rows (NN) or k-rows (NT)
     constexpr int SA = BK * LA, SB = NN ? BN * LB : BK * LB;   // doubles per stage
-    constexpr int IA = SA / 128, IB = SB / 128;             // 1 KB loads per stage
-    constexpr int NI = (IA + IB) / NW;                      // loads per wave per stage
+    constexpr int IA = SA / 128, IB0 = SB / 128;            // 1 KB loads per stage
+    // every wave issues NI loads per stage; spare loads land past SB
+    constexpr int NI = (IA + IB0 + NW - 1) / NW, IB = NI * NW - IA;
+    constexpr int SBP = IB * 128;                           // B stage with the spare slots
     static_assert(SA % 128 == 0 && SB % 128 == 0, "stage size");
-    static_assert((IA + IB) % NW == 0 && NI < 16, "same load count in every wave");
+    static_assert(NI < 16, "load count per wave");
     static_assert(TM % 16 == 0 && TN % 16 == 0 && BM % WM == 0 && BN % WN == 0, "wave tile");
     if (S && S->err) return;
-    __shared__ __attribute__((aligned(1024))) double S0[SA + SB], S1[SA + SB], S2[SA + SB];
+    __shared__ __attribute__((aligned(1024))) double S0[SA + SBP], S1[SA + SBP], S2[SA + SBP];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave % WM, wn = wave / WM;
     int bx = blockIdx.x, by = blockIdx.y;
@@ -207,8 +212,9 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, 
                 dst = As + ii * 128;
             } else {
                 const int jj = ii - IA, slot = jj * 64 + lane, row = slot / SLB, q = slot % SLB;
+                // rows past the tile (the spare slots) read clamped addresses
                 if (NN) src = B + (size_t)min(n0 + row, N - 1) * ldb + min(k0 + 2 * q, klast);
-                else src = B + (size_t)min(k0 + row, K - 1) * ldb + min(n0 + 2 * q, nlast);
+                else src = B + (size_t)min(k0 + min(row, BK - 1), K - 1) * ldb + min(n0 + 2 * q, nlast);
                 dst = Bs + jj * 128;
             }
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
@@ -294,8 +300,9 @@ static void launch_gemm(int M, int N, int K, const double* A, int lda, const dou
                         const DevScalars* S, hipStream_t st)
 {
     if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192 && dma_ok<NN>(M, N, K, A, lda, B, ldb)) {
-        hipLaunchKernelGGL((k_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN>), dim3(cdiv(M, 96), cdiv(N, 128)),
-                           dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), 0, st, M, N, K, A, lda, B, ldb, C, ldc, S);
+        hipLaunchKernelGGL((k_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN, GEMM_DMA_BN>),
+                           dim3(cdiv(M, 96), cdiv(N, GEMM_DMA_BN)), dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), 0, st, M, N,
+                           K, A, lda, B, ldb, C, ldc, S);
     } else if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
         hipLaunchKernelGGL((k_gemm_nt<96, 128, 2, 4, NN>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M, N,
                            K, A, lda, B, ldb, C, ldc, S);

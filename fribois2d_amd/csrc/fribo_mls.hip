@@ -21,6 +21,7 @@
 //   k_checkbcs   checkbcs (ckbcs.F90:5-86)
 // Every substep (interp_spread, forcing_MLS.F90:135-182):
 //   k_mls_interp  u_m = sum phi q*f, f = (V - u_m)/dt, contribution cl*phi*f
+//                 (sixteen lanes per marker and direction, one per support)
 //   k_mls_spread  one thread per touched cell sums its contributions in
 //                 marker order (the order the reference accumulates fey/fez
 //                 in), zeroes plane k = n3m (the single-rank ghost update of
@@ -591,37 +592,42 @@ void launch_mls_setup(const FluidDev& F, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
+// Sixteen lanes per (marker, direction), lane m < 9 owning support m: each
+// lane gathers its own q + rhs value, the group's first lane sums the nine
+// products in support order (the reference's order) from its neighbours, and
+// every support lane writes its contribution.
 __global__ __launch_bounds__(64) void k_mls_interp(const FluidDev F)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int tid = blockIdx.x * 64 + threadIdx.x;
-    if (tid >= 2 * F.nlm) return;
-    const int n = tid % F.nlm + 1, d = tid / F.nlm + 1;
-    if (!F.active[n - 1]) return;
+    const int lane = threadIdx.x;
+    const int g = blockIdx.x * 4 + (lane >> 4), m = lane & 15;   // group (n,d), support m
+    const bool ing = g < 2 * F.nlm;
+    const int n = (ing ? g % F.nlm : 0) + 1, d = (ing ? g / F.nlm : 0) + 1;
+    const bool act = ing && F.active[n - 1];
+    const bool sup = act && m < FRIBO_DIMSD;
     const double* q = (d == 1) ? F.q2 : F.q3;
     const double* r = (d == 1) ? F.rhs2 : F.rhs3;
-    // gather the nine supports first (independent loads), then sum in order
-    double qf[FRIBO_DIMSD], ph[FRIBO_DIMSD];
-#pragma unroll
-    for (int m = 1; m <= FRIBO_DIMSD; ++m) {
-        const int j = EULINS(n, d, 1, m), k = EULINS(n, d, 2, m);
+    double prod = 0.0, ph = 0.0;
+    if (sup) {
+        const int j = EULINS(n, d, 1, m + 1), k = EULINS(n, d, 2, m + 1);
         // q2f is written for 2<=j<=n2m (INVTR.F90:145-149), q3f for
         // 2<=k<=n3m (INVTR.F90:273-277); elsewhere it keeps its zero start
         const bool def = (d == 1) ? (j >= 2 && j <= F.n2m && k >= 1 && k <= F.n3m)
                                   : (j >= 1 && j <= F.n2m && k >= 2 && k <= F.n3m);
         const size_t c = def ? fidx(F, j, k) : 0;
-        qf[m - 1] = def ? q[c] + r[c] : 0.0;
-        ph[m - 1] = PHI(n, d, m);
+        const double qf = def ? q[c] + r[c] : 0.0;
+        ph = PHI(n, d, m + 1);
+        prod = ph * qf;
     }
+    const int base = lane & ~15;
     double u = 0.0;
 #pragma unroll
-    for (int m = 0; m < FRIBO_DIMSD; ++m) u = u + ph[m] * qf[m];
+    for (int t = 0; t < FRIBO_DIMSD; ++t) u = u + __shfl(prod, base + t, 64);
+    if (!sup) return;
     const double f = (LMVEL(n, d) - u) / S->dt;
     double* con = F.contrib + (size_t)(d - 1) * F.nlm * FRIBO_DIMSD;
-    const double cl = CL(n, d);
-#pragma unroll
-    for (int m = 0; m < FRIBO_DIMSD; ++m) con[(size_t)(n - 1) * FRIBO_DIMSD + m] = cl * ph[m] * f;
+    con[(size_t)(n - 1) * FRIBO_DIMSD + m] = CL(n, d) * ph * f;
 }
 
 // One thread per sorted entry; the first entry of each cell (segment head)
@@ -681,7 +687,7 @@ __global__ __launch_bounds__(64) void k_mls_spread(const FluidDev F)
 void launch_mls_force(const FluidDev& F, hipStream_t st)
 {
     if (F.nlm <= 0) return;
-    hipLaunchKernelGGL(k_mls_interp, dim3((2 * F.nlm + 63) / 64), dim3(64), 0, st, F);
+    hipLaunchKernelGGL(k_mls_interp, dim3((2 * F.nlm + 3) / 4), dim3(64), 0, st, F);
     hipLaunchKernelGGL(k_mls_spread, dim3((F.nlm * FRIBO_DIMSD + 63) / 64, 2), dim3(64), 0, st, F);
 }
 

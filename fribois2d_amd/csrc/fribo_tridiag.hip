@@ -114,7 +114,7 @@ __device__ inline void wave_lds_sync()
 size_t jsolve_h_lds_bytes(int maxr, int jw, int stride, bool fl)
 {
     const size_t nr = (size_t)maxr + 1;
-    return ((fl ? 6 * nr * 64 + 2 * 384 + 2 * 64 : 0) + (size_t)jw * 64 * stride) * sizeof(double);
+    return ((fl ? 4 * nr * 64 + 2 * 384 + 2 * 64 : 0) + (size_t)jw * 64 * stride) * sizeof(double);
 }
 
 size_t jsolve_p_lds_bytes(int maxr, int jw, int stride)
@@ -129,9 +129,14 @@ size_t jsolve_p_lds_bytes(int maxr, int jw, int stride)
 // first half of the grid takes the q2 lines, the second half the q3 lines.
 //
 // The workgroup copies the factors into LDS front-padded per chunk like the
-// staged line (padding rows: all factors zero, which keeps the recurrences at
-// their zero start), so every elimination step tt reads row tt of every lane
-// without clamps or selects.
+// staged line, as four arrays: S*inv and a*inv (the forward step becomes
+// y' = d*(S*inv) - (a*inv)*y'; the separator row keeps S and a themselves),
+// vp and cp.  The weights g of the separator sums are regenerated on the fly
+// (g_0 = 1, g_{t+1} = -cp_t g_t, the recurrence that produced them).  Padding
+// rows have S*inv = a*inv = vp = 0 and cp = -1, which keeps y' at zero and g
+// at one until the chunk's first row, so every step tt reads row tt of every
+// lane without clamps or selects.  Four arrays leave room for eight waves
+// (two per SIMD) on grids up to n2 = 1537.
 template <int MAXR, int JW>
 __global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
 {
@@ -155,11 +160,9 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
     const int m = own ? K.cs[p + 1] - K.cs[p] : 1;
     const int ie = (m - 1) * 64 + p;
 
-    double* fS = lds;
-    double* fa = fS + NR * 64;
-    double* fi = fa + NR * 64;
-    double* fg = fi + NR * 64;
-    double* fv = fg + NR * 64;
+    double* fSI = lds;
+    double* fAI = fSI + NR * 64;
+    double* fv = fAI + NR * 64;
     double* fc = fv + NR * 64;
     double* fk1 = fc + NR * 64;
     double* fk2 = fk1 + 384;
@@ -168,10 +171,10 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
     double* stage = fib + 64 + wave * 64 * stride;
     {
         // padded copy: LDS row tt of lane q holds factor row tt - off_q of
-        // chunk q (zero where that is negative); all loads before the stores
+        // chunk q (padding where that is negative); all loads before the stores
         constexpr int NCP = (NR * 64 + 64 * JW - 1) / (64 * JW);
         constexpr int NCK = (384 + 64 * JW - 1) / (64 * JW);
-        double tS[NCP], ta[NCP], ti[NCP], tg[NCP], tv[NCP], tc[NCP], t1[NCK], t2[NCK];
+        double tS[NCP], ta[NCP], ti[NCP], tv[NCP], tc[NCP], t1[NCK], t2[NCK];
         bool ok[NCP];
 #pragma unroll
         for (int u = 0; u < NCP; ++u) {
@@ -184,7 +187,6 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
             tS[u] = K.S[ic];
             ta[u] = K.a[ic];
             ti[u] = K.inv[ic];
-            tg[u] = K.g[ic];
             tv[u] = K.vp[ic];
             tc[u] = K.cp[ic];
         }
@@ -199,12 +201,11 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
         for (int u = 0; u < NCP; ++u) {
             const int i = tid + u * 64 * JW;
             if (i < NR * 64) {
-                fS[i] = ok[u] ? tS[u] : 0.0;
-                fa[i] = ok[u] ? ta[u] : 0.0;
-                fi[i] = ok[u] ? ti[u] : 0.0;
-                fg[i] = ok[u] ? tg[u] : 0.0;
-                fv[i] = ok[u] ? tv[u] : 0.0;
-                fc[i] = ok[u] ? tc[u] : 0.0;
+                const bool sep = (i >> 6) == MAXR;   // separator row: S and a themselves
+                fSI[i] = ok[u] ? (sep ? tS[u] : tS[u] * ti[u]) : 0.0;
+                fAI[i] = ok[u] ? (sep ? ta[u] : ta[u] * ti[u]) : 0.0;
+                fv[i] = ok[u] && !sep ? tv[u] : 0.0;
+                fc[i] = ok[u] ? (sep ? 0.0 : tc[u]) : -1.0;
             }
         }
 #pragma unroll
@@ -219,7 +220,7 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
             fwl[tid] = twl;
             fib[tid] = tib;
         }
-        // the padding rows of the staging area stay zero
+        // the padding rows of the staging area start at zero
         for (int i = lane; i < 64 * stride; i += 64) stage[i] = 0.0;
     }
     const double ce = own ? K.c[ie] : 0.0;
@@ -239,16 +240,17 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
         line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (nxt < nlines ? nxt : 0)), n, lane);
         wave_lds_sync();
         double y[MAXR];
-        double yl = 0.0, acc = 0.0;
+        double yl = 0.0, acc = 0.0, gg = 1.0;
 #pragma unroll
         for (int tt = 0; tt < MAXR; ++tt) {
             const int ix = tt * 64 + p;
-            yl = (row[tt] * fS[ix] - fa[ix] * yl) * fi[ix];
+            yl = row[tt] * fSI[ix] - fAI[ix] * yl;
             y[tt] = yl;
-            acc = acc + fg[ix] * yl;
+            acc = acc + gg * yl;
+            gg = -fc[ix] * gg;
         }
-        const double de = row[MAXR] * fS[MAXR * 64 + p];
-        const double ae = fa[MAXR * 64 + p];
+        const double de = row[MAXR] * fSI[MAXR * 64 + p];
+        const double ae = fAI[MAXR * 64 + p];
         const double ysn = __shfl_down(acc, 1, 64);
         double delta = own ? de - ae * yl - ((p + 1 < nch) ? ce * ysn : 0.0) : 0.0;
         int li = 0;
@@ -521,7 +523,10 @@ static void launch_h(const FluidDev& F, hipStream_t st)
 {
     const int lines = 2 * F.n3m - 1;
     const int sd = F.jstride;
-    if (jsolve_h_lds_bytes(MAXR, 4, sd, true) <= (size_t)F.lds_max) {
+    if (jsolve_h_lds_bytes(MAXR, 8, sd, true) <= (size_t)F.lds_max) {
+        const int g = std::max(2, std::min(256, (lines + 7) / 8) & ~1);
+        hipLaunchKernelGGL((k_jsolve_hp<MAXR, 8>), dim3(g), dim3(512), jsolve_h_lds_bytes(MAXR, 8, sd, true), st, F);
+    } else if (jsolve_h_lds_bytes(MAXR, 4, sd, true) <= (size_t)F.lds_max) {
         const int g = std::max(2, std::min(256, (lines + 3) / 4) & ~1);
         hipLaunchKernelGGL((k_jsolve_hp<MAXR, 4>), dim3(g), dim3(256), jsolve_h_lds_bytes(MAXR, 4, sd, true), st, F);
     } else if (jsolve_h_lds_bytes(MAXR, 2, sd, true) <= (size_t)F.lds_max) {

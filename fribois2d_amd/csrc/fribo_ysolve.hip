@@ -52,6 +52,19 @@ __device__ inline double yrow(const double (&v)[NG], int t)
     return lane_bcast(NG == 1 || t < 64 ? v[0] : v[NG - 1], t & 63);
 }
 
+// 1/x from the hardware estimate refined by two Newton steps (within an ulp
+// of the rounded quotient; the pivots of these diagonally dominant systems
+// are far from zero and from the denormal range).  The elimination chain of
+// a chunk is one reciprocal per row, so its latency sets the pass time.
+__device__ inline double recip(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // Chunk elimination of line j, chunk p (rows s..e-1 interior, e separator).
 // Returns the recurrence state; cp/vp/y are kept when STORE.
 template <int MAXM, bool STORE>
@@ -80,7 +93,7 @@ __device__ inline void ychunk(const FluidDev& F, const KSys& K, int p, double w,
     for (int t = 0; t < MAXM; ++t) {
         if (t < m) {
             const double a = yrow<NG>(ra, t), b = yrow<NG>(rb, t) + w, c = yrow<NG>(rc, t);
-            const double iv = 1.0 / (b - a * cpp);
+            const double iv = recip(b - a * cpp);
             cpp = c * iv;
             yl = (dv[t] - a * yl) * iv;
             vl = (-a * vl) * iv;
@@ -164,7 +177,7 @@ __global__ __launch_bounds__(64) void k_ysolve2(const FluidDev F)
         double gp = __shfl_down(gamma, h, 64), dp = __shfl_down(delta, h, 64);
         if (lane - h < 0) { am = 0.0; bm = 1.0; gm = 0.0; dm = 0.0; }
         if (lane + h >= 64) { ap = 0.0; bp = 1.0; gp = 0.0; dp = 0.0; }
-        const double m1 = alpha / bm, m2 = gamma / bp;
+        const double m1 = alpha * recip(bm), m2 = gamma * recip(bp);
         const double na = -m1 * am, ng = -m2 * gp;
         const double nb = beta - m1 * gm - m2 * ap;
         const double nd = delta - m1 * dm - m2 * dp;
@@ -173,7 +186,7 @@ __global__ __launch_bounds__(64) void k_ysolve2(const FluidDev F)
         gamma = ng;
         delta = nd;
     }
-    if (p < nch) F.ysum[yidx(F, 6, j, p)] = delta / beta;
+    if (p < nch) F.ysum[yidx(F, 6, j, p)] = delta * recip(beta);
 }
 
 template <int MAXM>
