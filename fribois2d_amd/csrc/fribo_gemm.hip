@@ -23,8 +23,10 @@
 // read as 16-double runs of each column and stored with a row pitch of BN+1,
 // which spreads the transposing LDS stores over the banks.
 //
-// Large problems (k_gemm_dma): 96 x 128 tiles, 16 waves of 48 x 16, and the
-// operand slices move from global memory straight into LDS
+// Large problems (k_gemm_dma): 96 x 64 tiles of 8 waves (48 x 16 each), two
+// workgroups per CU so that one's barrier waits overlap the other's MFMAs
+// (72 KB of LDS each), and the operand slices move from global memory straight
+// into LDS
 // (global_load_lds_dwordx4, 16 bytes per lane, no register staging) through
 // three stages, so two K-slices of loads are in flight while one is
 // multiplied.  Each stage is its own LDS array and the slice loop is unrolled
@@ -39,12 +41,13 @@
 // used 16-byte slots of 56 / 72), the NN-form B as n-rows of BK k-values with
 // pitch BK+2 (the fragment read then hits 16 distinct bank groups).  Clamped
 // loads past the matrix only feed rows and columns that are not stored, or
-// the k-rows past K of the last slice, which are cleared before use.  The
-// 96 x 128 shape gives exactly 256 workgroups (one per CU) for the 1536 x 2048
-// transforms of the 2048x1536-cell grid; workgroups are dispatched to the 8
-// XCDs round-robin, so when the grid is a multiple of 256 tiles in 4 x 8
-// patches the tile indices are permuted to give each XCD compact patches
-// (its L2 then serves 4 A- and 8 B-tiles to 32 workgroups).
+// the k-rows past K of the last slice, which are cleared before use; when
+// the load count of a stage does not divide evenly among the waves, the spare
+// loads land past the stage's B block.  The 1536 x 2048 transforms of the
+// 2048x1536-cell grid give 512 workgroups, two per CU; workgroups are
+// dispatched to the 8 XCDs round-robin, so when the grid is a multiple of 256
+// tiles in 4 x 8 patches the tile indices are permuted to give each XCD
+// compact patches (its L2 then serves 4 A- and 8 B-tiles to 32 workgroups).
 #include "fribo_internal.h"
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -57,10 +60,10 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 #define GEMM_DMA_WM 2
 #endif
 #ifndef GEMM_DMA_WN
-#define GEMM_DMA_WN 8
+#define GEMM_DMA_WN 4
 #endif
 #ifndef GEMM_DMA_BN
-#define GEMM_DMA_BN 128
+#define GEMM_DMA_BN 64
 #endif
 
 template <int BM, int BN, int WM, int WN, bool NN>

@@ -394,6 +394,13 @@ void launch_bc(const FluidDev& F, int n, hipStream_t st)
 #define MK FRIBO_MK
 #define MKR FRIBO_MK_RHS
 #define WJ FRIBO_WJ
+// Waves (tiles) per workgroup: adjacent tiles in j share their boundary
+// cache lines, which then come from the one L2 of the workgroup's XCD; the
+// measured gain holds for k_div only (k_upd runs faster with one wave per
+// workgroup, k_rhs is indifferent).
+#define SW_RHS 1
+#define SW_DIV 8
+#define SW_UPD 1
 static_assert(MK <= 64 && MKR <= 64, "one plane of a tile per lane");
 
 __device__ inline double up1(double v) { return __shfl_up(v, 1, 64); }
@@ -402,12 +409,14 @@ __device__ inline double dn1(double v) { return __shfl_down(v, 1, 64); }
 // Fused HDNL + INVTR.  Cell (j,k): q2 point (rc(j), zm(k)) for 2<=j<=n2m and
 // q3 point (rm(j), zz(k)) for 2<=k<=n3m; operation order as in the reference
 // loops (HDNL.F90:35-220, INVTR.F90:45-277).
-__global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
+__global__ __launch_bounds__(64 * SW_RHS) void k_rhs(const FluidDev F)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int lane = threadIdx.x;
-    const int j = WJ * blockIdx.x + lane;
+    const int lane = threadIdx.x & 63;
+    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)blockIdx.x * SW_RHS + (int)(threadIdx.x >> 6);
+    if (bx >= ntj) return;
+    const int j = WJ * bx + lane;
     const int k0 = MKR * blockIdx.y + 1;
     const int n2m = F.n2m, n3m = F.n3m;
     if (k0 > n3m) return;
@@ -534,18 +543,21 @@ __global__ __launch_bounds__(64) void k_rhs(const FluidDev F)
 
 void launch_rhs(const FluidDev& F, hipStream_t st)
 {
-    dim3 g((F.n2m + WJ - 1) / WJ, (F.n3m + MKR - 1) / MKR);
-    hipLaunchKernelGGL(k_rhs, g, dim3(64), 0, st, F);
+    const int ntj = (F.n2m + WJ - 1) / WJ;
+    dim3 g((ntj + SW_RHS - 1) / SW_RHS, (F.n3m + MKR - 1) / MKR);
+    hipLaunchKernelGGL(k_rhs, g, dim3(64 * SW_RHS), 0, st, F);
 }
 
 // ---------------------------------------------------------------------------
 // divergence.F90:26-39
-__global__ __launch_bounds__(64) void k_div(const FluidDev F)
+__global__ __launch_bounds__(64 * SW_DIV) void k_div(const FluidDev F)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int lane = threadIdx.x;
-    const int j = WJ * blockIdx.x + lane;
+    const int lane = threadIdx.x & 63;
+    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)blockIdx.x * SW_DIV + (int)(threadIdx.x >> 6);
+    if (bx >= ntj) return;
+    const int j = WJ * bx + lane;
     const int k0 = MK * blockIdx.y + 1;
     if (k0 > F.n3m) return;
     const int k1 = min(k0 + MK - 1, F.n3m);
@@ -579,20 +591,23 @@ __global__ __launch_bounds__(64) void k_div(const FluidDev F)
 
 void launch_div(const FluidDev& F, hipStream_t st)
 {
-    dim3 g((F.n2m + WJ - 1) / WJ, (F.n3m + MK - 1) / MK);
-    hipLaunchKernelGGL(k_div, g, dim3(64), 0, st, F);
+    const int ntj = (F.n2m + WJ - 1) / WJ;
+    dim3 g((ntj + SW_DIV - 1) / SW_DIV, (F.n3m + MK - 1) / MK);
+    hipLaunchKernelGGL(k_div, g, dim3(64 * SW_DIV), 0, st, F);
 }
 
 // ---------------------------------------------------------------------------
 // updvp.F90:27-56 (velocity correction) + 83-102 (prcalc) + partial sums of
 // the pressure mean (updvp.F90:127-133).  The stored pressure is "raw": the
 // previous mean (S->pend) is subtracted on read.
-__global__ __launch_bounds__(64) void k_upd(const FluidDev F)
+__global__ __launch_bounds__(64 * SW_UPD) void k_upd(const FluidDev F)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int lane = threadIdx.x;
-    const int j = WJ * blockIdx.x + lane;
+    const int lane = threadIdx.x & 63;
+    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)blockIdx.x * SW_UPD + (int)(threadIdx.x >> 6);
+    if (bx >= ntj) return;
+    const int j = WJ * bx + lane;
     const int k0 = MK * blockIdx.y + 1;
     double part = 0.0;
     if (k0 <= F.n3m) {
@@ -657,13 +672,14 @@ __global__ __launch_bounds__(64) void k_upd(const FluidDev F)
     }
     // wave reduction (fixed order) -> one partial per workgroup
     for (int off = 32; off > 0; off >>= 1) part = part + __shfl_xor(part, off, 64);
-    if (lane == 0) F.pr_part[blockIdx.y * gridDim.x + blockIdx.x] = part;
+    if (lane == 0) F.pr_part[blockIdx.y * ntj + bx] = part;
 }
 
 void launch_upd(const FluidDev& F, hipStream_t st)
 {
-    dim3 g((F.n2m + WJ - 1) / WJ, (F.n3m + MK - 1) / MK);
-    hipLaunchKernelGGL(k_upd, g, dim3(64), 0, st, F);
+    const int ntj = (F.n2m + WJ - 1) / WJ;
+    dim3 g((ntj + SW_UPD - 1) / SW_UPD, (F.n3m + MK - 1) / MK);
+    hipLaunchKernelGGL(k_upd, g, dim3(64 * SW_UPD), 0, st, F);
 }
 
 // ---------------------------------------------------------------------------
