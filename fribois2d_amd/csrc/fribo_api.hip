@@ -738,11 +738,13 @@ static void enqueue_poisson(fribo_ctx* c)
     });
 }
 
-static void enqueue_substep(fribo_ctx* c, int n)
+// Substep n.  fold: the previous substep's pressure mean runs inside this
+// substep's k_bc (no kernel of its own); last: the mean runs after k_upd.
+static void enqueue_substep(fribo_ctx* c, int n, bool fold, bool last)
 {
     FluidDev& F = c->F;
     hipStream_t st = c->st;
-    run(c, PC_BC, [&] { launch_bc(F, n, st); });
+    run(c, PC_BC, [&] { launch_bc(F, n, fold, st); });
     run(c, PC_RHS, [&] { launch_rhs(F, st); });
     if (F.ibact && F.nlm > 0) {
         if (n == 1) run(c, PC_MLS_SETUP, [&] { launch_mls_setup(F, st); });
@@ -753,7 +755,7 @@ static void enqueue_substep(fribo_ctx* c, int n)
     run(c, PC_DIV, [&] { launch_div(F, st); });
     enqueue_poisson(c);
     run(c, PC_UPD, [&] { launch_upd(F, st); });
-    run(c, PC_MEAN, [&] { launch_mean(F, st); });
+    if (last) run(c, PC_MEAN, [&] { launch_mean(F, st); });
 }
 
 // Step inputs (dt, dt_o, ns) and a cleared error flag, one small copy.
@@ -770,7 +772,7 @@ static int write_inputs(fribo_ctx* c, int ns, double dt, double dt_o)
 static void enqueue_step(fribo_ctx* c)
 {
     enqueue_cfl(c);
-    for (int n = 1; n <= c->F.nsst; ++n) enqueue_substep(c, n);
+    for (int n = 1; n <= c->F.nsst; ++n) enqueue_substep(c, n, n > 1, n == c->F.nsst);
 }
 
 // Capture the whole step once; every kernel reads its per-step inputs from
@@ -844,7 +846,7 @@ extern "C" int fribo_cfl_dt(fribo_handle h, int32_t ns, double* dt, double dt_o,
     enqueue_cfl(h);
     // finalize the CFL / dt control without running a substep: launch the
     // boundary kernel for substep 1 (it also prepares substep 1's factors)
-    run(h, PC_BC, [&] { launch_bc(h->F, 1, h->st); });
+    run(h, PC_BC, [&] { launch_bc(h->F, 1, false, h->st); });
     return finish(h, dt, cflm);
 }
 
@@ -871,7 +873,7 @@ extern "C" int fribo_substep(fribo_handle h, int32_t n, int32_t ns)
         run(h, PC_UPD, [&] { launch_upd(F, st); });
         run(h, PC_MEAN, [&] { launch_mean(F, st); });
     } else {
-        enqueue_substep(h, n);
+        enqueue_substep(h, n, false, true);
     }
     return finish(h, nullptr, nullptr);
 }

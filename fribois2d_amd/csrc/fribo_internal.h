@@ -176,7 +176,7 @@ enum {
 
 // Kernel launchers (each .hip file owns its kernels).
 void launch_cfl(const FluidDev& F, hipStream_t st);
-void launch_bc(const FluidDev& F, int n, hipStream_t st);
+void launch_bc(const FluidDev& F, int n, bool mean, hipStream_t st);
 void launch_rhs(const FluidDev& F, hipStream_t st);
 void launch_div(const FluidDev& F, hipStream_t st);
 void launch_upd(const FluidDev& F, hipStream_t st);

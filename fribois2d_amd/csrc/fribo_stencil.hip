@@ -296,9 +296,34 @@ size_t sysprep_lds_bytes(const FluidDev& F)
     return (size_t)3 * n * sizeof(double);
 }
 
-// Workgroup 0: finalize CFL/dt (substep 1), RK coefficients, boucdq.
+// pr_submean (updvp.F90:125-141) and boucqt (iobcs.F90:300-312) by one
+// workgroup of 1024 threads.  The mean is kept pending for the interior
+// (S->pend, subtracted on the next read) and accumulated for the entries no
+// kernel reads (row j=n2 and the ghost planes, S->goff, subtracted when the
+// state is read back).
+__device__ void mean_block(const FluidDev& F, double* sh)
+{
+    DevScalars* S = F.S;
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int i = t; i < F.npr_part; i += 1024) s = s + F.pr_part[i];
+    const double pravg = block_sum<1024>(sh, s) / (F.rext2 * F.alx3);
+    for (int j = 1 + t; j <= F.n2m; j += 1024) {
+        F.qb2s[j] = F.qb2s[j] + F.dqb2s[j];
+        F.qb3s[j] = F.qb3s[j] + F.dqb3s[j];
+        F.qb2n[j] = F.qb2n[j] + F.dqb2n[j];
+        F.qb3n[j] = F.qb3n[j] + F.dqb3n[j];
+    }
+    if (t == 0) {
+        S->pend = pravg;
+        S->goff = S->goff + pravg;
+    }
+}
+
+// Workgroup 0: the previous substep's pressure mean and boucqt (mean != 0),
+// finalize CFL/dt (substep 1), RK coefficients, boucdq.
 // Workgroups 1..4: factors of the four Helmholtz systems of this substep.
-__global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n)
+__global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int mean)
 {
     extern __shared__ double dyn[];
     __shared__ double sh[1024];
@@ -321,6 +346,10 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n)
         const KSys& K = (sys < 2) ? F.ks[sys] : F.js[sys - 2];
         sys_prepare(F, K, r.betadx, dyn);
         return;
+    }
+    if (mean) {
+        mean_block(F, sh);
+        __syncthreads();
     }
     if (t == 0) {
         if (n == 1) {
@@ -380,9 +409,9 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n)
     }
 }
 
-void launch_bc(const FluidDev& F, int n, hipStream_t st)
+void launch_bc(const FluidDev& F, int n, bool mean, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_bc, dim3(5), dim3(1024), sysprep_lds_bytes(F), st, F, n);
+    hipLaunchKernelGGL(k_bc, dim3(5), dim3(1024), sysprep_lds_bytes(F), st, F, n, mean ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -683,29 +712,15 @@ void launch_upd(const FluidDev& F, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// pr_submean (updvp.F90:125-141) and boucqt (iobcs.F90:300-312).  The mean
-// is kept pending for the interior (S->pend, subtracted on the next read)
-// and accumulated for the entries no kernel reads (row j=n2 and the ghost
-// planes, S->goff, subtracted when the state is read back).
+// pr_submean (updvp.F90:125-141) and boucqt (iobcs.F90:300-312) as a kernel
+// of its own (after the last substep of a step, and after every substep of
+// the substep-by-substep entry point); within a step, k_bc of the next
+// substep does the same work first (mean_block).
 __global__ __launch_bounds__(1024) void k_mean(const FluidDev F)
 {
-    __shared__ double sh[1024];
-    DevScalars* S = F.S;
-    if (S->err) return;
-    const int t = threadIdx.x;
-    double s = 0.0;
-    for (int i = t; i < F.npr_part; i += 1024) s = s + F.pr_part[i];
-    const double pravg = block_sum<1024>(sh, s) / (F.rext2 * F.alx3);
-    for (int j = 1 + t; j <= F.n2m; j += 1024) {
-        F.qb2s[j] = F.qb2s[j] + F.dqb2s[j];
-        F.qb3s[j] = F.qb3s[j] + F.dqb3s[j];
-        F.qb2n[j] = F.qb2n[j] + F.dqb2n[j];
-        F.qb3n[j] = F.qb3n[j] + F.dqb3n[j];
-    }
-    if (t == 0) {
-        S->pend = pravg;
-        S->goff = S->goff + pravg;
-    }
+    __shared__ double sh[16];
+    if (F.S->err) return;
+    mean_block(F, sh);
 }
 
 void launch_mean(const FluidDev& F, hipStream_t st)
