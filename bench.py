@@ -139,9 +139,7 @@ def main():
               "upd": 7, "cfl": 2}
     for k, nv in values.items():
         if k in kernels and kernels[k]["ms_per_step"] > 0:
-            # along y the class also holds the gauge kernel: one line solve per substep
-            nl = kernels[k]["launches_per_step"] / (2 if k == "jsolve_poisson" and axis == "y" else 1)
-            byts = nv * 8.0 * cells * nl
+            byts = nv * 8.0 * cells * kernels[k]["launches_per_step"]
             gbs = byts / (kernels[k]["ms_per_step"] * 1e-3) / 1e9
             kernels[k]["alg_GBs"] = round(gbs, 1)
             kernels[k]["hbm_frac"] = round(gbs / PEAK_HBM_GBS, 4)

@@ -321,6 +321,8 @@ static int setup_ytransform(fribo_ctx* c, const double* zmmt, int ldz)
     if (!e) e = dalloc(c, &F.wy, n);
     if (!e) e = dalloc(c, &F.zg, F.n3m);
     if (!e) e = dalloc(c, &F.ysum, (size_t)7 * n * FRIBO_KMAXCH);
+    F.ngpart = gemm_nn_tiles_n(n, F.n3m, n, F.ymm, F.ldy, F.hmn, F.P);
+    if (!e) e = dalloc(c, &F.gpart, F.ngpart);
     if (e) return e;
     HIPCHK(hipMemcpy(F.ymm, Y.data(), Y.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(F.ymmi, Yi.data(), Yi.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -712,8 +714,9 @@ static void enqueue_cfl(fribo_ctx* c)
 
 // phcalc_outflow (phcalc_outflow.F90:4-71).  Along z, the reference's order:
 // hmn = dph * zmmt^T, one j-line per z-mode, dph = V * zmm^T.  Along y:
-// hmn = Yinv * dph, one k-line per y-mode, dph = Y * V, and the constant that
-// restores the z-order gauge.
+// hmn = Yinv * dph, one k-line per y-mode, dph = Y * V, whose epilogue leaves
+// the partial sums of the constant that restores the z-order gauge (k_upd
+// adds them up and subtracts the constant).
 static void enqueue_poisson(fribo_ctx* c)
 {
     FluidDev& F = c->F;
@@ -724,9 +727,9 @@ static void enqueue_poisson(fribo_ctx* c)
         });
         run(c, PC_JP, [&] { launch_ysolve(F, st); });
         run(c, PC_GEMM, [&] {
-            launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ymm, F.ldy, F.hmn, F.P, F.dph + fidx(F, 1, 1), F.P, F.S, st);
+            launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ymm, F.ldy, F.hmn, F.P, F.dph + fidx(F, 1, 1), F.P, F.S, st, F.zg,
+                           F.gpart);
         });
-        run(c, PC_JP, [&] { launch_ygauge(F, st); });
         return;
     }
     run(c, PC_GEMM, [&] {

@@ -66,10 +66,43 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 #define GEMM_DMA_BN 64
 #endif
 
+// Gauge partial of the transform along y (zg != nullptr): the tile holding
+// row M-1 of C adds zg(col) * C(M-1, col) over its columns, in a fixed order
+// (columns of a lane, lanes by butterfly, waves in index order), and writes
+// it to part[tile column].  The step then sums the partials in tile order.
+template <int RM, int RN, int TM, int TN, int NW>
+__device__ inline void gauge_partial(const f64x4 (&acc)[RM][RN], int M, int N, int i0, int n0, int wm, int wn,
+                                     int lane, int wave, int by, const double* __restrict__ zg, double* part,
+                                     double* sh)
+{
+    double v = 0.0;
+#pragma unroll
+    for (int c = 0; c < RN; ++c) {
+        const int col = n0 + wn * TN + c * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < RM; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = i0 + wm * TM + r * 16 + (lane >> 4) + 4 * q;
+                if (row == M - 1 && col < N) v = v + zg[col] * acc[r][c][q];
+            }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sh[0];
+        for (int w = 1; w < NW; ++w) t = t + sh[w];
+        part[by] = t;
+    }
+}
+
 template <int BM, int BN, int WM, int WN, bool NN>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, const double* __restrict__ A, int lda,
                                                           const double* __restrict__ B, int ldb,
-                                                          double* __restrict__ C, int ldc, const DevScalars* S)
+                                                          double* __restrict__ C, int ldc, const DevScalars* S,
+                                                          const double* __restrict__ zg, double* gpart)
 {
     constexpr int NT = 64 * WM * WN;
     constexpr int TM = BM / WM, TN = BN / WN;     // wave tile
@@ -162,6 +195,10 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
                 if (row < M && col < N) C[(size_t)col * ldc + row] = acc[r][c][v];
             }
         }
+    if (zg && M - 1 >= i0 && M - 1 < i0 + BM) {
+        __shared__ double gsh[WM * WN];
+        gauge_partial<RM, RN, TM, TN, WM * WN>(acc, M, N, i0, n0, wm, wn, lane, wave, blockIdx.y, zg, gpart, gsh);
+    }
 }
 
 // s_barrier without the memory fence of __syncthreads (whose release would
@@ -172,7 +209,8 @@ __device__ inline void raw_barrier() { asm volatile("s_barrier" ::: "memory"); }
 template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A,
                                                            int lda, const double* __restrict__ B, int ldb,
-                                                           double* __restrict__ C, int ldc, const DevScalars* S)
+                                                           double* __restrict__ C, int ldc, const DevScalars* S,
+                                                           const double* __restrict__ zg, double* gpart)
 {
     constexpr int BM = 96, NW = WM * WN;
     constexpr int TM = BM / WM, TN = BN / WN, RM = TM / 16, RN = TN / 16;
@@ -283,6 +321,10 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, 
                 if (row < M && col < N) C[(size_t)col * ldc + row] = acc[r][c][v];
             }
         }
+    if (zg && M - 1 >= i0 && M - 1 < i0 + BM) {
+        __shared__ double gsh[NW];
+        gauge_partial<RM, RN, TM, TN, NW>(acc, M, N, i0, n0, wm, wn, lane, wave, by, zg, gpart, gsh);
+    }
 }
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -299,33 +341,47 @@ static bool dma_ok(int M, int N, int K, const double* A, int lda, const double* 
 }
 
 template <bool NN>
-static void launch_gemm(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
-                        const DevScalars* S, hipStream_t st)
+static int launch_gemm(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+                       const DevScalars* S, hipStream_t st, const double* zg, double* gpart, bool plan)
 {
     if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192 && dma_ok<NN>(M, N, K, A, lda, B, ldb)) {
-        hipLaunchKernelGGL((k_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN, GEMM_DMA_BN>),
-                           dim3(cdiv(M, 96), cdiv(N, GEMM_DMA_BN)), dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), 0, st, M, N,
-                           K, A, lda, B, ldb, C, ldc, S);
+        if (!plan)
+            hipLaunchKernelGGL((k_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN, GEMM_DMA_BN>),
+                               dim3(cdiv(M, 96), cdiv(N, GEMM_DMA_BN)), dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), 0, st, M,
+                               N, K, A, lda, B, ldb, C, ldc, S, zg, gpart);
+        return cdiv(N, GEMM_DMA_BN);
     } else if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
-        hipLaunchKernelGGL((k_gemm_nt<96, 128, 2, 4, NN>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M, N,
-                           K, A, lda, B, ldb, C, ldc, S);
+        if (!plan)
+            hipLaunchKernelGGL((k_gemm_nt<96, 128, 2, 4, NN>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M,
+                               N, K, A, lda, B, ldb, C, ldc, S, zg, gpart);
+        return cdiv(N, 128);
     } else if ((long)cdiv(M, 64) * cdiv(N, 64) >= 96) {
-        hipLaunchKernelGGL((k_gemm_nt<64, 64, 2, 2, NN>), dim3(cdiv(M, 64), cdiv(N, 64)), dim3(256), 0, st, M, N, K,
-                           A, lda, B, ldb, C, ldc, S);
-    } else {
-        hipLaunchKernelGGL((k_gemm_nt<32, 32, 2, 2, NN>), dim3(cdiv(M, 32), cdiv(N, 32)), dim3(256), 0, st, M, N, K,
-                           A, lda, B, ldb, C, ldc, S);
+        if (!plan)
+            hipLaunchKernelGGL((k_gemm_nt<64, 64, 2, 2, NN>), dim3(cdiv(M, 64), cdiv(N, 64)), dim3(256), 0, st, M, N,
+                               K, A, lda, B, ldb, C, ldc, S, zg, gpart);
+        return cdiv(N, 64);
     }
+    if (!plan)
+        hipLaunchKernelGGL((k_gemm_nt<32, 32, 2, 2, NN>), dim3(cdiv(M, 32), cdiv(N, 32)), dim3(256), 0, st, M, N, K,
+                           A, lda, B, ldb, C, ldc, S, zg, gpart);
+    return cdiv(N, 32);
 }
 
 void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                     const DevScalars* S, hipStream_t st)
 {
-    launch_gemm<false>(M, N, K, A, lda, B, ldb, C, ldc, S, st);
+    launch_gemm<false>(M, N, K, A, lda, B, ldb, C, ldc, S, st, nullptr, nullptr, false);
 }
 
 void launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
-                    const DevScalars* S, hipStream_t st)
+                    const DevScalars* S, hipStream_t st, const double* zg, double* gpart)
 {
-    launch_gemm<true>(M, N, K, A, lda, B, ldb, C, ldc, S, st);
+    launch_gemm<true>(M, N, K, A, lda, B, ldb, C, ldc, S, st, zg, gpart, false);
+}
+
+// Tile columns of the C = A * B launch these arguments get (the number of
+// gauge partials it writes).
+int gemm_nn_tiles_n(int M, int N, int K, const double* A, int lda, const double* B, int ldb)
+{
+    return launch_gemm<true>(M, N, K, A, lda, B, ldb, nullptr, 0, nullptr, 0, nullptr, nullptr, true);
 }

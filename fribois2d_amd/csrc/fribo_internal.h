@@ -108,6 +108,8 @@ struct FluidDev {
     double *ymm, *ymmi, *wy, *zg, *ysum;
     int ldy, ynull;
     double zginv;
+    double* gpart;           // gauge partials of the backward transform along y, one per tile column
+    int ngpart;
     // outflow arrays (1-based, n2+2)
     double *qb2s, *qb3s, *qb2n, *qb3n, *dqb2s, *dqb3s, *dqb2n, *dqb3n, *dq2x2o, *dq3x2o;
     // 1-D grid (1-based)
@@ -189,12 +191,13 @@ void launch_ksolve(const FluidDev& F, hipStream_t st);
 void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc, const DevScalars* S, hipStream_t st);
 void launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
-                    double* C, int ldc, const DevScalars* S, hipStream_t st);
+                    double* C, int ldc, const DevScalars* S, hipStream_t st, const double* zg = nullptr,
+                    double* gpart = nullptr);
+int gemm_nn_tiles_n(int M, int N, int K, const double* A, int lda, const double* B, int ldb);
 void launch_ysolve(const FluidDev& F, hipStream_t st);
 void launch_loads(const FluidDev& F, double hpr, hipStream_t st);
 void launch_checkbcs(const FluidDev& F, double* nb, hipStream_t st);
 void launch_diag(const FluidDev& F, double* part, int kv, double* out, hipStream_t st);
-void launch_ygauge(const FluidDev& F, hipStream_t st);
 int ytransform_build(int n, const double* am, const double* ac, const double* ap, int ld, double* Y, double* Yinv,
                      double* wy);
 int phini_sym(int n3, const double* zz, double alx3, int ld, double* zmm, double* zmmt, double* wr);

@@ -656,7 +656,15 @@ __global__ __launch_bounds__(64 * SW_UPD) void k_upd(const FluidDev F)
         double* __restrict__ q3 = F.q3;
         double* __restrict__ pr = F.pr;
         const int n3m = F.n3m;
-        const double doff = S->dphoff;   // gauge constant of the transform along y (0 along z)
+        // gauge constant of the transform along y (0 along z): the backward
+        // transform's partials, summed in tile order
+        double doff = 0.0;
+        if (F.taxis == 1) {
+            double g = 0.0;
+            for (int q = 0; q < F.ngpart; ++q) g = g + F.gpart[q];
+            doff = g * F.zginv;
+        }
+        if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0) F.S->dphoff = doff;
         double dm = (inj && k0 > 1) ? dph[c - P] - doff : 0.0;
         double dc = inj ? dph[c] - doff : 0.0;
         // plane k+1 of dph and plane k of q2/q3/pr are loaded one iteration ahead

@@ -10,13 +10,14 @@
 //   hmn = Yinv * dph           (GEMM, B operand k-contiguous)
 //   (A_z + wy(j)) v(j,:) = hmn(j,:)   for every y-mode j   (this file)
 //   dph = Y * v                (GEMM)
-//   dph -= c                   (gauge, k_ygauge)
+//   dph -= c                   (gauge: partials from the second GEMM, subtracted by k_upd)
 //
 // The Neumann problem is singular (constants); the reference's order pins
 // the z null mode's last unknown, which fixes the constant through
-// sum_k zmmt(k0,k) dph(n2m,k) = 0.  Here the y null mode is pinned instead
-// and k_ygauge computes the constant c that restores the reference's gauge;
-// the kernels that read dph subtract it (S->dphoff).
+// sum_k zmmt(k0,k) dph(n2m,k) = 0.  Here the y null mode is pinned instead;
+// the epilogue of the second GEMM leaves the partial sums of the constant c
+// that restores the reference's gauge, and k_upd subtracts c when it reads
+// dph (S->dphoff keeps c for the read-back of dph).
 //
 // The per-mode z-lines use the partition of the q2 k-system (ks[0]: n3m rows,
 // up to 64 chunks).  Lanes run along the y-mode index (coalesced), each
@@ -222,25 +223,6 @@ __global__ __launch_bounds__(64) void k_ysolve3(const FluidDev F)
     col[(size_t)e * P] = X;
 }
 
-// c = sum_k zg(k) dph(n2m,k) / sum_k zg(k): the constant that moves the
-// y-pinned solution to the reference's z-pinned gauge (fixed-order sum).
-__global__ __launch_bounds__(1024) void k_ygauge(const FluidDev F)
-{
-    __shared__ double sh[1024];
-    DevScalars* S = F.S;
-    if (S->err) return;
-    const int t = threadIdx.x;
-    double s = 0.0;
-    for (int k = 1 + t; k <= F.n3m; k += 1024) s = s + F.zg[k - 1] * F.dph[fidx(F, F.n2m, k)];
-    sh[t] = s;
-    __syncthreads();
-    for (int h = 512; h > 0; h >>= 1) {
-        if (t < h) sh[t] = sh[t] + sh[t + h];
-        __syncthreads();
-    }
-    if (t == 0) S->dphoff = sh[0] * F.zginv;
-}
-
 void launch_ysolve(const FluidDev& F, hipStream_t st)
 {
     const int mm = F.ks[0].maxm - 1;   // interior rows per chunk
@@ -254,9 +236,4 @@ void launch_ysolve(const FluidDev& F, hipStream_t st)
     else if (mm <= 32) hipLaunchKernelGGL(k_ysolve3<32>, g, dim3(64), 0, st, F);
     else if (mm <= 64) hipLaunchKernelGGL(k_ysolve3<64>, g, dim3(64), 0, st, F);
     else hipLaunchKernelGGL(k_ysolve3<128>, g, dim3(64), 0, st, F);
-}
-
-void launch_ygauge(const FluidDev& F, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_ygauge, dim3(1), dim3(1024), 0, st, F);
 }
