@@ -13,7 +13,7 @@
 #define FRIBO_DIMSD 9
 #define FRIBO_KMAXCH 64      // max chunks of a partitioned k-line (one wave)
 #define FRIBO_NPART 512      // partial sums of the CFL reduction
-#define FRIBO_LWORK 20
+#define FRIBO_LWORK 20       // per-marker record of the load computation
 
 // Marching tiles of the stencil kernels (k_rhs, k_div, k_upd): a wave owns
 // FRIBO_WJ output columns (plus a halo lane each side) and MK planes
@@ -30,7 +30,7 @@
 inline int fribo_npr_part(int n2m, int n3m)
 {
     return ((n2m + FRIBO_WJ - 1) / FRIBO_WJ) * ((n3m + FRIBO_MK - 1) / FRIBO_MK);
-}       // per-marker record of the load computation
+}
 
 struct DevScalars {
     // step inputs, written by the host before every step (one 24-byte copy)
@@ -157,6 +157,20 @@ __device__ inline double lane_bcast(double v, int l)
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
     const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// 1/x from the hardware estimate refined by two Newton steps (within an ulp
+// of the rounded quotient for the pivots of the diagonally dominant line
+// systems, far from zero and from the denormal range).  A fraction of the
+// latency of the IEEE division sequence, which matters on the serial
+// elimination chains.
+__device__ inline double recip(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
 }
 
 __host__ __device__ inline size_t fidx(const FluidDev& F, int j, int k)

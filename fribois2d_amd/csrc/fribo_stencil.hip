@@ -225,9 +225,10 @@ __device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, dou
     if (p < K.nch) {
         const int s = K.cs[p], e = K.cs[p + 1] - 1;
         double cpp = 0.0, vpp = 1.0, gg = 1.0, wl = 0.0, glast = 1.0, vsum = 0.0;
+#pragma unroll 4
         for (int i = s; i <= e - 1; ++i) {
             const double a = la[i], c = lc[i];
-            const double iv = 1.0 / (1.0 - a * cpp);
+            const double iv = recip(1.0 - a * cpp);
             const double cpi = c * iv;
             const double vpi = (-a * vpp) * iv;
             const int idx = K.inter ? (i - s) * 64 + p : i;
@@ -275,7 +276,7 @@ __device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, dou
         double apn = __shfl_down(alpha, h, 64), bp = __shfl_down(beta, h, 64), gp = __shfl_down(gamma, h, 64);
         if (p - h < 0) { amn = 0.0; bm = 1.0; gm = 0.0; }
         if (p + h >= 64) { apn = 0.0; bp = 1.0; gp = 0.0; }
-        const double m1 = alpha / bm, m2 = gamma / bp;
+        const double m1 = alpha * recip(bm), m2 = gamma * recip(bp);
         K.k1[li * 64 + p] = m1;
         K.k2[li * 64 + p] = m2;
         const double na = -m1 * amn, ng = -m2 * gp;
@@ -296,27 +297,146 @@ size_t sysprep_lds_bytes(const FluidDev& F)
     return (size_t)3 * n * sizeof(double);
 }
 
-// pr_submean (updvp.F90:125-141) and boucqt (iobcs.F90:300-312) by one
-// workgroup of 1024 threads.  The mean is kept pending for the interior
-// (S->pend, subtracted on the next read) and accumulated for the entries no
-// kernel reads (row j=n2 and the ghost planes, S->goff, subtracted when the
-// state is read back).
+// Pressure mean of the previous substep (pr_submean, updvp.F90:125-141) from
+// the npr_part partials of k_upd: thread t adds entries t, t+1024, ... in
+// turn (eight loads in flight at a time), then block_sum.  Fixed order.
+__device__ inline double pr_mean(const FluidDev& F, double* sh)
+{
+    const int t = threadIdx.x, n = F.npr_part;
+    double s = 0.0;
+    for (int b = t; b < n; b += 8 * 1024) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = F.pr_part[min(b + u * 1024, n - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (b + u * 1024 < n) s = s + v[u];
+    }
+    return block_sum<1024>(sh, s) / (F.rext2 * F.alx3);
+}
+
+// pr_submean and boucqt (iobcs.F90:300-312) by one workgroup of 1024
+// threads.  The mean is kept pending for the interior (S->pend, subtracted on
+// the next read) and accumulated for the entries no kernel reads (row j=n2
+// and the ghost planes, S->goff, subtracted when the state is read back).
+// Thread t owns rows j = 1+t, 1+t+1024, ...; the loads of two rows are issued
+// before their stores.
 __device__ void mean_block(const FluidDev& F, double* sh)
 {
     DevScalars* S = F.S;
-    const int t = threadIdx.x;
-    double s = 0.0;
-    for (int i = t; i < F.npr_part; i += 1024) s = s + F.pr_part[i];
-    const double pravg = block_sum<1024>(sh, s) / (F.rext2 * F.alx3);
-    for (int j = 1 + t; j <= F.n2m; j += 1024) {
-        F.qb2s[j] = F.qb2s[j] + F.dqb2s[j];
-        F.qb3s[j] = F.qb3s[j] + F.dqb3s[j];
-        F.qb2n[j] = F.qb2n[j] + F.dqb2n[j];
-        F.qb3n[j] = F.qb3n[j] + F.dqb3n[j];
+    const int t = threadIdx.x, n2m = F.n2m;
+    const double pravg = pr_mean(F, sh);
+    for (int j0 = 1 + t; j0 <= n2m; j0 += 2048) {
+        double a[2][8];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int j = min(j0 + 1024 * u, n2m);
+            a[u][0] = F.qb2s[j];
+            a[u][1] = F.dqb2s[j];
+            a[u][2] = F.qb3s[j];
+            a[u][3] = F.dqb3s[j];
+            a[u][4] = F.qb2n[j];
+            a[u][5] = F.dqb2n[j];
+            a[u][6] = F.qb3n[j];
+            a[u][7] = F.dqb3n[j];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int j = j0 + 1024 * u;
+            if (j <= n2m) {
+                F.qb2s[j] = a[u][0] + a[u][1];
+                F.qb3s[j] = a[u][2] + a[u][3];
+                F.qb2n[j] = a[u][4] + a[u][5];
+                F.qb3n[j] = a[u][6] + a[u][7];
+            }
+        }
     }
     if (t == 0) {
         S->pend = pravg;
         S->goff = S->goff + pravg;
+    }
+}
+
+// Workgroup 0 of k_bc: with mean, pr_submean/boucqt of the previous substep
+// (as mean_block), then boucdq (iobcs.F90:22-50) and the global mass balance
+// at the outflow (iobcs.F90:54-74).  Thread t owns rows j = 1+t, 1+t+1024,
+// ...: every row value is loaded once, two rows' loads in flight together,
+// and the boundary values updated by boucqt feed boucdq from registers.
+__device__ void bc_rows(const FluidDev& F, const StepScalars& r, int mean, double* sh, double* sh2)
+{
+    DevScalars* S = F.S;
+    const int t = threadIdx.x, n2m = F.n2m;
+    const double pravg = mean ? pr_mean(F, sh) : 0.0;
+    const double dt = r.dt, ga = r.ga, ro = r.ro;
+    const double udx3c_n3 = F.udx3c[F.n3];
+    double area = 0.0, qout = 0.0;
+    for (int j0 = 1 + t; j0 <= n2m; j0 += 2048) {
+        double qn2[2], qn3[2], q2e[2], q3e[2], o2[2], o3[2], g[2];
+        double qs2[2], qs3[2], dqs2[2], dqs3[2], dqn2[2], dqn3[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int j = min(j0 + 1024 * u, n2m);
+            qn2[u] = F.qb2n[j];
+            qn3[u] = F.qb3n[j];
+            q2e[u] = F.q2[fidx(F, j, F.n3m)];
+            q3e[u] = F.q3[fidx(F, j, F.n3m)];
+            o2[u] = F.dq2x2o[j];
+            o3[u] = F.dq3x2o[j];
+            g[u] = F.g2rm[j];
+            if (mean) {
+                qs2[u] = F.qb2s[j];
+                qs3[u] = F.qb3s[j];
+                dqs2[u] = F.dqb2s[j];
+                dqs3[u] = F.dqb3s[j];
+                dqn2[u] = F.dqb2n[j];
+                dqn3[u] = F.dqb3n[j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int j = j0 + 1024 * u;
+            if (j <= n2m) {
+                if (mean) {
+                    F.qb2s[j] = qs2[u] + dqs2[u];
+                    F.qb3s[j] = qs3[u] + dqs3[u];
+                    qn2[u] = qn2[u] + dqn2[u];
+                    qn3[u] = qn3[u] + dqn3[u];
+                    F.qb2n[j] = qn2[u];
+                    F.qb3n[j] = qn3[u];
+                }
+                F.dqb2s[j] = 0.0;
+                F.dqb3s[j] = 0.0;
+                const double dq2x2 = (qn2[u] - q2e[u]) * 2.0 * udx3c_n3;
+                const double d2 = -dt * (ga * dq2x2 + ro * o2[u]) * F.cou;
+                const double dq3x2 = (qn3[u] - q3e[u]) * udx3c_n3;
+                const double d3 = -dt * (ga * dq3x2 + ro * o3[u]) * F.cou;
+                F.dqb2n[j] = d2;
+                F.dqb3n[j] = d3;
+                F.dq2x2o[j] = dq2x2;
+                F.dq3x2o[j] = dq3x2;
+                const double darea = g[u] / F.dx2;
+                area = area + darea;
+                qout = qout + d3 * darea;
+            }
+        }
+    }
+    if (t == 0) {
+        F.dqb2s[F.n2] = 0.0;
+        F.dqb3s[F.n2] = 0.0;
+        if (mean) {
+            S->pend = pravg;
+            S->goff = S->goff + pravg;
+        }
+    }
+    const double tarea = block_sum<1024>(sh, area);
+    const double tqout = block_sum<1024>(sh2, qout);
+    const double qinf = 0.0 * tarea;   // dqb3s == 0 (steady inflow)
+    const double cor = (qinf - tqout) / tarea;
+    for (int j = 1 + t; j <= n2m; j += 1024) F.dqb3n[j] = F.dqb3n[j] + cor;
+    if (t == 0) {
+        S->qinf = qinf;
+        S->qout = tqout;
+        S->area = tarea;
     }
 }
 
@@ -347,10 +467,6 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int mean)
         sys_prepare(F, K, r.betadx, dyn);
         return;
     }
-    if (mean) {
-        mean_block(F, sh);
-        __syncthreads();
-    }
     if (t == 0) {
         if (n == 1) {
             S->q3avg = r.q3sum / (F.rext2 * F.alx3);
@@ -367,40 +483,7 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int mean)
         S->ugkks = 1.0 / (F.g3rm[1] * F.g3rc[1]) * F.dx3q * 2.0 * r.betadx;
         S->ugkkn = 1.0 / (F.g3rm[F.n3m] * F.g3rc[F.n3]) * F.dx3q * 2.0 * r.betadx;
     }
-    const double dt = r.dt, ga = r.ga, ro = r.ro;
-
-    // boucdq (iobcs.F90:22-50)
-    for (int j = 1 + t; j <= F.n2; j += 1024) {
-        F.dqb2s[j] = 0.0;
-        F.dqb3s[j] = 0.0;
-    }
-    const double udx3c_n3 = F.udx3c[F.n3];
-    for (int j = 1 + t; j <= F.n2m; j += 1024) {
-        double dq2x2 = (F.qb2n[j] - F.q2[fidx(F, j, F.n3m)]) * 2.0 * udx3c_n3;
-        F.dqb2n[j] = -dt * (ga * dq2x2 + ro * F.dq2x2o[j]) * F.cou;
-        double dq3x2 = (F.qb3n[j] - F.q3[fidx(F, j, F.n3m)]) * udx3c_n3;
-        F.dqb3n[j] = -dt * (ga * dq3x2 + ro * F.dq3x2o[j]) * F.cou;
-        F.dq2x2o[j] = dq2x2;
-        F.dq3x2o[j] = dq3x2;
-    }
-    __syncthreads();
-    // iobcs.F90:54-74: global mass balance at the outflow
-    double area = 0.0, qout = 0.0;
-    for (int j = 1 + t; j <= F.n2m; j += 1024) {
-        double darea = F.g2rm[j] / F.dx2;
-        area = area + darea;
-        qout = qout + F.dqb3n[j] * darea;
-    }
-    const double tarea = block_sum<1024>(sh, area);
-    const double tqout = block_sum<1024>(sh2, qout);
-    const double qinf = 0.0 * tarea;   // dqb3s == 0 (steady inflow)
-    const double cor = (qinf - tqout) / tarea;
-    for (int j = 1 + t; j <= F.n2m; j += 1024) F.dqb3n[j] = F.dqb3n[j] + cor;
-    if (t == 0) {
-        S->qinf = qinf;
-        S->qout = tqout;
-        S->area = tarea;
-    }
+    bc_rows(F, r, mean, sh, sh2);
     // q2 wall rows: q2(1,:) = q2(n2,:) = 0 on the ghost planes (helsolve.F90:32-33)
     if (t < 6) {
         int k = (t < 3) ? t - 2 : F.n3m + 1 + (t - 3);
@@ -723,7 +806,7 @@ void launch_upd(const FluidDev& F, hipStream_t st)
 // pr_submean (updvp.F90:125-141) and boucqt (iobcs.F90:300-312) as a kernel
 // of its own (after the last substep of a step, and after every substep of
 // the substep-by-substep entry point); within a step, k_bc of the next
-// substep does the same work first (mean_block).
+// substep does the same work first (bc_rows).
 __global__ __launch_bounds__(1024) void k_mean(const FluidDev F)
 {
     __shared__ double sh[16];

@@ -53,19 +53,6 @@ __device__ inline double yrow(const double (&v)[NG], int t)
     return lane_bcast(NG == 1 || t < 64 ? v[0] : v[NG - 1], t & 63);
 }
 
-// 1/x from the hardware estimate refined by two Newton steps (within an ulp
-// of the rounded quotient; the pivots of these diagonally dominant systems
-// are far from zero and from the denormal range).  The elimination chain of
-// a chunk is one reciprocal per row, so its latency sets the pass time.
-__device__ inline double recip(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
-
 // Chunk elimination of line j, chunk p (rows s..e-1 interior, e separator).
 // Returns the recurrence state; cp/vp/y are kept when STORE.
 template <int MAXM, bool STORE>
