@@ -270,6 +270,8 @@ class Case:
     center: tuple = (0.0, 0.0)           # (y, z) of the body reference point
     motion: Optional[Callable] = None    # t -> (dy, dz, dtheta, vy, vz, omega)
     seed: Optional[int] = None
+    # markers of the body at rest (markers_from_vertices), computed once
+    _rest: Optional[tuple] = dataclasses.field(default=None, repr=False, compare=False)
 
     @property
     def nlm(self):
@@ -277,9 +279,11 @@ class Case:
 
     def markers(self, t=0.0):
         """lmcc (nlm,2), lmvel (nlm,2), lmA (nlm,) for the body at time t."""
-        lmcc0, lmA = markers_from_vertices(self.verts)
+        if self._rest is None:
+            self._rest = markers_from_vertices(self.verts)
+        lmcc0, lmA = self._rest
         if self.motion is None:
-            return lmcc0, np.zeros_like(lmcc0), lmA
+            return lmcc0.copy(), np.zeros_like(lmcc0), lmA.copy()
         dy, dz, th, vy, vz, om = self.motion(t)
         yc, zc = self.center
         ry, rz = lmcc0[:, 0] - yc, lmcc0[:, 1] - zc
@@ -287,7 +291,7 @@ class Case:
         ryr, rzr = c * ry - s * rz, s * ry + c * rz
         lmcc = np.stack([yc + dy + ryr, zc + dz + rzr], axis=1)
         lmvel = np.stack([vy - om * rzr, vz + om * ryr], axis=1)
-        return lmcc, lmvel, lmA
+        return lmcc, lmvel, lmA.copy()
 
     def surface(self, t=0.0):
         """lmnor (nlm,2) and lmacc (nlm,2) as rigid() leaves them at time t:

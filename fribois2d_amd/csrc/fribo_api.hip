@@ -44,6 +44,11 @@ struct fribo_ctx {
     std::vector<double> lmcc;   // host copy for the domain check
     int nlm_alloc = 0;
     std::vector<void*> mk_allocs;
+    // marker upload: lmcc, lmvel, lmA staged together in pinned memory and
+    // sent in one copy; ev_mk marks the copy so the staging is reused only
+    // after it has been read
+    double* mk_stage = nullptr;
+    hipEvent_t ev_mk = nullptr;
     DevScalars hs;
     // profiling
     bool profiling = false;
@@ -503,6 +508,8 @@ extern "C" int fribo_finalize(fribo_handle h)
     if (h->in) hipHostFree(h->in);
     for (auto ev : h->evpool) hipEventDestroy(ev);
     for (void* q : h->mk_allocs) hipFree(q);
+    if (h->mk_stage) hipHostFree(h->mk_stage);
+    if (h->ev_mk) hipEventDestroy(h->ev_mk);
     for (void* q : h->allocs) hipFree(q);
     hipStreamDestroy(h->st);
     delete h;
@@ -611,6 +618,10 @@ extern "C" int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc
         HIPCHK(hipStreamSynchronize(h->st));
         for (void* q : h->mk_allocs) hipFree(q);
         h->mk_allocs.clear();
+        if (h->mk_stage) hipHostFree(h->mk_stage);
+        h->mk_stage = nullptr;
+        h->nlm_alloc = -1;   // no marker buffers until the allocation below succeeds
+        F.nlm = 0;
         int maxkeys = 2048;
         while (maxkeys < nlm * FRIBO_DIMSD) maxkeys <<= 1;
         if (nlm >= (1 << 20) / FRIBO_DIMSD)
@@ -618,9 +629,11 @@ extern "C" int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc
         F.maxkeys = maxkeys;
         auto* L = &h->mk_allocs;
         int e = 0;
-        if (!e) e = dalloc(h, &F.lmcc, 2 * (size_t)nlm, L);
-        if (!e) e = dalloc(h, &F.lmvel, 2 * (size_t)nlm, L);
-        if (!e) e = dalloc(h, &F.lmA, (size_t)nlm, L);
+        if (!e) e = dalloc(h, &F.lmcc, 5 * (size_t)nlm, L);
+        if (!e) {
+            F.lmvel = F.lmcc + 2 * (size_t)nlm;
+            F.lmA = F.lmcc + 4 * (size_t)nlm;
+        }
         if (!e) e = dalloc(h, &F.lmind, 4 * (size_t)nlm, L);
         if (!e) e = dalloc(h, &F.eulins, 4 * FRIBO_DIMSD * (size_t)nlm, L);
         if (!e) e = dalloc(h, &F.phi, 2 * FRIBO_DIMSD * (size_t)nlm, L);
@@ -636,16 +649,25 @@ extern "C" int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc
         if (!e) e = dalloc(h, &F.tau, 2 * (size_t)nlm, L);
         if (!e) e = dalloc(h, &F.lwork, FRIBO_LWORK * (size_t)nlm, L);
         if (!e) e = dalloc(h, &F.bcerr, 2 * (size_t)nlm, L);
+        if (!e && nlm > 0) {
+            hipError_t he = hipHostMalloc((void**)&h->mk_stage, 5 * (size_t)nlm * sizeof(double), hipHostMallocDefault);
+            if (he == hipSuccess && !h->ev_mk) he = hipEventCreateWithFlags(&h->ev_mk, hipEventDisableTiming);
+            if (he != hipSuccess) e = set_err(FRIBO_ERR_HIP, hipGetErrorString(he));
+        }
         if (e) return e;
         h->nlm_alloc = nlm;
         F.nlm = nlm;
     }
     if (nlm > 0) {
-        HIPCHK(hipMemcpyAsync(F.lmcc, lmcc, 2 * nlm * sizeof(double), hipMemcpyHostToDevice, h->st));
-        HIPCHK(hipMemcpyAsync(F.lmvel, lmvel, 2 * nlm * sizeof(double), hipMemcpyHostToDevice, h->st));
-        HIPCHK(hipMemcpyAsync(F.lmA, lmA, nlm * sizeof(double), hipMemcpyHostToDevice, h->st));
+        // the previous upload has left the staging buffer (normally long ago)
+        HIPCHK(hipEventSynchronize(h->ev_mk));
+        double* sbuf = h->mk_stage;
+        memcpy(sbuf, lmcc, 2 * (size_t)nlm * sizeof(double));
+        memcpy(sbuf + 2 * (size_t)nlm, lmvel, 2 * (size_t)nlm * sizeof(double));
+        memcpy(sbuf + 4 * (size_t)nlm, lmA, (size_t)nlm * sizeof(double));
+        HIPCHK(hipMemcpyAsync(F.lmcc, sbuf, 5 * (size_t)nlm * sizeof(double), hipMemcpyHostToDevice, h->st));
+        HIPCHK(hipEventRecord(h->ev_mk, h->st));
         h->lmcc.assign(lmcc, lmcc + 2 * (size_t)nlm);
-        HIPCHK(hipStreamSynchronize(h->st));
     }
     return FRIBO_OK;
 }

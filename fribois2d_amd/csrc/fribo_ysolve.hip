@@ -27,6 +27,13 @@
 // wave per line, PCR) -> chunk back-substitution.
 #include "fribo_internal.h"
 
+// Waves per workgroup (each wave its own 64 y-modes, or its own mode in the
+// separator pass).
+#ifndef FRIBO_YW
+#define FRIBO_YW 1
+#endif
+#define YW FRIBO_YW
+
 __device__ inline size_t yidx(const FluidDev& F, int q, int j, int p)
 {
     // summary q of chunk p of y-mode j: chunk fastest
@@ -105,14 +112,14 @@ __device__ inline void ychunk(const FluidDev& F, const KSys& K, int p, double w,
 // Lanes past the last y-mode work on mode n2m and store nothing (every lane
 // stays active through the row broadcasts).
 template <int MAXM>
-__global__ __launch_bounds__(64) void k_ysolve1(const FluidDev F)
+__global__ __launch_bounds__(64 * YW) void k_ysolve1(const FluidDev F)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const KSys& K = F.ks[0];
     const int p = blockIdx.y;
     if (p >= K.nch) return;
-    const int j = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = (blockIdx.x * YW + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63) + 1;
     const bool act = j <= F.n2m;
     const int jc = act ? j : F.n2m;
     const double* col = F.hmn + (jc - 1);
@@ -132,12 +139,13 @@ __global__ __launch_bounds__(64) void k_ysolve1(const FluidDev F)
     Y[yidx(F, 5, j, p)] = ws;
 }
 
-__global__ __launch_bounds__(64) void k_ysolve2(const FluidDev F)
+__global__ __launch_bounds__(64 * YW) void k_ysolve2(const FluidDev F)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const KSys& K = F.ks[0];
-    const int j = blockIdx.x + 1, p = threadIdx.x, lane = p;
+    const int j = blockIdx.x * YW + (threadIdx.x >> 6) + 1, p = threadIdx.x & 63, lane = p;
+    if (j > F.n2m) return;   // whole wave
     const int nch = K.nch;
     const double* Y = F.ysum;
     double alpha = 0.0, beta = 1.0, gamma = 0.0, delta = 0.0;
@@ -178,14 +186,14 @@ __global__ __launch_bounds__(64) void k_ysolve2(const FluidDev F)
 }
 
 template <int MAXM>
-__global__ __launch_bounds__(64) void k_ysolve3(const FluidDev F)
+__global__ __launch_bounds__(64 * YW) void k_ysolve3(const FluidDev F)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const KSys& K = F.ks[0];
     const int p = blockIdx.y;
     if (p >= K.nch) return;
-    const int j = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = (blockIdx.x * YW + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63) + 1;
     const bool act = j <= F.n2m;
     const int jc = act ? j : F.n2m;
     double* col = F.hmn + (jc - 1);
@@ -213,14 +221,14 @@ __global__ __launch_bounds__(64) void k_ysolve3(const FluidDev F)
 void launch_ysolve(const FluidDev& F, hipStream_t st)
 {
     const int mm = F.ks[0].maxm - 1;   // interior rows per chunk
-    dim3 g((F.n2m + 63) / 64, F.ks[0].nch);
-    if (mm <= 16) hipLaunchKernelGGL(k_ysolve1<16>, g, dim3(64), 0, st, F);
-    else if (mm <= 32) hipLaunchKernelGGL(k_ysolve1<32>, g, dim3(64), 0, st, F);
-    else if (mm <= 64) hipLaunchKernelGGL(k_ysolve1<64>, g, dim3(64), 0, st, F);
-    else hipLaunchKernelGGL(k_ysolve1<128>, g, dim3(64), 0, st, F);
-    hipLaunchKernelGGL(k_ysolve2, dim3(F.n2m), dim3(64), 0, st, F);
-    if (mm <= 16) hipLaunchKernelGGL(k_ysolve3<16>, g, dim3(64), 0, st, F);
-    else if (mm <= 32) hipLaunchKernelGGL(k_ysolve3<32>, g, dim3(64), 0, st, F);
-    else if (mm <= 64) hipLaunchKernelGGL(k_ysolve3<64>, g, dim3(64), 0, st, F);
-    else hipLaunchKernelGGL(k_ysolve3<128>, g, dim3(64), 0, st, F);
+    dim3 g((F.n2m + 64 * YW - 1) / (64 * YW), F.ks[0].nch), b(64 * YW);
+    if (mm <= 16) hipLaunchKernelGGL(k_ysolve1<16>, g, b, 0, st, F);
+    else if (mm <= 32) hipLaunchKernelGGL(k_ysolve1<32>, g, b, 0, st, F);
+    else if (mm <= 64) hipLaunchKernelGGL(k_ysolve1<64>, g, b, 0, st, F);
+    else hipLaunchKernelGGL(k_ysolve1<128>, g, b, 0, st, F);
+    hipLaunchKernelGGL(k_ysolve2, dim3((F.n2m + YW - 1) / YW), b, 0, st, F);
+    if (mm <= 16) hipLaunchKernelGGL(k_ysolve3<16>, g, b, 0, st, F);
+    else if (mm <= 32) hipLaunchKernelGGL(k_ysolve3<32>, g, b, 0, st, F);
+    else if (mm <= 64) hipLaunchKernelGGL(k_ysolve3<64>, g, b, 0, st, F);
+    else hipLaunchKernelGGL(k_ysolve3<128>, g, b, 0, st, F);
 }

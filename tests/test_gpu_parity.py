@@ -150,6 +150,44 @@ def test_multistep(name, nsteps, axis):
         assert relmax(sg[f], o.get(f)) <= 1e-10, f
 
 
+# Step options of fluid_par.in / global_par.in the workloads above leave at
+# their defaults: Adams-Bashforth stepping (nsst=1, fluid_tsch.F90:62-77),
+# the other MLS weight functions (MLSint.F90:441-486), free-slip walls
+# (coetar.F90 inslws/inslwn), moving walls (INVTR.F90 q3lw/q3uw) and fixed
+# dt with the CFL check (fluid_tsch.F90:50-55).
+OPTIONS = [
+    ("nsst1", {"nsst": 1}),
+    ("wfun2", {"wfun": 2}),
+    ("wfun3", {"wfun": 3}),
+    ("wfun4", {"wfun": 4}),
+    ("slipwalls", {"inslws": True, "inslwn": True}),
+    ("movingwalls", {"q3uw": 0.3, "q3lw": -0.2}),
+    ("fixeddt", {"idtv": True, "dt0": 2.0e-3}),
+]
+
+
+@pytest.mark.parametrize("axis", ["z", "y"])
+@pytest.mark.parametrize("label,opts", OPTIONS, ids=[o[0] for o in OPTIONS])
+def test_options_match_the_oracle(label, opts, axis):
+    """cyl64, 12 steps per option set: same bars as the multi-step runs."""
+    c = case("cyl64")
+    p = cases.FluidParams(**{**c.params.__dict__, **opts})
+    c = cases.Case(c.name, p, c.rc, c.zz, c.verts, c.center, c.motion, c.seed)
+    dec = decomposition(c)
+    o = oracle_for(c, dec)
+    g = gpu_for(c, dec, axis=axis)
+    ho, hg = [], []
+    run_steps(o, c, 12, hist=ho)
+    run_steps(g, c, 12, hist=hg)
+    assert relmax([h[1] for h in hg], [h[1] for h in ho]) <= 1e-12
+    assert relmax([h[2] for h in hg], [h[2] for h in ho]) <= 1e-12
+    so, sg = {k: o.get(k) for k in ("q2", "q3", "pr", "ru2", "ru3")}, g.get_state()
+    for f in ("q2", "q3", "pr", "ru2", "ru3"):
+        e = relmax(sg[f], so[f])
+        print(f"{label} {axis} {f}: {e:.2e}", flush=True)
+        assert e <= 1e-10, f
+
+
 @pytest.mark.parametrize("axis", ["z", "auto"])
 @pytest.mark.parametrize("name", ["osccyl", "gallrect"])
 def test_reference_examples(name, axis):
