@@ -206,6 +206,26 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
 // clobber keeps the compiler from moving memory accesses across it.
 __device__ inline void raw_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
+#ifndef GEMM_ASM_DMA
+#define GEMM_ASM_DMA 1
+#endif
+// One 16-byte-per-lane global -> LDS load (global_load_lds_dwordx4: lane l
+// writes dst + 16 l).  Issued as inline assembly, the compiler does not track
+// the LDS write, so it adds no wait for it before later LDS reads of other
+// stages; the kernel's own vmcnt wait and barrier order every read after the
+// loads of its slice.
+__device__ inline void dma16(const double* src, double* dst)
+{
+#if GEMM_ASM_DMA
+    const unsigned lds = __builtin_amdgcn_readfirstlane(
+        (unsigned)(uintptr_t)(__attribute__((address_space(3))) double*)dst);
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(src) : "memory");
+#else
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+#endif
+}
+
 template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A,
                                                            int lda, const double* __restrict__ B, int ldb,
@@ -258,8 +278,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, 
                 else src = B + (size_t)min(k0 + min(row, BK - 1), K - 1) * ldb + min(n0 + 2 * q, nlast);
                 dst = Bs + jj * 128;
             }
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                             (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+            dma16(src, dst);
         }
     };
 
