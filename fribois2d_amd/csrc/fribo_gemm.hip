@@ -49,6 +49,7 @@
 // tiles in 4 x 8 patches the tile indices are permuted to give each XCD
 // compact patches (its L2 then serves 4 A- and 8 B-tiles to 32 workgroups).
 #include "fribo_internal.h"
+#include <hip/hip_ext.h>
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -206,6 +207,13 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
 // clobber keeps the compiler from moving memory accesses across it.
 __device__ inline void raw_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
+// Wave priority around the issue of the next slice's loads (2; 1: around
+// the MFMA loop instead, 0: none): with two workgroups per CU, a wave that
+// has passed the barrier issues its loads ahead of the other workgroup's
+// MFMA waves, which keeps the next stages in flight (-2% GEMM time).
+#ifndef GEMM_PRIO
+#define GEMM_PRIO 2
+#endif
 #ifndef GEMM_ASM_DMA
 #define GEMM_ASM_DMA 1
 #endif
@@ -305,7 +313,16 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, 
             __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
             raw_barrier();
         }
+#if GEMM_PRIO == 2
+        __builtin_amdgcn_s_setprio(1);
+#endif
         if (s + 2 < nk) issue((s + 2) * BK, An, Bn);
+#if GEMM_PRIO == 2
+        __builtin_amdgcn_s_setprio(0);
+#endif
+#if GEMM_PRIO == 1
+        __builtin_amdgcn_s_setprio(1);
+#endif
 #pragma unroll
         for (int k4 = 0; k4 < BK; k4 += 4) {
             const int kr = k4 + (lane >> 4);
@@ -321,6 +338,9 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, 
                 for (int c = 0; c < RN; ++c)
                     acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
         }
+#if GEMM_PRIO == 1
+        __builtin_amdgcn_s_setprio(0);
+#endif
     };
     issue(0, S0, S0 + SA);
     if (nk > 1) issue(BK, S1, S1 + SA);
@@ -359,43 +379,54 @@ static bool dma_ok(int M, int N, int K, const double* A, int lda, const double* 
            ldb % 2 == 0 && lda >= M + (M & 1) && ldb >= bext + (bext & 1);
 }
 
+// One launch; with events, the dispatch itself records its start and end
+// (hipExtLaunchKernel), so timing adds no packets between kernels.
+template <class Kern, class... Args>
+static void launch_timed(Kern kern, dim3 g, dim3 b, hipStream_t st, hipEvent_t e0, hipEvent_t e1, Args... args)
+{
+    if (e0) hipExtLaunchKernelGGL(kern, g, b, 0, st, e0, e1, 0, args...);
+    else hipLaunchKernelGGL(kern, g, b, 0, st, args...);
+}
+
 template <bool NN>
 static int launch_gemm(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
-                       const DevScalars* S, hipStream_t st, const double* zg, double* gpart, bool plan)
+                       const DevScalars* S, hipStream_t st, const double* zg, double* gpart, bool plan,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192 && dma_ok<NN>(M, N, K, A, lda, B, ldb)) {
         if (!plan)
-            hipLaunchKernelGGL((k_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN, GEMM_DMA_BN>),
-                               dim3(cdiv(M, 96), cdiv(N, GEMM_DMA_BN)), dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), 0, st, M,
-                               N, K, A, lda, B, ldb, C, ldc, S, zg, gpart);
+            launch_timed(k_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN, GEMM_DMA_BN>,
+                         dim3(cdiv(M, 96), cdiv(N, GEMM_DMA_BN)), dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), st, e0, e1, M,
+                         N, K, A, lda, B, ldb, C, ldc, S, zg, gpart);
         return cdiv(N, GEMM_DMA_BN);
     } else if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
         if (!plan)
-            hipLaunchKernelGGL((k_gemm_nt<96, 128, 2, 4, NN>), dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), 0, st, M,
-                               N, K, A, lda, B, ldb, C, ldc, S, zg, gpart);
+            launch_timed(k_gemm_nt<96, 128, 2, 4, NN>, dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), st, e0, e1, M, N,
+                         K, A, lda, B, ldb, C, ldc, S, zg, gpart);
         return cdiv(N, 128);
     } else if ((long)cdiv(M, 64) * cdiv(N, 64) >= 96) {
         if (!plan)
-            hipLaunchKernelGGL((k_gemm_nt<64, 64, 2, 2, NN>), dim3(cdiv(M, 64), cdiv(N, 64)), dim3(256), 0, st, M, N,
-                               K, A, lda, B, ldb, C, ldc, S, zg, gpart);
+            launch_timed(k_gemm_nt<64, 64, 2, 2, NN>, dim3(cdiv(M, 64), cdiv(N, 64)), dim3(256), st, e0, e1, M, N,
+                         K, A, lda, B, ldb, C, ldc, S, zg, gpart);
         return cdiv(N, 64);
     }
     if (!plan)
-        hipLaunchKernelGGL((k_gemm_nt<32, 32, 2, 2, NN>), dim3(cdiv(M, 32), cdiv(N, 32)), dim3(256), 0, st, M, N, K,
-                           A, lda, B, ldb, C, ldc, S, zg, gpart);
+        launch_timed(k_gemm_nt<32, 32, 2, 2, NN>, dim3(cdiv(M, 32), cdiv(N, 32)), dim3(256), st, e0, e1, M, N, K,
+                     A, lda, B, ldb, C, ldc, S, zg, gpart);
     return cdiv(N, 32);
 }
 
 void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
-                    const DevScalars* S, hipStream_t st)
+                    const DevScalars* S, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
-    launch_gemm<false>(M, N, K, A, lda, B, ldb, C, ldc, S, st, nullptr, nullptr, false);
+    launch_gemm<false>(M, N, K, A, lda, B, ldb, C, ldc, S, st, nullptr, nullptr, false, e0, e1);
 }
 
 void launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
-                    const DevScalars* S, hipStream_t st, const double* zg, double* gpart)
+                    const DevScalars* S, hipStream_t st, const double* zg, double* gpart, hipEvent_t e0,
+                    hipEvent_t e1)
 {
-    launch_gemm<true>(M, N, K, A, lda, B, ldb, C, ldc, S, st, zg, gpart, false);
+    launch_gemm<true>(M, N, K, A, lda, B, ldb, C, ldc, S, st, zg, gpart, false, e0, e1);
 }
 
 // Tile columns of the C = A * B launch these arguments get (the number of

@@ -202,11 +202,13 @@ void launch_mls_force(const FluidDev& F, hipStream_t st);
 void launch_jsolve_helm(const FluidDev& F, hipStream_t st);
 void launch_jsolve_poisson(const FluidDev& F, hipStream_t st);
 void launch_ksolve(const FluidDev& F, hipStream_t st);
+// e0/e1 (optional): events the dispatch records at its start and end
 void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
-                    double* C, int ldc, const DevScalars* S, hipStream_t st);
+                    double* C, int ldc, const DevScalars* S, hipStream_t st, hipEvent_t e0 = nullptr,
+                    hipEvent_t e1 = nullptr);
 void launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc, const DevScalars* S, hipStream_t st, const double* zg = nullptr,
-                    double* gpart = nullptr);
+                    double* gpart = nullptr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 int gemm_nn_tiles_n(int M, int N, int K, const double* A, int lda, const double* B, int ldb);
 void launch_ysolve(const FluidDev& F, hipStream_t st);
 void launch_loads(const FluidDev& F, double hpr, hipStream_t st);
