@@ -126,8 +126,9 @@ def main():
     axis = g.transform
     gemm_flops = 2.0 * n3m * n2m * n2m if axis == "y" else 2.0 * n2m * n3m * n3m
     achieved = gemm_flops / gemm_avg_s / 1e12
-    # the transforms of this workload take the direct-to-LDS kernel (96x128
-    # tiles, >= 192 workgroups, operands with padded leading dimensions)
+    # the transforms of this workload take the direct-to-LDS kernel (at least
+    # 192 tiles of 96x128 in the library's size test, operands with padded
+    # leading dimensions; the kernel itself runs 96x64 tiles)
     gemm_kernel = "k_gemm_dma" if -(-n2m // 96) * -(-n3m // 128) >= 192 else "k_gemm_nt"
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_MFMA_TFLOPS, 4), "traffic": pmc_traffic(gemm_kernel),

@@ -29,13 +29,14 @@
 // into LDS
 // (global_load_lds_dwordx4, 16 bytes per lane, no register staging) through
 // three stages, so two K-slices of loads are in flight while one is
-// multiplied.  Each stage is its own LDS array and the slice loop is unrolled
-// by three, which lets the compiler see that the fragment reads of one stage
-// do not alias the loads still landing in the others (it would otherwise
-// wait for every outstanding load before each read).  Every wave issues the
-// same number of loads per slice (addresses clamped into the matrix), so
-// "slice s has landed" is vmcnt(loads per slice): slice s+1 may stay in
+// multiplied.  Each stage is its own LDS array, the slice loop is unrolled
+// by three, and the loads are issued as inline assembly: the compiler then
+// neither tracks them nor waits for every outstanding load before a
+// fragment read.  Every wave issues the same number of loads per slice
+// (addresses clamped into the matrix), so "slice s has landed" is
+// vmcnt(loads per slice), waited for explicitly: slice s+1 may stay in
 // flight; the workgroup barrier is a plain s_barrier for the same reason.
+// The loads of a slice are issued at raised wave priority.
 // A direct LDS write lands lane-contiguous, so the LDS layout follows the
 // global one: A and the NT-form B as k-rows (pitch 112 / 144 doubles: 48 / 64
 // used 16-byte slots of 56 / 72), the NN-form B as n-rows of BK k-values with
