@@ -767,34 +767,42 @@ static void mls_sd_ind(orc_state *s, int idir, const int indse[2], double posk[2
         }
 }
 
+/* One factor of weight_function (MLSint.F90:431-479) at the scaled distance
+ * rcv = |dif| / (radiusSD * SDd): cubic spline (1), exponential with
+ * wideness 0.3 (2), quartic spline (3), Roma's 3-point function (4). */
+static double weight1d(double rcv, int wfun)
+{
+    const double wcon = 0.3;
+    double rc2 = rcv * rcv, rc3 = rcv * rcv * rcv, rc4 = rcv * rcv * rcv * rcv;
+    if (wfun == 1) {
+        if (rcv > 1.0) return 0.0;
+        if (rcv <= 0.5) return (2.0 / 3.0) - 4.0 * rc2 + 4.0 * rc3;
+        return (4.0 / 3.0) - 4.0 * rcv + 4.0 * rc2 - (4.0 / 3.0) * rc3;
+    } else if (wfun == 2) {
+        double t = rcv / wcon;
+        return (rcv <= 1.0) ? exp(-(t * t)) : 0.0;
+    } else if (wfun == 3) {
+        return (rcv <= 1.0) ? 1.0 - 6.0 * rc2 + 8.0 * rc3 - 3.0 * rc4 : 0.0;
+    } else if (wfun == 4) {
+        if (rcv > 1.0) return 0.0;
+        if (rcv < 0.5) return 1.0 / 3.0 * (1.0 + sqrt(-3.0 * rc2 + 1.0));
+        double om = 1.0 - rcv;
+        return 1.0 / 6.0 * (5.0 - 3.0 * rcv - sqrt(-3.0 * (om * om) + 1.0));
+    }
+    return 0.0;
+}
+
+double orc_test_weight1d(double rcv, int wfun) { return weight1d(rcv, wfun); }
+
 /* MLSint.F90:406-486 (weight_function) */
 static void weight_function(const double dif[2][DIMSD], const double sdd[2][DIMSD], double *w,
                             int wfun, double radiusSD)
 {
-    const double wcon = 0.3;
     for (int i = 0; i < DIMSD; ++i) {
-        double wc[2] = {0.0, 0.0};
+        double wc[2];
         for (int k = 0; k < 2; ++k) {
             double sdds = radiusSD * sdd[k][i];
-            double rcv = fabs(dif[k][i]) / sdds;
-            double rc2 = rcv * rcv, rc3 = rcv * rcv * rcv, rc4 = rcv * rcv * rcv * rcv;
-            if (wfun == 1) {
-                if (rcv > 1.0) wc[k] = 0.0;
-                else if (rcv <= 0.5) wc[k] = (2.0 / 3.0) - 4.0 * rc2 + 4.0 * rc3;
-                else wc[k] = (4.0 / 3.0) - 4.0 * rcv + 4.0 * rc2 - (4.0 / 3.0) * rc3;
-            } else if (wfun == 2) {
-                double t = rcv / wcon;
-                wc[k] = (rcv <= 1.0) ? exp(-(t * t)) : 0.0;
-            } else if (wfun == 3) {
-                wc[k] = (rcv <= 1.0) ? 1.0 - 6.0 * rc2 + 8.0 * rc3 - 3.0 * rc4 : 0.0;
-            } else if (wfun == 4) {
-                if (rcv > 1.0) wc[k] = 0.0;
-                else if (rcv < 0.5) wc[k] = 1.0 / 3.0 * (1.0 + sqrt(-3.0 * rc2 + 1.0));
-                else {
-                    double om = 1.0 - rcv;
-                    wc[k] = 1.0 / 6.0 * (5.0 - 3.0 * rcv - sqrt(-3.0 * (om * om) + 1.0));
-                }
-            }
+            wc[k] = weight1d(fabs(dif[k][i]) / sdds, wfun);
         }
         w[i] = wc[0] * wc[1];
     }

@@ -93,6 +93,7 @@ int orc_diagnostics(orc_state *s, double *out);
 int orc_test_dgtsv(int n, const double *dl, const double *d, const double *du, double *b);
 int orc_test_gesolve(const double *a_colmajor, const double *b, double *x);
 int orc_test_phcalc(orc_state *s, double *field);
+double orc_test_weight1d(double rcv, int wfun);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,8 @@ def _lib():
         L.orc_test_dgtsv.argtypes = [C.c_int, dp, dp, dp, dp]
         L.orc_test_gesolve.argtypes = [dp, dp, dp]
         L.orc_test_phcalc.argtypes = [vp, dp]
+        L.orc_test_weight1d.argtypes = [C.c_double, C.c_int]
+        L.orc_test_weight1d.restype = C.c_double
         L.orc_loads_transfer.argtypes = [vp, C.c_double, dp, dp, dp, dp]
         L.orc_checkbcs.argtypes = [vp, dp, dp, dp, dp]
         L.orc_diagnostics.argtypes = [vp, dp]
@@ -219,3 +221,8 @@ def phcalc(orc, rhs):
     f = np.asfortranarray(rhs, dtype=np.float64).copy(order="F")
     _lib().orc_test_phcalc(orc.h, _dp(f))
     return f
+
+
+def weight1d(rc, wfun):
+    """One factor of the MLS weight function at scaled distance rc."""
+    return _lib().orc_test_weight1d(float(rc), int(wfun))

@@ -6,6 +6,8 @@ brute force, closed forms, invariants, and the reference's own input files.
 """
 import os
 
+import math
+
 import numpy as np
 import pytest
 import scipy.linalg as sla
@@ -147,6 +149,61 @@ def test_mls_weights_reproduce_linear_fields():
             A = P.T @ (w[:, None] * P)
             gam = np.linalg.solve(A, np.array([1.0, lmcc[n, 0], lmcc[n, 1]]))
             np.testing.assert_allclose(ph, (P @ gam) * w, rtol=1e-9, atol=1e-12)
+
+
+def _integral01(f, n=2000):
+    """Composite Simpson's rule on [0, 1]."""
+    x = np.linspace(0.0, 1.0, n + 1)
+    y = np.array([f(v) for v in x])
+    return (1.0 / n) / 3.0 * (y[0] + y[-1] + 4 * y[1:-1:2].sum() + 2 * y[2:-1:2].sum())
+
+
+@pytest.mark.parametrize("wfun", [1, 2, 3, 4])
+def test_weight_functions_closed_forms(wfun):
+    """The four factors of weight_function (MLSint.F90:431-479) against the
+    properties that fix their coefficients.  Cubic spline: W(0) = 2/3,
+    W = W' = W'' = 0 at r = 1, value, slope and curvature continuous at 1/2,
+    integral 1/4 (right branch (4/3)(1-r)^3).  Exponential: exp(-(r/0.3)^2), i.e. 1/e at 0.3 and e^-4 at
+    0.6.  Quartic spline: W(0) = 1, W'(0) = 0, W = W' = W'' = 0 at 1
+    ((1-r)^3 (1+3r)), integral 2/5.  Roma's 3-point function at a grid point and at a
+    half-point: W(0) + 2 W(1) = 1 and W(0)^2 + 2 W(1)^2 = 1/2,
+    2 W(1/2) = 1 (partition of unity and the square-sum identity of the
+    discrete delta), continuous at 1/2.  Every factor vanishes past r = 1."""
+    from oracle import weight1d
+    W = lambda r: weight1d(r, wfun)
+    assert W(1.0 + 1e-12) == 0.0 and W(1.5) == 0.0
+    h = 1e-6
+    if wfun == 1:
+        assert abs(W(0.0) - 2.0 / 3.0) < 1e-15
+        # triple zero at 1: W(1 - e) = (4/3) e^3
+        e = 1e-2
+        assert abs(W(1.0 - e) / e**3 - 4.0 / 3.0) < 1e-6
+        # value, slope -1 and curvature 4 from both sides of r = 1/2
+        assert abs(W(0.5) - W(0.5 + 1e-13)) < 1e-12
+        assert abs((W(0.5) - W(0.5 - h)) / h + 1.0) < 1e-5
+        assert abs((W(0.5 + h) - W(0.5)) / h + 1.0) < 1e-5
+        k = 1e-4
+        assert abs((W(0.5) - 2 * W(0.5 - k) + W(0.5 - 2 * k)) / k**2 - 4.0) < 1e-2
+        assert abs((W(0.5) - 2 * W(0.5 + k) + W(0.5 + 2 * k)) / k**2 - 4.0) < 1e-2
+        assert abs(_integral01(W) - 0.25) < 1e-9
+    elif wfun == 2:
+        assert W(0.0) == 1.0
+        assert abs(W(0.3) - math.exp(-1.0)) < 1e-15
+        assert abs(W(0.6) - math.exp(-4.0)) < 1e-16
+        assert abs(W(1.0) - math.exp(-1.0 / 0.09)) < 1e-18
+    elif wfun == 3:
+        assert W(0.0) == 1.0
+        assert abs((W(h) - W(0.0)) / h) < 1e-5           # no linear term
+        # triple zero at 1: W(1 - e) = e^3 (4 - 3 e)
+        e = 1e-2
+        assert abs(W(1.0 - e) / e**3 - (4.0 - 3.0 * e)) < 1e-6
+        assert abs(_integral01(W) - 0.4) < 1e-9
+    else:
+        w0, w1, wh = W(0.0), W(1.0), W(0.5)
+        assert abs(w0 + 2 * w1 - 1.0) < 1e-15
+        assert abs(w0 * w0 + 2 * w1 * w1 - 0.5) < 1e-15
+        assert abs(2 * wh - 1.0) < 1e-15
+        assert abs(W(0.5 - 1e-12) - wh) < 1e-6
 
 
 def test_lgrdistribution_is_fortran_minloc():
@@ -312,3 +369,43 @@ def test_viscous_and_convective_terms_closed_form():
     k = np.arange(2, n3m)        # interior k (0-based column k-1)
     expect = -2.0 * ga * q3z[k - 1] * b * dt
     np.testing.assert_allclose(rhs3[2:n2m - 2, k - 1], np.broadcast_to(expect, (n2m - 4, k.size)), rtol=1e-9)
+
+
+def test_q2_viscous_and_convective_terms_closed_form():
+    """invtr2/hdnl2 on a uniform grid, q3 = 0 and pr = 0: q2 = g(z)
+    quadratic gives rhs2 = dt*al/Re*g'' (the y terms and the convective
+    terms vanish); q2 = a + b y gives rhs2 = -2 ga q2 b dt (the discrete
+    d(q2^2)/dy of a linear field is exact, the viscous terms vanish).
+    Interior rows j = 3..n2m-1 and planes k = 2..n3m-1."""
+    c = case("tiny")
+    p = _uniform_params(c, dtmax=1.0)
+    n2m, n3m = p.n2 - 1, p.n3 - 1
+    zm = 0.5 * (c.zz[:-1] + c.zz[1:])
+    # quadratic in z
+    st = cases.initial_state(p, None)
+    st["q3"][:] = 0.0
+    g = 0.1 * zm * (p.alx3 - zm)
+    st["q2"][:, 3:3 + n3m] = g[None, :]          # planes k = 1..n3m hold q2(zm(k))
+    o = oracle_for(c, params=p, state=st)
+    o.cfl_dt(0, 1e-3)
+    assert o.substep(1, 0) == 0
+    sc = o.get("scalars")
+    dt, al, ga = sc[0], sc[2], sc[3]
+    rhs2 = o.get("rhs2")
+    np.testing.assert_allclose(rhs2[2:n2m - 1, 1:n3m - 1], dt * al * p.visc * (-0.2), rtol=1e-9)
+    # linear in y
+    st = cases.initial_state(p, None)
+    st["q3"][:] = 0.0
+    a, b = 0.3, 0.05
+    q2y = a + b * c.rc
+    st["q2"][:, :] = q2y[:, None]
+    o = oracle_for(c, params=p, state=st)
+    o.cfl_dt(0, 1e-3)
+    assert o.substep(1, 0) == 0
+    sc = o.get("scalars")
+    dt, ga = sc[0], sc[3]
+    rhs2 = o.get("rhs2")
+    j = np.arange(3, n2m)                         # 1-based rows
+    expect = -2.0 * ga * q2y[j - 1] * b * dt
+    np.testing.assert_allclose(rhs2[j - 1, 1:n3m - 1], np.broadcast_to(expect[:, None], (j.size, n3m - 2)),
+                               rtol=1e-9)
