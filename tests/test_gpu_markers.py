@@ -66,3 +66,51 @@ def test_edge_markers_match_the_oracle(axis):
         e = relmax(sg[f], so[f])
         print(f"edge markers {axis} {f}: {e:.2e}", flush=True)
         assert e <= 1e-10, f
+
+
+def _body_case(verts, center, motion):
+    c = cases.make_case("cyl384")
+    return cases.Case(c.name, c.params, c.rc, c.zz, verts, center, motion, c.seed)
+
+
+@pytest.mark.parametrize("which", ["many", "large"])
+def test_contribution_order_fallbacks_match_the_oracle(which):
+    """The two fallbacks of the (cell, marker) ordering of the forcing
+    contributions: more than 16384 keys (2400 markers, nine supports each:
+    LDS bitonic blocks and global merge passes) and a body whose support box
+    exceeds 16384 cells (a 7 x 2.5 rectangle, about 101 x 281 cells: LDS
+    radix sort instead of the cell counting sort).  cyl384 grid, three steps.
+    A lost or repeated contribution would show as an O(1) error in the
+    forcing.  Bars: indices bit-exact, weights 1e-14, forcing of the last
+    substep 1e-11, fields 1e-10."""
+    if which == "many":
+        c = _body_case(cases.cylinder_vertices(15.0, 10.0, 0.5, 2400), (15.0, 10.0),
+                       cases.make_case("cyl384").motion)
+    else:
+        c = _body_case(cases.rectangle_vertices(15.0, 12.0, 7.0, 2.5, 600), (15.0, 12.0), None)
+    dec = decomposition(c)
+    o = oracle_for(c, dec)
+    g = gpu_for(c, dec)
+    t, dt = 0.0, c.params.dt0
+    for ns in range(3):
+        lm = c.markers(t)
+        o.set_markers(*lm)
+        g.set_markers(*lm)
+        rc, dto, _ = o.fluid_tsch(ns, dt, dt)
+        assert rc == 0
+        dtg, _ = g.fluid_tsch(ns, dt, dt)
+        assert abs(dtg - dto) <= 1e-12 * dto
+        if ns == 0:
+            np.testing.assert_array_equal(g.get("lmind"), o.get("lmind"))
+            np.testing.assert_array_equal(g.get("eulins"), o.get("eulins"))
+            assert relmax(g.get("phi"), o.get("phi")) <= 1e-14
+        # the forcing of the last substep: its inputs carry the solvers' rounding
+        assert relmax(g.get("fey"), o.get("fey")) <= 1e-11
+        assert relmax(g.get("fez"), o.get("fez")) <= 1e-11
+        t += dto
+        dt = dto
+    so, sg = {k: o.get(k) for k in ("q2", "q3", "pr")}, g.get_state()
+    for f in ("q2", "q3", "pr"):
+        e = relmax(sg[f], so[f])
+        print(f"{which} {f}: {e:.2e}", flush=True)
+        assert e <= 1e-10, f
