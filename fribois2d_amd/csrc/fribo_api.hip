@@ -758,11 +758,16 @@ static void enqueue_poisson(fribo_ctx* c)
     FluidDev& F = c->F;
     hipStream_t st = c->st;
     if (F.taxis == 1) {
+        // the chunk pass of the per-mode z-lines runs in the forward
+        // transform's epilogue when the line partition has equal chunks
+        YFuse yf{F.wy, F.amphk, F.acphk, F.apphk, F.ysum, F.n2m, 0};
+        if (F.n3m % F.ks[0].nch == 0) yf.csz = F.n3m / F.ks[0].nch;
+        bool fused = false;
         run_ev(c, PC_GEMM, [&](hipEvent_t a, hipEvent_t b) {
-            launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ymmi, F.ldy, F.dph + fidx(F, 1, 1), F.P, F.hmn, F.P, F.S, st,
-                           nullptr, nullptr, a, b);
+            fused = launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ymmi, F.ldy, F.dph + fidx(F, 1, 1), F.P, F.hmn, F.P, F.S,
+                                   st, nullptr, nullptr, a, b, &yf);
         });
-        run(c, PC_JP, [&] { launch_ysolve(F, st); });
+        run(c, PC_JP, [&] { launch_ysolve(F, st, fused); });
         run_ev(c, PC_GEMM, [&](hipEvent_t a, hipEvent_t b) {
             launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ymm, F.ldy, F.hmn, F.P, F.dph + fidx(F, 1, 1), F.P, F.S, st, F.zg,
                            F.gpart, a, b);

@@ -49,7 +49,7 @@
 // dispatched to the 8 XCDs round-robin, so when the grid is a multiple of 256
 // tiles in 4 x 8 patches the tile indices are permuted to give each XCD
 // compact patches (its L2 then serves 4 A- and 8 B-tiles to 32 workgroups).
-#include "fribo_internal.h"
+#include "fribo_ychunk.h"
 #include <hip/hip_ext.h>
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -239,7 +239,7 @@ template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A,
                                                            int lda, const double* __restrict__ B, int ldb,
                                                            double* __restrict__ C, int ldc, const DevScalars* S,
-                                                           const double* __restrict__ zg, double* gpart)
+                                                           const double* __restrict__ zg, double* gpart, YFuse yf)
 {
     constexpr int BM = 96, NW = WM * WN;
     constexpr int TM = BM / WM, TN = BN / WN, RM = TM / 16, RN = TN / 16;
@@ -254,7 +254,11 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, 
     static_assert(NI < 16, "load count per wave");
     static_assert(TM % 16 == 0 && TN % 16 == 0 && BM % WM == 0 && BN % WN == 0, "wave tile");
     if (S && S->err) return;
-    __shared__ __attribute__((aligned(1024))) double S0[SA + SBP], S1[SA + SBP], S2[SA + SBP];
+    // the three stages in one block, which the fused epilogue reuses for the tile
+    __shared__ __attribute__((aligned(1024))) double SM[3 * (SA + SBP)];
+    double* const S0 = SM;
+    double* const S1 = SM + (SA + SBP);
+    double* const S2 = SM + 2 * (SA + SBP);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave % WM, wn = wave / WM;
     int bx = blockIdx.x, by = blockIdx.y;
@@ -365,6 +369,39 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, 
         __shared__ double gsh[NW];
         gauge_partial<RM, RN, TM, TN, NW>(acc, M, N, i0, n0, wm, wn, lane, wave, by, zg, gpart, gsh);
     }
+    if (NN && yf.csz > 0) {
+        // chunk elimination of the y-mode z-lines on this tile: rows i are
+        // y-modes, columns n the z rows; the tile holds BN / csz whole chunks
+        constexpr int LT = BM + 1;   // LDS row pitch of the staged tile (one z row per LDS row)
+        static_assert(BN * LT <= 3 * (SA + SBP), "tile staging");
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RM; ++r)
+#pragma unroll
+            for (int c = 0; c < RN; ++c)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    SM[(wn * TN + c * 16 + (lane & 15)) * LT + wm * TM + r * 16 + (lane >> 4) + 4 * v] = acc[r][c][v];
+        __syncthreads();
+        constexpr int WPC = (BM + 63) / 64;   // waves per chunk
+        const int csz = yf.csz, cc = wave / WPC, iw = (wave % WPC) * 64 + lane;
+        if (cc < BN / csz) {                  // wave-uniform
+            const int il = iw < BM ? iw : BM - 1;
+            const int ic = min(i0 + il, M - 1);
+            const int p = (n0 + cc * csz) / csz, s = p * csz, m = csz - 1;
+            const double* col = SM + (size_t)(cc * csz) * LT + il;
+            double dv[32];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) dv[q] = col[(q < m ? q : 0) * LT];
+            const double w = yf.wy[ic];
+            double yl, vl, ys, vs, ws, wl;
+            ychunk_rows<32, false>(yf.amphk, yf.acphk, yf.apphk, s, m, w, dv, nullptr, nullptr, nullptr, yl, vl,
+                                   ys, vs, ws, wl);
+            if (iw < BM && i0 + iw < M)
+                ysum_store(yf.ysum, yf.n2m, i0 + iw + 1, p, yf.amphk, yf.acphk, s + m, w, col[m * LT], yl, vl, wl,
+                           ys, vs, ws);
+        }
+    }
 }
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -392,13 +429,21 @@ static void launch_timed(Kern kern, dim3 g, dim3 b, hipStream_t st, hipEvent_t e
 template <bool NN>
 static int launch_gemm(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                        const DevScalars* S, hipStream_t st, const double* zg, double* gpart, bool plan,
-                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const YFuse* yfp = nullptr,
+                       bool* fused = nullptr)
 {
+    YFuse yf{};
+    // the fused chunk pass needs whole chunks of at most 33 rows in every
+    // output tile of the direct-to-LDS kernel
+    if (NN && yfp && yfp->csz > 1 && yfp->csz <= 33 && GEMM_DMA_BN % yfp->csz == 0 && N % GEMM_DMA_BN == 0)
+        yf = *yfp;
+    if (fused) *fused = false;
     if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192 && dma_ok<NN>(M, N, K, A, lda, B, ldb)) {
         if (!plan)
             launch_timed(k_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN, GEMM_DMA_BN>,
                          dim3(cdiv(M, 96), cdiv(N, GEMM_DMA_BN)), dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), st, e0, e1, M,
-                         N, K, A, lda, B, ldb, C, ldc, S, zg, gpart);
+                         N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, yf);
+        if (fused) *fused = yf.csz > 0;
         return cdiv(N, GEMM_DMA_BN);
     } else if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
         if (!plan)
@@ -423,11 +468,13 @@ void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double*
     launch_gemm<false>(M, N, K, A, lda, B, ldb, C, ldc, S, st, nullptr, nullptr, false, e0, e1);
 }
 
-void launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+bool launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                     const DevScalars* S, hipStream_t st, const double* zg, double* gpart, hipEvent_t e0,
-                    hipEvent_t e1)
+                    hipEvent_t e1, const YFuse* yf)
 {
-    launch_gemm<true>(M, N, K, A, lda, B, ldb, C, ldc, S, st, zg, gpart, false, e0, e1);
+    bool fused = false;
+    launch_gemm<true>(M, N, K, A, lda, B, ldb, C, ldc, S, st, zg, gpart, false, e0, e1, yf, &fused);
+    return fused;
 }
 
 // Tile columns of the C = A * B launch these arguments get (the number of

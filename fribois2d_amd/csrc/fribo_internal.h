@@ -202,15 +202,29 @@ void launch_mls_force(const FluidDev& F, hipStream_t st);
 void launch_jsolve_helm(const FluidDev& F, hipStream_t st);
 void launch_jsolve_poisson(const FluidDev& F, hipStream_t st);
 void launch_ksolve(const FluidDev& F, hipStream_t st);
+// First pass of the y-order z-lines (chunk elimination, fribo_ychunk.h) run
+// by the epilogue of the forward transform on its own output tile, when
+// every 64-column tile holds whole chunks of the line partition (csz rows
+// each, csz - 1 <= 32).  csz = 0: no fused pass.
+struct YFuse {
+    const double *wy, *amphk, *acphk, *apphk;
+    double* ysum;
+    int n2m, csz;
+};
+
 // e0/e1 (optional): events the dispatch records at its start and end
 void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc, const DevScalars* S, hipStream_t st, hipEvent_t e0 = nullptr,
                     hipEvent_t e1 = nullptr);
-void launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
+// Returns true when the fused chunk pass of yf ran (yf->csz > 0 and the
+// direct-to-LDS kernel with 64-column tiles took the launch).
+bool launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc, const DevScalars* S, hipStream_t st, const double* zg = nullptr,
-                    double* gpart = nullptr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                    double* gpart = nullptr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
+                    const YFuse* yf = nullptr);
 int gemm_nn_tiles_n(int M, int N, int K, const double* A, int lda, const double* B, int ldb);
-void launch_ysolve(const FluidDev& F, hipStream_t st);
+// first_done: the chunk pass already ran in the forward transform's epilogue
+void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done = false);
 void launch_loads(const FluidDev& F, double hpr, hipStream_t st);
 void launch_checkbcs(const FluidDev& F, double* nb, hipStream_t st);
 void launch_diag(const FluidDev& F, double* part, int kv, double* out, hipStream_t st);

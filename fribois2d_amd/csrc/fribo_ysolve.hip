@@ -25,7 +25,7 @@
 // apphk are read as wave-uniform scalars, the shift per lane.  Three passes
 // as in the Helmholtz k-solves: chunk elimination -> separator system (one
 // wave per line, PCR) -> chunk back-substitution.
-#include "fribo_internal.h"
+#include "fribo_ychunk.h"
 
 // Waves per workgroup (each wave its own 64 y-modes, or its own mode in the
 // separator pass).
@@ -34,79 +34,21 @@
 #endif
 #define YW FRIBO_YW
 
-__device__ inline size_t yidx(const FluidDev& F, int q, int j, int p)
-{
-    // summary q of chunk p of y-mode j: chunk fastest
-    return ((size_t)q * F.n2m + (size_t)(j - 1)) * FRIBO_KMAXCH + p;
-}
+__device__ inline size_t yidx(const FluidDev& F, int q, int j, int p) { return ysum_idx(F.n2m, q, j, p); }
 
-// Rows k = s+1 .. s+m of the z operator for the interior rows of chunk p,
-// one row per lane (every lane of a workgroup works on the same chunk), taken
-// by each step from its lane.
-template <int NG>
-__device__ inline void yrows(const double* __restrict__ a, int s, int m, double (&v)[NG])
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int g = 0; g < NG; ++g) {
-        const int r = lane + 64 * g;
-        v[g] = a[s + 1 + (r < m ? r : (m > 0 ? m - 1 : 0))];
-    }
-}
-
-template <int NG>
-__device__ inline double yrow(const double (&v)[NG], int t)
-{
-    return lane_bcast(NG == 1 || t < 64 ? v[0] : v[NG - 1], t & 63);
-}
-
-// Chunk elimination of line j, chunk p (rows s..e-1 interior, e separator).
-// Returns the recurrence state; cp/vp/y are kept when STORE.
+// Chunk elimination of line j, chunk p (rows s..e-1 interior, e separator),
+// the rows read from column col of hmn.
 template <int MAXM, bool STORE>
 __device__ inline void ychunk(const FluidDev& F, const KSys& K, int p, double w, const double* col, double* cp,
                               double* vp, double* y, double& yl, double& vl, double& ys, double& vs, double& ws,
                               double& wl)
 {
-    constexpr int NG = (MAXM + 63) / 64;
-    static_assert(NG <= 2, "chunk rows");
     const size_t P = F.P;
     const int s = K.cs[p], e = K.cs[p + 1] - 1, m = e - s;
-    double ra[NG], rb[NG], rc[NG];
-    yrows<NG>(F.amphk, s, m, ra);
-    yrows<NG>(F.acphk, s, m, rb);
-    yrows<NG>(F.apphk, s, m, rc);
     double dv[MAXM];
 #pragma unroll
     for (int t = 0; t < MAXM; ++t) dv[t] = col[(size_t)(s + (t < m ? t : 0)) * P];
-    double cpp = 0.0, gg = 1.0, glast = 1.0;
-    yl = 0.0;
-    vl = 1.0;
-    ys = 0.0;
-    vs = 0.0;
-    wl = 0.0;
-#pragma unroll
-    for (int t = 0; t < MAXM; ++t) {
-        if (t < m) {
-            const double a = yrow<NG>(ra, t), b = yrow<NG>(rb, t) + w, c = yrow<NG>(rc, t);
-            const double iv = recip(b - a * cpp);
-            cpp = c * iv;
-            yl = (dv[t] - a * yl) * iv;
-            vl = (-a * vl) * iv;
-            if (STORE) {
-                cp[t] = cpp;
-                vp[t] = vl;
-                y[t] = yl;
-            }
-            ys = ys + gg * yl;
-            vs = vs + gg * vl;
-            if (t == m - 1) {
-                wl = (-c) * iv;
-                glast = gg;
-            }
-            gg = -cpp * gg;
-        }
-    }
-    ws = glast * wl;
+    ychunk_rows<MAXM, STORE>(F.amphk, F.acphk, F.apphk, s, m, w, dv, cp, vp, y, yl, vl, ys, vs, ws, wl);
 }
 
 // Lanes past the last y-mode work on mode n2m and store nothing (every lane
@@ -127,16 +69,8 @@ __global__ __launch_bounds__(64 * YW) void k_ysolve1(const FluidDev F)
     double yl, vl, ys, vs, ws, wl;
     ychunk<MAXM, false>(F, K, p, w, col, nullptr, nullptr, nullptr, yl, vl, ys, vs, ws, wl);
     if (!act) return;
-    const int e = K.cs[p + 1] - 1, k = e + 1;
-    const double ae = F.amphk[k];
-    const double de = col[(size_t)e * F.P];
-    double* Y = F.ysum;
-    Y[yidx(F, 0, j, p)] = ae * vl;                        // alpha
-    Y[yidx(F, 1, j, p)] = (F.acphk[k] + w) + ae * wl;     // beta before the coupling to chunk p+1
-    Y[yidx(F, 2, j, p)] = de - ae * yl;                   // delta before the coupling
-    Y[yidx(F, 3, j, p)] = ys;
-    Y[yidx(F, 4, j, p)] = vs;
-    Y[yidx(F, 5, j, p)] = ws;
+    const int e = K.cs[p + 1] - 1;
+    ysum_store(F.ysum, F.n2m, j, p, F.amphk, F.acphk, e, w, col[(size_t)e * F.P], yl, vl, wl, ys, vs, ws);
 }
 
 __global__ __launch_bounds__(64 * YW) void k_ysolve2(const FluidDev F)
@@ -218,14 +152,16 @@ __global__ __launch_bounds__(64 * YW) void k_ysolve3(const FluidDev F)
     col[(size_t)e * P] = X;
 }
 
-void launch_ysolve(const FluidDev& F, hipStream_t st)
+void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done)
 {
     const int mm = F.ks[0].maxm - 1;   // interior rows per chunk
     dim3 g((F.n2m + 64 * YW - 1) / (64 * YW), F.ks[0].nch), b(64 * YW);
-    if (mm <= 16) hipLaunchKernelGGL(k_ysolve1<16>, g, b, 0, st, F);
-    else if (mm <= 32) hipLaunchKernelGGL(k_ysolve1<32>, g, b, 0, st, F);
-    else if (mm <= 64) hipLaunchKernelGGL(k_ysolve1<64>, g, b, 0, st, F);
-    else hipLaunchKernelGGL(k_ysolve1<128>, g, b, 0, st, F);
+    if (!first_done) {
+        if (mm <= 16) hipLaunchKernelGGL(k_ysolve1<16>, g, b, 0, st, F);
+        else if (mm <= 32) hipLaunchKernelGGL(k_ysolve1<32>, g, b, 0, st, F);
+        else if (mm <= 64) hipLaunchKernelGGL(k_ysolve1<64>, g, b, 0, st, F);
+        else hipLaunchKernelGGL(k_ysolve1<128>, g, b, 0, st, F);
+    }
     hipLaunchKernelGGL(k_ysolve2, dim3((F.n2m + YW - 1) / YW), b, 0, st, F);
     if (mm <= 16) hipLaunchKernelGGL(k_ysolve3<16>, g, b, 0, st, F);
     else if (mm <= 32) hipLaunchKernelGGL(k_ysolve3<32>, g, b, 0, st, F);
