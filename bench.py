@@ -139,8 +139,16 @@ def main():
     # HBM fraction of the memory-bound classes: compulsory FP64 field values
     # per cell per launch (reads + writes), at the 8 TB/s MI355X peak
     # (the per-mode Poisson lines: along z one read and one write of hmn; along
-    # y the two chunk passes read it twice and write it once)
-    values = {"rhs": 9, "jsolve_helm": 4, "ksolve": 6, "div": 3, "jsolve_poisson": 3 if axis == "y" else 2,
+    # y the two chunk passes read it twice and write it once, unless the first
+    # pass runs in the forward transform's epilogue: equal chunks of at most
+    # 33 rows dividing its 64-column tiles, which the direct-to-LDS kernel
+    # takes at >= 192 tiles of 96x128)
+    nch = max(1, min(n3m // 2, 64))
+    csz = n3m // nch
+    fused = (axis == "y" and n3m % nch == 0 and 1 < csz <= 33 and 64 % csz == 0 and n3m % 64 == 0
+             and -(-n2m // 96) * -(-n3m // 128) >= 192)
+    values = {"rhs": 9, "jsolve_helm": 4, "ksolve": 6, "div": 3,
+              "jsolve_poisson": (2 if fused else 3) if axis == "y" else 2,
               "upd": 7, "cfl": 2}
     for k, nv in values.items():
         if k in kernels and kernels[k]["ms_per_step"] > 0:

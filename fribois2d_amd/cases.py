@@ -278,19 +278,25 @@ class Case:
         return 0 if self.verts is None else self.verts.shape[0] - 1
 
     def markers(self, t=0.0):
-        """lmcc (nlm,2), lmvel (nlm,2), lmA (nlm,) for the body at time t."""
+        """lmcc (nlm,2), lmvel (nlm,2), lmA (nlm,) for the body at time t
+        (the 2-column arrays in Fortran order, as the library takes them)."""
         if self._rest is None:
-            self._rest = markers_from_vertices(self.verts)
-        lmcc0, lmA = self._rest
+            lmcc0, lmA = markers_from_vertices(self.verts)
+            yc, zc = self.center
+            self._rest = (lmcc0, lmA, lmcc0[:, 0] - yc, lmcc0[:, 1] - zc)
+        lmcc0, lmA, ry, rz = self._rest
         if self.motion is None:
-            return lmcc0.copy(), np.zeros_like(lmcc0), lmA.copy()
+            return np.asfortranarray(lmcc0), np.zeros((lmA.size, 2), order="F"), lmA.copy()
         dy, dz, th, vy, vz, om = self.motion(t)
         yc, zc = self.center
-        ry, rz = lmcc0[:, 0] - yc, lmcc0[:, 1] - zc
         c, s = math.cos(th), math.sin(th)
         ryr, rzr = c * ry - s * rz, s * ry + c * rz
-        lmcc = np.stack([yc + dy + ryr, zc + dz + rzr], axis=1)
-        lmvel = np.stack([vy - om * rzr, vz + om * ryr], axis=1)
+        lmcc = np.empty((lmA.size, 2), order="F")
+        lmvel = np.empty((lmA.size, 2), order="F")
+        np.add(yc + dy, ryr, out=lmcc[:, 0])
+        np.add(zc + dz, rzr, out=lmcc[:, 1])
+        np.subtract(vy, om * rzr, out=lmvel[:, 0])
+        np.add(vz, om * ryr, out=lmvel[:, 1])
         return lmcc, lmvel, lmA.copy()
 
     def surface(self, t=0.0):

@@ -63,6 +63,7 @@ struct fribo_ctx {
     // step inputs staged in pinned memory: {dt, dt_o, ns, err=0}
     struct StepIn { double dt, dto; int ns, err; };
     StepIn* in = nullptr;
+    DevScalars* hsp = nullptr;   // pinned landing buffer of the scalars read back after a step
     // captured step (hipGraph): rebuilt when the marker buffers change
     bool use_graph = false;
     bool yavail = false;     // transform along y set up
@@ -489,8 +490,10 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     memset(&c->hs, 0, sizeof(c->hs));
     {
         hipError_t he = hipHostMalloc((void**)&c->in, sizeof(fribo_ctx::StepIn), hipHostMallocDefault);
+        if (he == hipSuccess) he = hipHostMalloc((void**)&c->hsp, sizeof(DevScalars), hipHostMallocDefault);
         if (he != hipSuccess) {
             for (void* q : c->allocs) hipFree(q);
+            if (c->in) hipHostFree(c->in);
             hipStreamDestroy(c->st);
             delete c;
             return set_err(FRIBO_ERR_HIP, hipGetErrorString(he));
@@ -506,6 +509,7 @@ extern "C" int fribo_finalize(fribo_handle h)
     hipStreamSynchronize(h->st);
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->in) hipHostFree(h->in);
+    if (h->hsp) hipHostFree(h->hsp);
     for (auto ev : h->evpool) hipEventDestroy(ev);
     for (void* q : h->mk_allocs) hipFree(q);
     if (h->mk_stage) hipHostFree(h->mk_stage);
@@ -536,8 +540,9 @@ static int get_field(fribo_ctx* c, double* host, const double* dev, int kfirst, 
 
 static int read_scalars(fribo_ctx* c)
 {
-    HIPCHK(hipMemcpyAsync(&c->hs, c->F.S, sizeof(DevScalars), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->hsp, c->F.S, sizeof(DevScalars), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    c->hs = *c->hsp;
     return FRIBO_OK;
 }
 
