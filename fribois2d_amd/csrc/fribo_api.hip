@@ -64,6 +64,11 @@ struct fribo_ctx {
     struct StepIn { double dt, dto; int ns, err; };
     StepIn* in = nullptr;
     DevScalars* hsp = nullptr;   // pinned landing buffer of the scalars read back after a step
+    hipEvent_t ev_rb = nullptr;  // the read-back copy of those scalars
+    // k_cfl of the next step already enqueued behind the read-back of this
+    // one (it reads only q2/q3, so it runs while the host prepares the next
+    // step); cleared by anything that changes the velocities in between
+    bool cfl_ready = false;
     // captured step (hipGraph): rebuilt when the marker buffers change
     bool use_graph = false;
     bool yavail = false;     // transform along y set up
@@ -491,6 +496,7 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     {
         hipError_t he = hipHostMalloc((void**)&c->in, sizeof(fribo_ctx::StepIn), hipHostMallocDefault);
         if (he == hipSuccess) he = hipHostMalloc((void**)&c->hsp, sizeof(DevScalars), hipHostMallocDefault);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_rb, hipEventDisableTiming);
         if (he != hipSuccess) {
             for (void* q : c->allocs) hipFree(q);
             if (c->in) hipHostFree(c->in);
@@ -510,6 +516,7 @@ extern "C" int fribo_finalize(fribo_handle h)
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->in) hipHostFree(h->in);
     if (h->hsp) hipHostFree(h->hsp);
+    if (h->ev_rb) hipEventDestroy(h->ev_rb);
     for (auto ev : h->evpool) hipEventDestroy(ev);
     for (void* q : h->mk_allocs) hipFree(q);
     if (h->mk_stage) hipHostFree(h->mk_stage);
@@ -538,11 +545,22 @@ static int get_field(fribo_ctx* c, double* host, const double* dev, int kfirst, 
     return FRIBO_OK;
 }
 
-static int read_scalars(fribo_ctx* c)
+static void enqueue_cfl(fribo_ctx* c);
+
+// next_cfl: after the copy, enqueue the next step's k_cfl and wait for the
+// copy only.
+static int read_scalars(fribo_ctx* c, bool next_cfl = false)
 {
     HIPCHK(hipMemcpyAsync(c->hsp, c->F.S, sizeof(DevScalars), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    if (next_cfl) {
+        HIPCHK(hipEventRecord(c->ev_rb, c->st));
+        enqueue_cfl(c);
+        HIPCHK(hipEventSynchronize(c->ev_rb));
+    } else {
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
     c->hs = *c->hsp;
+    c->cfl_ready = next_cfl && c->hs.err == 0;
     return FRIBO_OK;
 }
 
@@ -551,6 +569,7 @@ extern "C" int fribo_set_state(fribo_handle h, const double* q2, const double* q
                                const double* qb2n, const double* qb3n, const double* dq2x2o, const double* dq3x2o)
 {
     if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    h->cfl_ready = false;
     FluidDev& F = h->F;
     int e = 0;
     if (q2 && !e) e = put_field(h, F.q2, q2, -2, F.nk);
@@ -819,9 +838,10 @@ static int write_inputs(fribo_ctx* c, int ns, double dt, double dt_o)
     return FRIBO_OK;
 }
 
-static void enqueue_step(fribo_ctx* c)
+// cfl: begin with k_cfl (not already enqueued behind the previous step).
+static void enqueue_step(fribo_ctx* c, bool cfl)
 {
-    enqueue_cfl(c);
+    if (cfl) enqueue_cfl(c);
     for (int n = 1; n <= c->F.nsst; ++n) enqueue_substep(c, n, n > 1, n == c->F.nsst);
 }
 
@@ -838,7 +858,7 @@ static int ensure_graph(fribo_ctx* c)
     const bool prof = c->profiling;
     c->profiling = false;
     HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    enqueue_step(c);
+    enqueue_step(c, false);
     hipError_t e = hipStreamEndCapture(c->st, &g);
     c->profiling = prof;
     if (e != hipSuccess) return set_err(FRIBO_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -849,9 +869,9 @@ static int ensure_graph(fribo_ctx* c)
     return FRIBO_OK;
 }
 
-static int finish(fribo_ctx* c, double* dt, double* cflm)
+static int finish(fribo_ctx* c, double* dt, double* cflm, bool next_cfl = false)
 {
-    int e = read_scalars(c);
+    int e = read_scalars(c, next_cfl);
     if (e) return e;
     HIPCHK(hipGetLastError());
     if (!c->pending.empty()) collect_profile(c);
@@ -872,13 +892,17 @@ extern "C" int fribo_fluid_tsch(fribo_handle h, int32_t ns, double* dt, double d
     int e = markers_inside(h);
     if (e) return e;
     if ((e = write_inputs(h, ns, *dt, dt_o))) return e;
+    const bool cfl = !h->cfl_ready;
+    h->cfl_ready = false;
     if (h->use_graph && !h->profiling) {
         if ((e = ensure_graph(h))) return e;
+        if (cfl) enqueue_cfl(h);
         HIPCHK(hipGraphLaunch(h->gexec, h->st));
     } else {
-        enqueue_step(h);
+        enqueue_step(h, cfl);
     }
-    return finish(h, dt, cflm);
+    // with per-class profiling the next step's k_cfl stays in that step
+    return finish(h, dt, cflm, !h->profiling);
 }
 
 extern "C" int fribo_set_graph(fribo_handle h, int32_t on)
@@ -1218,6 +1242,7 @@ extern "C" int fribo_restore_state(fribo_handle h)
 {
     if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
     if (!h->snap_valid) return set_err(FRIBO_ERR_ARG, "no saved state");
+    h->cfl_ready = false;
     return snap_copy(h, false);
 }
 
