@@ -48,6 +48,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--transform", default="auto", choices=["auto", "y", "z"],
                     help="axis of the Poisson eigen-transform (auto: the shorter one)")
+    ap.add_argument("--launch", default="stream", choices=["stream", "graph"],
+                    help="kernel launches of a step: one by one on the stream, or one captured HIP graph "
+                         "(steps with GEMM timing events always launch on the stream)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the library's plain 256 MiB copy kernel 3+1 times (HBM counter scale)")
     args = ap.parse_args()
@@ -77,6 +80,7 @@ def main():
     st0 = cases.initial_state(p, c.seed)
     g = FluidStep(p, c.rc, c.zz, zmm, zmmt, wr)
     g.set_transform(args.transform)
+    g.set_graph(args.launch == "graph")
     g.set_state(st0)
 
     t, dt = 0.0, p.dt0
@@ -185,7 +189,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (generated grid and markers, seeded 1e-3 velocity perturbation)",
         "config": {"workload": f"{args.workload}: {n3m}x{n2m} cells, {c.nlm} markers, Re={1.0 / p.visc:g}, CFL {p.cflmax}",
-                   "precision": "plain FP64", "poisson_transform": f"along {axis}"},
+                   "precision": "plain FP64", "poisson_transform": f"along {axis}", "launch": args.launch},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,
