@@ -14,7 +14,10 @@
 // Wilkinson shifts (Bowdler, Martin, Reinsch and Wilkinson, "The QR and QL
 // algorithms for symmetric matrices", Numer. Math. 11 (1968); EISPACK tql2),
 // so A_y = (D Q) diag(w) (Q^T D^-1): Y = D Q, Yinv = Q^T D^-1.
+#include "fribo_gpu.h"
 #include "fribo_internal.h"
+
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -109,11 +112,77 @@ int tridiag_ql(int n, double* d, double* e, double* z)
     return 0;
 }
 
+// Y = D Q and Yinv = Q^T D^-1 from the orthonormal eigenvectors Q (n x n,
+// column-major) of the symmetrised operator: thread j of column block m.
+__global__ __launch_bounds__(256) void k_eig_assemble(int n, int ld, const double* __restrict__ Q,
+                                                      const double* __restrict__ dg, double* __restrict__ Y,
+                                                      double* __restrict__ Yinv)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x, m = blockIdx.y;
+    if (j >= n) return;
+    const double q = Q[(size_t)m * n + j];
+    Y[(size_t)m * ld + j] = dg[j] * q;
+    Yinv[(size_t)j * ld + m] = q / dg[j];
+}
+
+// The decomposition on the device: the symmetrised tridiagonal problem by
+// rocSOLVER's divide-and-conquer eigensolver (STEDC: eigenvalues ascending,
+// orthonormal eigenvectors), the scaling back to Y and Yinv in a kernel.
+// Host arrays in and out, like the host path.
+static int eig_device(int n, const std::vector<double>& d, const std::vector<double>& e,
+                      const std::vector<double>& dg, int ld, double* Y, double* Yinv, double* wy)
+{
+    double *dD = nullptr, *dE = nullptr, *dQ = nullptr, *dG = nullptr, *dY = nullptr, *dYi = nullptr;
+    rocblas_int* dinfo = nullptr;
+    rocblas_handle hb = nullptr;
+    hipStream_t st = nullptr;
+    int rc = 0;
+    rocblas_int info = 0;
+    const size_t nn = (size_t)n * n, ln = (size_t)ld * n;
+    auto ok = [&](hipError_t he) {
+        if (he != hipSuccess && !rc) rc = FRIBO_ERR_HIP;
+        return rc == 0;
+    };
+    if (ok(hipStreamCreate(&st)) && ok(hipMalloc(&dD, n * sizeof(double))) &&
+        ok(hipMalloc(&dE, std::max(n - 1, 1) * sizeof(double))) && ok(hipMalloc(&dQ, nn * sizeof(double))) &&
+        ok(hipMalloc(&dG, n * sizeof(double))) && ok(hipMalloc(&dY, ln * sizeof(double))) &&
+        ok(hipMalloc(&dYi, ln * sizeof(double))) && ok(hipMalloc(&dinfo, sizeof(rocblas_int))) &&
+        ok(hipMemcpyAsync(dD, d.data(), n * sizeof(double), hipMemcpyHostToDevice, st)) &&
+        ok(hipMemcpyAsync(dE, e.data(), std::max(n - 1, 1) * sizeof(double), hipMemcpyHostToDevice, st)) &&
+        ok(hipMemcpyAsync(dG, dg.data(), n * sizeof(double), hipMemcpyHostToDevice, st)) &&
+        ok(hipMemsetAsync(dY, 0, ln * sizeof(double), st)) && ok(hipMemsetAsync(dYi, 0, ln * sizeof(double), st))) {
+        if (rocblas_create_handle(&hb) != rocblas_status_success || rocblas_set_stream(hb, st) != rocblas_status_success ||
+            rocsolver_dstedc(hb, rocblas_evect_tridiagonal, n, dD, dE, dQ, n, dinfo) != rocblas_status_success)
+            rc = FRIBO_ERR_HIP;
+        if (!rc) {
+            hipLaunchKernelGGL(k_eig_assemble, dim3((n + 255) / 256, n), dim3(256), 0, st, n, ld, dQ, dG, dY, dYi);
+            if (ok(hipGetLastError()) && ok(hipMemcpyAsync(&info, dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, st)) &&
+                ok(hipMemcpyAsync(wy, dD, n * sizeof(double), hipMemcpyDeviceToHost, st)) &&
+                ok(hipMemcpyAsync(Y, dY, ln * sizeof(double), hipMemcpyDeviceToHost, st)) &&
+                ok(hipMemcpyAsync(Yinv, dYi, ln * sizeof(double), hipMemcpyDeviceToHost, st)) &&
+                ok(hipStreamSynchronize(st)) && info != 0)
+                rc = FRIBO_ERR_HIP;   // STEDC did not converge
+        }
+    }
+    if (hb) rocblas_destroy_handle(hb);
+    if (st) hipStreamSynchronize(st);
+    hipFree(dD);
+    hipFree(dE);
+    hipFree(dQ);
+    hipFree(dG);
+    hipFree(dY);
+    hipFree(dYi);
+    hipFree(dinfo);
+    if (st) hipStreamDestroy(st);
+    return rc;
+}
+
 // Y (eigenvectors of A_y as columns), Yinv = Y^-1 and the eigenvalues wy of
 // the n x n operator with rows (am, ac, ap) (0-based, am[0] = ap[n-1] = 0).
-// Y and Yinv are column-major with leading dimension ld.
+// Y and Yinv are column-major with leading dimension ld.  device: solve the
+// symmetrised problem on the GPU (eig_device) instead of the host QL.
 int ytransform_build(int n, const double* am, const double* ac, const double* ap, int ld, double* Y, double* Yinv,
-                     double* wy)
+                     double* wy, bool device)
 {
     std::vector<double> d(n), e(n > 1 ? n - 1 : 1), dg(n), q((size_t)n * n, 0.0);
     dg[0] = 1.0;
@@ -123,10 +192,9 @@ int ytransform_build(int n, const double* am, const double* ac, const double* ap
         e[j] = std::sqrt(up * lo);
         dg[j + 1] = dg[j] * std::sqrt(lo / up);
     }
-    for (int j = 0; j < n; ++j) {
-        d[j] = ac[j];
-        q[(size_t)j * n + j] = 1.0;
-    }
+    for (int j = 0; j < n; ++j) d[j] = ac[j];
+    if (device) return eig_device(n, d, e, dg, ld, Y, Yinv, wy);
+    for (int j = 0; j < n; ++j) q[(size_t)j * n + j] = 1.0;
     const int rc = tridiag_ql(n, d.data(), e.data(), q.data());
     if (rc) return rc;
     for (int m = 0; m < n; ++m) {
@@ -153,7 +221,7 @@ extern "C" int fribo_test_ytransform(int32_t n, const double* am, const double* 
 // (eigenvectors as columns) and zmmt = zmm^-1 are n3m x n3m with leading
 // dimension ld; wr(n3m) the eigenvalues.  The eigen-pairs are those of
 // DGEEV up to order and scaling, which the Poisson solve does not see.
-int phini_sym(int n3, const double* zz, double alx3, int ld, double* zmm, double* zmmt, double* wr)
+int phini_sym(int n3, const double* zz, double alx3, int ld, double* zmm, double* zmmt, double* wr, bool device)
 {
     const int n3m = n3 - 1;
     if (n3m < 2 || ld < n3m) return -1;
@@ -172,7 +240,7 @@ int phini_sym(int n3, const double* zz, double alx3, int ld, double* zmm, double
         ap[kc - 1] = a33icp * ug;
         ac[kc - 1] = -(am[kc - 1] + ap[kc - 1]);
     }
-    return ytransform_build(n3m, am.data(), ac.data(), ap.data(), ld, zmm, zmmt, wr);
+    return ytransform_build(n3m, am.data(), ac.data(), ap.data(), ld, zmm, zmmt, wr, device);
 }
 
 extern "C" int fribo_phini(int32_t n3, const double* zz, double alx3, int32_t ld, double* zmm, double* zmmt,
@@ -180,4 +248,14 @@ extern "C" int fribo_phini(int32_t n3, const double* zz, double alx3, int32_t ld
 {
     if (n3 < 3 || !zz || !zmm || !zmmt || !wr || ld < n3 - 1 || !(alx3 > 0.0)) return 5;
     return phini_sym(n3, zz, alx3, ld, zmm, zmmt, wr) == 0 ? 0 : 7;
+}
+
+// phini on the GPU: the same operator and similarity, eigen-pairs by
+// rocSOLVER's STEDC (eigenvalues in increasing order).
+extern "C" int fribo_phini_device(int32_t n3, const double* zz, double alx3, int32_t ld, double* zmm, double* zmmt,
+                                  double* wr)
+{
+    if (n3 < 3 || !zz || !zmm || !zmmt || !wr || ld < n3 - 1 || !(alx3 > 0.0)) return FRIBO_ERR_ARG;
+    const int rc = phini_sym(n3, zz, alx3, ld, zmm, zmmt, wr, true);
+    return rc == 0 ? 0 : (rc == FRIBO_ERR_HIP ? FRIBO_ERR_HIP : FRIBO_ERR_LIMIT);
 }

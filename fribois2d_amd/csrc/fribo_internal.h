@@ -228,9 +228,12 @@ void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done = false);
 void launch_loads(const FluidDev& F, double hpr, hipStream_t st);
 void launch_checkbcs(const FluidDev& F, double* nb, hipStream_t st);
 void launch_diag(const FluidDev& F, double* part, int kv, double* out, hipStream_t st);
+// device: the symmetrised eigen-problem on the GPU (rocSOLVER STEDC) instead
+// of the host QL
 int ytransform_build(int n, const double* am, const double* ac, const double* ap, int ld, double* Y, double* Yinv,
-                     double* wy);
-int phini_sym(int n3, const double* zz, double alx3, int ld, double* zmm, double* zmmt, double* wr);
+                     double* wy, bool device = false);
+int phini_sym(int n3, const double* zz, double alx3, int ld, double* zmm, double* zmmt, double* wr,
+              bool device = false);
 int jsolve_stride(const FluidDev& F);
 size_t jsolve_lds_bytes(const FluidDev& F);
 size_t sysprep_lds_bytes(const FluidDev& F);

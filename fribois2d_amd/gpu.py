@@ -28,7 +28,7 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
            "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform", "fribo_loads_transfer",
            "fribo_checkbcs", "fribo_diagnostics", "fribo_save_state", "fribo_restore_state",
-           "fribo_phini"]
+           "fribo_phini", "fribo_phini_device"]
 
 NPROF = 12
 
@@ -99,6 +99,7 @@ def load():
     L.fribo_diagnostics.argtypes = [vp, dp]
     L.fribo_save_state.argtypes = [vp]
     L.fribo_phini.argtypes = [C.c_int32, dp, C.c_double, C.c_int32, dp, dp, dp]
+    L.fribo_phini_device.argtypes = [C.c_int32, dp, C.c_double, C.c_int32, dp, dp, dp]
     L.fribo_restore_state.argtypes = [vp]
     _lib = L
     return L
@@ -333,12 +334,14 @@ def ytransform(am, ac, ap):
     return Y, Yi, w
 
 
-def phini(zz, alx3):
-    """phini's (zmm, zmmt, wr) by the library's symmetrised QL decomposition."""
+def phini(zz, alx3, device=False):
+    """phini's (zmm, zmmt, wr) by the library's symmetrised decomposition:
+    implicit QL on the host, or (device=True) rocSOLVER's STEDC on the GPU."""
     zz = np.ascontiguousarray(zz, dtype=np.float64)
     n = zz.size - 1
     zmm = np.zeros((n, n), order="F")
     zmmt = np.zeros((n, n), order="F")
     wr = np.zeros(n)
-    _check(load().fribo_phini(zz.size, _dp(zz), float(alx3), n, _dp(zmm), _dp(zmmt), _dp(wr)))
+    fn = load().fribo_phini_device if device else load().fribo_phini
+    _check(fn(zz.size, _dp(zz), float(alx3), n, _dp(zmm), _dp(zmmt), _dp(wr)))
     return zmm, zmmt, wr

@@ -191,6 +191,14 @@ int fribo_get_transform(fribo_handle h);
  * O(n^3) eigenvector accumulation instead of DGEEV's dense O(n^3). */
 int fribo_phini(int32_t n3, const double* zz, double alx3, int32_t ld, double* zmm, double* zmmt, double* wr);
 
+/* fribo_phini on the GPU (phini.F90:4-99; no reference counterpart runs it on
+ * a device): the same operator and similarity, the symmetric tridiagonal
+ * eigen-problem solved by rocSOLVER's divide-and-conquer STEDC (eigenvalues
+ * in increasing order), Y and Y^-1 assembled by a kernel; host arrays in
+ * and out.  For many instances of uneven grids, one call per instance. */
+int fribo_phini_device(int32_t n3, const double* zz, double alx3, int32_t ld, double* zmm, double* zmmt,
+                       double* wr);
+
 /* Diagnostic entry point (no reference counterpart): eigen-decomposition of
  * an n x n tridiagonal operator with rows (am, ac, ap), 0-based, that is
  * similar to a symmetric one (am[j+1]*ap[j] > 0): Y (eigenvectors as
