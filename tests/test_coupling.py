@@ -15,7 +15,7 @@ from helpers import OracleFluid, case, oracle_for
 
 
 def cylinder_body(c, rbmcon=(0, 1, 1)):
-    # m* = 10, Ur = 5 (ky = 12.4025), the north-star cylinder set-up
+    # m* = 10, Ur = 5 (ky = 12.4025): the cyl384 cylinder of the 1000-step runs
     return coupling.RigidBody(vert=c.verts, cm=(c.center[0], c.center[1], 0.0), rbden=10.0,
                               rba=(0.7853982, 0.0981748), kk=(12.4025, 0.0, 0.0), rbmcon=rbmcon)
 

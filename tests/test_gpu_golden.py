@@ -1,7 +1,7 @@
 """1000-step parity on the 384x288 cylinder workload against the committed
 oracle trajectory (tests/golden/cyl384_1000steps.npz, make_golden.py).
 
-Bars (north star): velocity/pressure max difference <= 1e-10 relative to the
+Bars: velocity/pressure max difference <= 1e-10 relative to the
 field's maximum after 1000 steps; per-step histories (dt, CFL, immersed-boundary
 force sums) within 1e-9 relative to the largest value of each history.
 """

@@ -6,7 +6,7 @@ Bars (written per test):
   identical operation order in both codes, compared at 1e-14 relative;
 * solver outputs (tridiagonal sweeps, Poisson transform): different but exact
   elimination orders, 1e-11 relative after a substep, 1e-10 relative to the
-  field maximum after multi-step runs (the north-star velocity/pressure bar).
+  field maximum after multi-step runs (the velocity/pressure bar).
 """
 import numpy as np
 import pytest
