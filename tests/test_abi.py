@@ -53,6 +53,10 @@ def test_argument_errors_are_reported_without_a_device():
     assert L.fribo_fluid_tsch(None, 0, None, 0.0, None) == 5
     assert L.fribo_set_markers(None, -1, None, None, None) == 5
     assert L.fribo_array_size(None, b"q2") == 0
+    # phini on either side: too few nodes / missing outputs
+    zz = (ctypes.c_double * 2)(0.0, 1.0)
+    for fn in (L.fribo_phini, L.fribo_phini_device):
+        assert fn(2, zz, 1.0, 1, None, None, None) == 5
 
 
 def test_bench_refuses_multi_gpu():
