@@ -176,89 +176,92 @@ __device__ double weight1d(double dif, double sdd, int wfun, double radiusSD)
     return 0.0;
 }
 
+// Sixteen lanes per (marker, direction), lane m < 9 owning support m (MLS_SD_ind's
+// order: j outer, k inner): each lane finds its support node, cell size,
+// weight and monomials; the 3x3 moment matrix, the partition-of-unity sum and
+// the two momentum sums are summed in support order from the group's lanes,
+// so every value is the serial loop's; each lane of the group solves the same
+// 3x3 system and keeps its own shape-function value.
 __global__ __launch_bounds__(64) void k_mls_sup(const FluidDev F)
 {
     DevScalars* S = F.S;
     if (S->err) return;
-    const int tid = blockIdx.x * 64 + threadIdx.x;
-    if (tid >= 2 * F.nlm) return;
-    const int n = tid % F.nlm + 1, d = tid / F.nlm + 1;
+    const int lane = threadIdx.x;
+    const int g = blockIdx.x * 4 + (lane >> 4), m = lane & 15, base = lane & 48;
+    if (g >= 2 * F.nlm) return;   // whole group
+    const int n = g % F.nlm + 1, d = g / F.nlm + 1;
+    const bool sup = m < FRIBO_DIMSD;
     const double pos[2] = {LMCC(n, 1), LMCC(n, 2)};
     unsigned long long* keys = F.keys + (size_t)(d - 1) * F.maxkeys;
     const bool act = (F.zz[1] < pos[1]) && (pos[1] < F.zz[F.n3]);
-    if (d == 1) F.active[n - 1] = act ? 1 : 0;
-    if (!act) {
-        for (int m = 1; m <= FRIBO_DIMSD; ++m) {
-            PHI(n, d, m) = 0.0;
-            PHIPRED(n, d, m) = 0.0;
-            keys[(size_t)(n - 1) * FRIBO_DIMSD + (m - 1)] = KEY_NONE;
+    if (d == 1 && m == 0) F.active[n - 1] = act ? 1 : 0;
+    if (!act) {                   // whole group
+        if (sup) {
+            PHI(n, d, m + 1) = 0.0;
+            PHIPRED(n, d, m + 1) = 0.0;
+            keys[(size_t)(n - 1) * FRIBO_DIMSD + m] = KEY_NONE;
         }
-        CL(n, d) = 0.0;
+        if (m == 0) CL(n, d) = 0.0;
         return;
     }
-    // MLS_SD_ind
+    // MLS_SD_ind: support mm of the 3x3 block (lanes past the ninth repeat it)
     const int i1 = LMIND(n, d, 1), i2 = LMIND(n, d, 2);
     const double* ya = (d == 1) ? F.rc : F.rm;
     const double* za = (d == 1) ? F.zm : F.zz;
-    double sdc[2][FRIBO_DIMSD], sdd[2][FRIBO_DIMSD];
-    int eul[2][FRIBO_DIMSD];
-    int m = 0;
-    for (int j = 1; j <= 3; ++j)
-        for (int k = 1; k <= 3; ++k) {
-            int jj = (i1 == 1) ? j + 1 : ((i1 == F.n2m) ? j - 1 : j);
-            int kk = (i2 == 1) ? k + 1 : ((i2 == F.n3m) ? k - 1 : k);
-            int ej = i1 + jj - 2, ek = i2 + kk - 2;
-            sdc[0][m] = ya[ej];
-            sdd[0][m] = ya[ej + 1] - ya[ej];
-            sdc[1][m] = za[ek];
-            sdd[1][m] = za[ek + 1] - za[ek];
-            eul[0][m] = ej;
-            eul[1][m] = ek;
-            ++m;
-        }
+    const int mm = sup ? m : FRIBO_DIMSD - 1, j = mm / 3 + 1, k = mm % 3 + 1;
+    const int jj = (i1 == 1) ? j + 1 : ((i1 == F.n2m) ? j - 1 : j);
+    const int kk = (i2 == 1) ? k + 1 : ((i2 == F.n3m) ? k - 1 : k);
+    const int ej = i1 + jj - 2, ek = i2 + kk - 2;
+    const double sdc0 = ya[ej], sdd0 = ya[ej + 1] - ya[ej];
+    const double sdc1 = za[ek], sdd1 = za[ek + 1] - za[ek];
     // MLS_SF
-    double w[FRIBO_DIMSD], A[3][3], pk[3][FRIBO_DIMSD];
+    const double pk0 = 1.0, pk1 = sdc0, pk2 = sdc1;
+    const double w = weight1d(pos[0] - sdc0, sdd0, F.wfun, F.radiusSD) *
+                     weight1d(pos[1] - sdc1, sdd1, F.wfun, F.radiusSD);
+    const double t00 = w * (pk0 * pk0), t01 = w * (pk0 * pk1), t02 = w * (pk0 * pk2);
+    const double t11 = w * (pk1 * pk1), t12 = w * (pk1 * pk2), t22 = w * (pk2 * pk2);
+    double a00 = 0.0, a01 = 0.0, a02 = 0.0, a11 = 0.0, a12 = 0.0, a22 = 0.0;
+#pragma unroll
     for (int i = 0; i < FRIBO_DIMSD; ++i) {
-        pk[0][i] = 1.0;
-        pk[1][i] = sdc[0][i];
-        pk[2][i] = sdc[1][i];
-        w[i] = weight1d(pos[0] - sdc[0][i], sdd[0][i], F.wfun, F.radiusSD) *
-               weight1d(pos[1] - sdc[1][i], sdd[1][i], F.wfun, F.radiusSD);
+        a00 = a00 + __shfl(t00, base + i, 64);
+        a01 = a01 + __shfl(t01, base + i, 64);
+        a02 = a02 + __shfl(t02, base + i, 64);
+        a11 = a11 + __shfl(t11, base + i, 64);
+        a12 = a12 + __shfl(t12, base + i, 64);
+        a22 = a22 + __shfl(t22, base + i, 64);
     }
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) A[a][b] = 0.0;
-    for (int i = 0; i < FRIBO_DIMSD; ++i)
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) A[a][b] = A[a][b] + w[i] * (pk[a][i] * pk[b][i]);
+    // A is symmetric entry by entry: w*(pa*pb) == w*(pb*pa)
+    const double A[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
     const double gps[3] = {1.0, pos[0], pos[1]};
     double ptA[3] = {0.0, 0.0, 0.0};
     gesolve3(A, gps, ptA);
-    double phl[FRIBO_DIMSD], sumphi = 0.0;
-    for (int i = 0; i < FRIBO_DIMSD; ++i)
-        phl[i] = ptA[0] * (pk[0][i] * w[i]) + ptA[1] * (pk[1][i] * w[i]) + ptA[2] * (pk[2][i] * w[i]);
-    for (int i = 0; i < FRIBO_DIMSD; ++i) sumphi = sumphi + phl[i];
-    if (fabs(1.0 - fabs(sumphi)) > 1.0e-10) atomicCAS(&S->err, 0, FRIBO_E_PARTITION_UNITY);
-    // forcing_MLS.F90:72-81
-    double dvel = 0.0, lgr_h = 0.0;
+    const double phl = ptA[0] * (pk0 * w) + ptA[1] * (pk1 * w) + ptA[2] * (pk2 * w);
+    // forcing_MLS.F90:72-81 terms of this support
+    const double tv = phl * (sdd0 * sdd1);
+    const double tl = 1.0 / 2.0 * phl * (sdd0 + sdd1);
+    double sumphi = 0.0, dvel = 0.0, lgr_h = 0.0;
+#pragma unroll
     for (int i = 0; i < FRIBO_DIMSD; ++i) {
-        double vol = sdd[0][i] * sdd[1][i];
-        dvel = dvel + phl[i] * vol;
-        lgr_h = lgr_h + 1.0 / 2.0 * phl[i] * (sdd[0][i] + sdd[1][i]);
+        sumphi = sumphi + __shfl(phl, base + i, 64);
+        dvel = dvel + __shfl(tv, base + i, 64);
+        lgr_h = lgr_h + __shfl(tl, base + i, 64);
     }
-    CL(n, d) = F.lmA[n - 1] * lgr_h / dvel;
-    for (int i = 0; i < FRIBO_DIMSD; ++i) {
-        PHI(n, d, i + 1) = phl[i];
-        PHIPRED(n, d, i + 1) = phl[i];
-        EULINS(n, d, 1, i + 1) = eul[0][i];
-        EULINS(n, d, 2, i + 1) = eul[1][i];
-        // supports past the last node exist only for markers within half a
-        // cell of the top wall / outflow, where the reference writes outside
-        // its arrays; they are not spread
-        const bool inb = eul[0][i] >= 1 && eul[0][i] <= F.n2 && eul[1][i] >= 1 && eul[1][i] <= F.n3;
-        unsigned long long cell = (unsigned long long)(eul[0][i] - 1) + (unsigned long long)(eul[1][i] - 1) * F.n2;
-        keys[(size_t)(n - 1) * FRIBO_DIMSD + i] =
-            inb ? ((cell << ENT_BITS) | (unsigned long long)((n - 1) * FRIBO_DIMSD + i)) : KEY_NONE;
+    if (m == 0) {
+        if (fabs(1.0 - fabs(sumphi)) > 1.0e-10) atomicCAS(&S->err, 0, FRIBO_E_PARTITION_UNITY);
+        CL(n, d) = F.lmA[n - 1] * lgr_h / dvel;
     }
+    if (!sup) return;
+    PHI(n, d, m + 1) = phl;
+    PHIPRED(n, d, m + 1) = phl;
+    EULINS(n, d, 1, m + 1) = ej;
+    EULINS(n, d, 2, m + 1) = ek;
+    // supports past the last node exist only for markers within half a cell
+    // of the top wall / outflow, where the reference writes outside its
+    // arrays; they are not spread
+    const bool inb = ej >= 1 && ej <= F.n2 && ek >= 1 && ek <= F.n3;
+    const unsigned long long cell = (unsigned long long)(ej - 1) + (unsigned long long)(ek - 1) * F.n2;
+    keys[(size_t)(n - 1) * FRIBO_DIMSD + m] =
+        inb ? ((cell << ENT_BITS) | (unsigned long long)((n - 1) * FRIBO_DIMSD + m)) : KEY_NONE;
 }
 
 // Ordering of the (cell, entry) keys of each direction, ascending.
@@ -573,7 +576,7 @@ void launch_mls_setup(const FluidDev& F, hipStream_t st)
 {
     if (F.nlm <= 0) return;
     hipLaunchKernelGGL(k_mls_idx, dim3(F.nlm, 4), dim3(64), 0, st, F);
-    hipLaunchKernelGGL(k_mls_sup, dim3((2 * F.nlm + 63) / 64), dim3(64), 0, st, F);
+    hipLaunchKernelGGL(k_mls_sup, dim3((2 * F.nlm + 3) / 4), dim3(64), 0, st, F);
     const unsigned long long ncell = (unsigned long long)F.n2 * (unsigned long long)F.n3;
     if (F.nlm * FRIBO_DIMSD <= RADIX_MAX && ncell < 0xffffffffull) {
         int nbits = 1;
