@@ -117,8 +117,8 @@ struct StepScalars {
     int err;
 };
 
-__device__ StepScalars step_scalars(const FluidDev& F, const DevScalars* S, int n, int ns, double dt_in,
-                                    double dt_o, double* sh, double* sh2)
+__device__ StepScalars step_scalars(const FluidDev& F, int n, int ns, double dt_in, double dt_o, double s_dt,
+                                    double s_cflm, double s_beta, int s_err, double* sh, double* sh2)
 {
     __shared__ StepScalars out;
     const int t = threadIdx.x;
@@ -154,11 +154,11 @@ __device__ StepScalars step_scalars(const FluidDev& F, const DevScalars* S, int 
             r.cflm = cflm;
             r.beta = dt / F.ren * 0.5;
         } else {
-            r.dt = S->dt;
-            r.cflm = S->cflm;
+            r.dt = s_dt;
+            r.cflm = s_cflm;
             r.q3sum = 0.0;
-            r.beta = S->beta;
-            if (S->err) r.err = S->err;
+            r.beta = s_beta;
+            if (s_err) r.err = s_err;
         }
         double al, ga, ro;
         if (F.nsst == 3) {
@@ -297,14 +297,26 @@ size_t sysprep_lds_bytes(const FluidDev& F)
     return (size_t)3 * n * sizeof(double);
 }
 
+// First two pressure partials of thread t (entries t and t+1024, clamped to
+// the last), loaded ahead so that they travel with the kernel's other loads.
+__device__ inline void pr_load(const FluidDev& F, double* v)
+{
+    const int t = threadIdx.x, n = F.npr_part;
+    v[0] = F.pr_part[min(t, n - 1)];
+    v[1] = F.pr_part[min(t + 1024, n - 1)];
+}
+
 // Pressure mean of the previous substep (pr_submean, updvp.F90:125-141) from
 // the npr_part partials of k_upd: thread t adds entries t, t+1024, ... in
-// turn (eight loads in flight at a time), then block_sum.  Fixed order.
-__device__ inline double pr_mean(const FluidDev& F, double* sh)
+// turn (the first two from pr_load, then eight loads in flight at a time),
+// then block_sum.  Fixed order.
+__device__ inline double pr_mean(const FluidDev& F, double* sh, const double* v0)
 {
     const int t = threadIdx.x, n = F.npr_part;
     double s = 0.0;
-    for (int b = t; b < n; b += 8 * 1024) {
+    if (t < n) s = s + v0[0];
+    if (t + 1024 < n) s = s + v0[1];
+    for (int b = t + 2048; b < n; b += 8 * 1024) {
         double v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) v[u] = F.pr_part[min(b + u * 1024, n - 1)];
@@ -325,7 +337,9 @@ __device__ void mean_block(const FluidDev& F, double* sh)
 {
     DevScalars* S = F.S;
     const int t = threadIdx.x, n2m = F.n2m;
-    const double pravg = pr_mean(F, sh);
+    double prv[2];
+    pr_load(F, prv);
+    const double pravg = pr_mean(F, sh, prv);
     for (int j0 = 1 + t; j0 <= n2m; j0 += 2048) {
         double a[2][8];
 #pragma unroll
@@ -357,68 +371,91 @@ __device__ void mean_block(const FluidDev& F, double* sh)
     }
 }
 
+// Row values read by boucqt/boucdq for rows j0 and j0 + 1024 (clamped to n2m).
+struct BcRows {
+    double qn2[2], qn3[2], q2e[2], q3e[2], o2[2], o3[2], g[2];
+    double qs2[2], qs3[2], dqs2[2], dqs3[2], dqn2[2], dqn3[2];
+};
+
+__device__ inline void bc_load(const FluidDev& F, int mean, int j0, BcRows& b)
+{
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int j = min(j0 + 1024 * u, F.n2m);
+        b.qn2[u] = F.qb2n[j];
+        b.qn3[u] = F.qb3n[j];
+        b.q2e[u] = F.q2[fidx(F, j, F.n3m)];
+        b.q3e[u] = F.q3[fidx(F, j, F.n3m)];
+        b.o2[u] = F.dq2x2o[j];
+        b.o3[u] = F.dq3x2o[j];
+        b.g[u] = F.g2rm[j];
+        if (mean) {
+            b.qs2[u] = F.qb2s[j];
+            b.qs3[u] = F.qb3s[j];
+            b.dqs2[u] = F.dqb2s[j];
+            b.dqs3[u] = F.dqb3s[j];
+            b.dqn2[u] = F.dqb2n[j];
+            b.dqn3[u] = F.dqb3n[j];
+        }
+    }
+}
+
 // Workgroup 0 of k_bc: with mean, pr_submean/boucqt of the previous substep
 // (as mean_block), then boucdq (iobcs.F90:22-50) and the global mass balance
 // at the outflow (iobcs.F90:54-74).  Thread t owns rows j = 1+t, 1+t+1024,
-// ...: every row value is loaded once, two rows' loads in flight together,
-// and the boundary values updated by boucqt feed boucdq from registers.
-__device__ void bc_rows(const FluidDev& F, const StepScalars& r, int mean, double* sh, double* sh2)
+// ...: every row value is loaded once, and the boundary values updated by
+// boucqt feed boucdq from registers.  The rows of the first pass (pre) and
+// the first pressure partials (prv) are loaded at kernel entry, before the
+// step scalars are known, so that a single memory round trip serves the
+// common case n2m <= 2048; the outflow increments of those rows stay in
+// registers for the mass correction.
+__device__ void bc_rows(const FluidDev& F, const StepScalars& r, int mean, double* sh, double* sh2,
+                        const BcRows& pre, const double* prv)
 {
     DevScalars* S = F.S;
     const int t = threadIdx.x, n2m = F.n2m;
-    const double pravg = mean ? pr_mean(F, sh) : 0.0;
+    const double pravg = mean ? pr_mean(F, sh, prv) : 0.0;
     const double dt = r.dt, ga = r.ga, ro = r.ro;
     const double udx3c_n3 = F.udx3c[F.n3];
     double area = 0.0, qout = 0.0;
-    for (int j0 = 1 + t; j0 <= n2m; j0 += 2048) {
-        double qn2[2], qn3[2], q2e[2], q3e[2], o2[2], o3[2], g[2];
-        double qs2[2], qs3[2], dqs2[2], dqs3[2], dqn2[2], dqn3[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int j = min(j0 + 1024 * u, n2m);
-            qn2[u] = F.qb2n[j];
-            qn3[u] = F.qb3n[j];
-            q2e[u] = F.q2[fidx(F, j, F.n3m)];
-            q3e[u] = F.q3[fidx(F, j, F.n3m)];
-            o2[u] = F.dq2x2o[j];
-            o3[u] = F.dq3x2o[j];
-            g[u] = F.g2rm[j];
-            if (mean) {
-                qs2[u] = F.qb2s[j];
-                qs3[u] = F.qb3s[j];
-                dqs2[u] = F.dqb2s[j];
-                dqs3[u] = F.dqb3s[j];
-                dqn2[u] = F.dqb2n[j];
-                dqn3[u] = F.dqb3n[j];
-            }
-        }
+    double keep[2] = {0.0, 0.0};
+    // one pass over rows j0 and j0 + 1024 from the values in b
+    auto pass = [&](const BcRows& b, int j0, bool first) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int j = j0 + 1024 * u;
             if (j <= n2m) {
+                double qn2 = b.qn2[u], qn3 = b.qn3[u];
                 if (mean) {
-                    F.qb2s[j] = qs2[u] + dqs2[u];
-                    F.qb3s[j] = qs3[u] + dqs3[u];
-                    qn2[u] = qn2[u] + dqn2[u];
-                    qn3[u] = qn3[u] + dqn3[u];
-                    F.qb2n[j] = qn2[u];
-                    F.qb3n[j] = qn3[u];
+                    F.qb2s[j] = b.qs2[u] + b.dqs2[u];
+                    F.qb3s[j] = b.qs3[u] + b.dqs3[u];
+                    qn2 = qn2 + b.dqn2[u];
+                    qn3 = qn3 + b.dqn3[u];
+                    F.qb2n[j] = qn2;
+                    F.qb3n[j] = qn3;
                 }
                 F.dqb2s[j] = 0.0;
                 F.dqb3s[j] = 0.0;
-                const double dq2x2 = (qn2[u] - q2e[u]) * 2.0 * udx3c_n3;
-                const double d2 = -dt * (ga * dq2x2 + ro * o2[u]) * F.cou;
-                const double dq3x2 = (qn3[u] - q3e[u]) * udx3c_n3;
-                const double d3 = -dt * (ga * dq3x2 + ro * o3[u]) * F.cou;
+                const double dq2x2 = (qn2 - b.q2e[u]) * 2.0 * udx3c_n3;
+                const double d2 = -dt * (ga * dq2x2 + ro * b.o2[u]) * F.cou;
+                const double dq3x2 = (qn3 - b.q3e[u]) * udx3c_n3;
+                const double d3 = -dt * (ga * dq3x2 + ro * b.o3[u]) * F.cou;
                 F.dqb2n[j] = d2;
                 F.dqb3n[j] = d3;
                 F.dq2x2o[j] = dq2x2;
                 F.dq3x2o[j] = dq3x2;
-                const double darea = g[u] / F.dx2;
+                if (first) keep[u] = d3;
+                const double darea = b.g[u] / F.dx2;
                 area = area + darea;
                 qout = qout + d3 * darea;
             }
         }
+    };
+    pass(pre, 1 + t, true);
+    for (int j0 = 1 + t + 2048; j0 <= n2m; j0 += 2048) {
+        BcRows b;
+        bc_load(F, mean, j0, b);
+        pass(b, j0, false);
     }
     if (t == 0) {
         F.dqb2s[F.n2] = 0.0;
@@ -432,7 +469,8 @@ __device__ void bc_rows(const FluidDev& F, const StepScalars& r, int mean, doubl
     const double tqout = block_sum<1024>(sh2, qout);
     const double qinf = 0.0 * tarea;   // dqb3s == 0 (steady inflow)
     const double cor = (qinf - tqout) / tarea;
-    for (int j = 1 + t; j <= n2m; j += 1024) F.dqb3n[j] = F.dqb3n[j] + cor;
+    int i = 0;
+    for (int j = 1 + t; j <= n2m; j += 1024, ++i) F.dqb3n[j] = ((i < 2) ? keep[i] : F.dqb3n[j]) + cor;
     if (t == 0) {
         S->qinf = qinf;
         S->qout = tqout;
@@ -452,7 +490,16 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int mean)
     const int t = threadIdx.x;
     const int ns = S->in_ns;
     const double dt_in = S->in_dt, dt_o_in = S->in_dto;
-    const StepScalars r = step_scalars(F, S, n, ns, dt_in, dt_o_in, sh, sh2);
+    const double s_dt = S->dt, s_cflm = S->cflm, s_beta = S->beta;
+    const int s_err = S->err;
+    // workgroup 0: its row values and pressure partials, in flight with the above
+    BcRows pre;
+    double prv[2] = {0.0, 0.0};
+    if (blockIdx.x == 0) {
+        bc_load(F, mean, 1 + t, pre);
+        if (mean) pr_load(F, prv);
+    }
+    const StepScalars r = step_scalars(F, n, ns, dt_in, dt_o_in, s_dt, s_cflm, s_beta, s_err, sh, sh2);
     if (r.err) {
         if (blockIdx.x == 0 && t == 0 && n == 1) {
             S->cflm = r.cflm;
@@ -483,7 +530,7 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int mean)
         S->ugkks = 1.0 / (F.g3rm[1] * F.g3rc[1]) * F.dx3q * 2.0 * r.betadx;
         S->ugkkn = 1.0 / (F.g3rm[F.n3m] * F.g3rc[F.n3]) * F.dx3q * 2.0 * r.betadx;
     }
-    bc_rows(F, r, mean, sh, sh2);
+    bc_rows(F, r, mean, sh, sh2, pre, prv);
     // q2 wall rows: q2(1,:) = q2(n2,:) = 0 on the ghost planes (helsolve.F90:32-33)
     if (t < 6) {
         int k = (t < 3) ? t - 2 : F.n3m + 1 + (t - 3);
