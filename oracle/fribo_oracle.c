@@ -1776,3 +1776,38 @@ int orc_test_phcalc(orc_state *s, double *field)
         for (int j = 1; j <= s->n2m; ++j) field[(size_t)(j - 1) + (size_t)(k - 1) * s->n2m] = DPH(j, k);
     return ORC_OK;
 }
+
+/* The step scalars {dt, beta, al, ga, ro} a stage reads (fluid_tsch.F90:57,
+ * 62-77), set directly for the stage-level pins. */
+int orc_set_scalars(orc_state *s, const double *v)
+{
+    if (!s || !v) return ORC_ERR_ARG;
+    s->dt = v[0]; s->beta = v[1]; s->al = v[2]; s->ga = v[3]; s->ro = v[4];
+    return ORC_OK;
+}
+
+/* One stage of the substep on the current state, by name: "cfl", "boucdq",
+ * "boucqt", "explicit" (hdnl2, hdnl3, invtr2, invtr3), "lgrdistribution",
+ * "forcing1" (lgrdistribution + forcing_MLS of substep 1), "helsolve", "divg", "updvp",
+ * "prcalc", "pr_submean". */
+int orc_test_stage(orc_state *s, const char *name)
+{
+    if (!s || !name) return ORC_ERR_ARG;
+    if (!strcmp(name, "cfl")) cfl(s);
+    else if (!strcmp(name, "boucdq")) boucdq(s);
+    else if (!strcmp(name, "boucqt")) boucqt(s);
+    else if (!strcmp(name, "explicit")) { hdnl2(s); hdnl3(s); invtr2(s); invtr3(s); }
+    else if (!strcmp(name, "lgrdistribution")) return lgrdistribution(s);
+    else if (!strcmp(name, "forcing1")) {
+        int rc = lgrdistribution(s);
+        if (rc != ORC_OK) return rc;
+        return forcing_mls(s, 1);
+    }
+    else if (!strcmp(name, "helsolve")) helsolve(s);
+    else if (!strcmp(name, "divg")) divg(s);
+    else if (!strcmp(name, "updvp")) updvp(s);
+    else if (!strcmp(name, "prcalc")) prcalc(s);
+    else if (!strcmp(name, "pr_submean")) pr_submean(s);
+    else return ORC_ERR_ARG;
+    return ORC_OK;
+}

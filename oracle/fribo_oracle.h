@@ -95,6 +95,13 @@ int orc_test_gesolve(const double *a_colmajor, const double *b, double *x);
 int orc_test_phcalc(orc_state *s, double *field);
 double orc_test_weight1d(double rcv, int wfun);
 
+/* Stage-level test hooks: set the step scalars {dt, beta, al, ga, ro}, and
+ * run one named stage of the substep ("cfl", "boucdq", "boucqt", "explicit",
+ * "lgrdistribution", "forcing1", "helsolve", "divg", "updvp", "prcalc", "pr_submean") on the
+ * current state. */
+int orc_set_scalars(orc_state *s, const double *v);
+int orc_test_stage(orc_state *s, const char *name);
+
 #ifdef __cplusplus
 }
 #endif

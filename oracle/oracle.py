@@ -58,6 +58,8 @@ def _lib():
         L.orc_loads_transfer.argtypes = [vp, C.c_double, dp, dp, dp, dp]
         L.orc_checkbcs.argtypes = [vp, dp, dp, dp, dp]
         L.orc_diagnostics.argtypes = [vp, dp]
+        L.orc_set_scalars.argtypes = [vp, dp]
+        L.orc_test_stage.argtypes = [vp, C.c_char_p]
         _LIB = L
     return _LIB
 
@@ -183,6 +185,15 @@ class Oracle:
         tau = np.zeros((nlm, 2), order="F")
         rc = _lib().orc_loads_transfer(self.h, float(kprobe), _dp(lmnor), _dp(lmacc), _dp(press), _dp(tau))
         return rc, press, tau
+
+    def set_scalars(self, dt, beta, al, ga, ro):
+        """The step scalars a stage reads (test hook)."""
+        v = np.array([dt, beta, al, ga, ro], dtype=np.float64)
+        _lib().orc_set_scalars(self.h, _dp(v))
+
+    def stage(self, name):
+        """Run one named stage of the substep on the current state (test hook)."""
+        return _lib().orc_test_stage(self.h, name.encode())
 
     def diagnostics(self):
         """ttime's per-step diagnostics as a dict."""
