@@ -439,10 +439,12 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     if (!e) e = dalloc(c, &F.zmm, (size_t)F.ldz * F.n3m);
     if (!e) e = dalloc(c, &F.zmmt, (size_t)F.ldz * F.n3m);
     if (!e) {
-        HIPCHK(hipMemcpy2D(F.zmm, F.ldz * sizeof(double), zmm, ldz * sizeof(double), F.n3m * sizeof(double), F.n3m,
-                           hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy2D(F.zmmt, F.ldz * sizeof(double), zmmt, ldz * sizeof(double), F.n3m * sizeof(double),
-                           F.n3m, hipMemcpyHostToDevice));
+        hipError_t he = hipMemcpy2D(F.zmm, F.ldz * sizeof(double), zmm, ldz * sizeof(double), F.n3m * sizeof(double),
+                                    F.n3m, hipMemcpyHostToDevice);
+        if (he == hipSuccess)
+            he = hipMemcpy2D(F.zmmt, F.ldz * sizeof(double), zmmt, ldz * sizeof(double), F.n3m * sizeof(double), F.n3m,
+                             hipMemcpyHostToDevice);
+        if (he != hipSuccess) e = set_err(FRIBO_ERR_HIP, std::string("hipMemcpy2D (zmm, zmmt): ") + hipGetErrorString(he));
     }
     if (!e) e = dalloc(c, &F.cfl_part, 2 * FRIBO_NPART);
     F.npr_part = fribo_npr_part(F.n2m, F.n3m);
@@ -500,6 +502,8 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
         if (he != hipSuccess) {
             for (void* q : c->allocs) hipFree(q);
             if (c->in) hipHostFree(c->in);
+            if (c->hsp) hipHostFree(c->hsp);
+            if (c->ev_rb) hipEventDestroy(c->ev_rb);
             hipStreamDestroy(c->st);
             delete c;
             return set_err(FRIBO_ERR_HIP, hipGetErrorString(he));
@@ -640,6 +644,13 @@ extern "C" int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc
     FluidDev& F = h->F;
     if (nlm != h->nlm_alloc) {
         HIPCHK(hipStreamSynchronize(h->st));
+        // the captured step holds the old marker buffers in its kernel
+        // arguments: it is rebuilt on the next graph launch
+        if (h->gexec) {
+            hipGraphExecDestroy(h->gexec);
+            h->gexec = nullptr;
+        }
+        h->graph_nlm = -1;
         for (void* q : h->mk_allocs) hipFree(q);
         h->mk_allocs.clear();
         if (h->mk_stage) hipHostFree(h->mk_stage);

@@ -434,8 +434,12 @@ static int launch_gemm(int M, int N, int K, const double* A, int lda, const doub
 {
     YFuse yf{};
     // the fused chunk pass needs whole chunks of at most 33 rows in every
-    // output tile of the direct-to-LDS kernel
-    if (NN && yfp && yfp->csz > 1 && yfp->csz <= 33 && GEMM_DMA_BN % yfp->csz == 0 && N % GEMM_DMA_BN == 0)
+    // output tile of the direct-to-LDS kernel, and no more chunks per tile
+    // than the epilogue has wave groups (one group of (96 + 63) / 64 waves
+    // per chunk)
+    constexpr int kChunkGroups = (GEMM_DMA_WM * GEMM_DMA_WN) / ((96 + 63) / 64);
+    if (NN && yfp && yfp->csz > 1 && yfp->csz <= 33 && GEMM_DMA_BN % yfp->csz == 0 && N % GEMM_DMA_BN == 0 &&
+        GEMM_DMA_BN / yfp->csz <= kChunkGroups)
         yf = *yfp;
     if (fused) *fused = false;
     if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192 && dma_ok<NN>(M, N, K, A, lda, B, ldb)) {

@@ -304,6 +304,32 @@ def test_graph_replay_matches_stream_launches():
         dt = dte
 
 
+def test_graph_replay_after_marker_count_changes():
+    """A new marker count reallocates the marker buffers; the captured step
+    is rebuilt, so a graph replay after A -> B -> A marker counts equals the
+    stream launches bit for bit."""
+    c = case("cyl64")
+    dec = decomposition(c)
+    ge = gpu_for(c, dec)
+    gg = gpu_for(c, dec)
+    gg.set_graph(True)
+    t, dt = 0.0, c.params.dt0
+    for ns, keep in enumerate([None, None, 20, 20, 0, None, None]):
+        lmcc, lmvel, lmA = c.markers(t)
+        if keep is not None:
+            lmcc, lmvel, lmA = lmcc[:keep], lmvel[:keep], lmA[:keep]
+        for g in (ge, gg):
+            g.set_markers(np.asfortranarray(lmcc), np.asfortranarray(lmvel), np.ascontiguousarray(lmA))
+        dte, cfe = ge.fluid_tsch(ns, dt, dt)
+        dtg, cfg = gg.fluid_tsch(ns, dt, dt)
+        assert dte == dtg and cfe == cfg
+        se, sg = ge.get_state(), gg.get_state()
+        for f in ("q2", "q3", "pr", "ru2", "ru3"):
+            np.testing.assert_array_equal(se[f], sg[f], err_msg=f"{f} step {ns}")
+        t += dte
+        dt = dte
+
+
 def test_transform_axes_agree():
     """The y-order and z-order Poisson solves give the same step (same
     discrete system and gauge; only the rounding of the transforms differs),
