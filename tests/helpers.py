@@ -119,3 +119,17 @@ class OracleFluid:
 
     def restore_state(self):
         self.o.restore_state()
+
+
+def offset_grid_case():
+    """tiny with every node but the first moved 2^30 up in y: the MLS moment
+    matrices of the markers near y = 2^30 lose all their digits, so the
+    partition of unity fails."""
+    c = case("tiny")
+    off = 2.0 ** 30
+    c.rc = np.concatenate([[0.0], off + np.linspace(0.0, 6.0, c.params.n2 - 1)])
+    p = cases.FluidParams(**{**c.params.__dict__, "rext2": float(c.rc[-1])})
+    lmcc, lmvel, lmA = c.markers(0.0)
+    lmcc = lmcc.copy()
+    lmcc[:, 0] += off
+    return c, p, (lmcc, lmvel, lmA)

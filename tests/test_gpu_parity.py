@@ -12,7 +12,8 @@ import numpy as np
 import pytest
 
 from fribois2d_amd import cases
-from helpers import case, decomposition, divergence, gpu_for, oracle_for, poisson_kappa, relmax, run_steps
+from helpers import (case, decomposition, divergence, gpu_for, offset_grid_case, oracle_for, poisson_kappa, relmax,
+                     run_steps)
 
 pytestmark = pytest.mark.gpu
 
@@ -279,6 +280,16 @@ def test_error_codes():
     assert rc == 2
     with pytest.raises(gpu.FriboError):
         g.fluid_tsch(0, 10.0)
+    # partition of unity violated (MLSint.F90:157-163): moment matrices of
+    # markers near y = 2^30 on a grid offset there lose all their digits
+    c4, p4, mk = offset_grid_case()
+    o = oracle_for(c4, params=p4)
+    o.set_markers(*mk)
+    assert o.fluid_tsch(0, 1e-3)[0] == 4
+    g = gpu_for(c4, params=p4, axis="z")
+    g.set_markers(*mk)
+    rc, _, _ = g.fluid_tsch_rc(0, 1e-3)
+    assert rc == 4
 
 
 def test_graph_replay_matches_stream_launches():

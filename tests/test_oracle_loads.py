@@ -31,7 +31,7 @@ def uniform_case():
     return cases.Case("uniform", p, rc, zz, verts, (2.0, 2.4), cases._heave(0.05, 0.5), 20261202)
 
 
-def linear_fields(c):
+def linear_fields(c, a=A, b=B, pp=P):
     p = c.params
     n2, n3m = p.n2, p.n3 - 1
     rc, zz = c.rc, c.zz
@@ -43,7 +43,7 @@ def linear_fields(c):
     Y3, Z3 = np.meshgrid(rm, zz[kc], indexing="ij")
     Yp, Zp = np.meshgrid(rm, zm[np.clip(ks, 1, n3m) - 1], indexing="ij")
     f = lambda C, y, z: np.asfortranarray(C[0] + C[1] * y + C[2] * z)
-    return f(A, Y2, Z2), f(B, Y3, Z3), f(P, Yp, Zp)
+    return f(a, Y2, Z2), f(b, Y3, Z3), f(pp, Yp, Zp)
 
 
 def test_loads_closed_form_linear_fields():
@@ -140,3 +140,179 @@ def test_diagnostics_closed_forms():
     o.set("q3", q3s)
     d = o.diagnostics()
     assert abs(d["vorti"] - (s * rm[-1] - s * rm[0])) < 1e-12
+
+
+def test_loads_on_a_stretched_grid_linear_fields():
+    """The pressure closed form of linear fields holds on a stretched grid
+    too: getpbup differences the pressure across the cell-centre spacing
+    dx/g2rc, dx/g3rc (loads_transfer.F90:223-246), which is the pressure
+    grid's own, and the MLS fit reproduces linear functions and their
+    derivatives on any grid.  Also a non-zero wall acceleration, which
+    enters the wall pressure gradient with a minus sign
+    (loads_transfer.F90:110-112).  The shear has no closed form here: the
+    reference differences q2 along y and q3 along z with the same
+    cell-centre spacing, which on a stretched grid is not their node
+    spacing, so their fitted second derivatives do not vanish; with q2
+    varying only along z and q3 only along y every difference is taken
+    across the right spacing and the shear closed form holds as well."""
+    c = uniform_case()
+    s2 = np.linspace(0.0, 1.0, c.params.n2)
+    s3 = np.linspace(0.0, 1.0, c.params.n3)
+    c.rc = c.params.rext2 * (s2 + 0.05 * np.sin(2 * np.pi * s2))
+    c.zz = c.params.alx3 * (s3 + 0.04 * np.sin(2 * np.pi * s3))
+    o = oracle_for(c)
+    lm = c.markers(0.0)
+    o.set_markers(*lm)
+    assert o.fluid_tsch(0, 1e-3)[0] == 0
+    q2, q3, pr = linear_fields(c)
+    ya, yb = linear_fields(c, (A[0], 0.0, A[2]), (B[0], B[1], 0.0))[:2]
+    o.set("q2", q2)
+    o.set("q3", q3)
+    o.set("pr", pr)
+    nor, _ = c.surface(0.0)
+    acc = np.asfortranarray(np.stack([0.3 + 0.1 * np.arange(nor.shape[0]) / nor.shape[0],
+                                      np.full(nor.shape[0], -0.45)], axis=1))
+    kprobe = 1.2
+    rc_, press, tau = o.loads_transfer(kprobe, nor, acc)
+    assert rc_ == 0
+    h = min(np.diff(c.rc).min(), np.diff(c.zz).min())
+    hpr = h * kprobe * kprobe
+    prb = lm[0] + nor * hpr
+    pp = P[0] + P[1] * prb[:, 0] + P[2] * prb[:, 1]
+    dpdnw = -(acc[:, 0] * nor[:, 0] + acc[:, 1] * nor[:, 1])
+    dpdne = P[1] * nor[:, 0] + P[2] * nor[:, 1]
+    np.testing.assert_allclose(press, pp - 0.5 * (dpdnw + dpdne) * hpr, rtol=0, atol=1e-11)
+    # shear: q2 = A0 + A2 z, q3 = B0 + B1 y
+    o.set("q2", ya)
+    o.set("q3", yb)
+    rc_, press, tau = o.loads_transfer(kprobe, nor, acc)
+    assert rc_ == 0
+    uy = A[0] + A[2] * prb[:, 1]
+    uz = B[0] + B[1] * prb[:, 0]
+    vn = uy * nor[:, 0] + uz * nor[:, 1]
+    ty, tz = uy - vn * nor[:, 0], uz - vn * nor[:, 1]
+    tn = np.hypot(ty, tz)
+    ty, tz = ty / tn, tz / tn
+    ducdn = A[2] * ty * nor[:, 1] + B[1] * tz * nor[:, 0]
+    np.testing.assert_allclose(tau[:, 0], c.params.visc * ducdn * ty, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(tau[:, 1], c.params.visc * ducdn * tz, rtol=0, atol=1e-12)
+
+
+def test_wall_shear_extrapolation_from_the_probe():
+    """Probes placed exactly on q2 nodes of a uniform grid (markers at
+    probe - n hpr, oblique normals), q2 = a0 + a1 y + a2 z + a4 y z, q3
+    linear and a quadratic pressure.  The 3x3 MLS support is then symmetric
+    about the probe, so the fit of q2 and its first derivatives are exact
+    there, and the backward differences of quadratic fields are linear, so
+    their fitted derivatives (the second derivatives of getpbup) are exact
+    anywhere.  The wall shear is the Taylor extrapolation of du_t/dn from
+    the probe to the wall, du_t/dn - hpr d2u_t/dn2 (exact for a quadratic
+    profile), with the third-order term the reference takes from the wall
+    pressure, - hpr^2/2 (d2p/dn2)/visc (loads_transfer.F90:113-143)."""
+    c = uniform_case()
+    p = c.params
+    n2, n3m = p.n2, p.n3 - 1
+    rc, zz = c.rc, c.zz
+    h = rc[1] - rc[0]
+    kprobe = 1.1
+    hpr = min(np.diff(rc).min(), np.diff(zz).min()) * kprobe * kprobe
+    zm = 0.5 * (zz[:-1] + zz[1:])
+    # probes on q2 nodes around the middle of the grid
+    js = np.array([14, 17, 20, 23, 26, 18])
+    ks = np.array([22, 25, 28, 31, 26, 33])
+    th = np.array([0.3, 1.1, 2.0, 2.9, 4.0, 5.3])
+    nor = np.asfortranarray(np.stack([np.cos(th), np.sin(th)], axis=1))
+    prb = np.stack([rc[js - 1], zm[ks - 1]], axis=1)
+    lmcc = np.asfortranarray(prb - nor * hpr)
+    nlm = lmcc.shape[0]
+    o = oracle_for(c)
+    o.set_markers(lmcc, np.zeros((nlm, 2), order="F"), np.full(nlm, 0.05))
+    assert o.fluid_tsch(0, 1e-3)[0] == 0
+    a0, a1, a2, a4 = 0.4, 0.2, -0.3, 0.15
+    b0, b1, b2 = 0.7, -0.1, 0.2
+    pc = (0.2, 0.1, -0.2, 0.35, -0.25, 0.45)   # p0 + p1 y + p2 z + p3 y^2 + p4 y z + p5 z^2
+    rm = np.append((rc[:-1] + rc[1:]) * 0.5, rc[-1])
+    zmx = np.append(zm, zz[-1])
+    kk = np.clip(np.arange(-2, n3m + 4), 1, n3m)
+    Y2, Z2 = np.meshgrid(rc, zmx[kk - 1], indexing="ij")
+    Y3, Z3 = np.meshgrid(rm, zz[np.clip(np.arange(-2, n3m + 4), 1, n3m + 1) - 1], indexing="ij")
+    Yp, Zp = np.meshgrid(rm, zmx[kk - 1], indexing="ij")
+    o.set("q2", np.asfortranarray(a0 + a1 * Y2 + a2 * Z2 + a4 * Y2 * Z2))
+    o.set("q3", np.asfortranarray(b0 + b1 * Y3 + b2 * Z3))
+    o.set("pr", np.asfortranarray(pc[0] + pc[1] * Yp + pc[2] * Zp + pc[3] * Yp ** 2 + pc[4] * Yp * Zp
+                                  + pc[5] * Zp ** 2))
+    rc_, press, tau = o.loads_transfer(kprobe, nor, np.zeros((nlm, 2), order="F"))
+    assert rc_ == 0
+    y, z = prb[:, 0], prb[:, 1]
+    ny, nz = nor[:, 0], nor[:, 1]
+    uy, uz = a0 + a1 * y + a2 * z + a4 * y * z, b0 + b1 * y + b2 * z
+    vn = uy * ny + uz * nz
+    ty, tz = uy - vn * ny, uz - vn * nz
+    tn = np.hypot(ty, tz)
+    ty, tz = ty / tn, tz / tn
+    # gradients at the probe: d(q2)/dy, d(q2)/dz, d(q3)/dy, d(q3)/dz
+    g2y, g2z, g3y, g3z = a1 + a4 * z, a2 + a4 * y, b1, b2
+    dut_dn = ty * (g2y * ny + g2z * nz) + tz * (g3y * ny + g3z * nz)
+    d2ut_dn2 = ty * (2.0 * a4 * ny * nz)
+    d2p_dn2 = 2 * pc[3] * ny * ny + 2 * pc[4] * ny * nz + 2 * pc[5] * nz * nz
+    wall = dut_dn - hpr * d2ut_dn2 - 0.5 * hpr * hpr * d2p_dn2 / p.visc
+    assert np.min(np.abs(d2ut_dn2)) > 1e-3
+    np.testing.assert_allclose(tau[:, 0], p.visc * wall * ty, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(tau[:, 1], p.visc * wall * tz, rtol=0, atol=1e-12)
+
+
+def test_zero_velocity_keeps_a_zero_tangent():
+    """No probe velocity at all: no marker ever gets a tangent
+    (loads_transfer.F90:94-96 keeps the previous one, initially zero), so
+    the shear vanishes and nothing is divided by zero."""
+    c = uniform_case()
+    o = oracle_for(c)
+    lm = c.markers(0.0)
+    o.set_markers(*lm)
+    assert o.fluid_tsch(0, 1e-3)[0] == 0
+    n2, n3m = c.params.n2, c.params.n3 - 1
+    o.set("q2", np.zeros((n2, n3m + 6), order="F"))
+    o.set("q3", np.zeros((n2, n3m + 6), order="F"))
+    nor, acc = c.surface(0.0)
+    rc_, press, tau = o.loads_transfer(1.0, nor, acc)
+    assert rc_ == 0 and np.all(tau == 0.0)
+
+
+def test_diagnostics_signs_extrema_and_vorticity_line():
+    """vmin is the signed minimum, divgck the largest |div| (a compression
+    q2 = -s y), the vorticity integral takes dv/dz with a minus sign
+    (q2 = s z gives -s rext2) on the node line nearest z = 18.5, and a q3
+    shear growing with z is read on that line (vcheck.F90, divergence.F90:
+    49-78, utils_proc.F90:582-629)."""
+    c = cases.make_case("cyl384")
+    o = oracle_for(c)
+    p = c.params
+    n2, n3m = p.n2, p.n3 - 1
+    rng = np.random.default_rng(21)
+    q2 = np.asfortranarray(rng.standard_normal((n2, n3m + 6)))
+    q3 = np.asfortranarray(rng.standard_normal((n2, n3m + 6)) - 0.5)
+    o.set("q2", q2)
+    o.set("q3", q3)
+    d = o.diagnostics()
+    inner = (slice(0, n2 - 1), slice(3, 3 + n3m))
+    assert d["vmin2"] == q2[inner].min() and d["vmin3"] == q3[inner].min()
+    assert d["vmax2"] == np.abs(q2[inner]).max() and d["vmax3"] == np.abs(q3[inner]).max()
+    rc, zz = c.rc, c.zz
+    s = 0.21
+    o.set("q2", np.asfortranarray(np.repeat((-s * rc)[:, None], n3m + 6, axis=1)))
+    o.set("q3", np.zeros((n2, n3m + 6), order="F"))
+    d = o.diagnostics()
+    assert abs(d["divmax"] - s) < 1e-12
+    zm = np.append(0.5 * (zz[:-1] + zz[1:]), zz[-1])
+    kk = np.clip(np.arange(-2, n3m + 4), 1, n3m + 1)
+    o.set("q2", np.asfortranarray(np.repeat((s * zm[kk - 1])[None, :], n2, axis=0)))
+    d = o.diagnostics()
+    indz = int(np.argmin(np.abs(zz - 18.5))) + 1
+    dz_line = zm[indz] - zm[indz - 1]
+    assert abs(d["vorti"] - (-s * (zm[indz] - zm[indz - 1]) / (zz[indz] - zz[indz - 1]) * p.rext2)) < 1e-12
+    rm = np.append((rc[:-1] + rc[1:]) * 0.5, rc[-1])
+    zkk = zz[np.clip(np.arange(-2, n3m + 4), 1, n3m + 1) - 1]
+    o.set("q2", np.zeros((n2, n3m + 6), order="F"))
+    o.set("q3", np.asfortranarray(rm[:, None] * zkk[None, :]))
+    d = o.diagnostics()
+    assert abs(d["vorti"] - zz[indz - 1] * (rm[-1] - rm[0])) < 1e-11

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from fribois2d_amd import cases
-from helpers import case, oracle_for
+from helpers import case, offset_grid_case, oracle_for
 
 def _params(c, **kw):
     d = dict(c.params.__dict__)
@@ -574,3 +574,187 @@ def test_self_advection_flux_form_on_a_stretched_grid():
     q = a + b * c.rc
     exp2 = -ga * b * (q[j] + 2 * q[j - 1] + q[j - 2]) / 2 * dt
     np.testing.assert_allclose(o.get("rhs2")[j - 1, :], np.broadcast_to(exp2[:, None], (j.size, n3m)), rtol=1e-10)
+
+
+# -- time discretisation (fluid_tsch.F90:34-77) --------------------------------
+
+def test_viscous_half_step_is_crank_nicolson():
+    """q3 = f(y) constant in z, with f an eigenvector (eigenvalue lam) of
+    the discrete slip-wall operator L_y assembled here from the geometry, and
+    q2 = pr = ru = 0: the substep's explicit viscous term is z f with
+    z = al dt lam / Re, and the implicit j-solve divides by (1 - al beta lam).
+    With beta as set by the step's dt control, the increment away from the
+    inflow and outflow rows is (R(z) - 1) f with R(z) = (1 + z/2)/(1 - z/2),
+    the (1,1) Pade approximant of exp(z): the Crank-Nicolson weight
+    beta = dt / (2 Re)."""
+    c = case("tiny")
+    p = _params(c, ibact=False, idtv=True, cfllim=1e9)
+    o = oracle_for(c, params=p)
+    _blank(o, p)
+    n2, n2m, n3m = p.n2, p.n2 - 1, p.n3 - 1
+    rm = _centres(c.rc)
+    Ly = _second_difference(rm, c.rc, 0.0, 0.0)
+    lam, vec = np.linalg.eig(Ly)
+    order = np.argsort(lam.real)
+    i = order[n2m // 3]
+    lam_i, f = float(lam[i].real), vec[:, i].real
+    f = 0.05 * f / np.max(np.abs(f))
+    q3 = np.zeros((n2, n3m + 6), order="F")
+    q3[:n2m, :] = f[:, None]
+    o.set("q3", q3)
+    o.set("qb3s", np.append(f, 0.0))
+    o.set("qb3n", np.append(f, 0.0))
+    dt = 1.5
+    rc, dt_out, _ = o.cfl_dt(3, dt)
+    assert rc == 0 and dt_out == dt
+    sc = o.get("scalars")
+    al = 8.0 / 15.0
+    o.set_scalars(dt, sc[1], al, al, 0.0)
+    assert o.stage("explicit") == 0
+    assert o.stage("helsolve") == 0
+    z = al * dt * lam_i / sc[11]
+    assert -2.0 < z < -0.1
+    R = (1.0 + z / 2.0) / (1.0 - z / 2.0)
+    mid = slice(3 + n3m // 2 - 2, 3 + n3m // 2 + 2)
+    d = o.get("q3")[:n2m, mid] - q3[:n2m, mid]
+    np.testing.assert_allclose(d, np.broadcast_to(((R - 1.0) * f)[:, None], d.shape), rtol=0,
+                               atol=1e-12 * np.max(np.abs(f)))
+
+
+def test_dt_control_takes_dtmax_for_slow_flow():
+    """fluid_tsch.F90:38-42: dt = min(cflmax / cflm, dtmax)."""
+    c = case("tiny")
+    for dtmax in (0.01, 1e9):
+        p = _params(c, ibact=False, dtmax=dtmax)
+        o = oracle_for(c, params=p)
+        n2, n3m = p.n2, p.n3 - 1
+        o.set("q2", np.zeros((n2, n3m + 6), order="F"))
+        o.set("q3", np.full((n2, n3m + 6), 1e-3, order="F"))
+        rc, dt, cflm = o.cfl_dt(2, 1e-3)
+        assert rc == 0
+        h = c.zz[1] - c.zz[0]
+        assert abs(cflm - 1e-3 / h) < 1e-15
+        assert dt == min(p.cflmax / cflm, dtmax)
+
+
+def _lagrange_weights(nodes, a, b):
+    """Integrals over [a, b] of the Lagrange basis polynomials on `nodes`."""
+    w = []
+    for i in range(len(nodes)):
+        others = [x for j, x in enumerate(nodes) if j != i]
+        poly = np.poly1d([1.0])
+        for x in others:
+            poly = poly * np.poly1d([1.0, -x]) / (nodes[i] - x)
+        P = poly.integ()
+        w.append(P(b) - P(a))
+    return w
+
+
+def test_adams_bashforth_weights_for_a_variable_step():
+    """nsst = 1 (fluid_tsch.F90:66-76): ga and ro are the variable-step
+    Adams-Bashforth weights, i.e. (1/dt) times the integrals over
+    [t_n, t_n + dt] of the Lagrange basis on (t_n - dt_o, t_n); the first
+    step (ns = 0) is forward Euler."""
+    c = case("tiny")
+    p = _params(c, ibact=False, nsst=1, idtv=True)
+    o = oracle_for(c, params=p)
+    for ns, dt, dt_o in ((0, 1e-3, 1e-3), (4, 1.3e-3, 0.8e-3), (9, 0.5e-3, 1.1e-3)):
+        assert o.cfl_dt(ns, dt, dt_o)[0] == 0
+        assert o.substep(1, ns) == 0
+        sc = o.get("scalars")
+        if ns == 0:
+            w_old, w_new = 0.0, 1.0
+        else:
+            w_old, w_new = (x / dt for x in _lagrange_weights([-dt_o, 0.0], 0.0, dt))
+        assert abs(sc[3] - w_new) < 1e-14 and abs(sc[4] - w_old) < 1e-14
+        assert abs(sc[2] - (sc[3] + sc[4])) < 1e-15
+
+
+def test_rk3_substeps_are_third_order():
+    """nsst = 3: y' = F applied as y += dt (ga_n F(y) + ro_n F(y_prev)) over
+    the three substeps, with the substep scalars the step uses, multiplies
+    y by 1 + z + z^2/2 + z^3/6 for F = lam y (z = lam dt), and the al_n sum
+    to one (the substep times add up to dt)."""
+    c = case("tiny")
+    p = _params(c, ibact=False, idtv=True)
+    o = oracle_for(c, params=p)
+    dt = 1e-3
+    assert o.cfl_dt(1, dt)[0] == 0
+    g, r, a = [], [], []
+    for n in (1, 2, 3):
+        assert o.substep(n, 1) == 0
+        sc = o.get("scalars")
+        a.append(float(sc[2])); g.append(float(sc[3])); r.append(float(sc[4]))
+    assert abs(sum(a) - 1.0) < 1e-15
+    z = np.poly1d([1.0, 0.0])
+    y, y_prev = np.poly1d([1.0]), np.poly1d([0.0])
+    for n in range(3):
+        y, y_prev = y + g[n] * z * y + r[n] * z * y_prev, y
+    np.testing.assert_allclose(y.coeffs[::-1], [1.0, 1.0, 0.5, 1.0 / 6.0], rtol=0, atol=1e-15)
+    for n in range(3):
+        np.testing.assert_allclose(a[n], g[n] + r[n], rtol=0, atol=1e-16)
+
+
+def test_marker_ties_take_the_first_node():
+    """lgrdistribution.F90:38-67 searches with MINLOC, which returns the
+    first of equal minima: on a dyadic grid a marker exactly half-way
+    between two nodes takes the lower one (all distances are exact)."""
+    c = case("tiny")
+    h = 0.25
+    c.rc = np.arange(c.params.n2) * h
+    c.zz = np.arange(c.params.n3) * h
+    p = _params(c, rext2=float(c.rc[-1]), alx3=float(c.zz[-1]))
+    o = oracle_for(c, params=p)
+    # y half-way between nodes 5 and 6 of rc (1-based) and on a node of rm;
+    # z on a node of zz and half-way between cells 8 and 9 of zm
+    lmcc = np.asfortranarray([[1.125, 2.0], [2.375, 1.5], [1.0, 2.125]])
+    o.set_markers(lmcc, np.zeros((3, 2), order="F"), np.full(3, 0.1))
+    assert o.stage("lgrdistribution") == 0
+    lmind = o.get("lmind").reshape((3, 2, 2), order="F")
+    rm, zm = _centres(c.rc), _centres(c.zz)
+    for n, (y, z) in enumerate(lmcc):
+        assert lmind[n, 0, 0] == np.argmin(np.abs(c.rc - y)) + 1
+        assert lmind[n, 0, 1] == np.argmin(np.abs(zm - z)) + 1
+        assert lmind[n, 1, 0] == np.argmin(np.abs(rm - y)) + 1
+        assert lmind[n, 1, 1] == np.argmin(np.abs(c.zz - z)) + 1
+    assert lmind[0, 0, 0] == 5 and lmind[0, 0, 1] == 8
+    assert lmind[1, 0, 0] == 10 and lmind[1, 1, 1] == 7
+    assert lmind[2, 1, 0] == 4 and lmind[2, 1, 1] == 9
+
+
+# -- the aborts of stopcase (fluid_tsch.F90:43-54, lgrdistribution.F90:32-36,
+#    MLSint.F90:157-163) ----------------------------------------------------------
+
+def test_oracle_error_codes():
+    c = case("tiny")
+    lmcc, lmvel, lmA = c.markers(0.0)
+    # a marker below z = 0
+    o = oracle_for(c)
+    bad = lmcc.copy()
+    bad[0, 1] = -0.5
+    o.set_markers(bad, lmvel, lmA)
+    assert o.fluid_tsch(0, 1e-3)[0] == 3
+    # cflmax so small that dt < 1e-8
+    o = oracle_for(c, params=_params(c, cflmax=1e-12))
+    o.set_markers(lmcc, lmvel, lmA)
+    assert o.fluid_tsch(1, 1e-3)[0] == 1
+    # fixed dt with cflm dt above cfllim; at ns = 0 cflm is reported times dt
+    o = oracle_for(c, params=_params(c, idtv=True))
+    o.set_markers(lmcc, lmvel, lmA)
+    rc, dt, cflm = o.fluid_tsch(0, 10.0)
+    assert rc == 2 and cflm > c.params.cfllim
+    o = oracle_for(c, params=_params(c, idtv=True))
+    rc, dt, cflm_small = o.cfl_dt(0, 1e-3)
+    assert rc == 0 and abs(cflm_small / cflm - 1e-4) < 1e-15
+    o = oracle_for(c)
+    rc, dt, cflm0 = o.cfl_dt(0, 1e-3)
+    assert rc == 0 and dt == 1e-3 and abs(cflm0 / cflm - 1e-4) < 1e-15
+    # partition of unity
+    c2, p2, mk = offset_grid_case()
+    o = oracle_for(c2, params=p2)
+    o.set_markers(*mk)
+    assert o.fluid_tsch(0, 1e-3)[0] == 4
+    # and in the loads' shape functions (MLSint.F90:560-566)
+    nor = np.zeros_like(mk[0])
+    nor[:, 0] = 1.0
+    assert o.loads_transfer(1.0, nor, np.zeros_like(nor))[0] == 4
