@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -81,6 +82,8 @@ struct fribo_ctx {
     int vort_k = 0;          // plane of getintvort's z = 18.5 line (0: none)
     hipGraphExec_t gexec = nullptr;
     int graph_nlm = -1;
+    int dev = 0;                 // device of the handle
+    double* scratch = nullptr;   // device address of the static loads scratch on that device
 };
 
 template <class T>
@@ -346,6 +349,35 @@ static int setup_ytransform(fribo_ctx* c, const double* zmmt, int ldz)
     return FRIBO_OK;
 }
 
+// Static device scratch of fribo_loads_transfer_static: the probe inputs
+// lmnor, lmacc (2 nlm each), the outputs press (nlm) and tau (2 nlm) and the
+// per-marker work records (FRIBO_LWORK nlm) for up to
+// FRIBO_STATIC_LOADS_MAX_NLM markers, 1.8 MB held on every device the
+// library runs on until the process ends.  Host code never passes the
+// array's name to a kernel or a copy: it uses the device address that
+// hipGetSymbolAddress returns, once per device, kept in g_scratch_addr.
+static __device__ __align__(256) double scratch[(size_t)FRIBO_STATIC_LOADS_MAX_NLM * (7 + FRIBO_LWORK)];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_addr[kMaxDevices] = {nullptr};
+
+// Device address of `scratch` on device `dev`, which must be current.
+static int scratch_address(int dev, double** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return set_err(FRIBO_ERR_LIMIT, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        const hipError_t he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (he != hipSuccess || !p)
+            return set_err(FRIBO_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(he));
+        g_scratch_addr[dev] = (double*)p;
+    }
+    *out = g_scratch_addr[dev];
+    return FRIBO_OK;
+}
+
 extern "C" int fribo_init(const fribo_params* p, const double* rc, const double* zz, const double* zmm,
                           const double* zmmt, int ldz, const double* wr, fribo_handle* out)
 {
@@ -375,8 +407,12 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
+    double* scratch_dev = nullptr;
+    if (int se = scratch_address(dev, &scratch_dev)) return se;
     fribo_ctx* c = new fribo_ctx();
     c->p = *p;
+    c->dev = dev;
+    c->scratch = scratch_dev;
     c->lds_limit = prop.sharedMemPerBlock;
     FluidDev& F = c->F;
     memset(&F, 0, sizeof(F));
@@ -1170,6 +1206,40 @@ extern "C" int fribo_loads_transfer(fribo_handle h, double kprobe, const double*
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(press, F.press, nb, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipMemcpyAsync(tau, F.tau, 2 * nb, hipMemcpyDeviceToHost, h->st));
+    int e = read_scalars(h);
+    if (e) return e;
+    if (h->hs.err) return set_err(h->hs.err, "MLS partition of unity violated at a load probe");
+    return FRIBO_OK;
+}
+
+// fribo_loads_transfer with every device buffer of the call (probe inputs,
+// loads, per-marker work records) in the static scratch of the handle's
+// device: no allocation.  Same kernels, same results.
+extern "C" int fribo_loads_transfer_static(fribo_handle h, double kprobe, const double* lmnor, const double* lmacc,
+                                           double* press, double* tau)
+{
+    if (!h || !lmnor || !lmacc || !press || !tau) return set_err(FRIBO_ERR_ARG, "null argument");
+    if (h->F.nlm <= 0) return set_err(FRIBO_ERR_ARG, "no markers set");
+    if (h->F.nlm > FRIBO_STATIC_LOADS_MAX_NLM)
+        return set_err(FRIBO_ERR_LIMIT, "fribo_loads_transfer_static: " + std::to_string(h->F.nlm) +
+                                            " markers, the static scratch holds at most " +
+                                            std::to_string(FRIBO_STATIC_LOADS_MAX_NLM));
+    if (!h->scratch) return set_err(FRIBO_ERR_HIP, "static scratch address not set");
+    FluidDev G = h->F;
+    const size_t n = (size_t)G.nlm;
+    G.lmnor = h->scratch;
+    G.lmacc = G.lmnor + 2 * n;
+    G.press = G.lmacc + 2 * n;
+    G.tau = G.press + n;
+    G.lwork = G.tau + 2 * n;
+    const size_t nb = n * sizeof(double);
+    HIPCHK(hipMemcpyAsync(G.lmnor, lmnor, 2 * nb, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(G.lmacc, lmacc, 2 * nb, hipMemcpyHostToDevice, h->st));
+    const double hpr = (h->dmin * kprobe) * kprobe;
+    launch_loads(G, hpr, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(press, G.press, nb, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(tau, G.tau, 2 * nb, hipMemcpyDeviceToHost, h->st));
     int e = read_scalars(h);
     if (e) return e;
     if (h->hs.err) return set_err(h->hs.err, "MLS partition of unity violated at a load probe");

@@ -14,6 +14,7 @@
 #define FRIBO_KMAXCH 64      // max chunks of a partitioned k-line (one wave)
 #define FRIBO_NPART 512      // partial sums of the CFL reduction
 #define FRIBO_LWORK 20       // per-marker record of the load computation
+#define FRIBO_STATIC_LOADS_MAX_NLM 8192   // markers fribo_loads_transfer_static serves
 
 // Marching tiles of the stencil kernels (k_rhs, k_div, k_upd): a wave owns
 // FRIBO_WJ output columns (plus a halo lane each side) and MK planes

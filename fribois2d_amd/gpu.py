@@ -28,7 +28,7 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
            "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform", "fribo_loads_transfer",
            "fribo_checkbcs", "fribo_diagnostics", "fribo_save_state", "fribo_restore_state",
-           "fribo_phini", "fribo_phini_device"]
+           "fribo_phini", "fribo_phini_device", "fribo_loads_transfer_static"]
 
 NPROF = 12
 
@@ -95,6 +95,7 @@ def load():
     L.fribo_get_transform.argtypes = [vp]
     L.fribo_test_ytransform.argtypes = [C.c_int32, dp, dp, dp, dp, dp, dp]
     L.fribo_loads_transfer.argtypes = [vp, C.c_double, dp, dp, dp, dp]
+    L.fribo_loads_transfer_static.argtypes = [vp, C.c_double, dp, dp, dp, dp]
     L.fribo_checkbcs.argtypes = [vp, dp, dp, dp, dp]
     L.fribo_diagnostics.argtypes = [vp, dp]
     L.fribo_save_state.argtypes = [vp]

@@ -125,6 +125,18 @@ int fribo_sync(fribo_handle h);
 int fribo_loads_transfer(fribo_handle h, double kprobe, const double* lmnor, const double* lmacc,
                          double* press, double* tau);
 
+/* No counterpart in the reference: fribo_loads_transfer for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * coupling loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results; every device buffer of the call lives in a static device
+ * array of the library (1.8 MB per device, held until the process ends), so
+ * it serves at most 8192 markers and returns FRIBO_ERR_LIMIT above that.
+ * Calls on one device share that array: calls that overlap on one GPU (two
+ * host threads, two handles on one device) race and must be serialised by
+ * the caller. */
+int fribo_loads_transfer_static(fribo_handle h, double kprobe, const double* lmnor, const double* lmacc,
+                                double* press, double* tau);
+
 /* The per-step diagnostics of SUBROUTINE ttime (ttime.F90:29-69,136) on the
  * device, so that the fields need not cross PCIe for them: out[9] =
  * {vmaxv(1), vmaxv(2) (vcheck.F90:3-36), vminv(1), vminv(2)
