@@ -400,6 +400,26 @@ def make_case(name: str, instance: int = 0) -> Case:
     raise KeyError(name)
 
 
+def ensemble_case(instance: int, n3m: int, n2m: int, re: float, amp: float = 0.1, freq: float = 0.2) -> Case:
+    """One member of a batched ensemble: cylinder D = 1 at (y, z) = (15, 10)
+    in [0, 30] x [0, 40] on an n3m x n2m grid whose uniform core covers
+    z in [8, 16], y in [13, 17] with about 83% / 55% of the nodes (the
+    proportions of ``cyl384``), Reynolds number re, heaving with amplitude
+    amp and frequency freq; markers about pi D / dx, clipped to 63..250."""
+    n3, n2 = n3m + 1, n2m + 1
+    zc = max(int(round(0.83 * n3)), 8)
+    zb = max(int(round(0.045 * n3)), 3)
+    yc = max(int(round(0.55 * n2)), 8)
+    yb = max((n2 - yc) // 2, 3)
+    zz = cubicstr([8.0, 16.0], [zb, zc, n3 - zb - zc], 0.0, 40.0)
+    rc = cubicstr([13.0, 17.0], [yb, yc, n2 - yb - yc], 0.0, 30.0)
+    dx = 8.0 / (zc - 1)
+    nlm = int(min(250, max(63, round(math.pi / dx))))
+    p = FluidParams(n2=rc.size, n3=zz.size, alx3=40.0, rext2=30.0, visc=1.0 / re)
+    verts = cylinder_vertices(15.0, 10.0, 0.5, nlm)
+    return Case(f"ens{instance}", p, rc, zz, verts, (15.0, 10.0), _heave(amp, freq), 20261202 + instance)
+
+
 def reference_example(directory, which="osccyl"):
     """The reference's own example inputs (test_OscCyl1DOF / test_GallRect1DOF).
 

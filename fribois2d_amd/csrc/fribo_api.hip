@@ -5,6 +5,7 @@
 // synchronisation per step is the final read-back of dt / cflm / the error
 // flag (the values the Fortran driver needs before rigid() and ttime()).
 #include "fribo_internal.h"
+#include "fribo_batch.h"
 #include "../../include/fribo_gpu.h"
 
 #include <algorithm>
@@ -82,6 +83,8 @@ struct fribo_ctx {
     int vort_k = 0;          // plane of getintvort's z = 18.5 line (0: none)
     hipGraphExec_t gexec = nullptr;
     int graph_nlm = -1;
+    int mk_gen = 0;              // bumped whenever the marker buffers are reallocated
+    struct fribo_batch_ctx* batch = nullptr;   // batch this instance belongs to (shares its stream)
     int dev = 0;                 // device of the handle
     double* scratch = nullptr;   // device address of the static loads scratch on that device
 };
@@ -336,7 +339,8 @@ static int setup_ytransform(fribo_ctx* c, const double* zmmt, int ldz)
     if (!e) e = dalloc(c, &F.zg, F.n3m);
     if (!e) e = dalloc(c, &F.ysum, (size_t)7 * n * FRIBO_KMAXCH);
     F.ngpart = gemm_nn_tiles_n(n, F.n3m, n, F.ymm, F.ldy, F.hmn, F.P);
-    if (!e) e = dalloc(c, &F.gpart, F.ngpart);
+    // room for the gauge partials of any tile shape (a batch may use 32-column tiles)
+    if (!e) e = dalloc(c, &F.gpart, std::max(F.ngpart, (F.n3m + 31) / 32));
     if (e) return e;
     HIPCHK(hipMemcpy(F.ymm, Y.data(), Y.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(F.ymmi, Yi.data(), Yi.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -552,6 +556,7 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
 extern "C" int fribo_finalize(fribo_handle h)
 {
     if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    if (h->batch) return set_err(FRIBO_ERR_ARG, "instance belongs to a batch: destroy the batch first");
     hipStreamSynchronize(h->st);
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->in) hipHostFree(h->in);
@@ -687,6 +692,7 @@ extern "C" int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc
             h->gexec = nullptr;
         }
         h->graph_nlm = -1;
+        ++h->mk_gen;
         for (void* q : h->mk_allocs) hipFree(q);
         h->mk_allocs.clear();
         if (h->mk_stage) hipHostFree(h->mk_stage);
@@ -1383,4 +1389,471 @@ extern "C" int fribo_reset_profile(fribo_handle h)
 extern "C" const char* fribo_profile_name(int32_t cls)
 {
     return (cls >= 0 && cls < FRIBO_NPROF) ? kProfNames[cls] : "";
+}
+
+// ---------------------------------------------------------------------------
+// Batches of independent instances (no counterpart in the reference: many
+// runs of the fluid step, on uneven grids, advanced together).  Every stage
+// of the step is one launch over all instances (fribo_batch.h); the
+// instances keep their own device state, created by fribo_init and filled
+// by the single-instance calls, and share the batch's stream.
+
+struct BatchOut {
+    double dt, cflm;
+    int err, pad;
+};
+
+__global__ __launch_bounds__(64) void k_batch_inputs(const FluidDev* __restrict__ F,
+                                                      const double* __restrict__ in, int n)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    DevScalars* S = F[i].S;
+    S->in_dt = in[3 * i];
+    S->in_dto = in[3 * i + 1];
+    const long long w = __double_as_longlong(in[3 * i + 2]);
+    S->in_ns = (int)(w & 0xffffffff);
+    S->err = (int)(w >> 32);
+}
+
+__global__ __launch_bounds__(64) void k_batch_outputs(const FluidDev* __restrict__ F, BatchOut* __restrict__ out,
+                                                       int n)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    const DevScalars* S = F[i].S;
+    out[i] = BatchOut{S->dt, S->cflm, S->err, 0};
+}
+
+// marker kinematics of all instances from one staged copy: instance
+// blockIdx.y takes n5[i] doubles from src + off[i] into its lmcc buffer
+__global__ __launch_bounds__(256) void k_batch_markers(const FluidDev* __restrict__ F, const long* __restrict__ off,
+                                                       const double* __restrict__ src)
+{
+    const int i = blockIdx.y;
+    const long n5 = 5 * (long)F[i].nlm;
+    double* dst = F[i].lmcc;
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n5; q += (long)gridDim.x * 256) dst[q] = src[off[i] + q];
+}
+
+struct BTable {
+    std::vector<int> dims;
+    std::vector<GemmProb> probs;
+    int* dbuf = nullptr;         // off [n+1], dims [3n]
+    GemmProb* dprob = nullptr;
+};
+
+struct fribo_batch_ctx {
+    std::vector<fribo_handle> inst;
+    std::vector<int> gen;              // marker generation of each instance at the last refresh
+    BatchSet B;
+    hipStream_t st = nullptr;
+    BTable tab[BK_COUNT];
+    int jclass = 8;                    // j-line chunk class of the batch
+    int gemm_fixed = -1;               // tile shape set by the caller or the autotuner (-1: default)
+    double* in_h = nullptr;            // pinned step inputs [3n]
+    double* in_d = nullptr;
+    BatchOut* out_h = nullptr;         // pinned step outputs [n]
+    BatchOut* out_d = nullptr;
+    // batched marker upload
+    double* mk_h = nullptr;
+    double* mk_d = nullptr;
+    long* mk_off = nullptr;
+    long mk_cap = 0;
+    hipEvent_t ev_mk = nullptr;
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    bool warm = false;                 // an eager step ran with the current tables
+};
+
+static void batch_drop_graph(fribo_batch_ctx* b);
+
+static BGrid<FluidDev> batch_grid(BatchSet* B, int key, const std::vector<int>& dims)
+{
+    fribo_batch_ctx* b = (fribo_batch_ctx*)B->owner;
+    BTable& T = b->tab[key];
+    if (!T.dbuf || T.dims != dims) {
+        if (T.dbuf) hipFree(T.dbuf);
+        T.dbuf = nullptr;
+        std::vector<int> h(B->n + 1 + dims.size());
+        int acc = 0;
+        for (int i = 0; i < B->n; ++i) {
+            h[i] = acc;
+            acc += dims[3 * i] * dims[3 * i + 1] * dims[3 * i + 2];
+        }
+        h[B->n] = acc;
+        std::copy(dims.begin(), dims.end(), h.begin() + B->n + 1);
+        if (hipMalloc(&T.dbuf, h.size() * sizeof(int)) != hipSuccess ||
+            hipMemcpy(T.dbuf, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            T.dbuf = nullptr;
+            return BGrid<FluidDev>{B->dF, nullptr, nullptr, 0};
+        }
+        T.dims = dims;
+        batch_drop_graph(b);
+    }
+    return BGrid<FluidDev>{B->dF, T.dbuf, T.dbuf + B->n + 1, B->n};
+}
+
+static bool same_prob(const GemmProb& a, const GemmProb& b)
+{
+    return a.M == b.M && a.N == b.N && a.K == b.K && a.lda == b.lda && a.ldb == b.ldb && a.ldc == b.ldc &&
+           a.A == b.A && a.B == b.B && a.C == b.C && a.S == b.S && a.zg == b.zg && a.gpart == b.gpart &&
+           a.yf.wy == b.yf.wy && a.yf.amphk == b.yf.amphk && a.yf.acphk == b.yf.acphk && a.yf.apphk == b.yf.apphk &&
+           a.yf.ysum == b.yf.ysum && a.yf.n2m == b.yf.n2m && a.yf.csz == b.yf.csz;
+}
+
+static bool same_probs(const std::vector<GemmProb>& a, const std::vector<GemmProb>& b)
+{
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (!same_prob(a[i], b[i])) return false;
+    return true;
+}
+
+// A rebuilt table invalidates the captured step, whose kernel arguments
+// point to the old one.
+static void batch_drop_graph(fribo_batch_ctx* b)
+{
+    if (b->gexec) {
+        hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    b->warm = false;
+}
+
+static BGrid<GemmProb> batch_gemm(BatchSet* B, int key, const std::vector<GemmProb>& probs,
+                                  const std::vector<int>& dims)
+{
+    fribo_batch_ctx* b = (fribo_batch_ctx*)B->owner;
+    BTable& T = b->tab[key];
+    const BGrid<FluidDev> g = batch_grid(B, key, dims);
+    if (!g.off) return BGrid<GemmProb>{nullptr, nullptr, nullptr, 0};
+    if (!T.dprob || !same_probs(T.probs, probs)) {
+        if (T.dprob) hipFree(T.dprob);
+        T.dprob = nullptr;
+        if (hipMalloc(&T.dprob, probs.size() * sizeof(GemmProb)) != hipSuccess ||
+            hipMemcpy(T.dprob, probs.data(), probs.size() * sizeof(GemmProb), hipMemcpyHostToDevice) != hipSuccess) {
+            T.dprob = nullptr;
+            return BGrid<GemmProb>{nullptr, nullptr, nullptr, 0};
+        }
+        T.probs = probs;
+        batch_drop_graph(b);
+    }
+    return BGrid<GemmProb>{T.dprob, g.off, g.dims, B->n};
+}
+
+// Default tile shape of the batch's transforms: the one the largest
+// instance would take on its own, among those every instance can run.
+static int batch_default_gemm(const fribo_batch_ctx* b)
+{
+    int mask = 15;
+    long best = -1;
+    const FluidDev* big = nullptr;
+    for (auto h : b->inst) {
+        mask &= gemm_cfg_mask(h->F);
+        const long cells = (long)h->F.n2m * h->F.n3m;
+        if (cells > best) {
+            best = cells;
+            big = &h->F;
+        }
+    }
+    const int M = big->n2m, N = big->n3m;
+    const auto cdiv = [](int a, int c) { return (a + c - 1) / c; };
+    int cfg = 0;
+    if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) cfg = (mask & 8) ? 3 : 2;
+    else if ((long)cdiv(M, 64) * cdiv(N, 64) >= 96) cfg = 1;
+    while (cfg > 0 && !((mask >> cfg) & 1)) --cfg;
+    return cfg;
+}
+
+// The instances' FluidDev as the batched kernels see them: the j-line
+// staging stride of the batch's chunk class, the gauge partial count of its
+// tile shape.  Uploaded as one array.
+static int batch_refresh(fribo_batch_ctx* b)
+{
+    BatchSet& B = b->B;
+    B.gemm_cfg = b->gemm_fixed >= 0 ? b->gemm_fixed : batch_default_gemm(b);
+    bool fuse = B.gemm_cfg == 3;
+    for (auto h : b->inst) fuse = fuse && gemm_fuse_ok(h->F);
+    B.ysolve_fused = fuse;
+    for (int i = 0; i < B.n; ++i) {
+        FluidDev F = b->inst[i]->F;
+        F.jstride = jsolve_stride_for(F, b->jclass);
+        if (F.taxis == 1) F.ngpart = gemm_cfg_parts(F, B.gemm_cfg);
+        B.hF[i] = F;
+        b->gen[i] = b->inst[i]->mk_gen;
+    }
+    HIPCHK(hipMemcpyAsync(B.dF, B.hF.data(), B.n * sizeof(FluidDev), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    if (b->gexec) {
+        hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    b->warm = false;
+    return FRIBO_OK;
+}
+
+static void batch_free(fribo_batch_ctx* b)
+{
+    if (b->st) hipStreamSynchronize(b->st);
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    for (auto& T : b->tab) {
+        if (T.dbuf) hipFree(T.dbuf);
+        if (T.dprob) hipFree(T.dprob);
+    }
+    if (b->B.dF) hipFree(b->B.dF);
+    if (b->in_h) hipHostFree(b->in_h);
+    if (b->in_d) hipFree(b->in_d);
+    if (b->out_h) hipHostFree(b->out_h);
+    if (b->out_d) hipFree(b->out_d);
+    if (b->mk_h) hipHostFree(b->mk_h);
+    if (b->mk_d) hipFree(b->mk_d);
+    if (b->mk_off) hipFree(b->mk_off);
+    if (b->ev_mk) hipEventDestroy(b->ev_mk);
+    // the instances get streams of their own again
+    for (auto h : b->inst) {
+        if (h->batch != b) continue;
+        hipStreamCreate(&h->st);
+        h->batch = nullptr;
+    }
+    if (b->st) hipStreamDestroy(b->st);
+    delete b;
+}
+
+extern "C" int fribo_batch_create(const fribo_handle* hs, int32_t n, fribo_batch* out)
+{
+    if (!hs || !out || n < 1) return set_err(FRIBO_ERR_ARG, "bad batch arguments");
+    *out = nullptr;
+    bool all_y = true;
+    for (int i = 0; i < n; ++i) {
+        fribo_handle h = hs[i];
+        if (!h) return set_err(FRIBO_ERR_ARG, "null instance handle");
+        if (h->batch) return set_err(FRIBO_ERR_ARG, "instance already belongs to a batch");
+        for (int j = 0; j < i; ++j)
+            if (hs[j] == h) return set_err(FRIBO_ERR_ARG, "instance listed twice");
+        if (h->dev != hs[0]->dev) return set_err(FRIBO_ERR_ARG, "instances on different devices");
+        if (h->F.nsst != hs[0]->F.nsst) return set_err(FRIBO_ERR_ARG, "instances with different nsst");
+        if (!mls_batchable(h->F)) return set_err(FRIBO_ERR_LIMIT, "marker count beyond the batched marker order");
+        all_y = all_y && h->yavail;
+    }
+    fribo_batch_ctx* b = new fribo_batch_ctx();
+    b->inst.assign(hs, hs + n);
+    b->gen.assign(n, -1);
+    b->B.n = n;
+    b->B.hF.resize(n);
+    b->B.owner = b;
+    b->B.grid_fn = batch_grid;
+    b->B.gemm_fn = batch_gemm;
+    for (auto h : b->inst) b->jclass = std::max(b->jclass, jsolve_class_h(h->F));
+    int e = 0;
+    hipError_t he = hipStreamCreate(&b->st);
+    if (he == hipSuccess) he = hipMalloc(&b->B.dF, n * sizeof(FluidDev));
+    if (he == hipSuccess) he = hipHostMalloc((void**)&b->in_h, 3 * (size_t)n * sizeof(double), hipHostMallocDefault);
+    if (he == hipSuccess) he = hipMalloc(&b->in_d, 3 * (size_t)n * sizeof(double));
+    if (he == hipSuccess) he = hipHostMalloc((void**)&b->out_h, n * sizeof(BatchOut), hipHostMallocDefault);
+    if (he == hipSuccess) he = hipMalloc(&b->out_d, n * sizeof(BatchOut));
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&b->ev_mk, hipEventDisableTiming);
+    if (he != hipSuccess) {
+        e = set_err(FRIBO_ERR_HIP, hipGetErrorString(he));
+        b->inst.clear();
+        batch_free(b);
+        return e;
+    }
+    // one stream for the batch and its instances; every instance takes the
+    // same transform axis
+    for (auto h : b->inst) {
+        hipStreamSynchronize(h->st);
+        if (h->gexec) {
+            hipGraphExecDestroy(h->gexec);
+            h->gexec = nullptr;
+            h->graph_nlm = -1;
+        }
+        h->cfl_ready = false;
+        fribo_set_transform(h, all_y ? 1 : 0);
+        hipStreamDestroy(h->st);
+        h->st = b->st;
+        h->batch = b;
+    }
+    if ((e = batch_refresh(b))) {
+        batch_free(b);
+        return e;
+    }
+    *out = b;
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_batch_destroy(fribo_batch b)
+{
+    if (!b) return set_err(FRIBO_ERR_ARG, "null batch");
+    batch_free(b);
+    return FRIBO_OK;
+}
+
+// Marker kinematics of every instance in one copy: lmcc, lmvel (nlm_i, 2)
+// and lmA (nlm_i) of instance i at the offsets 5*sum(nlm_<i), in the
+// reference layouts, instance after instance (lmcc_i, lmvel_i, lmA_i).  The
+// instances' marker counts must be the ones of their last
+// fribo_set_markers.
+extern "C" int fribo_batch_set_markers(fribo_batch b, const double* mk)
+{
+    if (!b || !mk) return set_err(FRIBO_ERR_ARG, "null argument");
+    const int n = b->B.n;
+    long tot = 0;
+    std::vector<long> off(n);
+    for (int i = 0; i < n; ++i) {
+        off[i] = tot;
+        tot += 5 * (long)b->inst[i]->F.nlm;
+    }
+    if (tot == 0) return FRIBO_OK;
+    if (tot > b->mk_cap) {
+        HIPCHK(hipStreamSynchronize(b->st));
+        if (b->mk_h) hipHostFree(b->mk_h);
+        if (b->mk_d) hipFree(b->mk_d);
+        if (b->mk_off) hipFree(b->mk_off);
+        b->mk_h = nullptr;
+        b->mk_d = nullptr;
+        b->mk_off = nullptr;
+        b->mk_cap = 0;
+        HIPCHK(hipHostMalloc((void**)&b->mk_h, tot * sizeof(double), hipHostMallocDefault));
+        HIPCHK(hipMalloc(&b->mk_d, tot * sizeof(double)));
+        HIPCHK(hipMalloc(&b->mk_off, n * sizeof(long)));
+        b->mk_cap = tot;
+    }
+    HIPCHK(hipEventSynchronize(b->ev_mk));
+    memcpy(b->mk_h, mk, tot * sizeof(double));
+    for (int i = 0; i < n; ++i) {
+        fribo_ctx* h = b->inst[i];
+        const long nl = h->F.nlm;
+        h->lmcc.assign(mk + off[i], mk + off[i] + 2 * nl);
+    }
+    HIPCHK(hipMemcpyAsync(b->mk_off, off.data(), n * sizeof(long), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipMemcpyAsync(b->mk_d, b->mk_h, tot * sizeof(double), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipEventRecord(b->ev_mk, b->st));
+    for (int i = 0; i < n; ++i)
+        if (b->gen[i] != b->inst[i]->mk_gen) {
+            int e = batch_refresh(b);
+            if (e) return e;
+            break;
+        }
+    int mx = 1;
+    for (auto h : b->inst) mx = std::max<long>(mx, (5 * (long)h->F.nlm + 255) / 256);
+    hipLaunchKernelGGL(k_batch_markers, dim3(std::min(mx, 64), n), dim3(256), 0, b->st, (const FluidDev*)b->B.dF,
+                       (const long*)b->mk_off, (const double*)b->mk_d);
+    HIPCHK(hipGetLastError());
+    return FRIBO_OK;
+}
+
+static void batch_enqueue(fribo_batch_ctx* b)
+{
+    BatchSet& B = b->B;
+    hipStream_t st = b->st;
+    const int nsst = B.hF[0].nsst;
+    bool any_mk = false;
+    for (const auto& F : B.hF) any_mk = any_mk || (F.ibact && F.nlm > 0);
+    launch_cfl_b(B, st);
+    for (int n = 1; n <= nsst; ++n) {
+        launch_bc_b(B, n, n > 1, st);
+        launch_rhs_b(B, st);
+        if (any_mk) {
+            if (n == 1) launch_mls_setup_b(B, st);
+            launch_mls_force_b(B, st);
+        }
+        launch_jsolve_helm_b(B, st);
+        launch_ksolve_b(B, st);
+        launch_div_b(B, st);
+        const bool fused = launch_poisson_gemm_b(B, true, st);
+        if (B.hF[0].taxis == 1) launch_ysolve_b(B, st, fused);
+        else launch_jsolve_poisson_b(B, st);
+        launch_poisson_gemm_b(B, false, st);
+        launch_upd_b(B, st);
+        if (n == nsst) launch_mean_b(B, st);
+    }
+}
+
+// One time step of every instance (fluid_tsch.F90:25-166 each): dt[i]
+// in/out, dt_o[i] (nsst = 1), cflm[i] and err[i] out (the codes of
+// fribo_fluid_tsch; an instance with an error leaves its state as its own
+// step would).  Returns FRIBO_OK when the launches went through, whatever
+// the instances' codes.
+extern "C" int fribo_batch_step(fribo_batch b, int32_t ns, double* dt, const double* dt_o, double* cflm,
+                                int32_t* err)
+{
+    if (!b || !dt) return set_err(FRIBO_ERR_ARG, "null argument");
+    const int n = b->B.n;
+    for (int i = 0; i < n; ++i)
+        if (b->gen[i] != b->inst[i]->mk_gen) {
+            int e = batch_refresh(b);
+            if (e) return e;
+            break;
+        }
+    for (int i = 0; i < n; ++i) {
+        fribo_ctx* h = b->inst[i];
+        h->cfl_ready = false;
+        const int pre = markers_inside(h) ? FRIBO_E_MARKER_OUTSIDE : 0;
+        b->in_h[3 * i] = dt[i];
+        b->in_h[3 * i + 1] = dt_o ? dt_o[i] : dt[i];
+        const long long w = ((long long)pre << 32) | (unsigned)ns;
+        memcpy(&b->in_h[3 * i + 2], &w, sizeof(w));
+    }
+    HIPCHK(hipMemcpyAsync(b->in_d, b->in_h, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, b->st));
+    hipLaunchKernelGGL(k_batch_inputs, dim3((n + 63) / 64), dim3(64), 0, b->st, (const FluidDev*)b->B.dF,
+                       (const double*)b->in_d, n);
+    if (b->use_graph && b->warm) {
+        if (!b->gexec) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+            batch_enqueue(b);
+            hipError_t e = hipStreamEndCapture(b->st, &g);
+            if (e != hipSuccess) return set_err(FRIBO_ERR_HIP, std::string("batch capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) return set_err(FRIBO_ERR_HIP, std::string("batch graph: ") + hipGetErrorString(e));
+        }
+        HIPCHK(hipGraphLaunch(b->gexec, b->st));
+    } else {
+        batch_enqueue(b);
+        b->warm = true;
+    }
+    hipLaunchKernelGGL(k_batch_outputs, dim3((n + 63) / 64), dim3(64), 0, b->st, (const FluidDev*)b->B.dF,
+                       b->out_d, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->out_h, b->out_d, n * sizeof(BatchOut), hipMemcpyDeviceToHost, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    for (int i = 0; i < n; ++i) {
+        const long long w = *(const long long*)&b->in_h[3 * i + 2];
+        const bool outside = (w >> 32) != 0;   // not stepped: dt stays as given
+        if (!outside) dt[i] = b->out_h[i].dt;
+        if (cflm) cflm[i] = b->out_h[i].cflm;
+        if (err) err[i] = b->out_h[i].err;
+    }
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_batch_set_graph(fribo_batch b, int32_t on)
+{
+    if (!b) return set_err(FRIBO_ERR_ARG, "null batch");
+    b->use_graph = on != 0;
+    if (!b->use_graph && b->gexec) {
+        hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    return FRIBO_OK;
+}
+
+// Tile shape of the batch's transforms: 0 (32x32), 1 (64x64), 2 (96x128),
+// 3 (direct-to-LDS 96x64) or -1 for the default.
+extern "C" int fribo_batch_set_gemm_shape(fribo_batch b, int32_t cfg)
+{
+    if (!b || cfg < -1 || cfg > 3) return set_err(FRIBO_ERR_ARG, "bad tile shape");
+    if (cfg >= 0)
+        for (auto h : b->inst)
+            if (!((gemm_cfg_mask(h->F) >> cfg) & 1))
+                return set_err(FRIBO_ERR_LIMIT, "tile shape not available for every instance");
+    b->gemm_fixed = cfg;
+    return batch_refresh(b);
+}
+
+extern "C" int fribo_batch_get_gemm_shape(fribo_batch b)
+{
+    return b ? b->B.gemm_cfg : -1;
 }

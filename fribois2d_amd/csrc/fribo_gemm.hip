@@ -50,6 +50,7 @@
 // tiles in 4 x 8 patches the tile indices are permuted to give each XCD
 // compact patches (its L2 then serves 4 A- and 8 B-tiles to 32 workgroups).
 #include "fribo_ychunk.h"
+#include "fribo_batch.h"
 #include <hip/hip_ext.h>
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -101,10 +102,7 @@ __device__ inline void gauge_partial(const f64x4 (&acc)[RM][RN], int M, int N, i
 }
 
 template <int BM, int BN, int WM, int WN, bool NN>
-__global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, const double* __restrict__ A, int lda,
-                                                          const double* __restrict__ B, int ldb,
-                                                          double* __restrict__ C, int ldc, const DevScalars* S,
-                                                          const double* __restrict__ zg, double* gpart)
+__device__ __forceinline__ void gemm_nt_body(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, const VBlock& vb)
 {
     constexpr int NT = 64 * WM * WN;
     constexpr int TM = BM / WM, TN = BN / WN;     // wave tile
@@ -120,7 +118,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave % WM, wn = wave / WM;
-    const int i0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int i0 = vb.x * BM, n0 = vb.y * BN;
     double ra[NA], rb[NB];
 
     auto load = [&](int k0) {
@@ -199,8 +197,20 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, c
         }
     if (zg && M - 1 >= i0 && M - 1 < i0 + BM) {
         __shared__ double gsh[WM * WN];
-        gauge_partial<RM, RN, TM, TN, WM * WN>(acc, M, N, i0, n0, wm, wn, lane, wave, blockIdx.y, zg, gpart, gsh);
+        gauge_partial<RM, RN, TM, TN, WM * WN>(acc, M, N, i0, n0, wm, wn, lane, wave, vb.y, zg, gpart, gsh);
     }
+}
+template <int BM, int BN, int WM, int WN, bool NN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart)
+{
+    gemm_nt_body<BM, BN, WM, WN, NN>(M, N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, vb_hw());
+}
+template <int BM, int BN, int WM, int WN, bool NN>
+__global__ __launch_bounds__(64 * WM * WN) void kb_gemm_nt(const BGrid<GemmProb> G)
+{
+    VBlock vb;
+    const GemmProb& P = *bg_block(G, vb);
+    gemm_nt_body<BM, BN, WM, WN, NN>(P.M, P.N, P.K, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, P.S, P.zg, P.gpart, vb);
 }
 
 // s_barrier without the memory fence of __syncthreads (whose release would
@@ -236,10 +246,7 @@ __device__ inline void dma16(const double* src, double* dst)
 }
 
 template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128>
-__global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A,
-                                                           int lda, const double* __restrict__ B, int ldb,
-                                                           double* __restrict__ C, int ldc, const DevScalars* S,
-                                                           const double* __restrict__ zg, double* gpart, YFuse yf)
+__device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, YFuse yf, const VBlock& vb)
 {
     constexpr int BM = 96, NW = WM * WN;
     constexpr int TM = BM / WM, TN = BN / WN, RM = TM / 16, RN = TN / 16;
@@ -261,12 +268,12 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, 
     double* const S2 = SM + 2 * (SA + SBP);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave % WM, wn = wave / WM;
-    int bx = blockIdx.x, by = blockIdx.y;
-    if (gridDim.x % 4 == 0 && gridDim.y % 8 == 0 && (gridDim.x * gridDim.y) % 256 == 0) {
+    int bx = vb.x, by = vb.y;
+    if (vb.gx % 4 == 0 && vb.gy % 8 == 0 && (vb.gx * vb.gy) % 256 == 0) {
         // dispatch id -> XCD (id % 8); 32 consecutive ids of one XCD form a 4 x 8
         // patch (a bijection when every XCD receives whole patches)
-        const int id = blockIdx.x + gridDim.x * blockIdx.y, x = id % 8, l = id / 8;
-        const int npm = gridDim.x / 4;
+        const int id = vb.x + vb.gx * vb.y, x = id % 8, l = id / 8;
+        const int npm = vb.gx / 4;
         const int patch = x + 8 * (l / 32), ll = l % 32;
         bx = (patch % npm) * 4 + ll % 4;
         by = (patch / npm) * 8 + ll / 4;
@@ -403,6 +410,18 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, 
         }
     }
 }
+template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, YFuse yf)
+{
+    gemm_dma_body<BK, NN, WM, WN, BN>(M, N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, yf, vb_hw());
+}
+template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128>
+__global__ __launch_bounds__(64 * WM * WN) void kb_gemm_dma(const BGrid<GemmProb> G)
+{
+    VBlock vb;
+    const GemmProb& P = *bg_block(G, vb);
+    gemm_dma_body<BK, NN, WM, WN, BN>(P.M, P.N, P.K, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, P.S, P.zg, P.gpart, P.yf, vb);
+}
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
@@ -486,4 +505,120 @@ bool launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double*
 int gemm_nn_tiles_n(int M, int N, int K, const double* A, int lda, const double* B, int ldb)
 {
     return launch_gemm<true>(M, N, K, A, lda, B, ldb, nullptr, 0, nullptr, 0, nullptr, nullptr, true);
+}
+
+// ---------------------------------------------------------------------------
+// Batched transforms of the Poisson solve: one grouped launch over the
+// instances' problems (each instance's tiles are its own grid), one tile
+// shape for the whole batch (B.gemm_cfg).  Every tile shape accumulates each
+// output element over k in the same order, so the transform results do not
+// depend on the shape; the gauge partials are one per tile column of the
+// shape in use (gemm_cfg_tile_n).
+int gemm_cfg_tile_n(int cfg)
+{
+    return cfg == 0 ? 32 : cfg == 1 ? 64 : cfg == 2 ? 128 : GEMM_DMA_BN;
+}
+
+// The shapes every instance of the batch can run: bit c set for cfg c.
+int gemm_cfg_mask(const FluidDev& F)
+{
+    int m = 7;
+    const bool yax = F.taxis == 1;
+    const double* A = yax ? F.ymmi : F.dph + fidx(F, 1, 1);
+    const double* Bm = yax ? F.dph + fidx(F, 1, 1) : F.zmmt;
+    const int lda = yax ? F.ldy : F.P, ldb = yax ? F.P : F.ldz;
+    const int K = yax ? F.n2m : F.n3m;
+    const bool ok1 = yax ? dma_ok<true>(F.n2m, F.n3m, K, A, lda, Bm, ldb) : dma_ok<false>(F.n2m, F.n3m, K, A, lda, Bm, ldb);
+    const double* A2 = yax ? F.ymm : F.hmn;
+    const double* B2 = yax ? F.hmn : F.zmm;
+    const bool ok2 = yax ? dma_ok<true>(F.n2m, F.n3m, K, A2, lda, B2, ldb) : dma_ok<false>(F.n2m, F.n3m, K, A2, F.P, B2, ldb);
+    if (ok1 && ok2) m |= 8;
+    return m;
+}
+
+// Tile columns (gauge partials) of the backward transform with shape cfg.
+int gemm_cfg_parts(const FluidDev& F, int cfg) { return cdiv(F.n3m, gemm_cfg_tile_n(cfg)); }
+
+template <bool NN>
+static void launch_gemm_b(BatchSet& B, int key, const std::vector<GemmProb>& pr, hipStream_t st)
+{
+    std::vector<int> d(3 * (size_t)B.n);
+    int bm = 32, bn = 32;
+    switch (B.gemm_cfg) {
+    case 1: bm = 64; bn = 64; break;
+    case 2: bm = 96; bn = 128; break;
+    case 3: bm = 96; bn = GEMM_DMA_BN; break;
+    default: break;
+    }
+    for (int i = 0; i < B.n; ++i) {
+        d[3 * i] = cdiv(pr[i].M, bm);
+        d[3 * i + 1] = cdiv(pr[i].N, bn);
+        d[3 * i + 2] = 1;
+    }
+    const int tot = bg_total(d);
+    if (tot <= 0) return;
+    const BGrid<GemmProb> G = B.gemm(key, pr, d);
+    switch (B.gemm_cfg) {
+    case 3:
+        hipLaunchKernelGGL((kb_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN, GEMM_DMA_BN>), dim3(tot),
+                           dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), 0, st, G);
+        break;
+    case 2: hipLaunchKernelGGL((kb_gemm_nt<96, 128, 2, 4, NN>), dim3(tot), dim3(512), 0, st, G); break;
+    case 1: hipLaunchKernelGGL((kb_gemm_nt<64, 64, 2, 2, NN>), dim3(tot), dim3(256), 0, st, G); break;
+    default: hipLaunchKernelGGL((kb_gemm_nt<32, 32, 2, 2, NN>), dim3(tot), dim3(256), 0, st, G); break;
+    }
+}
+
+bool launch_poisson_gemm_b(BatchSet& B, bool fwd, hipStream_t st)
+{
+    const bool yax = B.hF[0].taxis == 1;
+    std::vector<GemmProb> pr(B.n);
+    bool fused = false;
+    for (int i = 0; i < B.n; ++i) {
+        const FluidDev& F = B.hF[i];
+        GemmProb& q = pr[i];
+        q = GemmProb{};
+        q.M = F.n2m;
+        q.N = F.n3m;
+        q.S = F.S;
+        double* d = F.dph + fidx(F, 1, 1);
+        if (yax) {
+            q.K = F.n2m;
+            q.lda = F.ldy;
+            q.ldb = F.P;
+            q.ldc = F.P;
+            q.A = fwd ? F.ymmi : F.ymm;
+            q.B = fwd ? d : F.hmn;
+            q.C = fwd ? F.hmn : d;
+            if (!fwd) {
+                q.zg = F.zg;
+                q.gpart = F.gpart;
+            } else if (B.ysolve_fused) {
+                q.yf = YFuse{F.wy, F.amphk, F.acphk, F.apphk, F.ysum, F.n2m, F.n3m / F.ks[0].nch};
+                fused = true;
+            }
+        } else {
+            q.K = F.n3m;
+            q.lda = F.P;
+            q.ldb = F.ldz;
+            q.ldc = F.P;
+            q.A = fwd ? d : F.hmn;
+            q.B = fwd ? F.zmmt : F.zmm;
+            q.C = fwd ? F.hmn : d;
+        }
+    }
+    if (yax) launch_gemm_b<true>(B, fwd ? BK_GEMM_F : BK_GEMM_B, pr, st);
+    else launch_gemm_b<false>(B, fwd ? BK_GEMM_F : BK_GEMM_B, pr, st);
+    return fused;
+}
+
+// The fused chunk pass is possible for an instance when its line partition
+// has equal chunks that fill the direct-to-LDS kernel's output tiles.
+bool gemm_fuse_ok(const FluidDev& F)
+{
+    constexpr int kChunkGroups = (GEMM_DMA_WM * GEMM_DMA_WN) / ((96 + 63) / 64);
+    if (F.taxis != 1 || F.n3m % F.ks[0].nch != 0) return false;
+    const int csz = F.n3m / F.ks[0].nch;
+    return csz > 1 && csz <= 33 && GEMM_DMA_BN % csz == 0 && F.n3m % GEMM_DMA_BN == 0 &&
+           GEMM_DMA_BN / csz <= kChunkGroups;
 }

@@ -27,7 +27,9 @@
 //                 in), zeroes plane k = n3m (the single-rank ghost update of
 //                 mpi_routines.F90:324-339) and adds f*dt to rhs2/rhs3
 //                 (helsolve.F90:20-49).  No atomics, bit-reproducible.
-#include "fribo_internal.h"
+#include "fribo_batch.h"
+
+#include <algorithm>
 
 #define LMCC(n, c) F.lmcc[(size_t)((n) - 1) + (size_t)((c) - 1) * F.nlm]
 #define LMVEL(n, c) F.lmvel[(size_t)((n) - 1) + (size_t)((c) - 1) * F.nlm]
@@ -79,10 +81,10 @@ __device__ int wave_minloc(const double* x, int len, double v)
     return bi;
 }
 
-__global__ __launch_bounds__(64) void k_mls_idx(const FluidDev F)
+__device__ __forceinline__ void mls_idx_body(const FluidDev& F, const VBlock& vb)
 {
     if (F.S->err) return;
-    const int n = blockIdx.x + 1, s = blockIdx.y;
+    const int n = vb.x + 1, s = vb.y;
     const double y = LMCC(n, 1), z = LMCC(n, 2);
     int ind;
     if (s == 0) {
@@ -102,6 +104,16 @@ __global__ __launch_bounds__(64) void k_mls_idx(const FluidDev F)
         int d = (s < 2) ? 1 : 2, c = (s & 1) + 1;
         LMIND(n, d, c) = ind;
     }
+}
+__global__ __launch_bounds__(64) void k_mls_idx(const FluidDev F)
+{
+    mls_idx_body(F, vb_hw());
+}
+__global__ __launch_bounds__(64) void kb_mls_idx(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    mls_idx_body(*Fp, vb);
 }
 
 // utils_math.f90:637-725, 3x3 scaled partial pivoting; returns info
@@ -182,12 +194,12 @@ __device__ double weight1d(double dif, double sdd, int wfun, double radiusSD)
 // the two momentum sums are summed in support order from the group's lanes,
 // so every value is the serial loop's; each lane of the group solves the same
 // 3x3 system and keeps its own shape-function value.
-__global__ __launch_bounds__(64) void k_mls_sup(const FluidDev F)
+__device__ __forceinline__ void mls_sup_body(const FluidDev& F, const VBlock& vb)
 {
     DevScalars* S = F.S;
     if (S->err) return;
     const int lane = threadIdx.x;
-    const int g = blockIdx.x * 4 + (lane >> 4), m = lane & 15, base = lane & 48;
+    const int g = vb.x * 4 + (lane >> 4), m = lane & 15, base = lane & 48;
     if (g >= 2 * F.nlm) return;   // whole group
     const int n = g % F.nlm + 1, d = g / F.nlm + 1;
     const bool sup = m < FRIBO_DIMSD;
@@ -262,6 +274,16 @@ __global__ __launch_bounds__(64) void k_mls_sup(const FluidDev F)
     const unsigned long long cell = (unsigned long long)(ej - 1) + (unsigned long long)(ek - 1) * F.n2;
     keys[(size_t)(n - 1) * FRIBO_DIMSD + m] =
         inb ? ((cell << ENT_BITS) | (unsigned long long)((n - 1) * FRIBO_DIMSD + m)) : KEY_NONE;
+}
+__global__ __launch_bounds__(64) void k_mls_sup(const FluidDev F)
+{
+    mls_sup_body(F, vb_hw());
+}
+__global__ __launch_bounds__(64) void kb_mls_sup(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    mls_sup_body(*Fp, vb);
 }
 
 // Ordering of the (cell, entry) keys of each direction, ascending.
@@ -419,14 +441,14 @@ __device__ inline unsigned int block_exscan(unsigned int v, unsigned int* wsum, 
 // its rank among the cell's entries -- the (cell, entry) order, whatever the
 // order of the claims.  Larger boxes take the radix sort.
 #define CELL_MAX 16384
-__global__ __launch_bounds__(1024) void k_mls_order(const FluidDev F, int nbits)
+__device__ __forceinline__ void mls_order_body(const FluidDev& F, const VBlock& vb, int nbits)
 {
     __shared__ unsigned long long sk[RADIX_MAX];         // radix keys, or cnt[CELL_MAX] + ent[RADIX_MAX] (u32)
     __shared__ unsigned int hist[256 * RADIX_WAVES];
     __shared__ unsigned int wsum[RADIX_WAVES];
     __shared__ int box[4];
     if (F.S->err) return;
-    const int d = blockIdx.x, t = threadIdx.x, lane = t & 63;
+    const int d = vb.x, t = threadIdx.x, lane = t & 63;
     const int n = F.nlm * FRIBO_DIMSD;
     unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
     const unsigned int n2u = (unsigned int)F.n2;
@@ -520,6 +542,16 @@ __global__ __launch_bounds__(1024) void k_mls_order(const FluidDev F, int nbits)
         }
     for (int i = (int)nvalid + t; i < n; i += 1024) gk[i] = KEY_NONE;
 }
+__global__ __launch_bounds__(1024) void k_mls_order(const FluidDev F, int nbits)
+{
+    mls_order_body(F, vb_hw(), nbits);
+}
+__global__ __launch_bounds__(1024) void kb_mls_order(const BGrid<FluidDev> G, int nbits)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    mls_order_body(*Fp, vb, nbits);
+}
 
 __global__ __launch_bounds__(1024) void k_mls_sort_local(const FluidDev F, int size0, int first)
 {
@@ -599,12 +631,12 @@ void launch_mls_setup(const FluidDev& F, hipStream_t st)
 // lane gathers its own q + rhs value, the group's first lane sums the nine
 // products in support order (the reference's order) from its neighbours, and
 // every support lane writes its contribution.
-__global__ __launch_bounds__(64) void k_mls_interp(const FluidDev F)
+__device__ __forceinline__ void mls_interp_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const int lane = threadIdx.x;
-    const int g = blockIdx.x * 4 + (lane >> 4), m = lane & 15;   // group (n,d), support m
+    const int g = vb.x * 4 + (lane >> 4), m = lane & 15;   // group (n,d), support m
     const bool ing = g < 2 * F.nlm;
     const int n = (ing ? g % F.nlm : 0) + 1, d = (ing ? g / F.nlm : 0) + 1;
     const bool act = ing && F.active[n - 1];
@@ -632,15 +664,25 @@ __global__ __launch_bounds__(64) void k_mls_interp(const FluidDev F)
     double* con = F.contrib + (size_t)(d - 1) * F.nlm * FRIBO_DIMSD;
     con[(size_t)(n - 1) * FRIBO_DIMSD + m] = CL(n, d) * ph * f;
 }
+__global__ __launch_bounds__(64) void k_mls_interp(const FluidDev F)
+{
+    mls_interp_body(F, vb_hw());
+}
+__global__ __launch_bounds__(64) void kb_mls_interp(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    mls_interp_body(*Fp, vb);
+}
 
 // One thread per sorted entry; the first entry of each cell (segment head)
 // sums the cell's contributions in marker order.
-__global__ __launch_bounds__(64) void k_mls_spread(const FluidDev F)
+__device__ __forceinline__ void mls_spread_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int d = blockIdx.y;
-    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int d = vb.y;
+    const int i = vb.x * 64 + threadIdx.x;
     if (i >= F.nlm * FRIBO_DIMSD) return;
     const unsigned long long* gk = F.keys + (size_t)d * F.maxkeys;
     const unsigned long long key = gk[i];
@@ -686,12 +728,66 @@ __global__ __launch_bounds__(64) void k_mls_spread(const FluidDev F)
         }
     }
 }
+__global__ __launch_bounds__(64) void k_mls_spread(const FluidDev F)
+{
+    mls_spread_body(F, vb_hw());
+}
+__global__ __launch_bounds__(64) void kb_mls_spread(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    mls_spread_body(*Fp, vb);
+}
 
 void launch_mls_force(const FluidDev& F, hipStream_t st)
 {
     if (F.nlm <= 0) return;
     hipLaunchKernelGGL(k_mls_interp, dim3((2 * F.nlm + 3) / 4), dim3(64), 0, st, F);
     hipLaunchKernelGGL(k_mls_spread, dim3((F.nlm * FRIBO_DIMSD + 63) / 64, 2), dim3(64), 0, st, F);
+}
+
+// Batched forms: instances without markers (or without the immersed
+// boundary) get no blocks.
+static bool has_markers(const FluidDev& F) { return F.ibact && F.nlm > 0; }
+
+static int order_bits(const FluidDev& F)
+{
+    const unsigned long long ncell = (unsigned long long)F.n2 * (unsigned long long)F.n3;
+    int nbits = 1;
+    while (nbits < 32 && (1ull << nbits) <= ncell) ++nbits;   // 2^nbits > every cell + 1
+    return nbits;
+}
+
+bool mls_batchable(const FluidDev& F)
+{
+    const unsigned long long ncell = (unsigned long long)F.n2 * (unsigned long long)F.n3;
+    return !has_markers(F) || (F.nlm * FRIBO_DIMSD <= RADIX_MAX && ncell < 0xffffffffull);
+}
+
+void launch_mls_setup_b(BatchSet& B, hipStream_t st)
+{
+    int nbits = 1;
+    for (const auto& F : B.hF)
+        if (has_markers(F)) nbits = std::max(nbits, order_bits(F));
+    blaunch(B, BK_MLS_IDX, bdims(B, [](const FluidDev& F) { return has_markers(F) ? dim3(F.nlm, 4) : dim3(0); }),
+            kb_mls_idx, dim3(64), 0, st);
+    blaunch(B, BK_MLS_SUP,
+            bdims(B, [](const FluidDev& F) { return has_markers(F) ? dim3((2 * F.nlm + 3) / 4) : dim3(0); }),
+            kb_mls_sup, dim3(64), 0, st);
+    // a larger bit count than an instance's own only adds empty radix passes
+    blaunch(B, BK_MLS_ORDER, bdims(B, [](const FluidDev& F) { return has_markers(F) ? dim3(2) : dim3(0); }),
+            kb_mls_order, dim3(1024), 0, st, nbits);
+}
+
+void launch_mls_force_b(BatchSet& B, hipStream_t st)
+{
+    blaunch(B, BK_MLS_INTERP,
+            bdims(B, [](const FluidDev& F) { return has_markers(F) ? dim3((2 * F.nlm + 3) / 4) : dim3(0); }),
+            kb_mls_interp, dim3(64), 0, st);
+    blaunch(B, BK_MLS_SPREAD, bdims(B, [](const FluidDev& F) {
+                return has_markers(F) ? dim3((F.nlm * FRIBO_DIMSD + 63) / 64, 2) : dim3(0);
+            }),
+            kb_mls_spread, dim3(64), 0, st);
 }
 
 // ---------------------------------------------------------------------------

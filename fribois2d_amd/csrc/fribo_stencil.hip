@@ -16,7 +16,9 @@
 // exactly like the FP64 reference loops.  Deviations from the reference's
 // sequential sums: the CFL/pressure-mean/outflow-flux reductions use a fixed
 // tree order (deterministic run to run).
-#include "fribo_internal.h"
+#include "fribo_batch.h"
+
+#include <algorithm>
 
 
 // Fixed-order reductions over the NT threads of a block: a butterfly inside
@@ -60,12 +62,12 @@ __device__ inline double block_max(double* sh, double v)
 // and q3 are each read from HBM once.  Blocks without planes write the
 // neutral partials (max seed 1e-8, sum 0).
 #define CFL_NJ 16  // columns per thread: n2m <= 256*CFL_NJ
-__global__ __launch_bounds__(256) void k_cfl(const FluidDev F)
+__device__ __forceinline__ void cfl_body(const FluidDev& F, const VBlock& vb)
 {
     __shared__ double sh[256];
     const int t = threadIdx.x;
     const int per = (F.n3m + FRIBO_NPART - 1) / FRIBO_NPART;
-    const int k0 = blockIdx.x * per + 1;
+    const int k0 = vb.x * per + 1;
     const int k1 = min(k0 + per - 1, F.n3m);
     const double* __restrict__ q2 = F.q2;
     const double* __restrict__ q3 = F.q3;
@@ -98,14 +100,29 @@ __global__ __launch_bounds__(256) void k_cfl(const FluidDev F)
     const double bm = block_max<256>(sh, m);
     const double bs = block_sum<256>(sh, s);
     if (t == 0) {
-        F.cfl_part[2 * blockIdx.x] = bm;
-        F.cfl_part[2 * blockIdx.x + 1] = bs;
+        F.cfl_part[2 * vb.x] = bm;
+        F.cfl_part[2 * vb.x + 1] = bs;
     }
+}
+__global__ __launch_bounds__(256) void k_cfl(const FluidDev F)
+{
+    cfl_body(F, vb_hw());
+}
+__global__ __launch_bounds__(256) void kb_cfl(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    cfl_body(*Fp, vb);
 }
 
 void launch_cfl(const FluidDev& F, hipStream_t st)
 {
     hipLaunchKernelGGL(k_cfl, dim3(FRIBO_NPART), dim3(256), 0, st, F);
+}
+
+void launch_cfl_b(BatchSet& B, hipStream_t st)
+{
+    blaunch(B, BK_CFL, bdims(B, [](const FluidDev&) { return dim3(FRIBO_NPART); }), kb_cfl, dim3(256), 0, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -150,6 +167,9 @@ __device__ StepScalars step_scalars(const FluidDev& F, int n, int ns, double dt_
                 cflm = cflm * dt;
                 if (cflm > F.cfllim) r.err = FRIBO_E_CFL_TOO_LARGE;
             }
+            // an error set before the step (a batched instance whose markers
+            // left the domain) stops it like the host check does
+            if (s_err) r.err = s_err;
             r.dt = dt;
             r.cflm = cflm;
             r.beta = dt / F.ren * 0.5;
@@ -481,7 +501,7 @@ __device__ void bc_rows(const FluidDev& F, const StepScalars& r, int mean, doubl
 // Workgroup 0: the previous substep's pressure mean and boucqt (mean != 0),
 // finalize CFL/dt (substep 1), RK coefficients, boucdq.
 // Workgroups 1..4: factors of the four Helmholtz systems of this substep.
-__global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int mean)
+__device__ __forceinline__ void bc_body(const FluidDev& F, const VBlock& vb, int n, int mean)
 {
     extern __shared__ double dyn[];
     __shared__ double sh[1024];
@@ -495,21 +515,21 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int mean)
     // workgroup 0: its row values and pressure partials, in flight with the above
     BcRows pre;
     double prv[2] = {0.0, 0.0};
-    if (blockIdx.x == 0) {
+    if (vb.x == 0) {
         bc_load(F, mean, 1 + t, pre);
         if (mean) pr_load(F, prv);
     }
     const StepScalars r = step_scalars(F, n, ns, dt_in, dt_o_in, s_dt, s_cflm, s_beta, s_err, sh, sh2);
     if (r.err) {
-        if (blockIdx.x == 0 && t == 0 && n == 1) {
+        if (vb.x == 0 && t == 0 && n == 1) {
             S->cflm = r.cflm;
             S->dt = r.dt;
             S->err = r.err;
         }
         return;
     }
-    if (blockIdx.x > 0) {
-        const int sys = blockIdx.x - 1;
+    if (vb.x > 0) {
+        const int sys = vb.x - 1;
         const KSys& K = (sys < 2) ? F.ks[sys] : F.js[sys - 2];
         sys_prepare(F, K, r.betadx, dyn);
         return;
@@ -538,10 +558,28 @@ __global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int mean)
         F.q2[fidx(F, F.n2, k)] = 0.0;
     }
 }
+__global__ __launch_bounds__(1024) void k_bc(const FluidDev F, int n, int mean)
+{
+    bc_body(F, vb_hw(), n, mean);
+}
+__global__ __launch_bounds__(1024) void kb_bc(const BGrid<FluidDev> G, int n, int mean)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    bc_body(*Fp, vb, n, mean);
+}
 
 void launch_bc(const FluidDev& F, int n, bool mean, hipStream_t st)
 {
     hipLaunchKernelGGL(k_bc, dim3(5), dim3(1024), sysprep_lds_bytes(F), st, F, n, mean ? 1 : 0);
+}
+
+void launch_bc_b(BatchSet& B, int n, bool mean, hipStream_t st)
+{
+    size_t lds = 0;
+    for (const auto& F : B.hF) lds = std::max(lds, sysprep_lds_bytes(F));
+    blaunch(B, BK_BC, bdims(B, [](const FluidDev&) { return dim3(5); }), kb_bc, dim3(1024), lds, st, n,
+            mean ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -568,15 +606,15 @@ __device__ inline double dn1(double v) { return __shfl_down(v, 1, 64); }
 // Fused HDNL + INVTR.  Cell (j,k): q2 point (rc(j), zm(k)) for 2<=j<=n2m and
 // q3 point (rm(j), zz(k)) for 2<=k<=n3m; operation order as in the reference
 // loops (HDNL.F90:35-220, INVTR.F90:45-277).
-__global__ __launch_bounds__(64 * SW_RHS) void k_rhs(const FluidDev F)
+__device__ __forceinline__ void rhs_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const int lane = threadIdx.x & 63;
-    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)blockIdx.x * SW_RHS + (int)(threadIdx.x >> 6);
+    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)vb.x * SW_RHS + (int)(threadIdx.x >> 6);
     if (bx >= ntj) return;
     const int j = WJ * bx + lane;
-    const int k0 = MKR * blockIdx.y + 1;
+    const int k0 = MKR * vb.y + 1;
     const int n2m = F.n2m, n3m = F.n3m;
     if (k0 > n3m) return;
     const int k1 = min(k0 + MKR - 1, n3m);
@@ -699,25 +737,44 @@ __global__ __launch_bounds__(64 * SW_RHS) void k_rhs(const FluidDev F)
         prm = prc;
     }
 }
+__global__ __launch_bounds__(64 * SW_RHS) void k_rhs(const FluidDev F)
+{
+    rhs_body(F, vb_hw());
+}
+__global__ __launch_bounds__(64 * SW_RHS) void kb_rhs(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    rhs_body(*Fp, vb);
+}
+
+static dim3 grid_rhs(const FluidDev& F)
+{
+    const int ntj = (F.n2m + WJ - 1) / WJ;
+    return dim3((ntj + SW_RHS - 1) / SW_RHS, (F.n3m + MKR - 1) / MKR);
+}
 
 void launch_rhs(const FluidDev& F, hipStream_t st)
 {
-    const int ntj = (F.n2m + WJ - 1) / WJ;
-    dim3 g((ntj + SW_RHS - 1) / SW_RHS, (F.n3m + MKR - 1) / MKR);
-    hipLaunchKernelGGL(k_rhs, g, dim3(64 * SW_RHS), 0, st, F);
+    hipLaunchKernelGGL(k_rhs, grid_rhs(F), dim3(64 * SW_RHS), 0, st, F);
+}
+
+void launch_rhs_b(BatchSet& B, hipStream_t st)
+{
+    blaunch(B, BK_RHS, bdims(B, grid_rhs), kb_rhs, dim3(64 * SW_RHS), 0, st);
 }
 
 // ---------------------------------------------------------------------------
 // divergence.F90:26-39
-__global__ __launch_bounds__(64 * SW_DIV) void k_div(const FluidDev F)
+__device__ __forceinline__ void div_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const int lane = threadIdx.x & 63;
-    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)blockIdx.x * SW_DIV + (int)(threadIdx.x >> 6);
+    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)vb.x * SW_DIV + (int)(threadIdx.x >> 6);
     if (bx >= ntj) return;
     const int j = WJ * bx + lane;
-    const int k0 = MK * blockIdx.y + 1;
+    const int k0 = MK * vb.y + 1;
     if (k0 > F.n3m) return;
     const int k1 = min(k0 + MK - 1, F.n3m);
     const bool inj = (j >= 1 && j <= F.n2);
@@ -747,27 +804,46 @@ __global__ __launch_bounds__(64 * SW_DIV) void k_div(const FluidDev F)
         q3c = q3p;
     }
 }
+__global__ __launch_bounds__(64 * SW_DIV) void k_div(const FluidDev F)
+{
+    div_body(F, vb_hw());
+}
+__global__ __launch_bounds__(64 * SW_DIV) void kb_div(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    div_body(*Fp, vb);
+}
+
+static dim3 grid_div(const FluidDev& F)
+{
+    const int ntj = (F.n2m + WJ - 1) / WJ;
+    return dim3((ntj + SW_DIV - 1) / SW_DIV, (F.n3m + MK - 1) / MK);
+}
 
 void launch_div(const FluidDev& F, hipStream_t st)
 {
-    const int ntj = (F.n2m + WJ - 1) / WJ;
-    dim3 g((ntj + SW_DIV - 1) / SW_DIV, (F.n3m + MK - 1) / MK);
-    hipLaunchKernelGGL(k_div, g, dim3(64 * SW_DIV), 0, st, F);
+    hipLaunchKernelGGL(k_div, grid_div(F), dim3(64 * SW_DIV), 0, st, F);
+}
+
+void launch_div_b(BatchSet& B, hipStream_t st)
+{
+    blaunch(B, BK_DIV, bdims(B, grid_div), kb_div, dim3(64 * SW_DIV), 0, st);
 }
 
 // ---------------------------------------------------------------------------
 // updvp.F90:27-56 (velocity correction) + 83-102 (prcalc) + partial sums of
 // the pressure mean (updvp.F90:127-133).  The stored pressure is "raw": the
 // previous mean (S->pend) is subtracted on read.
-__global__ __launch_bounds__(64 * SW_UPD) void k_upd(const FluidDev F)
+__device__ __forceinline__ void upd_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const int lane = threadIdx.x & 63;
-    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)blockIdx.x * SW_UPD + (int)(threadIdx.x >> 6);
+    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)vb.x * SW_UPD + (int)(threadIdx.x >> 6);
     if (bx >= ntj) return;
     const int j = WJ * bx + lane;
-    const int k0 = MK * blockIdx.y + 1;
+    const int k0 = MK * vb.y + 1;
     double part = 0.0;
     if (k0 <= F.n3m) {
         const int k1 = min(k0 + MK - 1, F.n3m);
@@ -794,7 +870,7 @@ __global__ __launch_bounds__(64 * SW_UPD) void k_upd(const FluidDev F)
             for (int q = 0; q < F.ngpart; ++q) g = g + F.gpart[q];
             doff = g * F.zginv;
         }
-        if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0) F.S->dphoff = doff;
+        if (vb.x == 0 && vb.y == 0 && lane == 0) F.S->dphoff = doff;
         double dm = (inj && k0 > 1) ? dph[c - P] - doff : 0.0;
         double dc = inj ? dph[c] - doff : 0.0;
         // plane k+1 of dph and plane k of q2/q3/pr are loaded one iteration ahead
@@ -839,14 +915,33 @@ __global__ __launch_bounds__(64 * SW_UPD) void k_upd(const FluidDev F)
     }
     // wave reduction (fixed order) -> one partial per workgroup
     for (int off = 32; off > 0; off >>= 1) part = part + __shfl_xor(part, off, 64);
-    if (lane == 0) F.pr_part[blockIdx.y * ntj + bx] = part;
+    if (lane == 0) F.pr_part[vb.y * ntj + bx] = part;
+}
+__global__ __launch_bounds__(64 * SW_UPD) void k_upd(const FluidDev F)
+{
+    upd_body(F, vb_hw());
+}
+__global__ __launch_bounds__(64 * SW_UPD) void kb_upd(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    upd_body(*Fp, vb);
+}
+
+static dim3 grid_upd(const FluidDev& F)
+{
+    const int ntj = (F.n2m + WJ - 1) / WJ;
+    return dim3((ntj + SW_UPD - 1) / SW_UPD, (F.n3m + MK - 1) / MK);
 }
 
 void launch_upd(const FluidDev& F, hipStream_t st)
 {
-    const int ntj = (F.n2m + WJ - 1) / WJ;
-    dim3 g((ntj + SW_UPD - 1) / SW_UPD, (F.n3m + MK - 1) / MK);
-    hipLaunchKernelGGL(k_upd, g, dim3(64 * SW_UPD), 0, st, F);
+    hipLaunchKernelGGL(k_upd, grid_upd(F), dim3(64 * SW_UPD), 0, st, F);
+}
+
+void launch_upd_b(BatchSet& B, hipStream_t st)
+{
+    blaunch(B, BK_UPD, bdims(B, grid_upd), kb_upd, dim3(64 * SW_UPD), 0, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -854,16 +949,31 @@ void launch_upd(const FluidDev& F, hipStream_t st)
 // of its own (after the last substep of a step, and after every substep of
 // the substep-by-substep entry point); within a step, k_bc of the next
 // substep does the same work first (bc_rows).
-__global__ __launch_bounds__(1024) void k_mean(const FluidDev F)
+__device__ __forceinline__ void mean_body(const FluidDev& F, const VBlock& vb)
 {
     __shared__ double sh[16];
     if (F.S->err) return;
     mean_block(F, sh);
 }
+__global__ __launch_bounds__(1024) void k_mean(const FluidDev F)
+{
+    mean_body(F, vb_hw());
+}
+__global__ __launch_bounds__(1024) void kb_mean(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    mean_body(*Fp, vb);
+}
 
 void launch_mean(const FluidDev& F, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mean, dim3(1), dim3(1024), 0, st, F);
+}
+
+void launch_mean_b(BatchSet& B, hipStream_t st)
+{
+    blaunch(B, BK_MEAN, bdims(B, [](const FluidDev&) { return dim3(1); }), kb_mean, dim3(1024), 0, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -25,7 +25,7 @@
 // is computed once per substep (k_bc) and the per-line work is three passes:
 // chunk elimination -> separator system (one wave per column, PCR with
 // precomputed multipliers) -> chunk back-substitution fused with q += dq.
-#include "fribo_internal.h"
+#include "fribo_batch.h"
 
 #include <algorithm>
 
@@ -138,17 +138,17 @@ size_t jsolve_p_lds_bytes(int maxr, int jw, int stride)
 // lane without clamps or selects.  Four arrays leave room for eight waves
 // (two per SIMD) on grids up to n2 = 1537.
 template <int MAXR, int JW>
-__global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
+__device__ __forceinline__ void jsolve_hp_body(const FluidDev& F, const VBlock& vb)
 {
     constexpr int NR = MAXR + 1;
     extern __shared__ double lds[];
     const DevScalars* S = F.S;
     if (S->err) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int half = (int)gridDim.x >> 1;
-    const int f = (int)blockIdx.x < half ? 0 : 1;
-    const int wg = f == 0 ? (int)blockIdx.x : (int)blockIdx.x - half;
-    const int nwgf = f == 0 ? half : (int)gridDim.x - half;
+    const int half = (int)vb.gx >> 1;
+    const int f = (int)vb.x < half ? 0 : 1;
+    const int wg = f == 0 ? (int)vb.x : (int)vb.x - half;
+    const int nwgf = f == 0 ? half : (int)vb.gx - half;
     const KSys& K = F.js[f];
     const int nlines = f == 0 ? F.n3m : F.n3m - 1;
     const int kfirst = f == 0 ? 1 : 2;
@@ -278,21 +278,33 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
         wave_lds_sync();
     }
 }
+template <int MAXR, int JW>
+__global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
+{
+    jsolve_hp_body<MAXR, JW>(F, vb_hw());
+}
+template <int MAXR, int JW>
+__global__ __launch_bounds__(64 * JW) void kb_jsolve_hp(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    jsolve_hp_body<MAXR, JW>(*Fp, vb);
+}
 
 // The same solve with the factors read from global memory (grids whose chunk
 // factors do not fit LDS).
 template <int MAXR>
-__global__ __launch_bounds__(64) void k_jsolve_h(const FluidDev F)
+__device__ __forceinline__ void jsolve_h_body(const FluidDev& F, const VBlock& vb)
 {
     constexpr int NR = MAXR + 1;
     extern __shared__ double lds[];
     const DevScalars* S = F.S;
     if (S->err) return;
     const int lane = threadIdx.x & 63;
-    const int half = (int)gridDim.x >> 1;
-    const int f = (int)blockIdx.x < half ? 0 : 1;
-    const int wg = f == 0 ? (int)blockIdx.x : (int)blockIdx.x - half;
-    const int nwgf = f == 0 ? half : (int)gridDim.x - half;
+    const int half = (int)vb.gx >> 1;
+    const int f = (int)vb.x < half ? 0 : 1;
+    const int wg = f == 0 ? (int)vb.x : (int)vb.x - half;
+    const int nwgf = f == 0 ? half : (int)vb.gx - half;
     const KSys& K = F.js[f];
     const int nlines = f == 0 ? F.n3m : F.n3m - 1;
     const int kfirst = f == 0 ? 1 : 2;
@@ -366,6 +378,18 @@ __global__ __launch_bounds__(64) void k_jsolve_h(const FluidDev F)
         wave_lds_sync();
     }
 }
+template <int MAXR>
+__global__ __launch_bounds__(64) void k_jsolve_h(const FluidDev F)
+{
+    jsolve_h_body<MAXR>(F, vb_hw());
+}
+template <int MAXR>
+__global__ __launch_bounds__(64) void kb_jsolve_h(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    jsolve_h_body<MAXR>(*Fp, vb);
+}
 
 // Poisson j-lines (phcalc_outflow.F90:34-56): mode m solves
 // (A_y + wr(m)) v = hmn(:,m); each lane factors its chunk in registers
@@ -374,7 +398,7 @@ __global__ __launch_bounds__(64) void k_jsolve_h(const FluidDev F)
 // and the singular Neumann mode is gauge-fixed by pinning its last unknown
 // to 0.
 template <int MAXR, int JW>
-__global__ __launch_bounds__(64 * JW) void k_jsolve_p(const FluidDev F)
+__device__ __forceinline__ void jsolve_p_body(const FluidDev& F, const VBlock& vb)
 {
     constexpr int NR = MAXR + 1;
     extern __shared__ double lds[];
@@ -419,9 +443,9 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_p(const FluidDev F)
     line_map<NR>(slot, Q.cs, n, nch, stride, lane);
     double pre[NR];
     const int nlines = F.n3m;
-    const int step = (int)gridDim.x * JW;
+    const int step = (int)vb.gx * JW;
     const int iters = (nlines + step - 1) / step;
-    int line = (int)blockIdx.x * JW + wave;
+    int line = (int)vb.x * JW + wave;
     line_fetch<NR>(pre, F.hmn + (size_t)(line < nlines ? line : 0) * F.P, n, lane);
     __syncthreads();
     double* row = stage + p * stride;
@@ -517,6 +541,18 @@ __global__ __launch_bounds__(64 * JW) void k_jsolve_p(const FluidDev F)
         wave_lds_sync();
     }
 }
+template <int MAXR, int JW>
+__global__ __launch_bounds__(64 * JW) void k_jsolve_p(const FluidDev F)
+{
+    jsolve_p_body<MAXR, JW>(F, vb_hw());
+}
+template <int MAXR, int JW>
+__global__ __launch_bounds__(64 * JW) void kb_jsolve_p(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    jsolve_p_body<MAXR, JW>(*Fp, vb);
+}
 
 template <int MAXR>
 static void launch_h(const FluidDev& F, hipStream_t st)
@@ -600,6 +636,87 @@ void launch_jsolve_poisson(const FluidDev& F, hipStream_t st)
     }
 }
 
+// Batched j-lines: one kernel shape for the whole batch, the largest
+// chunk-length class of its instances; the instances' LDS strides (set by
+// the batch for that class) give the largest staging size.
+template <int MAXR>
+static void launch_h_b(BatchSet& B, hipStream_t st)
+{
+    int sd = 1;
+    for (const auto& F : B.hF) sd = std::max(sd, F.jstride);
+    const size_t lim = (size_t)B.hF[0].lds_max;
+    auto lines = [](const FluidDev& F) { return 2 * F.n3m - 1; };
+    if (jsolve_h_lds_bytes(MAXR, 8, sd, true) <= lim) {
+        blaunch(B, BK_JH, bdims(B, [&](const FluidDev& F) { return dim3(std::max(2, std::min(256, (lines(F) + 7) / 8) & ~1)); }),
+                kb_jsolve_hp<MAXR, 8>, dim3(512), jsolve_h_lds_bytes(MAXR, 8, sd, true), st);
+    } else if (jsolve_h_lds_bytes(MAXR, 4, sd, true) <= lim) {
+        blaunch(B, BK_JH, bdims(B, [&](const FluidDev& F) { return dim3(std::max(2, std::min(256, (lines(F) + 3) / 4) & ~1)); }),
+                kb_jsolve_hp<MAXR, 4>, dim3(256), jsolve_h_lds_bytes(MAXR, 4, sd, true), st);
+    } else if (jsolve_h_lds_bytes(MAXR, 2, sd, true) <= lim) {
+        blaunch(B, BK_JH, bdims(B, [&](const FluidDev& F) { return dim3(std::max(2, std::min(512, (lines(F) + 1) / 2) & ~1)); }),
+                kb_jsolve_hp<MAXR, 2>, dim3(128), jsolve_h_lds_bytes(MAXR, 2, sd, true), st);
+    } else {
+        blaunch(B, BK_JH, bdims(B, [&](const FluidDev& F) { return dim3(std::max(2, std::min(2048, lines(F)) & ~1)); }),
+                kb_jsolve_h<MAXR>, dim3(64), jsolve_h_lds_bytes(MAXR, 1, sd, false), st);
+    }
+}
+
+template <int MAXR>
+static void launch_p_b(BatchSet& B, hipStream_t st)
+{
+    int sd = 1;
+    for (const auto& F : B.hF) sd = std::max(sd, F.jstride);
+    const size_t lim = (size_t)B.hF[0].lds_max;
+    if (MAXR <= 8 && jsolve_p_lds_bytes(MAXR, 8, sd) <= lim) {
+        blaunch(B, BK_JP, bdims(B, [](const FluidDev& F) { return dim3(std::min(256, (F.n3m + 7) / 8)); }),
+                kb_jsolve_p<MAXR, 8>, dim3(512), jsolve_p_lds_bytes(MAXR, 8, sd), st);
+    } else if (jsolve_p_lds_bytes(MAXR, 4, sd) <= lim) {
+        blaunch(B, BK_JP, bdims(B, [](const FluidDev& F) { return dim3(std::min(256, (F.n3m + 3) / 4)); }),
+                kb_jsolve_p<MAXR, 4>, dim3(256), jsolve_p_lds_bytes(MAXR, 4, sd), st);
+    } else {
+        blaunch(B, BK_JP, bdims(B, [](const FluidDev& F) { return dim3(std::min(1024, F.n3m)); }),
+                kb_jsolve_p<MAXR, 1>, dim3(64), jsolve_p_lds_bytes(MAXR, 1, sd), st);
+    }
+}
+
+// Chunk-length classes of the j-line kernels: Helmholtz lines, Poisson lines.
+int jsolve_class_h(const FluidDev& F) { return jsolve_maxr(std::max(F.js[0].maxm, F.js[1].maxm)); }
+int jsolve_class_p(const FluidDev& F) { return jsolve_maxr(F.ps.maxm); }
+
+// LDS stride of an instance's j-lines when the batch runs chunk class mh
+// (Helmholtz) for every instance.
+int jsolve_stride_for(const FluidDev& F, int mh)
+{
+    const int mm = std::max(std::max(F.js[0].maxm, F.js[1].maxm), F.ps.maxm);
+    return std::max(mm, mh + 1) | 1;
+}
+
+void launch_jsolve_helm_b(BatchSet& B, hipStream_t st)
+{
+    int c = 8;
+    for (const auto& F : B.hF) c = std::max(c, jsolve_class_h(F));
+    switch (c) {
+    case 8: launch_h_b<8>(B, st); break;
+    case 16: launch_h_b<16>(B, st); break;
+    case 24: launch_h_b<24>(B, st); break;
+    case 32: launch_h_b<32>(B, st); break;
+    default: launch_h_b<64>(B, st); break;
+    }
+}
+
+void launch_jsolve_poisson_b(BatchSet& B, hipStream_t st)
+{
+    int c = 8;
+    for (const auto& F : B.hF) c = std::max(c, jsolve_class_p(F));
+    switch (c) {
+    case 8: launch_p_b<8>(B, st); break;
+    case 16: launch_p_b<16>(B, st); break;
+    case 24: launch_p_b<24>(B, st); break;
+    case 32: launch_p_b<32>(B, st); break;
+    default: launch_p_b<64>(B, st); break;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k-direction solves.  Field f = 0: q2 (rows kc=1..n3m), f = 1: q3 (interior
 // rows kc=2..n3m of the n3 system; the Dirichlet rows dqb3s/dqb3n enter the
@@ -652,16 +769,16 @@ __device__ inline double chunk_row(const double (&v)[NG], int t)
 // loads of the chunk are issued together (clamped rows past the chunk end);
 // lanes past the last column load column n2m and store nothing.
 template <int MAXM>
-__global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
+__device__ __forceinline__ void ksolve1_body(const FluidDev& F, const VBlock& vb)
 {
     constexpr int NG = (MAXM + 63) / 64;
     static_assert(NG <= 2, "chunk rows");
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int f = blockIdx.z, p = blockIdx.y;
+    const int f = vb.z, p = vb.y;
     const KSys& K = F.ks[f];
     if (p >= K.nch) return;
-    const int j = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = vb.x * 64 + threadIdx.x + 1;
     const bool act = j <= F.n2m;
     const int jc = act ? j : F.n2m;
     const double* __restrict__ src = (f == 0 ? F.sA : F.sB) + fidx(F, jc, K.kofs);
@@ -699,14 +816,26 @@ __global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
     F.kYe[o] = yp;
     F.kDe[o] = de;
 }
+template <int MAXM>
+__global__ __launch_bounds__(64) void k_ksolve1(const FluidDev F)
+{
+    ksolve1_body<MAXM>(F, vb_hw());
+}
+template <int MAXM>
+__global__ __launch_bounds__(64) void kb_ksolve1(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    ksolve1_body<MAXM>(*Fp, vb);
+}
 
 // Pass 2: one wave per column solves the nch separators (PCR over lanes with
 // the precomputed line-invariant multipliers).
-__global__ __launch_bounds__(64) void k_ksolve2(const FluidDev F)
+__device__ __forceinline__ void ksolve2_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int f = blockIdx.y, j = blockIdx.x + 1, p = threadIdx.x;
+    const int f = vb.y, j = vb.x + 1, p = threadIdx.x;
     const KSys& K = F.ks[f];
     const size_t o = kidx(F, f, j, p);
     double delta = 0.0;
@@ -724,19 +853,29 @@ __global__ __launch_bounds__(64) void k_ksolve2(const FluidDev F)
     }
     if (p < K.nch) F.kX[o] = delta * K.ib[p];
 }
+__global__ __launch_bounds__(64) void k_ksolve2(const FluidDev F)
+{
+    ksolve2_body(F, vb_hw());
+}
+__global__ __launch_bounds__(64) void kb_ksolve2(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    ksolve2_body(*Fp, vb);
+}
 
 // Pass 3: chunk back-substitution fused with q += dq (solve_fluid.F90:246-250).
 template <int MAXM>
-__global__ __launch_bounds__(64) void k_ksolve3(const FluidDev F)
+__device__ __forceinline__ void ksolve3_body(const FluidDev& F, const VBlock& vb)
 {
     constexpr int NG = (MAXM + 63) / 64;
     static_assert(NG <= 2, "chunk rows");
     const DevScalars* S = F.S;
     if (S->err) return;
-    const int f = blockIdx.z, p = blockIdx.y;
+    const int f = vb.z, p = vb.y;
     const KSys& K = F.ks[f];
     if (p >= K.nch) return;
-    const int j = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = vb.x * 64 + threadIdx.x + 1;
     const bool act = j <= F.n2m;
     const int jc = act ? j : F.n2m;
     const size_t P = F.P;
@@ -804,6 +943,18 @@ __global__ __launch_bounds__(64) void k_ksolve3(const FluidDev F)
         }
     }
 }
+template <int MAXM>
+__global__ __launch_bounds__(64) void k_ksolve3(const FluidDev F)
+{
+    ksolve3_body<MAXM>(F, vb_hw());
+}
+template <int MAXM>
+__global__ __launch_bounds__(64) void kb_ksolve3(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    ksolve3_body<MAXM>(*Fp, vb);
+}
 
 void launch_ksolve(const FluidDev& F, hipStream_t st)
 {
@@ -822,4 +973,24 @@ void launch_ksolve(const FluidDev& F, hipStream_t st)
     else if (maxm <= 32) hipLaunchKernelGGL(k_ksolve3<32>, g, dim3(64), 0, st, F);
     else if (maxm <= 64) hipLaunchKernelGGL(k_ksolve3<64>, g, dim3(64), 0, st, F);
     else hipLaunchKernelGGL(k_ksolve3<128>, g, dim3(64), 0, st, F);
+}
+
+void launch_ksolve_b(BatchSet& B, hipStream_t st)
+{
+    int maxm = 0;
+    for (const auto& F : B.hF) maxm = std::max(maxm, std::max(F.ks[0].maxm, F.ks[1].maxm));
+    const auto g = bdims(B, [](const FluidDev& F) {
+        return dim3((F.n2m + 63) / 64, std::max(F.ks[0].nch, F.ks[1].nch), 2);
+    });
+    if (maxm - 1 <= 8) blaunch(B, BK_K1, g, kb_ksolve1<8>, dim3(64), 0, st);
+    else if (maxm - 1 <= 16) blaunch(B, BK_K1, g, kb_ksolve1<16>, dim3(64), 0, st);
+    else if (maxm - 1 <= 32) blaunch(B, BK_K1, g, kb_ksolve1<32>, dim3(64), 0, st);
+    else if (maxm - 1 <= 64) blaunch(B, BK_K1, g, kb_ksolve1<64>, dim3(64), 0, st);
+    else blaunch(B, BK_K1, g, kb_ksolve1<128>, dim3(64), 0, st);
+    blaunch(B, BK_K2, bdims(B, [](const FluidDev& F) { return dim3(F.n2m, 2); }), kb_ksolve2, dim3(64), 0, st);
+    if (maxm <= 8) blaunch(B, BK_K3, g, kb_ksolve3<8>, dim3(64), 0, st);
+    else if (maxm <= 16) blaunch(B, BK_K3, g, kb_ksolve3<16>, dim3(64), 0, st);
+    else if (maxm <= 32) blaunch(B, BK_K3, g, kb_ksolve3<32>, dim3(64), 0, st);
+    else if (maxm <= 64) blaunch(B, BK_K3, g, kb_ksolve3<64>, dim3(64), 0, st);
+    else blaunch(B, BK_K3, g, kb_ksolve3<128>, dim3(64), 0, st);
 }

@@ -26,6 +26,7 @@
 // as in the Helmholtz k-solves: chunk elimination -> separator system (one
 // wave per line, PCR) -> chunk back-substitution.
 #include "fribo_ychunk.h"
+#include "fribo_batch.h"
 
 // Waves per workgroup (each wave its own 64 y-modes, or its own mode in the
 // separator pass).
@@ -54,14 +55,14 @@ __device__ inline void ychunk(const FluidDev& F, const KSys& K, int p, double w,
 // Lanes past the last y-mode work on mode n2m and store nothing (every lane
 // stays active through the row broadcasts).
 template <int MAXM>
-__global__ __launch_bounds__(64 * YW) void k_ysolve1(const FluidDev F)
+__device__ __forceinline__ void ysolve1_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const KSys& K = F.ks[0];
-    const int p = blockIdx.y;
+    const int p = vb.y;
     if (p >= K.nch) return;
-    const int j = (blockIdx.x * YW + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63) + 1;
+    const int j = (vb.x * YW + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63) + 1;
     const bool act = j <= F.n2m;
     const int jc = act ? j : F.n2m;
     const double* col = F.hmn + (jc - 1);
@@ -72,13 +73,25 @@ __global__ __launch_bounds__(64 * YW) void k_ysolve1(const FluidDev F)
     const int e = K.cs[p + 1] - 1;
     ysum_store(F.ysum, F.n2m, j, p, F.amphk, F.acphk, e, w, col[(size_t)e * F.P], yl, vl, wl, ys, vs, ws);
 }
+template <int MAXM>
+__global__ __launch_bounds__(64 * YW) void k_ysolve1(const FluidDev F)
+{
+    ysolve1_body<MAXM>(F, vb_hw());
+}
+template <int MAXM>
+__global__ __launch_bounds__(64 * YW) void kb_ysolve1(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    ysolve1_body<MAXM>(*Fp, vb);
+}
 
-__global__ __launch_bounds__(64 * YW) void k_ysolve2(const FluidDev F)
+__device__ __forceinline__ void ysolve2_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const KSys& K = F.ks[0];
-    const int j = blockIdx.x * YW + (threadIdx.x >> 6) + 1, p = threadIdx.x & 63, lane = p;
+    const int j = vb.x * YW + (threadIdx.x >> 6) + 1, p = threadIdx.x & 63, lane = p;
     if (j > F.n2m) return;   // whole wave
     const int nch = K.nch;
     const double* Y = F.ysum;
@@ -118,16 +131,26 @@ __global__ __launch_bounds__(64 * YW) void k_ysolve2(const FluidDev F)
     }
     if (p < nch) F.ysum[yidx(F, 6, j, p)] = delta * recip(beta);
 }
+__global__ __launch_bounds__(64 * YW) void k_ysolve2(const FluidDev F)
+{
+    ysolve2_body(F, vb_hw());
+}
+__global__ __launch_bounds__(64 * YW) void kb_ysolve2(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    ysolve2_body(*Fp, vb);
+}
 
 template <int MAXM>
-__global__ __launch_bounds__(64 * YW) void k_ysolve3(const FluidDev F)
+__device__ __forceinline__ void ysolve3_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const KSys& K = F.ks[0];
-    const int p = blockIdx.y;
+    const int p = vb.y;
     if (p >= K.nch) return;
-    const int j = (blockIdx.x * YW + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63) + 1;
+    const int j = (vb.x * YW + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63) + 1;
     const bool act = j <= F.n2m;
     const int jc = act ? j : F.n2m;
     double* col = F.hmn + (jc - 1);
@@ -151,6 +174,18 @@ __global__ __launch_bounds__(64 * YW) void k_ysolve3(const FluidDev F)
     }
     col[(size_t)e * P] = X;
 }
+template <int MAXM>
+__global__ __launch_bounds__(64 * YW) void k_ysolve3(const FluidDev F)
+{
+    ysolve3_body<MAXM>(F, vb_hw());
+}
+template <int MAXM>
+__global__ __launch_bounds__(64 * YW) void kb_ysolve3(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    ysolve3_body<MAXM>(*Fp, vb);
+}
 
 void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done)
 {
@@ -167,4 +202,23 @@ void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done)
     else if (mm <= 32) hipLaunchKernelGGL(k_ysolve3<32>, g, b, 0, st, F);
     else if (mm <= 64) hipLaunchKernelGGL(k_ysolve3<64>, g, b, 0, st, F);
     else hipLaunchKernelGGL(k_ysolve3<128>, g, b, 0, st, F);
+}
+
+void launch_ysolve_b(BatchSet& B, hipStream_t st, bool first_done)
+{
+    int mm = 0;
+    for (const auto& F : B.hF) mm = std::max(mm, F.ks[0].maxm - 1);
+    const dim3 b(64 * YW);
+    const auto g = bdims(B, [](const FluidDev& F) { return dim3((F.n2m + 64 * YW - 1) / (64 * YW), F.ks[0].nch); });
+    if (!first_done) {
+        if (mm <= 16) blaunch(B, BK_Y1, g, kb_ysolve1<16>, b, 0, st);
+        else if (mm <= 32) blaunch(B, BK_Y1, g, kb_ysolve1<32>, b, 0, st);
+        else if (mm <= 64) blaunch(B, BK_Y1, g, kb_ysolve1<64>, b, 0, st);
+        else blaunch(B, BK_Y1, g, kb_ysolve1<128>, b, 0, st);
+    }
+    blaunch(B, BK_Y2, bdims(B, [](const FluidDev& F) { return dim3((F.n2m + YW - 1) / YW); }), kb_ysolve2, b, 0, st);
+    if (mm <= 16) blaunch(B, BK_Y3, g, kb_ysolve3<16>, b, 0, st);
+    else if (mm <= 32) blaunch(B, BK_Y3, g, kb_ysolve3<32>, b, 0, st);
+    else if (mm <= 64) blaunch(B, BK_Y3, g, kb_ysolve3<64>, b, 0, st);
+    else blaunch(B, BK_Y3, g, kb_ysolve3<128>, b, 0, st);
 }

@@ -28,7 +28,9 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
            "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform", "fribo_loads_transfer",
            "fribo_checkbcs", "fribo_diagnostics", "fribo_save_state", "fribo_restore_state",
-           "fribo_phini", "fribo_phini_device", "fribo_loads_transfer_static"]
+           "fribo_phini", "fribo_phini_device", "fribo_loads_transfer_static",
+           "fribo_batch_create", "fribo_batch_destroy", "fribo_batch_set_markers", "fribo_batch_step",
+           "fribo_batch_set_graph", "fribo_batch_set_gemm_shape", "fribo_batch_get_gemm_shape"]
 
 NPROF = 12
 
@@ -102,6 +104,13 @@ def load():
     L.fribo_phini.argtypes = [C.c_int32, dp, C.c_double, C.c_int32, dp, dp, dp]
     L.fribo_phini_device.argtypes = [C.c_int32, dp, C.c_double, C.c_int32, dp, dp, dp]
     L.fribo_restore_state.argtypes = [vp]
+    L.fribo_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+    L.fribo_batch_destroy.argtypes = [vp]
+    L.fribo_batch_set_markers.argtypes = [vp, dp]
+    L.fribo_batch_step.argtypes = [vp, C.c_int32, dp, dp, dp, C.POINTER(C.c_int32)]
+    L.fribo_batch_set_graph.argtypes = [vp, C.c_int32]
+    L.fribo_batch_set_gemm_shape.argtypes = [vp, C.c_int32]
+    L.fribo_batch_get_gemm_shape.argtypes = [vp]
     _lib = L
     return L
 
@@ -346,3 +355,59 @@ def phini(zz, alx3, device=False):
     fn = load().fribo_phini_device if device else load().fribo_phini
     _check(fn(zz.size, _dp(zz), float(alx3), n, _dp(zmm), _dp(zmmt), _dp(wr)))
     return zmm, zmmt, wr
+
+
+class FluidBatch:
+    """Independent FluidStep instances advanced together, one kernel launch
+    per stage of the step for the whole batch (fribo_batch_*)."""
+
+    def __init__(self, steps):
+        self.steps = list(steps)
+        L = load()
+        hs = (C.c_void_p * len(self.steps))(*[s.h.value for s in self.steps])
+        b = C.c_void_p()
+        _check(L.fribo_batch_create(hs, len(self.steps), C.byref(b)))
+        self.b = b
+
+    def close(self):
+        if getattr(self, "b", None) is not None and self.b.value:
+            load().fribo_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_markers(self, markers):
+        """markers[i] = (lmcc, lmvel, lmA) of instance i, with the counts of
+        the instance's last set_markers: one staged copy for the batch."""
+        parts = []
+        for (lmcc, lmvel, lmA), s in zip(markers, self.steps):
+            if np.size(lmA) != s.nlm:
+                raise ValueError("marker count differs from the instance's last set_markers")
+            parts += [np.asarray(lmcc, dtype=np.float64).ravel(order="F"),
+                      np.asarray(lmvel, dtype=np.float64).ravel(order="F"),
+                      np.asarray(lmA, dtype=np.float64).ravel()]
+        buf = np.ascontiguousarray(np.concatenate(parts)) if parts else np.zeros(1)
+        _check(load().fribo_batch_set_markers(self.b, _dp(buf)))
+
+    def step(self, ns, dt, dt_o=None):
+        """One step of every instance: (dt, cflm, err) arrays."""
+        n = len(self.steps)
+        d = np.array(dt, dtype=np.float64).reshape(n).copy()
+        do = None if dt_o is None else np.ascontiguousarray(dt_o, dtype=np.float64)
+        cf = np.zeros(n)
+        err = (C.c_int32 * n)()
+        _check(load().fribo_batch_step(self.b, ns, _dp(d), _dp(do), _dp(cf), err))
+        return d, cf, np.array(err[:], dtype=np.int32)
+
+    def set_graph(self, on):
+        _check(load().fribo_batch_set_graph(self.b, 1 if on else 0))
+
+    def set_gemm_shape(self, cfg):
+        _check(load().fribo_batch_set_gemm_shape(self.b, int(cfg)))
+
+    def gemm_shape(self):
+        return load().fribo_batch_get_gemm_shape(self.b)

@@ -236,6 +236,36 @@ int fribo_get_profile(fribo_handle h, double* ms, int32_t* launches, int32_t ncl
 int fribo_reset_profile(fribo_handle h);
 const char* fribo_profile_name(int32_t cls);
 
+/* Batches of independent instances (no counterpart in the reference: many
+ * runs of fluid_tsch -- sweeps over Re, reduced velocity, mass ratio -- on
+ * uneven grids, advanced together).  The instances are ordinary handles
+ * made by fribo_init and filled by the calls above; fribo_batch_create
+ * groups them: from then on they share one stream, and fribo_batch_step
+ * advances all of them by one time step (fluid_tsch.F90:25-166 each) with
+ * one kernel launch per stage for the whole batch.  Each instance computes
+ * exactly what its own fribo_fluid_tsch would, up to the rounding of the
+ * Poisson gauge sums when the batch's transform tiles differ from the
+ * instance's own (1e-15 relative).  Every instance takes the transform
+ * along y when all of them can, else along z.
+ *   fribo_batch_set_markers: the marker kinematics of every instance in one
+ *     copy, lmcc_i, lmvel_i, lmA_i of instance i at offset 5*sum(nlm_<i)
+ *     (the counts of each instance's last fribo_set_markers);
+ *   fribo_batch_step: dt[i] in/out, dt_o[i] (NULL: dt), cflm[i] and err[i]
+ *     out (NULL to skip): the codes fribo_fluid_tsch would return;
+ *   fribo_batch_set_graph: replay the batch's launches as one HIP graph;
+ *   fribo_batch_set_gemm_shape: tile shape of the transforms (0: 32x32,
+ *     1: 64x64, 2: 96x128, 3: direct-to-LDS 96x64, -1: default), and
+ *     fribo_batch_get_gemm_shape the one in use.
+ * Destroy the batch before finalizing its instances. */
+typedef struct fribo_batch_ctx* fribo_batch;
+int fribo_batch_create(const fribo_handle* h, int32_t n, fribo_batch* out);
+int fribo_batch_destroy(fribo_batch b);
+int fribo_batch_set_markers(fribo_batch b, const double* mk);
+int fribo_batch_step(fribo_batch b, int32_t ns, double* dt, const double* dt_o, double* cflm, int32_t* err);
+int fribo_batch_set_graph(fribo_batch b, int32_t on);
+int fribo_batch_set_gemm_shape(fribo_batch b, int32_t cfg);
+int fribo_batch_get_gemm_shape(fribo_batch b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,149 @@
+"""Batched steps of uneven instances (fribo_batch_*) against the same
+instances stepped one by one (fribo_fluid_tsch).
+
+Bar: bit-identical fields and step scalars when the batch's transform tiles
+are the instances' own (every kernel runs the same arithmetic on the same
+data; only the launch grouping differs).  With other transform tiles the
+transforms themselves are still bit-identical (each output element sums
+over k in the same order) but the Poisson gauge constant is summed by tile
+column, so it rounds differently; the pressure accumulates that constant's
+rounding over the steps (1.6e-14 of its maximum after 3 steps), so the bar
+is 1e-14 of the largest value for the velocities and dt, 1e-13 for the
+pressure.  The captured (graph) batch equals the eager one bit for bit.
+"""
+import numpy as np
+import pytest
+
+from fribois2d_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(96, 72, 100.0), (128, 96, 150.0), (160, 120, 60.0), (224, 168, 200.0), (256, 192, 120.0)]
+FIELDS = ("q2", "q3", "pr", "ru2", "ru3")
+
+
+def _members():
+    out = []
+    for i, (n3m, n2m, re) in enumerate(SIZES):
+        out.append(cases.ensemble_case(i, n3m, n2m, re, amp=0.05 + 0.02 * i, freq=0.15 + 0.05 * i))
+    return out
+
+
+def _step(c):
+    from fribois2d_amd.gpu import FluidStep
+    g = FluidStep(c.params, c.rc, c.zz)   # the library's own phini decomposition
+    g.set_state(cases.initial_state(c.params, c.seed))
+    g.set_markers(*c.markers(0.0))
+    return g
+
+
+def _run_single(cs, nsteps):
+    out = []
+    for c in cs:
+        g = _step(c)
+        t, dt, hist = 0.0, c.params.dt0, []
+        for ns in range(nsteps):
+            g.set_markers(*c.markers(t))
+            dt, cf = g.fluid_tsch(ns, dt, dt)
+            hist.append((dt, cf))
+            t += dt
+        out.append((g.get_state(), np.array(hist)))
+        g.close()
+    return out
+
+
+def _run_batch(cs, nsteps, graph=False, shape=None, bad=None):
+    from fribois2d_amd.gpu import FluidBatch
+    gs = [_step(c) for c in cs]
+    b = FluidBatch(gs)
+    if graph:
+        b.set_graph(True)
+    if shape is not None:
+        b.set_gemm_shape(shape)
+    n = len(cs)
+    t = np.zeros(n)
+    dt = np.array([c.params.dt0 for c in cs])
+    hist = [[] for _ in cs]
+    errs = []
+    for ns in range(nsteps):
+        mk = [c.markers(t[i]) for i, c in enumerate(cs)]
+        if bad is not None and ns == nsteps - 1:
+            lmcc = mk[bad][0].copy()
+            lmcc[0, 1] = -1.0
+            mk[bad] = (lmcc, mk[bad][1], mk[bad][2])
+        b.set_markers(mk)
+        dtn, cf, err = b.step(ns, dt, dt)
+        errs.append(err)
+        for i in range(n):
+            if err[i] == 0:
+                hist[i].append((dtn[i], cf[i]))
+        t += dtn
+        dt = dtn
+    states = [g.get_state() for g in gs]
+    shape_used = b.gemm_shape()
+    b.close()
+    for g in gs:
+        g.close()
+    return states, [np.array(h) for h in hist], errs, shape_used
+
+
+def _compare(single, batch, bar):
+    worst = 0.0
+    for (st1, h1), st2, h2 in zip(single, batch[0], batch[1]):
+        if bar == 0.0:
+            np.testing.assert_array_equal(h1[:, 0], h2[:, 0])
+        else:
+            np.testing.assert_allclose(h2[:, 0], h1[:, 0], rtol=bar, atol=0)
+        for k in FIELDS:
+            a, b = st1[k], st2[k]
+            d = float(np.max(np.abs(a - b)) / max(np.max(np.abs(a)), 1e-300))
+            worst = max(worst, d)
+            if bar == 0.0:
+                assert d == 0.0, (k, d)
+            else:
+                assert d <= (10 * bar if k == "pr" else bar), (k, d)
+    return worst
+
+
+def test_batch_equals_single_instances():
+    cs = _members()
+    single = _run_single(cs, 4)
+    batch = _run_batch(cs, 4)
+    print(f"batch tile shape {batch[3]}", flush=True)
+    # every member is small enough to take the 32x32 transform tiles on its own
+    assert batch[3] == 0
+    _compare(single, batch, 0.0)
+
+
+def test_batch_with_other_transform_tiles():
+    cs = _members()
+    single = _run_single(cs, 3)
+    for shape in (1, 2):
+        batch = _run_batch(cs, 3, shape=shape)
+        worst = _compare(single, batch, 1e-14)
+        print(f"tile shape {shape}: largest relative difference {worst:.2e}", flush=True)
+
+
+def test_batch_graph_replay_equals_eager():
+    cs = _members()[:3]
+    eager = _run_batch(cs, 6)
+    graph = _run_batch(cs, 6, graph=True)
+    for s1, s2 in zip(eager[0], graph[0]):
+        for k in FIELDS:
+            np.testing.assert_array_equal(s1[k], s2[k])
+    for h1, h2 in zip(eager[1], graph[1]):
+        np.testing.assert_array_equal(h1, h2)
+
+
+def test_batch_instance_error_leaves_the_others():
+    """One member's marker leaves the domain on the last step: it reports
+    FRIBO_ERR_MARKER_OUTSIDE and keeps its state; the others step on."""
+    cs = _members()[:3]
+    ok = _run_batch(cs, 3)
+    bad = _run_batch(cs, 3, bad=1)
+    assert list(bad[2][-1]) == [0, 3, 0]
+    ref2 = _run_batch(cs, 2)
+    for k in FIELDS:
+        np.testing.assert_array_equal(bad[0][0][k], ok[0][0][k])
+        np.testing.assert_array_equal(bad[0][2][k], ok[0][2][k])
+        np.testing.assert_array_equal(bad[0][1][k], ref2[0][1][k])
