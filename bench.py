@@ -38,6 +38,161 @@ def pmc_traffic(name):
         return None
 
 
+def ensemble_members(n, seed=20261202):
+    """The batched ensemble: n cylinders, n3m uniform in 256..1024 (multiples
+    of 8), n2m = 3/4 n3m (multiple of 8), Re uniform in [60, 200], reduced
+    velocity Ur uniform in [3, 10] giving a heave of 0.1 D at frequency 1/Ur."""
+    import numpy as np
+    from fribois2d_amd import cases
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        n3m = 8 * int(rng.integers(32, 129))
+        n2m = 8 * int(round(0.75 * n3m / 8))
+        re = float(rng.uniform(60.0, 200.0))
+        ur = float(rng.uniform(3.0, 10.0))
+        out.append(cases.ensemble_case(i, n3m, n2m, re, amp=0.1, freq=1.0 / ur))
+    return out
+
+
+class EnsembleMarkers:
+    """Marker kinematics of every heaving member at time t[i], laid out as
+    fribo_batch_set_markers takes them, with the arithmetic of
+    Case.markers (so each member sees exactly its single-instance input)."""
+
+    def __init__(self, cs):
+        import numpy as np
+        self.cs = cs
+        rest = [c.markers(0.0) for c in cs]
+        self.nl = np.array([r[2].size for r in rest])
+        self.off = np.concatenate([[0], np.cumsum(5 * self.nl)])
+        self.buf = np.zeros(int(self.off[-1]))
+        self.iy, self.iz, self.ivy, self.ivz = [], [], [], []
+        ry, rz, yc, zc, inst = [], [], [], [], []
+        for i, (c, r) in enumerate(zip(cs, rest)):
+            o, n = int(self.off[i]), int(self.nl[i])
+            self.buf[o + 4 * n:o + 5 * n] = r[2]
+            self.iy.append(np.arange(o, o + n))
+            self.iz.append(np.arange(o + n, o + 2 * n))
+            self.ivy.append(np.arange(o + 2 * n, o + 3 * n))
+            self.ivz.append(np.arange(o + 3 * n, o + 4 * n))
+            l0 = c._rest
+            ry.append(l0[2])
+            rz.append(l0[3])
+            yc.append(np.full(n, c.center[0]))
+            zc.append(np.full(n, c.center[1]))
+            inst.append(np.full(n, i))
+        cat = np.concatenate
+        self.iy, self.iz, self.ivy, self.ivz = cat(self.iy), cat(self.iz), cat(self.ivy), cat(self.ivz)
+        self.ry, self.rz, self.yc, self.zc, self.inst = cat(ry), cat(rz), cat(yc), cat(zc), cat(inst)
+        self.buf[self.iz] = (self.zc + 0.0) + self.rz
+        self.buf[self.ivz] = 0.0
+        self.amp = np.array([c.motion.amp for c in cs])
+        self.w = np.array([c.motion.w for c in cs])
+
+    def at(self, t):
+        import numpy as np
+        wt = self.w * t
+        dy = self.amp * np.sin(wt)
+        vy = self.amp * self.w * np.cos(wt)
+        self.buf[self.iy] = (self.yc + dy[self.inst]) + self.ry
+        self.buf[self.ivy] = vy[self.inst]
+        return self.buf
+
+
+def run_ensemble(args):
+    """Batched ensemble of uneven cylinders: one launch per stage for the
+    whole batch, against the same instances stepped one at a time."""
+    import numpy as np
+    from fribois2d_amd import cases
+    from fribois2d_amd.gpu import FluidBatch, FluidStep
+    n = int(args.workload[len("ensemble"):] or 256)
+    cs = ensemble_members(n)
+    t0 = time.perf_counter()
+    gs = []
+    for c in cs:
+        g = FluidStep(c.params, c.rc, c.zz)
+        g.set_state(cases.initial_state(c.params, c.seed))
+        g.set_markers(*c.markers(0.0))
+        gs.append(g)
+    t_init = time.perf_counter() - t0
+    cells = np.array([(c.params.n2 - 1) * (c.params.n3 - 1) for c in cs], dtype=np.float64)
+    mk = EnsembleMarkers(cs)
+    b = FluidBatch(gs)
+    best, tune_ms = b.autotune(5)
+    b.set_graph(args.launch == "graph")
+    t = np.zeros(n)
+    dt = np.array([c.params.dt0 for c in cs])
+    ns = 0
+
+    def bstep():
+        nonlocal t, dt, ns
+        b.set_markers_flat(mk.at(t))
+        dtn, _, err = b.step(ns, dt, dt)
+        if err.any():
+            raise RuntimeError(f"instance errors {np.nonzero(err)[0][:8]} at step {ns}")
+        t = t + dtn
+        dt = dtn
+        ns += 1
+
+    for _ in range(args.warmup):
+        bstep()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        bstep()
+    el = time.perf_counter() - t1
+    value = float(cells.sum()) * args.steps / el / 1e6
+    b.close()
+    # the same instances, one fribo_fluid_tsch at a time (their own streams again)
+    nst = max(1, min(3, args.steps))
+    t2 = time.perf_counter()
+    for i, (c, g) in enumerate(zip(cs, gs)):
+        ti, di = float(t[i]), float(dt[i])
+        for k in range(nst):
+            g.set_markers(*c.markers(ti))
+            dn, _ = g.fluid_tsch(ns + k, di, di)
+            ti += dn
+            di = dn
+    g.sync()
+    el1 = time.perf_counter() - t2
+    single = float(cells.sum()) * nst / el1 / 1e6
+    for g in gs:
+        g.close()
+    # transform GEMM of the batch (forward, the autotuned shape): flops of
+    # every member's 2*n3m*n2m^2 (along y) per grouped launch
+    flops = float(sum(2.0 * (c.params.n3 - 1) * (c.params.n2 - 1) ** 2 for c in cs))
+    gemm_ms = float(tune_ms[best])
+    achieved = flops / (gemm_ms * 1e-3) / 1e12
+    shapes = {0: "32x32", 1: "64x64", 2: "96x128", 3: "direct-to-LDS 96x64"}
+    out = {
+        "metric": "Mcell-steps/s (batched ensemble, plain FP64)",
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated grids and markers, seeded 1e-3 velocity perturbation per member)",
+        "config": {"workload": f"ensemble{n}: {n} heaving cylinders, grids 256x192..1024x768 (uneven, "
+                               f"{int(cells.sum())} cells in all), Re 60-200, Ur 3-10, CFL 0.3",
+                   "precision": "plain FP64", "launch": f"batched, one launch per stage ({args.launch})"},
+        "unbatched": {"value": round(single, 3), "unit": "Mcell-steps/s",
+                      "how": f"the same {n} instances, {nst} fribo_fluid_tsch calls each, one after another"},
+        "batched_over_unbatched": round(value / single, 2),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_MFMA_TFLOPS, 4), "traffic": None,
+                     "kernel": "kb_gemm (grouped forward transform)", "avg_ms": round(gemm_ms, 4)},
+        "autotune": {"shape": shapes.get(best, best),
+                     "ms_per_launch": {shapes[k]: round(float(v), 4) for k, v in enumerate(tune_ms) if v > 0}},
+        "init_s": round(t_init, 1),
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +221,10 @@ def main():
         print(json.dumps({"kernel": "k_plain_copy", "bytes_read": nbytes, "bytes_written": nbytes,
                           "dispatches": 4, "ms_per_copy": round(ms, 4),
                           "GB/s": round(2 * nbytes / ms / 1e6, 1)}))
+        return
+
+    if args.workload.startswith("ensemble"):
+        run_ensemble(args)
         return
 
     import numpy as np

@@ -346,6 +346,7 @@ def _heave(amp, freq):
     def motion(t):
         return (amp * math.sin(w * t), 0.0, 0.0, amp * w * math.cos(w * t), 0.0, 0.0)
     motion.accel = lambda t: (-amp * w * w * math.sin(w * t), 0.0, 0.0)
+    motion.amp, motion.w = amp, w
     return motion
 
 

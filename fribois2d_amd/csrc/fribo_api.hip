@@ -1593,6 +1593,18 @@ static int batch_refresh(fribo_batch_ctx* b)
     return FRIBO_OK;
 }
 
+// The marker-staging event of an instance whose stream is about to be
+// destroyed: its last record would be on that stream, so a fresh
+// (unrecorded) event takes its place.
+static void rearm_marker_event(fribo_ctx* h)
+{
+    if (!h->ev_mk) return;
+    hipEventSynchronize(h->ev_mk);
+    hipEventDestroy(h->ev_mk);
+    h->ev_mk = nullptr;
+    hipEventCreateWithFlags(&h->ev_mk, hipEventDisableTiming);
+}
+
 static void batch_free(fribo_batch_ctx* b)
 {
     if (b->st) hipStreamSynchronize(b->st);
@@ -1613,6 +1625,7 @@ static void batch_free(fribo_batch_ctx* b)
     // the instances get streams of their own again
     for (auto h : b->inst) {
         if (h->batch != b) continue;
+        rearm_marker_event(h);
         hipStreamCreate(&h->st);
         h->batch = nullptr;
     }
@@ -1670,6 +1683,7 @@ extern "C" int fribo_batch_create(const fribo_handle* hs, int32_t n, fribo_batch
         }
         h->cfl_ready = false;
         fribo_set_transform(h, all_y ? 1 : 0);
+        rearm_marker_event(h);
         hipStreamDestroy(h->st);
         h->st = b->st;
         h->batch = b;
@@ -1856,4 +1870,52 @@ extern "C" int fribo_batch_set_gemm_shape(fribo_batch b, int32_t cfg)
 extern "C" int fribo_batch_get_gemm_shape(fribo_batch b)
 {
     return b ? b->B.gemm_cfg : -1;
+}
+
+// Autotuner of the batch's transform tile shape: the forward transform of
+// every instance (its output hmn is scratch between steps; nothing of the
+// state changes) with each shape all instances can run, reps timed launches
+// after one untimed, on the batch's own problems; the fastest shape is kept.
+// ms[4] receives the mean time per grouped launch of shapes 0..3 (0 where a
+// shape is not available).
+extern "C" int fribo_batch_autotune(fribo_batch b, int32_t reps, double* ms, int32_t* best_out)
+{
+    if (!b || reps < 1) return set_err(FRIBO_ERR_ARG, "bad autotune arguments");
+    int mask = 15;
+    for (auto h : b->inst) mask &= gemm_cfg_mask(h->F);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int best = -1;
+    float best_ms = 0.0f;
+    const int keep = b->gemm_fixed;
+    for (int cfg = 0; cfg < 4; ++cfg) {
+        if (ms) ms[cfg] = 0.0;
+        if (!((mask >> cfg) & 1)) continue;
+        b->gemm_fixed = cfg;
+        int e = batch_refresh(b);
+        if (e) {
+            b->gemm_fixed = keep;
+            return e;
+        }
+        launch_poisson_gemm_b(b->B, true, b->st);
+        HIPCHK(hipEventRecord(e0, b->st));
+        for (int r = 0; r < reps; ++r) launch_poisson_gemm_b(b->B, true, b->st);
+        HIPCHK(hipEventRecord(e1, b->st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipGetLastError());
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        t /= (float)reps;
+        if (ms) ms[cfg] = t;
+        if (best < 0 || t < best_ms) {
+            best = cfg;
+            best_ms = t;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    b->gemm_fixed = best;
+    if (best_out) *best_out = best;
+    return batch_refresh(b);
 }

@@ -30,7 +30,8 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_checkbcs", "fribo_diagnostics", "fribo_save_state", "fribo_restore_state",
            "fribo_phini", "fribo_phini_device", "fribo_loads_transfer_static",
            "fribo_batch_create", "fribo_batch_destroy", "fribo_batch_set_markers", "fribo_batch_step",
-           "fribo_batch_set_graph", "fribo_batch_set_gemm_shape", "fribo_batch_get_gemm_shape"]
+           "fribo_batch_set_graph", "fribo_batch_set_gemm_shape", "fribo_batch_get_gemm_shape",
+           "fribo_batch_autotune"]
 
 NPROF = 12
 
@@ -111,6 +112,7 @@ def load():
     L.fribo_batch_set_graph.argtypes = [vp, C.c_int32]
     L.fribo_batch_set_gemm_shape.argtypes = [vp, C.c_int32]
     L.fribo_batch_get_gemm_shape.argtypes = [vp]
+    L.fribo_batch_autotune.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int32)]
     _lib = L
     return L
 
@@ -411,3 +413,16 @@ class FluidBatch:
 
     def gemm_shape(self):
         return load().fribo_batch_get_gemm_shape(self.b)
+
+    def autotune(self, reps=5):
+        """Pick the transform tile shape by timing; returns (best, ms per shape)."""
+        ms = np.zeros(4)
+        best = C.c_int32(-1)
+        _check(load().fribo_batch_autotune(self.b, int(reps), _dp(ms), C.byref(best)))
+        return best.value, ms
+
+    def set_markers_flat(self, buf):
+        """Marker kinematics of all instances, already laid out as
+        fribo_batch_set_markers takes them."""
+        buf = np.ascontiguousarray(buf, dtype=np.float64)
+        _check(load().fribo_batch_set_markers(self.b, _dp(buf)))

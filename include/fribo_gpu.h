@@ -265,6 +265,11 @@ int fribo_batch_step(fribo_batch b, int32_t ns, double* dt, const double* dt_o, 
 int fribo_batch_set_graph(fribo_batch b, int32_t on);
 int fribo_batch_set_gemm_shape(fribo_batch b, int32_t cfg);
 int fribo_batch_get_gemm_shape(fribo_batch b);
+/* Autotuner (no reference counterpart): times the batch's forward transform
+ * with every tile shape all its instances can run (reps launches each, no
+ * change to any state) and keeps the fastest; ms[4] gets the time per
+ * launch of shapes 0..3 (0: not available), best the shape kept. */
+int fribo_batch_autotune(fribo_batch b, int32_t reps, double* ms, int32_t* best);
 
 #ifdef __cplusplus
 }

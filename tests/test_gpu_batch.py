@@ -147,3 +147,32 @@ def test_batch_instance_error_leaves_the_others():
         np.testing.assert_array_equal(bad[0][0][k], ok[0][0][k])
         np.testing.assert_array_equal(bad[0][2][k], ok[0][2][k])
         np.testing.assert_array_equal(bad[0][1][k], ref2[0][1][k])
+
+
+def test_instances_step_on_alone_after_the_batch():
+    """Two batched steps, the batch destroyed, two more steps of each
+    instance on its own: the same as four single-instance steps."""
+    from fribois2d_amd.gpu import FluidBatch
+    cs = _members()[:3]
+    single = _run_single(cs, 4)
+    gs = [_step(c) for c in cs]
+    b = FluidBatch(gs)
+    t = np.zeros(len(cs))
+    dt = np.array([c.params.dt0 for c in cs])
+    for ns in range(2):
+        b.set_markers([c.markers(t[i]) for i, c in enumerate(cs)])
+        dtn, _, err = b.step(ns, dt, dt)
+        assert not err.any()
+        t += dtn
+        dt = dtn
+    b.close()
+    for i, (c, g) in enumerate(zip(cs, gs)):
+        ti, di = t[i], dt[i]
+        for ns in range(2, 4):
+            g.set_markers(*c.markers(ti))
+            di, _ = g.fluid_tsch(ns, di, di)
+            ti += di
+        st = g.get_state()
+        for k in FIELDS:
+            np.testing.assert_array_equal(st[k], single[i][0][k])
+        g.close()
