@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_MFMA_TFLOPS = 78.6
+PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
@@ -203,6 +204,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--transform", default="auto", choices=["auto", "y", "z"],
                     help="axis of the Poisson eigen-transform (auto: the shorter one)")
+    ap.add_argument("--precision", type=int, default=-1,
+                    help="Poisson transforms: -1 FP64; n >= 0 FP32 MFMA + n FP64-residual refinement sweeps")
     ap.add_argument("--launch", default="stream", choices=["stream", "graph"],
                     help="kernel launches of a step: one by one on the stream, or one captured HIP graph "
                          "(steps with GEMM timing events always launch on the stream)")
@@ -239,6 +242,8 @@ def main():
     st0 = cases.initial_state(p, c.seed)
     g = FluidStep(p, c.rc, c.zz, zmm, zmmt, wr)
     g.set_transform(args.transform)
+    if args.precision >= 0:
+        g.set_precision(args.precision)
     g.set_graph(args.launch == "graph")
     g.set_state(st0)
 
@@ -293,8 +298,11 @@ def main():
     # 192 tiles of 96x128 in the library's size test, operands with padded
     # leading dimensions; the kernel itself runs 96x64 tiles)
     gemm_kernel = "k_gemm_dma" if -(-n2m // 96) * -(-n3m // 128) >= 192 else "k_gemm_nt"
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_MFMA_TFLOPS, 4), "traffic": pmc_traffic(gemm_kernel),
+    peak = PEAK_FP64_MFMA_TFLOPS
+    if args.precision >= 0:
+        gemm_kernel, peak = "k_gemm32", PEAK_FP32_MFMA_TFLOPS
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": pmc_traffic(gemm_kernel),
             "kernel": gemm_kernel, "avg_ms": round(gemm_avg_s * 1e3, 4), "transform_axis": axis,
             "flops_per_launch": gemm_flops}
     kernels = {k: {"ms_per_step": round(v[0] / nb, 4), "launches_per_step": v[1] / nb}
@@ -348,7 +356,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (generated grid and markers, seeded 1e-3 velocity perturbation)",
         "config": {"workload": f"{args.workload}: {n3m}x{n2m} cells, {c.nlm} markers, Re={1.0 / p.visc:g}, CFL {p.cflmax}",
-                   "precision": "plain FP64", "poisson_transform": f"along {axis}", "launch": args.launch},
+                   "precision": "plain FP64" if args.precision < 0 else
+                   f"FP32 MFMA transforms + {args.precision} FP64-residual refinement sweeps",
+                   "poisson_transform": f"along {axis}", "launch": args.launch},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -84,6 +84,7 @@ struct fribo_ctx {
     hipGraphExec_t gexec = nullptr;
     int graph_nlm = -1;
     int mk_gen = 0;              // bumped whenever the marker buffers are reallocated
+    int ngpart64 = 0;            // gauge partials of the FP64 backward transform along y
     struct fribo_batch_ctx* batch = nullptr;   // batch this instance belongs to (shares its stream)
     int dev = 0;                 // device of the handle
     double* scratch = nullptr;   // device address of the static loads scratch on that device
@@ -339,6 +340,7 @@ static int setup_ytransform(fribo_ctx* c, const double* zmmt, int ldz)
     if (!e) e = dalloc(c, &F.zg, F.n3m);
     if (!e) e = dalloc(c, &F.ysum, (size_t)7 * n * FRIBO_KMAXCH);
     F.ngpart = gemm_nn_tiles_n(n, F.n3m, n, F.ymm, F.ldy, F.hmn, F.P);
+    c->ngpart64 = F.ngpart;
     // room for the gauge partials of any tile shape (a batch may use 32-column tiles)
     if (!e) e = dalloc(c, &F.gpart, std::max(F.ngpart, (F.n3m + 31) / 32));
     if (e) return e;
@@ -420,6 +422,7 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     c->lds_limit = prop.sharedMemPerBlock;
     FluidDev& F = c->F;
     memset(&F, 0, sizeof(F));
+    F.refine = -1;
     F.n2 = p->n2; F.n3 = p->n3; F.n2m = p->n2 - 1; F.n3m = p->n3 - 1;
     F.lds_max = (int)c->lds_limit;
     F.P = (F.n2 + 15) / 16 * 16;
@@ -834,6 +837,33 @@ static void enqueue_poisson(fribo_ctx* c)
 {
     FluidDev& F = c->F;
     hipStream_t st = c->st;
+    if (F.taxis == 1 && F.refine >= 0) {
+        // mixed precision: x0 = P32(rhs), then F.refine sweeps of
+        // x += P32(rhs - L x) with the residual in FP64; P32 is the solve with
+        // FP32 transforms and the FP64 per-mode z-lines.  The iterate lives
+        // in sA, the right-hand side (for a second sweep) in sB, the final
+        // solution in dph.
+        double* d = F.dph + fidx(F, 1, 1);
+        double* xa = F.sA + fidx(F, 1, 1);
+        double* rb = F.sB + fidx(F, 1, 1);
+        const size_t fbytes = (size_t)F.P * F.n3m * sizeof(double);
+        if (F.refine >= 2) hipMemcpyAsync(rb, d, fbytes, hipMemcpyDeviceToDevice, st);
+        for (int it = 0; it <= F.refine; ++it) {
+            if (it > 0)
+                run(c, PC_JP, [&] { launch_pres_residual(F, F.sA, it == 1 ? F.dph : F.sB, F.dph, st); });
+            run_ev(c, PC_GEMM, [&](hipEvent_t a, hipEvent_t b) {
+                launch_gemm32(F.n2m, F.n3m, F.n2m, F.ymmi, F.ldy, d, F.P, F.hmn, F.P, nullptr, F.S, st, nullptr,
+                              nullptr, a, b);
+            });
+            run(c, PC_JP, [&] { launch_ysolve(F, st, false); });
+            const bool last = it == F.refine;
+            run_ev(c, PC_GEMM, [&](hipEvent_t a, hipEvent_t b) {
+                launch_gemm32(F.n2m, F.n3m, F.n2m, F.ymm, F.ldy, F.hmn, F.P, last ? d : xa, F.P,
+                              it > 0 ? xa : nullptr, F.S, st, last ? F.zg : nullptr, last ? F.gpart : nullptr, a, b);
+            });
+        }
+        return;
+    }
     if (F.taxis == 1) {
         // the chunk pass of the per-mode z-lines runs in the forward
         // transform's epilogue when the line partition has equal chunks
@@ -1342,6 +1372,8 @@ extern "C" int fribo_set_transform(fribo_handle h, int32_t axis)
     if (want != h->F.taxis) {
         HIPCHK(hipStreamSynchronize(h->st));
         h->F.taxis = want;
+        if (want != 1) h->F.refine = -1;
+        if (want == 1 && h->F.refine < 0) h->F.ngpart = h->ngpart64;
         const double zero = 0.0;
         HIPCHK(hipMemcpy(&h->F.S->dphoff, &zero, sizeof(double), hipMemcpyHostToDevice));
         if (h->gexec) {
@@ -1918,4 +1950,30 @@ extern "C" int fribo_batch_autotune(fribo_batch b, int32_t reps, double* ms, int
     b->gemm_fixed = best;
     if (best_out) *best_out = best;
     return batch_refresh(b);
+}
+
+// Precision of the Poisson transforms (no counterpart in the reference, whose
+// DGEMMs are FP64): refine < 0, FP64 (the default); refine = n >= 0, FP32
+// MFMA transforms with FP32 accumulation followed by n sweeps of FP64-residual
+// refinement (x += solve(rhs - L x)).  Transforms along y only.
+extern "C" int fribo_set_precision(fribo_handle h, int32_t refine)
+{
+    if (!h || refine > 3) return set_err(FRIBO_ERR_ARG, "refine must be < 0 (FP64) or 0..3");
+    if (refine >= 0 && h->F.taxis != 1)
+        return set_err(FRIBO_ERR_LIMIT, "the mixed-precision solve needs the transform along y");
+    if (h->batch) return set_err(FRIBO_ERR_ARG, "instance belongs to a batch");
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->F.refine = refine < 0 ? -1 : refine;
+    if (h->F.taxis == 1) h->F.ngpart = refine < 0 ? h->ngpart64 : gemm32_tiles_n(h->F.n3m);
+    if (h->gexec) {
+        hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+        h->graph_nlm = -1;
+    }
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_get_precision(fribo_handle h)
+{
+    return h ? h->F.refine : -2;
 }

@@ -622,3 +622,167 @@ bool gemm_fuse_ok(const FluidDev& F)
     return csz > 1 && csz <= 33 && GEMM_DMA_BN % csz == 0 && F.n3m % GEMM_DMA_BN == 0 &&
            GEMM_DMA_BN / csz <= kChunkGroups;
 }
+
+// ---------------------------------------------------------------------------
+// FP32 transforms of the mixed-precision Poisson solve: C = X0 + A * B with
+// the operands read in FP64 (the FP64 eigen-matrices and fields), rounded to
+// FP32 as they are staged in LDS, multiplied on v_mfma_f32_16x16x4_f32 with
+// FP32 accumulation, and the result written back in FP64 (added to X0 when
+// X0 is given).  Only the C = A * B form of the transforms along y ("NN": B
+// is k-contiguous).  Fragment layout of v_mfma_f32_16x16x4_f32: A lane l
+// holds A[l%16][l/16], B lane l holds B[l/16][l%16]; accumulator element v
+// of lane l is C[4*(l/16) + v][l%16].  The gauge partials (zg) are taken
+// from the final FP64 values of row M-1.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm32(int M, int N, int K, const double* __restrict__ A, int lda,
+                                                         const double* __restrict__ B, int ldb, double* __restrict__ C,
+                                                         int ldc, const double* __restrict__ X0,
+                                                         const DevScalars* S, const double* __restrict__ zg,
+                                                         double* gpart)
+{
+    constexpr int NT = 64 * WM * WN;
+    constexpr int TM = BM / WM, TN = BN / WN;
+    constexpr int RM = TM / 16, RN = TN / 16;
+    constexpr int LA = BM + 4;                   // A: k-rows of BM floats
+    constexpr int LB = BN + 1;                   // B: k-rows of BN floats (transposing store)
+    constexpr int NA = (BM * GBK + NT - 1) / NT;
+    constexpr int NB = (BN * GBK + NT - 1) / NT;
+    static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile");
+    if (S && S->err) return;
+    __shared__ float As[2][GBK][LA];
+    __shared__ float Bs[2][GBK][LB];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int i0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    float ra[NA], rb[NB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = t + u * NT, kk = e / BM, r = e % BM;
+            const bool ok = (e < BM * GBK) && (k0 + kk < K) && (i0 + r < M);
+            ra[u] = ok ? (float)A[(size_t)(k0 + kk) * lda + i0 + r] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = t + u * NT, kk = e % GBK, r = e / GBK;
+            const bool ok = (e < BN * GBK) && (k0 + kk < K) && (n0 + r < N);
+            rb[u] = ok ? (float)B[(size_t)(n0 + r) * ldb + k0 + kk] : 0.0f;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = t + u * NT;
+            if (e < BM * GBK) As[buf][e / BM][e % BM] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = t + u * NT;
+            if (e < BN * GBK) Bs[buf][e % GBK][e / GBK] = rb[u];
+        }
+    };
+    f32x4 acc[RM][RN];
+#pragma unroll
+    for (int r = 0; r < RM; ++r)
+#pragma unroll
+        for (int c = 0; c < RN; ++c) acc[r][c] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    load(0);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += GBK) {
+        const bool more = k0 + GBK < K;
+        if (more) load(k0 + GBK);
+#pragma unroll
+        for (int k4 = 0; k4 < GBK; k4 += 4) {
+            const int kr = k4 + (lane >> 4);
+            float a[RM], b[RN];
+#pragma unroll
+            for (int r = 0; r < RM; ++r) a[r] = As[buf][kr][wm * TM + r * 16 + (lane & 15)];
+#pragma unroll
+            for (int c = 0; c < RN; ++c) b[c] = Bs[buf][kr][wn * TN + c * 16 + (lane & 15)];
+#pragma unroll
+            for (int r = 0; r < RM; ++r)
+#pragma unroll
+                for (int c = 0; c < RN; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], b[c], acc[r][c], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    double g = 0.0;
+#pragma unroll
+    for (int r = 0; r < RM; ++r)
+#pragma unroll
+        for (int c = 0; c < RN; ++c) {
+            const int col = n0 + wn * TN + c * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = i0 + wm * TM + r * 16 + 4 * (lane >> 4) + v;
+                if (row < M && col < N) {
+                    const size_t q = (size_t)col * ldc + row;
+                    const double x = (X0 ? X0[q] : 0.0) + (double)acc[r][c][v];
+                    C[q] = x;
+                    if (zg && row == M - 1) g = g + zg[col] * x;
+                }
+            }
+        }
+    if (zg && M - 1 >= i0 && M - 1 < i0 + BM) {
+        __shared__ double gsh[WM * WN];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) g = g + __shfl_xor(g, o, 64);
+        if (lane == 0) gsh[wave] = g;
+        __syncthreads();
+        if (t == 0) {
+            double s = gsh[0];
+            for (int w = 1; w < WM * WN; ++w) s = s + gsh[w];
+            gpart[blockIdx.y] = s;
+        }
+    }
+}
+
+// Tile columns of the FP32 transform (the gauge partials it writes).
+int gemm32_tiles_n(int N) { return cdiv(N, 128); }
+
+void launch_gemm32(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+                   const double* X0, const DevScalars* S, hipStream_t st, const double* zg, double* gpart,
+                   hipEvent_t e0, hipEvent_t e1)
+{
+    launch_timed(k_gemm32<128, 128, 2, 4>, dim3(cdiv(M, 128), cdiv(N, 128)), dim3(512), st, e0, e1, M, N, K, A, lda,
+                 B, ldb, C, ldc, X0, S, zg, gpart);
+}
+
+// Diagnostic entry point: the FP32 transform on host arrays, C = A * B
+// (column-major, lda/ldb/ldc), for the layout and accuracy test.
+extern "C" int fribo_test_gemm32(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
+                                 int32_t ldb, double* C, int32_t ldc)
+{
+    if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C || lda < M || ldb < K || ldc < M) return 5;
+    double *a = nullptr, *b = nullptr, *c = nullptr;
+    const size_t na = (size_t)lda * K, nb = (size_t)ldb * N, nc = (size_t)ldc * N;
+    if (hipMalloc(&a, na * 8) != hipSuccess || hipMalloc(&b, nb * 8) != hipSuccess ||
+        hipMalloc(&c, nc * 8) != hipSuccess) {
+        hipFree(a);
+        hipFree(b);
+        hipFree(c);
+        return 6;
+    }
+    int rc = 0;
+    if (hipMemcpy(a, A, na * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b, B, nb * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c, C, nc * 8, hipMemcpyHostToDevice) != hipSuccess)
+        rc = 6;
+    if (!rc) {
+        launch_gemm32(M, N, K, a, lda, b, ldb, c, ldc, nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(C, c, nc * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = 6;
+    }
+    hipFree(a);
+    hipFree(b);
+    hipFree(c);
+    return rc;
+}

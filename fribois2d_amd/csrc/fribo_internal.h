@@ -94,6 +94,7 @@ struct FluidDev {
     int nlm;
     int null_mode;           // 1-based index of the singular Poisson mode, 0 if none
     int taxis;               // eigen-transform axis of the Poisson solve: 0 z (zmm), 1 y (ymm)
+    int refine;              // -1: FP64 transforms; n >= 0: FP32 transforms + n FP64-residual sweeps
     double alx3, rext2, ren, visc, cou, q3uw, q3lw, dtmax, cflmax, cfllim, radiusSD;
     double dx2, dx3, dx2q, dx3q;
     double gam[4], rom[4], alm[4];
@@ -227,6 +228,13 @@ int gemm_nn_tiles_n(int M, int N, int K, const double* A, int lda, const double*
 // first_done: the chunk pass already ran in the forward transform's epilogue
 void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done = false);
 void launch_loads(const FluidDev& F, double hpr, hipStream_t st);
+// mixed-precision Poisson solve: r = rhs - (A_y + A_z) x (r may alias rhs),
+// and the FP32 transform C = X0 + A * B (X0 optional, may alias C)
+void launch_pres_residual(const FluidDev& F, const double* x, const double* rhs, double* r, hipStream_t st);
+void launch_gemm32(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+                   const double* X0, const DevScalars* S, hipStream_t st, const double* zg, double* gpart,
+                   hipEvent_t e0, hipEvent_t e1);
+int gemm32_tiles_n(int N);
 void launch_checkbcs(const FluidDev& F, double* nb, hipStream_t st);
 void launch_diag(const FluidDev& F, double* part, int kv, double* out, hipStream_t st);
 // device: the symmetrised eigen-problem on the GPU (rocSOLVER STEDC) instead

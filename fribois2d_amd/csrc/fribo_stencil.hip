@@ -1079,3 +1079,41 @@ void launch_diag(const FluidDev& F, double* part, int kv, double* out, hipStream
     hipLaunchKernelGGL(k_diag, dim3(DIAG_NB), dim3(256), 0, st, F, part);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, F, (const double*)part, kv, out);
 }
+
+// ---------------------------------------------------------------------------
+// Residual of the pressure Poisson system for the mixed-precision solve:
+// r = rhs - (A_y + A_z) x over the n2m x n3m cells, with the tridiagonal
+// operators the transforms invert (phini.F90:121-142: rows amphj/acphj/apphj
+// along y, amphk/acphk/apphk along z, the couplings past the first and last
+// rows dropped).  One thread per column j marches RES_MK planes with x of
+// planes k-1, k, k+1 in registers; r may alias rhs.
+#define RES_MK 16
+__global__ __launch_bounds__(256) void k_pres_residual(const FluidDev F, const double* __restrict__ x,
+                                                       const double* rhs, double* r)
+{
+    if (F.S->err) return;
+    const int j = blockIdx.x * 256 + threadIdx.x + 1;
+    if (j > F.n2m) return;
+    const int k0 = blockIdx.y * RES_MK + 1, k1 = min(k0 + RES_MK - 1, F.n3m);
+    const double am = j > 1 ? F.amphj[j] : 0.0, ac = F.acphj[j], ap = j < F.n2m ? F.apphj[j] : 0.0;
+    const bool jl = j > 1, jr = j < F.n2m;
+    double xm = k0 > 1 ? x[fidx(F, j, k0 - 1)] : 0.0;
+    double xc = x[fidx(F, j, k0)];
+    for (int k = k0; k <= k1; ++k) {
+        const size_t c = fidx(F, j, k);
+        const double xp = k < F.n3m ? x[c + F.P] : 0.0;
+        const double xw = jl ? x[c - 1] : 0.0, xe = jr ? x[c + 1] : 0.0;
+        const double bk = k > 1 ? F.amphk[k] : 0.0, ck = F.acphk[k], tk = k < F.n3m ? F.apphk[k] : 0.0;
+        const double ly = am * xw + ac * xc + ap * xe;
+        const double lz = bk * xm + ck * xc + tk * xp;
+        r[c] = rhs[c] - (ly + lz);
+        xm = xc;
+        xc = xp;
+    }
+}
+
+void launch_pres_residual(const FluidDev& F, const double* x, const double* rhs, double* r, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pres_residual, dim3((F.n2m + 255) / 256, (F.n3m + RES_MK - 1) / RES_MK), dim3(256), 0, st,
+                       F, x, rhs, r);
+}

@@ -31,7 +31,7 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_phini", "fribo_phini_device", "fribo_loads_transfer_static",
            "fribo_batch_create", "fribo_batch_destroy", "fribo_batch_set_markers", "fribo_batch_step",
            "fribo_batch_set_graph", "fribo_batch_set_gemm_shape", "fribo_batch_get_gemm_shape",
-           "fribo_batch_autotune"]
+           "fribo_batch_autotune", "fribo_set_precision", "fribo_get_precision", "fribo_test_gemm32"]
 
 NPROF = 12
 
@@ -113,6 +113,9 @@ def load():
     L.fribo_batch_set_gemm_shape.argtypes = [vp, C.c_int32]
     L.fribo_batch_get_gemm_shape.argtypes = [vp]
     L.fribo_batch_autotune.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int32)]
+    L.fribo_set_precision.argtypes = [vp, C.c_int32]
+    L.fribo_get_precision.argtypes = [vp]
+    L.fribo_test_gemm32.argtypes = [C.c_int32, C.c_int32, C.c_int32, dp, C.c_int32, dp, C.c_int32, dp, C.c_int32]
     _lib = L
     return L
 
@@ -270,6 +273,15 @@ class FluidStep:
     def transform(self):
         return "y" if load().fribo_get_transform(self.h) == 1 else "z"
 
+    def set_precision(self, refine):
+        """Poisson transforms: refine < 0 FP64; n >= 0 FP32 MFMA plus n
+        FP64-residual refinement sweeps."""
+        _check(load().fribo_set_precision(self.h, int(refine)))
+
+    @property
+    def precision(self):
+        return load().fribo_get_precision(self.h)
+
     def set_graph(self, on=True):
         """Replay the step as one captured HIP graph (same results)."""
         _check(load().fribo_set_graph(self.h, 1 if on else 0))
@@ -325,6 +337,19 @@ def gemm_nn(A, B, pad=0):
     Cm = np.zeros((la, N), order="F")
     _check(L.fribo_test_gemm_nn(M, N, K, _dp(Ap), la, _dp(Bp), lb, _dp(Cm), la))
     return Cm[:M]
+
+
+def gemm32(A, B):
+    """C = A @ B by the FP32 transform kernel (operands rounded to FP32,
+    FP32 accumulation; test hook)."""
+    L = load()
+    M, K = A.shape
+    N = B.shape[1]
+    Ap = np.asfortranarray(A, dtype=np.float64)
+    Bp = np.asfortranarray(B, dtype=np.float64)
+    Cm = np.zeros((M, N), order="F")
+    _check(L.fribo_test_gemm32(M, N, K, _dp(Ap), M, _dp(Bp), K, _dp(Cm), M))
+    return Cm
 
 
 def plain_copy(nbytes, reps):

@@ -194,6 +194,19 @@ int fribo_test_gemm_nn(int32_t M, int32_t N, int32_t K, const double* A, int32_t
 int fribo_set_transform(fribo_handle h, int32_t axis);
 int fribo_get_transform(fribo_handle h);
 
+/* Execution option (no reference counterpart; the reference's DGEMMs are
+ * FP64): precision of the Poisson eigen-transforms along y.  refine < 0:
+ * FP64 MFMA (the default).  refine = n in 0..3: the transforms on FP32 MFMA
+ * with FP32 accumulation (operands rounded to FP32, results back in FP64),
+ * the per-mode line solves in FP64, then n sweeps of FP64-residual iterative
+ * refinement x += solve(rhs - (A_y + A_z) x).  FRIBO_ERR_LIMIT when the
+ * transform is along z.  fribo_get_precision returns the setting. */
+int fribo_set_precision(fribo_handle h, int32_t refine);
+int fribo_get_precision(fribo_handle h);
+/* Diagnostic entry point: the FP32 transform C = A * B on host arrays. */
+int fribo_test_gemm32(int32_t M, int32_t N, int32_t K, const double* A, int32_t lda, const double* B,
+                      int32_t ldb, double* C, int32_t ldc);
+
 /* phini (phini.F90:4-99) without LAPACK: the z pressure operator of
  * tridiag_matrices (phini.F90:121-142) built from zz(1:n3) and alx3 is
  * similar to a symmetric tridiagonal matrix, decomposed by implicit QL;
