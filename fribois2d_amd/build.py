@@ -20,7 +20,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfribo_gpu.so")
 SOURCES = ["fribo_api.hip", "fribo_stencil.hip", "fribo_tridiag.hip", "fribo_mls.hip", "fribo_gemm.hip",
-           "fribo_eig.hip", "fribo_ysolve.hip"]
+           "fribo_eig.hip", "fribo_ysolve.hip", "fribo_mlp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",

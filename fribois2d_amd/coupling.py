@@ -77,18 +77,14 @@ class RigidBody:
         fp = -press[:, None] * self.lmnor * self.lmA[:, None]
         fv = tau * self.lmA[:, None]
         ftot = fp + fv
-        fyp = fzp = fyv = fzv = 0.0
-        for i in range(self.nlm):
-            fyp += fp[i, 0]
-            fzp += fp[i, 1]
-            fyv += fv[i, 0]
-            fzv += fv[i, 1]
+        # the marker sums in marker order (cumulative sums run left to
+        # right, like the reference's loops)
+        fyp, fzp = np.cumsum(fp[:, 0])[-1], np.cumsum(fp[:, 1])[-1]
+        fyv, fzv = np.cumsum(fv[:, 0])[-1], np.cumsum(fv[:, 1])[-1]
         forg = (1.0 - 1.0 / self.rbden) * self.rba[0] * self.rbgravF
         forcm = np.array([fyp + fyv, fzp + fzv]) + forg
-        mom = 0.0
-        for i in range(self.nlm):
-            ry, rz = self.lmcc[i, 0] - self.cm[0], self.lmcc[i, 1] - self.cm[1]
-            mom += ry * ftot[i, 1] - rz * ftot[i, 0]
+        ry, rz = self.lmcc[:, 0] - self.cm[0], self.lmcc[:, 1] - self.cm[1]
+        mom = float(np.cumsum(ry * ftot[:, 1] - rz * ftot[:, 0])[-1])
         return forcm, mom
 
     # rigid.F90:341-369

@@ -31,7 +31,9 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_phini", "fribo_phini_device", "fribo_loads_transfer_static",
            "fribo_batch_create", "fribo_batch_destroy", "fribo_batch_set_markers", "fribo_batch_step",
            "fribo_batch_set_graph", "fribo_batch_set_gemm_shape", "fribo_batch_get_gemm_shape",
-           "fribo_batch_autotune", "fribo_set_precision", "fribo_get_precision", "fribo_test_gemm32"]
+           "fribo_batch_autotune", "fribo_set_precision", "fribo_get_precision", "fribo_test_gemm32",
+           "fribo_mlp_create", "fribo_mlp_destroy", "fribo_mlp_size", "fribo_mlp_get_params", "fribo_mlp_grad",
+           "fribo_mlp_train", "fribo_mlp_predict"]
 
 NPROF = 12
 
@@ -115,6 +117,14 @@ def load():
     L.fribo_batch_autotune.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int32)]
     L.fribo_set_precision.argtypes = [vp, C.c_int32]
     L.fribo_get_precision.argtypes = [vp]
+    fp = C.POINTER(C.c_float)
+    L.fribo_mlp_create.argtypes = [C.c_int32, C.c_int32, fp, C.c_uint64, C.POINTER(vp)]
+    L.fribo_mlp_destroy.argtypes = [vp]
+    L.fribo_mlp_size.argtypes = [vp]
+    L.fribo_mlp_get_params.argtypes = [vp, fp]
+    L.fribo_mlp_grad.argtypes = [vp, C.c_int32, fp, fp, fp, fp]
+    L.fribo_mlp_train.argtypes = [vp, C.c_int32, fp, fp, C.c_int32, C.c_int32, C.c_float, fp]
+    L.fribo_mlp_predict.argtypes = [vp, C.c_int32, fp, fp]
     L.fribo_test_gemm32.argtypes = [C.c_int32, C.c_int32, C.c_int32, dp, C.c_int32, dp, C.c_int32, dp, C.c_int32]
     _lib = L
     return L
@@ -451,3 +461,61 @@ class FluidBatch:
         fribo_batch_set_markers takes them."""
         buf = np.ascontiguousarray(buf, dtype=np.float64)
         _check(load().fribo_batch_set_markers(self.b, _dp(buf)))
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The library's coupling-predictor network (fribo_mlp_*): FP32 training
+    on the GPU, BF16 inference."""
+
+    def __init__(self, ni, no, params=None, seed=1):
+        L = load()
+        h = C.c_void_p()
+        init = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        _check(L.fribo_mlp_create(int(ni), int(no), _fp(init), int(seed), C.byref(h)))
+        self.h, self.ni, self.no = h, int(ni), int(no)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            load().fribo_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def size(self):
+        return load().fribo_mlp_size(self.h)
+
+    def params(self):
+        out = np.zeros(self.size, dtype=np.float32)
+        _check(load().fribo_mlp_get_params(self.h, _fp(out)))
+        return out
+
+    def grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.size, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        _check(load().fribo_mlp_grad(self.h, X.shape[0], _fp(X), _fp(T), _fp(g), _fp(loss)))
+        return g, float(loss[0])
+
+    def train(self, X, T, epochs, batch, lr):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        _check(load().fribo_mlp_train(self.h, X.shape[0], _fp(X), _fp(T), int(epochs), int(batch), float(lr),
+                                      _fp(loss)))
+        return float(loss[0])
+
+    def predict(self, X):
+        X = np.ascontiguousarray(np.atleast_2d(X), dtype=np.float32)
+        Y = np.zeros((X.shape[0], self.no), dtype=np.float32)
+        _check(load().fribo_mlp_predict(self.h, X.shape[0], _fp(X), _fp(Y)))
+        return Y

@@ -284,6 +284,25 @@ int fribo_batch_get_gemm_shape(fribo_batch b);
  * launch of shapes 0..3 (0: not available), best the shape kept. */
 int fribo_batch_autotune(fribo_batch b, int32_t reps, double* ms, int32_t* best);
 
+/* Learned first iterate of the strong coupling loop (no counterpart in the
+ * reference, whose coupling is loose; its unused iterated-coupling state is
+ * declarations.F90:250,271-278): an MLP ni -> 64 tanh -> 64 tanh -> no with
+ * FP32 parameters [W1 (64 x ni, row-major), b1, W2 (64 x 64), b2,
+ * W3 (no x 64), b3] (fribo_mlp_size floats), trained by the library's own
+ * forward, backward and Adam kernels (FP32, gradients summed over a
+ * minibatch in sample order) and run in BF16 with FP32 accumulation.
+ * Samples are row-major (n x ni inputs, n x no targets).  Codes are the
+ * FRIBO_ERR_* values. */
+typedef struct fribo_mlp_ctx* fribo_mlp;
+int fribo_mlp_create(int32_t ni, int32_t no, const float* init, uint64_t seed, fribo_mlp* out);
+int fribo_mlp_destroy(fribo_mlp m);
+int fribo_mlp_size(fribo_mlp m);
+int fribo_mlp_get_params(fribo_mlp m, float* out);
+int fribo_mlp_grad(fribo_mlp m, int32_t n, const float* X, const float* T, float* grad, float* loss);
+int fribo_mlp_train(fribo_mlp m, int32_t n, const float* X, const float* T, int32_t epochs, int32_t batch, float lr,
+                    float* loss);
+int fribo_mlp_predict(fribo_mlp m, int32_t n, const float* X, float* Y);
+
 #ifdef __cplusplus
 }
 #endif
