@@ -65,6 +65,7 @@ def build_fortran(verbose=False):
     os.makedirs(bdir, exist_ok=True)
     exe = os.path.join(fdir, "fribo_demo")
     cmd = [fc, "-O2", "-module-dir", bdir, os.path.join(fdir, "fribo_gpu_mod.f90"),
+           os.path.join(fdir, "fribo_ttime_mod.f90"),
            os.path.join(fdir, "fribo_demo.f90"), "-o", exe + ".tmp", "-L", HERE, "-lfribo_gpu",
            "-Wl,-rpath," + HERE]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)

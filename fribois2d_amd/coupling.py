@@ -146,18 +146,30 @@ def weak_step(fluid, body, loads, ns, dt, kprobe=1.0):
     return body.int_stress(press, tau), dtn, cflm
 
 
-def strong_step(fluid, body, guess, ns, dt, kprobe=1.0, tol=1e-8, maxit=50, omega0=0.5):
+OMEGA_MIN, OMEGA_MAX = 0.01, 2.0
+
+
+def strong_step(fluid, body, guess, ns, dt, kprobe=1.0, tol=1e-8, maxit=50, omega0=0.5, state=None,
+                strict=True):
     """One strongly coupled step: fixed-point iteration on the loads
     (forcm, momcm) with Aitken relaxation until the relative load change is
     below tol.  ``guess`` is the first load iterate (e.g. the previous step's
     loads, or a predictor's).  The fluid state is rolled back on the device
     before every repeat.  Returns (loads, body after the step, iterations,
-    dt, cflm).  dt must not be changed by the step (fixed dt, idtv)."""
+    dt, cflm).  dt must not be changed by the step (fixed dt, idtv).
+    The relaxation factor stays within [OMEGA_MIN, OMEGA_MAX] (an Aitken
+    update outside it restarts from omega0); with a ``state`` dict the
+    factor the previous step converged with starts the next one.
+    The loads are a discontinuous function of the body position where a
+    marker's support or probe changes cell; an iterate that straddles such a
+    switch oscillates between two load levels and cannot converge.  With
+    strict=False the step then ends after maxit sub-iterations with the last
+    iterate (state["failed"] counts such steps) instead of raising."""
     fluid.save_state()
     body0 = body.copy()
     x = np.array([guess[0][0], guess[0][1], guess[1]], dtype=np.float64)
     r_prev = None
-    omega = omega0
+    omega = state.get("omega", omega0) if state is not None else omega0
     for it in range(1, maxit + 1):
         b = body0.copy()
         b.advance(dt, x[:2], x[2])
@@ -170,12 +182,21 @@ def strong_step(fluid, body, guess, ns, dt, kprobe=1.0, tol=1e-8, maxit=50, omeg
         y = np.array([f[0], f[1], m])
         r = y - x
         if np.linalg.norm(r) <= tol * max(np.linalg.norm(y), 1e-300):
+            if state is not None:
+                state["omega"] = omega
             return (y[:2], y[2]), b, it, dtn, cflm
         if r_prev is not None:
             d = r - r_prev
             dd = float(d @ d)
             if dd > 0.0:
                 omega = -omega * float(r_prev @ d) / dd
+            if not OMEGA_MIN <= omega <= OMEGA_MAX:
+                omega = omega0
         x = x + omega * r
         r_prev = r
+    if not strict:
+        if state is not None:
+            state["failed"] = state.get("failed", 0) + 1
+            state["omega"] = omega0
+        return (y[:2], y[2]), b, maxit, dtn, cflm
     raise RuntimeError(f"strong coupling did not converge in {maxit} iterations")

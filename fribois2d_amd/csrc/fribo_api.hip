@@ -752,6 +752,45 @@ extern "C" int fribo_set_markers(fribo_handle h, int32_t nlm, const double* lmcc
     return FRIBO_OK;
 }
 
+// Windowed read-back: planes kfirst..klast (reference k indices) of the
+// fields selected by mask (1 q2, 2 q3, 4 pr, 8 ru2, 16 ru3) into the host
+// arrays of reference extents (q2, q3, pr: (1:n2, -2:n3m+3); ru2, ru3:
+// (1:n2, 1:n3m), where the window is clipped to 1..n3m); the other planes of
+// the host arrays are left as they are.  The pressure gets the pending
+// means as in fribo_get_state.
+extern "C" int fribo_fetch_fields(fribo_handle h, int32_t mask, int32_t kfirst, int32_t klast, double* q2,
+                                  double* q3, double* pr, double* ru2, double* ru3)
+{
+    if (!h) return set_err(FRIBO_ERR_ARG, "null handle");
+    FluidDev& F = h->F;
+    if (kfirst < -2 || klast > F.n3m + 3 || kfirst > klast) return set_err(FRIBO_ERR_ARG, "bad plane window");
+    double* hf[5] = {q2, q3, pr, ru2, ru3};
+    const double* df[5] = {F.q2, F.q3, F.pr, F.ru2, F.ru3};
+    int e = 0;
+    for (int f = 0; f < 5 && !e; ++f) {
+        if (!((mask >> f) & 1)) continue;
+        if (!hf[f]) return set_err(FRIBO_ERR_ARG, "null array for a selected field");
+        const bool ru = f >= 3;
+        const int a = ru ? std::max<int>(kfirst, 1) : kfirst, b = ru ? std::min<int>(klast, F.n3m) : klast;
+        if (a > b) continue;
+        const int base = ru ? 1 : -2;
+        e = get_field(h, hf[f] + (size_t)(a - base) * F.n2, df[f], a, b - a + 1);
+    }
+    if (e) return e;
+    if ((e = read_scalars(h))) return e;
+    if ((mask & 4) && pr) {
+        const double pend = h->hs.pend, goff = h->hs.goff;
+        for (int k = kfirst; k <= klast; ++k) {
+            const bool ghost = (k < 1 || k > F.n3m);
+            for (int j = 1; j <= F.n2; ++j) {
+                double& v = pr[(size_t)(j - 1) + (size_t)(k + 2) * F.n2];
+                v = v - ((ghost || j == F.n2) ? goff : pend);
+            }
+        }
+    }
+    return FRIBO_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Step orchestration
 static hipEvent_t next_event(fribo_ctx* c)

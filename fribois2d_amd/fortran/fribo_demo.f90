@@ -6,13 +6,19 @@
 !  the Neumann z operator of phini.F90:133-142 is dx3q times the 1-D
 !  Neumann Laplacian, whose eigen-pairs are the DCT-II vectors
 !  cos(pi*m*(k-1/2)/n3m) with eigenvalues -4*dx3q*sin(pi*m/(2*n3m))**2, so no
-!  LAPACK is needed here.  All inputs and the final state are written to
-!  fribo_demo_out.bin (stream access) for comparison with other drivers.
+!  LAPACK is needed here.  After every step the host reads ttime's
+!  diagnostics through the fribo_ttime shims (vmaxv, vminv, divgck, kinen,
+!  getintvort, cfl_diag) instead of copying the fields back; at the end the
+!  fields come back once, whole (fribo_get_state) and as a window of planes
+!  (fribo_fetch_fields).  All inputs, the diagnostics and the final state are
+!  written to fribo_demo_out.bin (stream access) for comparison with other
+!  drivers.
 
       PROGRAM fribo_demo
 
       USE iso_c_binding
       USE fribo_gpu_mod
+      USE fribo_ttime
 
       implicit none
 
@@ -24,6 +30,8 @@
       real(c_double) :: rc(n2), zz(n3), wr(n3m), zmm(n3m,n3m), zmmt(n3m,n3m)
       real(c_double) :: q2(n2,-2:n3m+3), q3(n2,-2:n3m+3), pr(n2,-2:n3m+3)
       real(c_double) :: ru2(n2,n3m), ru3(n2,n3m)
+      real(c_double) :: diag(9,nsteps), w2(n2,-2:n3m+3), w3(n2,-2:n3m+3), wp(n2,-2:n3m+3)
+      real(c_double) :: wru2(n2,n3m), wru3(n2,n3m), vmx(2), vmn(2), dmx, ek, vo, cfd, q3a
       real(c_double) :: qb2s(n2), qb3s(n2), qb2n(n2), qb3n(n2), dq2x2o(n2), dq3x2o(n2)
       real(c_double) :: vert(nlm+1,2), lmcc(nlm,2), lmvel(nlm,2), lmA(nlm)
       real(c_double) :: dt, dt_o, cflm, dx3q, th, nrm, dthist(nsteps)
@@ -101,6 +109,7 @@
 
       rc_ = fribo_init(p, rc, zz, zmm, zmmt, int(n3m,c_int), wr, h)
       call check(rc_, 'fribo_init')
+      fribo_diag_h = h
       rc_ = fribo_set_state(h, q2, q3, pr, ru2, ru3, qb2s, qb3s, qb2n, qb3n, dq2x2o, dq3x2o)
       call check(rc_, 'fribo_set_state')
 
@@ -113,15 +122,38 @@
         call check(rc_, 'fribo_fluid_tsch')
         dthist(ns+1) = dt
         dt_o = dt
+       !ttime's diagnostics of this step from the device
+        fribo_diag_valid = .false.
+        call vmaxv(vmx)
+        call vminv(vmn)
+        call cfl_diag(cfd, q3a)
+        call divgck(dmx)
+        call getintvort(vo)
+        call kinen(ek)
+        diag(:,ns+1) = (/ vmx(1), vmx(2), vmn(1), vmn(2), dmx, ek, vo, cfd, q3a /)
       enddo
 
       rc_ = fribo_get_state(h, q2, q3, pr, ru2, ru3, qb2s, qb3s, qb2n, qb3n, dq2x2o, dq3x2o)
       call check(rc_, 'fribo_get_state')
+     !planes 5..17 of every field through the windowed read-back
+      w2 = 0.0d0
+      w3 = 0.0d0
+      wp = 0.0d0
+      wru2 = 0.0d0
+      wru3 = 0.0d0
+      rc_ = fribo_fetch_fields(h, 31_c_int32_t, 5_c_int32_t, 17_c_int32_t, w2, w3, wp, wru2, wru3)
+      call check(rc_, 'fribo_fetch_fields')
+      if (any(w2(:,5:17) /= q2(:,5:17)) .or. any(w3(:,5:17) /= q3(:,5:17)) .or. &
+          any(wp(:,5:17) /= pr(:,5:17)) .or. any(wru2(:,5:17) /= ru2(:,5:17)) .or. &
+          any(wru3(:,5:17) /= ru3(:,5:17)) .or. any(w2(:,-2:4) /= 0.0d0) .or. any(wp(:,18:) /= 0.0d0)) then
+        write(*,*) 'fribo_fetch_fields: window differs from fribo_get_state'
+        stop 1
+      endif
       rc_ = fribo_finalize(h)
 
       open(21, file='fribo_demo_out.bin', access='stream', form='unformatted', status='replace')
       write(21) n2, n3, nlm, nsteps
-      write(21) rc, zz, zmm, zmmt, wr, lmcc, lmvel, lmA, dthist, q2, q3, pr
+      write(21) rc, zz, zmm, zmmt, wr, lmcc, lmvel, lmA, dthist, q2, q3, pr, diag
       close(21)
 
       write(*,'(a,i3,a,es12.5,a,es12.5)') ' fribo_demo: ', nsteps, ' steps, last dt =', dt, &

@@ -140,6 +140,20 @@
           integer(c_int32_t), value :: axis
         end function fribo_set_transform
 
+        integer(c_int) function fribo_diagnostics(h, out) bind(c, name="fribo_diagnostics")
+          import :: c_int, c_double, c_ptr
+          type(c_ptr), value :: h
+          real(c_double), intent(out) :: out(9)
+        end function fribo_diagnostics
+
+        integer(c_int) function fribo_fetch_fields(h, mask, kfirst, klast, q2, q3, pr, ru2, ru3) &
+            bind(c, name="fribo_fetch_fields")
+          import :: c_int, c_int32_t, c_double, c_ptr
+          type(c_ptr), value :: h
+          integer(c_int32_t), value :: mask, kfirst, klast
+          real(c_double), intent(inout) :: q2(*), q3(*), pr(*), ru2(*), ru3(*)
+        end function fribo_fetch_fields
+
         type(c_ptr) function fribo_last_error() bind(c, name="fribo_last_error")
           import :: c_ptr
         end function fribo_last_error

@@ -33,7 +33,7 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_batch_set_graph", "fribo_batch_set_gemm_shape", "fribo_batch_get_gemm_shape",
            "fribo_batch_autotune", "fribo_set_precision", "fribo_get_precision", "fribo_test_gemm32",
            "fribo_mlp_create", "fribo_mlp_destroy", "fribo_mlp_size", "fribo_mlp_get_params", "fribo_mlp_grad",
-           "fribo_mlp_train", "fribo_mlp_predict"]
+           "fribo_mlp_train", "fribo_mlp_predict", "fribo_fetch_fields"]
 
 NPROF = 12
 
@@ -115,6 +115,7 @@ def load():
     L.fribo_batch_set_gemm_shape.argtypes = [vp, C.c_int32]
     L.fribo_batch_get_gemm_shape.argtypes = [vp]
     L.fribo_batch_autotune.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int32)]
+    L.fribo_fetch_fields.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, dp, dp, dp, dp, dp]
     L.fribo_set_precision.argtypes = [vp, C.c_int32]
     L.fribo_get_precision.argtypes = [vp]
     fp = C.POINTER(C.c_float)
@@ -193,6 +194,16 @@ class FluidStep:
         keys = ("q2", "q3", "pr", "ru2", "ru3", "qb2s", "qb3s", "qb2n", "qb3n", "dq2x2o", "dq3x2o")
         _check(load().fribo_get_state(self.h, *[_dp(st[k]) for k in keys]))
         return st
+
+    def fetch_fields(self, kfirst, klast, st, fields=("q2", "q3", "pr", "ru2", "ru3")):
+        """Planes kfirst..klast of the named fields into the arrays of st (the
+        dict get_state returns; other planes untouched)."""
+        bits = {"q2": 1, "q3": 2, "pr": 4, "ru2": 8, "ru3": 16}
+        mask = sum(bits[f] for f in fields)
+        arrs = [st[k] if k in fields else None for k in ("q2", "q3", "pr", "ru2", "ru3")]
+        for a in arrs:
+            assert a is None or (a.flags.f_contiguous and a.dtype == np.float64)
+        _check(load().fribo_fetch_fields(self.h, mask, int(kfirst), int(klast), *[_dp(a) for a in arrs]))
 
     def set_markers(self, lmcc, lmvel, lmA):
         lmcc = np.asfortranarray(lmcc, dtype=np.float64)

@@ -11,9 +11,9 @@ HIP forward, backward and Adam kernels, FP32 training, BF16 inference) from
 the last HIST steps of the body's displacement, velocity and lift, trained
 on coupling histories the solver itself generates.
 
-Features of step n: (disp, vel, lift) of steps n-HIST .. n-1 relative to
-step n-1's values (plus step n-1's values themselves), scaled by the
-training set's spread; target: the change of the converged loads
+Features of step n: (disp, vel, lift, drag, moment) of steps n-HIST .. n-1
+relative to step n-1's values (plus step n-1's values themselves), scaled
+by the training set's spread; target: the change of the converged loads
 (lift, drag, moment) from step n-1 to step n, scaled likewise.
 """
 from __future__ import annotations
@@ -25,7 +25,7 @@ import numpy as np
 from . import cases, coupling
 
 HIST = 8
-NI = 3 * HIST + 3
+NI = 5 * HIST + 5
 NO = 3
 
 
@@ -48,8 +48,10 @@ def coupled_case(run: int, n3m: int = 512, n2m: int = 384, re: float = 150.0, dt
 
 def features(hist):
     """Input vector of the step after the recorded ones: hist is a list of
-    (disp, vel, lift, drag, moment) of the previous steps (at least HIST)."""
-    h = np.asarray(hist[-HIST:], dtype=np.float64)[:, :3]
+    (disp, vel, lift, drag, moment) of the previous steps (at least HIST).
+    Drag and moment enter too: the coupling tolerance applies to the whole
+    load vector, whose size the drag dominates."""
+    h = np.asarray(hist[-HIST:], dtype=np.float64)
     last = h[-1]
     return np.concatenate([(h - last).ravel(), last])
 
@@ -63,18 +65,26 @@ def samples(hist):
     return np.array(X).reshape(-1, NI), np.array(T).reshape(-1, NO)
 
 
-def run_coupled(fluid, body, nsteps, dt, guess=None, tol=1e-8, maxit=100):
+def run_coupled(fluid, body, nsteps, dt, guess=None, tol=1e-8, maxit=40, stats=None):
     """Strongly coupled run; guess(hist, loads) proposes the first iterate of
-    a step (None: the previous step's converged loads).  Returns the history
-    rows (disp, vel, lift, drag, moment) and the sub-iterations per step."""
+    a step (None: the previous step's converged loads).  Each step's Aitken
+    factor starts from the one the previous step converged with.  Returns
+    the history rows (disp, vel, lift, drag, moment) and the sub-iterations
+    per step.  A step whose iterates straddle a switch of the marker
+    supports (see coupling.strong_step) ends after maxit sub-iterations;
+    stats (a dict) gets the count of such steps under "failed"."""
     y0 = body.cmposi[0]
     loads = (np.zeros(2), 0.0)
     hist, its = [], []
+    state = {}
     for ns in range(nsteps):
         g = guess(hist, loads) if guess is not None else loads
-        loads, body, it, _, _ = coupling.strong_step(fluid, body, g, ns, dt, tol=tol, maxit=maxit)
+        loads, body, it, _, _ = coupling.strong_step(fluid, body, g, ns, dt, tol=tol, maxit=maxit, state=state,
+                                                     strict=False)
         hist.append((body.cm[0] - y0, body.cmv[0], loads[0][0], loads[0][1], loads[1]))
         its.append(it)
+    if stats is not None:
+        stats["failed"] = stats.get("failed", 0) + state.get("failed", 0)
     return hist, np.array(its)
 
 

@@ -112,6 +112,15 @@ int fribo_get_array(fribo_handle h, const char* name, void* out, long n);
 
 int fribo_sync(fribo_handle h);
 
+/* Windowed read-back for host consumers that need only some planes (output
+ * slices, restart in pieces, probes): planes kfirst..klast (reference k
+ * indices, -2..n3m+3) of the fields selected by mask (1 q2, 2 q3, 4 pr,
+ * 8 ru2, 16 ru3) into the host arrays of reference extents; ru2/ru3 take the
+ * window clipped to 1..n3m; the other planes of the host arrays are left as
+ * they are.  Same values as fribo_get_state on those planes. */
+int fribo_fetch_fields(fribo_handle h, int32_t mask, int32_t kfirst, int32_t klast, double* q2, double* q3,
+                       double* pr, double* ru2, double* ru3);
+
 /* Loads on the body after a step: SUBROUTINE loads_transfer_c(ns)
  * (loads_transfer.F90:4-162, called at fluid_tsch.F90:175-177).  Probes at
  * ddmin*kprobe^2 along the markers' normals (ddmin = smallest node spacing,

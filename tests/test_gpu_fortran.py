@@ -1,7 +1,11 @@
 """The Fortran binding (fribois2d_amd/fortran/fribo_gpu_mod.f90) drives the same
 library as the Python one: the Fortran demo driver (main.F90-style loop,
 20 CFL-controlled steps of a cylinder) and a Python run on the inputs it
-wrote must agree bit for bit, and the CPU oracle to 1e-10."""
+wrote must agree bit for bit, and the CPU oracle to 1e-10; ttime's
+diagnostics the driver read through the Fortran shims every step
+(fribo_ttime_mod.f90) equal the library's bit for bit and the oracle's to
+1e-12 of each value (the rounding-level divergence and vorticity integral
+to 1e-11 and 1e-12 absolute)."""
 import os
 import subprocess
 
@@ -24,7 +28,8 @@ def _read(path):
     out, o = {}, 0
     for name, shape in (("rc", (n2,)), ("zz", (n3,)), ("zmm", (n3m, n3m)), ("zmmt", (n3m, n3m)), ("wr", (n3m,)),
                         ("lmcc", (nlm, 2)), ("lmvel", (nlm, 2)), ("lmA", (nlm,)), ("dt", (nsteps,)),
-                        ("q2", (n2, n3m + 6)), ("q3", (n2, n3m + 6)), ("pr", (n2, n3m + 6))):
+                        ("q2", (n2, n3m + 6)), ("q3", (n2, n3m + 6)), ("pr", (n2, n3m + 6)),
+                        ("diag", (9, nsteps))):
         size = int(np.prod(shape))
         out[name] = f[o:o + size].reshape(shape, order="F")
         o += size
@@ -54,6 +59,7 @@ def test_fortran_driver_matches_python_driver(tmp_path):
     o.set_state(st)
     dt = dto = 1e-3
     dts = []
+    from oracle import DIAG_KEYS
     for ns in range(nsteps):
         g.set_markers(d["lmcc"], d["lmvel"], d["lmA"])
         o.set_markers(d["lmcc"], d["lmvel"], d["lmA"])
@@ -62,6 +68,16 @@ def test_fortran_driver_matches_python_driver(tmp_path):
         assert rc == 0
         dts.append(dtg)
         dt = dto = dtg
+        # ttime's diagnostics through the Fortran shims: the library's
+        # values bit for bit, the oracle's within 1e-12 of each value (the
+        # largest divergence and the vorticity integral are rounding-level
+        # quantities here: within 1e-11 and 1e-12 absolute)
+        dg = g.diagnostics()
+        do = o.diagnostics()
+        atol = {"divmax": 1e-11, "vorti": 1e-12}
+        for i, k in enumerate(DIAG_KEYS):
+            assert d["diag"][i, ns] == dg[k], (ns, k)
+            assert abs(d["diag"][i, ns] - do[k]) <= 1e-12 * abs(do[k]) + atol.get(k, 0.0), (ns, k)
     np.testing.assert_array_equal(np.array(dts), d["dt"])
     sg = g.get_state()
     for k in ("q2", "q3", "pr"):
