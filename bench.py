@@ -421,6 +421,28 @@ def main():
             kernels[k]["alg_GBs"] = round(gbs, 1)
             kernels[k]["hbm_frac"] = round(gbs / PEAK_HBM_GBS, 4)
 
+    # the drop-in fluid_tsch body per step (markers, step, checkbcs, loads,
+    # ttime's diagnostics on the device), and the same with q2, q3, pr, ru2,
+    # ru3 copied back every step as a host-side ttime would need them
+    nor, acc = c.surface(t)
+
+    def dropin(copy_state, n=10):
+        g.sync()
+        t2 = time.perf_counter()
+        for _ in range(n):
+            step()
+            g.checkbcs()
+            g.loads_transfer(1.0, nor, acc)
+            if copy_state:
+                g.get_state()
+            else:
+                g.diagnostics()
+        g.sync()
+        return (time.perf_counter() - t2) * 1e3 / n
+
+    drop_in = {"ms_per_step_device_diagnostics": round(dropin(False), 4),
+               "ms_per_step_full_state_copied_back": round(dropin(True), 4)}
+
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -428,12 +450,27 @@ def main():
         o = Oracle(p, c.rc, c.zz, zmm, zmmt, wr)
         o.set_state(st0)
         o.set_markers(*c.markers(0.0))
+        ncpu = 2
         t1 = time.perf_counter()
-        rc, _, _ = o.fluid_tsch(0, p.dt0, p.dt0)
+        tt, dtc = 0.0, p.dt0
+        for k in range(ncpu):
+            o.set_markers(*c.markers(tt))
+            rc, dtc, _ = o.fluid_tsch(k, dtc, dtc)
+            tt += dtc
         cel = time.perf_counter() - t1
-        cpu = {"value": round(cells / cel / 1e6, 4), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-               "sample": f"1 time step (3 substeps) of the same {args.workload} workload, single-threaded C "
-                         f"oracle, {cel:.1f} s"}
+        model = ""
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        cpu = {"value": round(cells * ncpu / cel / 1e6, 4), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{ncpu} time steps (3 substeps each) of the same {args.workload} workload by the "
+                         f"single-threaded C oracle, {cel:.1f} s, on one core of the GPU host "
+                         f"({os.cpu_count()} cores visible, {model})",
+               "host_cores": os.cpu_count()}
 
     out = {
         "metric": "Mcell-steps/s (plain FP64 GPU fluid step)",
@@ -454,6 +491,9 @@ def main():
                    "poisson_transform": f"along {axis}", "launch": args.launch},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "drop_in": drop_in,
+        "chosen_form": "plain FP64 transforms: FP32 MFMA transforms with FP64-residual refinement measured "
+                       "slower on MI355X (bench.py --precision n)",
         "kernels": kernels,
     }
     print(json.dumps(out))
