@@ -496,6 +496,7 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
     if (!e) e = setup_ksys(c, F.ks[1], F.n3m - 1, 2, 0, 0);
     if (!e) e = setup_ksys(c, F.js[0], F.n2 - 2, 2, 1, 1);
     if (!e) e = setup_ksys(c, F.js[1], F.n2m, 1, 2, 1);
+    F.kform = ksolve_fused_fits(F) ? 1 : 0;
     if (!e) e = setup_psys(c, F.ps, F.n2m, c->h1[A_AMPHJ], c->h1[A_ACPHJ], c->h1[A_APPHJ]);
     if (!e) {
         F.jstride = jsolve_stride(F);
@@ -1046,13 +1047,18 @@ extern "C" int fribo_cfl_dt(fribo_handle h, int32_t ns, double* dt, double dt_o,
     return finish(h, dt, cflm);
 }
 
-extern "C" int fribo_substep(fribo_handle h, int32_t n, int32_t ns)
+// Substep n after fribo_cfl_dt.  folded: the launch sequence of
+// fribo_fluid_tsch (substep n's k_bc also runs substep n-1's pressure mean and
+// boundary update; only the last substep ends with the mean kernel);
+// otherwise every substep ends with the mean kernel.
+static int substep(fribo_handle h, int n, bool folded)
 {
     if (!h || n < 1 || n > h->F.nsst) return set_err(FRIBO_ERR_ARG, "bad substep");
     int e = 0;
     if (n == 1 && (e = markers_inside(h))) return e;
     FluidDev& F = h->F;
     hipStream_t st = h->st;
+    const bool mean = !folded || n == F.nsst;
     // substep 1's boundary kernel already ran in fribo_cfl_dt; for n == 1 run
     // the body with the stored dt (launch_bc with n > 1 semantics would
     // redo boucdq), so the body starts after k_bc.
@@ -1067,11 +1073,23 @@ extern "C" int fribo_substep(fribo_handle h, int32_t n, int32_t ns)
         run(h, PC_DIV, [&] { launch_div(F, st); });
         enqueue_poisson(h);
         run(h, PC_UPD, [&] { launch_upd(F, st); });
-        run(h, PC_MEAN, [&] { launch_mean(F, st); });
+        if (mean) run(h, PC_MEAN, [&] { launch_mean(F, st); });
     } else {
-        enqueue_substep(h, n, false, true);
+        enqueue_substep(h, n, folded, mean);
     }
     return finish(h, nullptr, nullptr);
+}
+
+extern "C" int fribo_substep(fribo_handle h, int32_t n, int32_t ns)
+{
+    (void)ns;
+    return substep(h, n, false);
+}
+
+extern "C" int fribo_substep_folded(fribo_handle h, int32_t n, int32_t ns)
+{
+    (void)ns;
+    return substep(h, n, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1427,6 +1445,29 @@ extern "C" int fribo_set_transform(fribo_handle h, int32_t axis)
 extern "C" int fribo_get_transform(fribo_handle h)
 {
     return h ? h->F.taxis : -1;
+}
+
+extern "C" int fribo_set_ksolve(fribo_handle h, int32_t form)
+{
+    if (!h || form < 0 || form > 1) return set_err(FRIBO_ERR_ARG, "bad k-line solve form");
+    if (h->batch) return set_err(FRIBO_ERR_ARG, "instance is in a batch");
+    if (form == 1 && !ksolve_fused_fits(h->F))
+        return set_err(FRIBO_ERR_LIMIT, "k-lines too long for the one-pass solve (chunks of more than 32 rows)");
+    if (form != h->F.kform) {
+        HIPCHK(hipStreamSynchronize(h->st));
+        h->F.kform = form;
+        if (h->gexec) {
+            hipGraphExecDestroy(h->gexec);
+            h->gexec = nullptr;
+            h->graph_nlm = -1;
+        }
+    }
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_get_ksolve(fribo_handle h)
+{
+    return h ? h->F.kform : -1;
 }
 
 extern "C" int fribo_set_profiling(fribo_handle h, int32_t on)

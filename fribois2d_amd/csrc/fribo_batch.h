@@ -127,7 +127,7 @@ inline void blaunch(BatchSet& B, int key, const std::vector<int>& dims, Kern ker
 // Stage keys of the launch tables.
 enum {
     BK_CFL, BK_BC, BK_RHS, BK_DIV, BK_UPD, BK_MEAN, BK_MLS_IDX, BK_MLS_SUP, BK_MLS_ORDER, BK_MLS_INTERP,
-    BK_MLS_SPREAD, BK_JH, BK_JP, BK_K1, BK_K2, BK_K3, BK_Y1, BK_Y2, BK_Y3, BK_GEMM_F, BK_GEMM_B, BK_IO,
+    BK_MLS_SPREAD, BK_JH, BK_JP, BK_K1, BK_K2, BK_K3, BK_KF, BK_Y1, BK_Y2, BK_Y3, BK_GEMM_F, BK_GEMM_B, BK_IO,
     BK_COUNT
 };
 

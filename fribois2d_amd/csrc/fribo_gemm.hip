@@ -245,12 +245,12 @@ __device__ inline void dma16(const double* src, double* dst)
 #endif
 }
 
-template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128>
+template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128, int BM = 96>
 __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, YFuse yf, const VBlock& vb)
 {
-    constexpr int BM = 96, NW = WM * WN;
+    constexpr int NW = WM * WN;
     constexpr int TM = BM / WM, TN = BN / WN, RM = TM / 16, RN = TN / 16;
-    constexpr int LA = 112, SLA = LA / 2;                   // A: k-rows, 56 slots (48 used)
+    constexpr int LA = BM + 16, SLA = LA / 2;               // A: k-rows, BM / 2 of SLA slots used
     constexpr int LB = NN ? BK + 2 : BN + 16, SLB = LB / 2; // B: n-rows (NN) or k-rows (NT)
     constexpr int SA = BK * LA, SB = NN ? BN * LB : BK * LB;   // doubles per stage
     constexpr int IA = SA / 128, IB0 = SB / 128;            // 1 KB loads per stage
@@ -260,9 +260,12 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
     static_assert(SA % 128 == 0 && SB % 128 == 0, "stage size");
     static_assert(NI < 16, "load count per wave");
     static_assert(TM % 16 == 0 && TN % 16 == 0 && BM % WM == 0 && BN % WN == 0, "wave tile");
+    static_assert(BM % 32 == 0, "A pitch 16 (mod 32)");
     if (S && S->err) return;
-    // the three stages in one block, which the fused epilogue reuses for the tile
-    __shared__ __attribute__((aligned(1024))) double SM[3 * (SA + SBP)];
+    // the three stages in one block, which the fused epilogue of the NN form
+    // reuses for the tile (BN rows of pitch BM + 1)
+    constexpr int SMN = 3 * (SA + SBP), STN = NN ? BN * (BM + 1) : 0;
+    __shared__ __attribute__((aligned(1024))) double SM[SMN > STN ? SMN : STN];
     double* const S0 = SM;
     double* const S1 = SM + (SA + SBP);
     double* const S2 = SM + 2 * (SA + SBP);
@@ -380,7 +383,6 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
         // chunk elimination of the y-mode z-lines on this tile: rows i are
         // y-modes, columns n the z rows; the tile holds BN / csz whole chunks
         constexpr int LT = BM + 1;   // LDS row pitch of the staged tile (one z row per LDS row)
-        static_assert(BN * LT <= 3 * (SA + SBP), "tile staging");
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < RM; ++r)
@@ -410,17 +412,17 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
         }
     }
 }
-template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128>
+template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128, int BM = 96>
 __global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, YFuse yf)
 {
-    gemm_dma_body<BK, NN, WM, WN, BN>(M, N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, yf, vb_hw());
+    gemm_dma_body<BK, NN, WM, WN, BN, BM>(M, N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, yf, vb_hw());
 }
-template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128>
+template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128, int BM = 96>
 __global__ __launch_bounds__(64 * WM * WN) void kb_gemm_dma(const BGrid<GemmProb> G)
 {
     VBlock vb;
     const GemmProb& P = *bg_block(G, vb);
-    gemm_dma_body<BK, NN, WM, WN, BN>(P.M, P.N, P.K, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, P.S, P.zg, P.gpart, P.yf, vb);
+    gemm_dma_body<BK, NN, WM, WN, BN, BM>(P.M, P.N, P.K, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, P.S, P.zg, P.gpart, P.yf, vb);
 }
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }

@@ -130,6 +130,7 @@ struct FluidDev {
     KSys js[2];
     PSys ps;
     int jstride;             // LDS chunk stride of the j-line kernels (odd)
+    int kform;               // k-line solve: 1 one fused pass (chunks of <= 32 rows), 0 three passes
     int lds_max;             // LDS bytes a workgroup may use on this device
     double *kYs, *kYe, *kDe, *kX;  // [2][n2m][64] reduced-system data (chunk fastest)
     // markers
@@ -204,6 +205,8 @@ void launch_mls_force(const FluidDev& F, hipStream_t st);
 void launch_jsolve_helm(const FluidDev& F, hipStream_t st);
 void launch_jsolve_poisson(const FluidDev& F, hipStream_t st);
 void launch_ksolve(const FluidDev& F, hipStream_t st);
+// true when both k-systems have chunks of at most 32 rows (the one-pass form)
+bool ksolve_fused_fits(const FluidDev& F);
 // First pass of the y-order z-lines (chunk elimination, fribo_ychunk.h) run
 // by the epilogue of the forward transform on its own output tile, when
 // every 64-column tile holds whole chunks of the line partition (csz rows

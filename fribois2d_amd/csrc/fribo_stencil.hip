@@ -347,7 +347,7 @@ __device__ inline double pr_mean(const FluidDev& F, double* sh, const double* v0
     return block_sum<1024>(sh, s) / (F.rext2 * F.alx3);
 }
 
-// pr_submean and boucqt (iobcs.F90:300-312) by one workgroup of 1024
+// pr_submean and boucqt (iobcs.F90:81-108) by one workgroup of 1024
 // threads.  The mean is kept pending for the interior (S->pend, subtracted on
 // the next read) and accumulated for the entries no kernel reads (row j=n2
 // and the ghost planes, S->goff, subtracted when the state is read back).
@@ -945,7 +945,7 @@ void launch_upd_b(BatchSet& B, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// pr_submean (updvp.F90:125-141) and boucqt (iobcs.F90:300-312) as a kernel
+// pr_submean (updvp.F90:125-141) and boucqt (iobcs.F90:81-108) as a kernel
 // of its own (after the last substep of a step, and after every substep of
 // the substep-by-substep entry point); within a step, k_bc of the next
 // substep does the same work first (bc_rows).

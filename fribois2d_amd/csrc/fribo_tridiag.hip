@@ -956,10 +956,169 @@ __global__ __launch_bounds__(64) void kb_ksolve3(const BGrid<FluidDev> G)
     ksolve3_body<MAXM>(*Fp, vb);
 }
 
+// One-pass form of the three passes above for chunks of at most 32 rows
+// (lines of up to 2048 rows): a workgroup of 1024 threads owns KF_COLS = 16
+// columns of one field and every chunk of them; thread (column jl, chunk p)
+// with jl = lane % 16, p = 4 * wave + lane / 16, so a wave's loads cover 16
+// columns (128 bytes) of 4 chunks.  The chunk's forward-eliminated rows stay
+// in registers across the separator solve, so the right-hand side is read
+// once: read src, read q, write q.  The coefficient rows of every chunk are
+// staged in LDS lane-interleaved ([t * 64 + p]); the chunk summaries go to
+// LDS, wave w solves the separator system of column w with the multipliers
+// and operation order of k_ksolve2, and the back-substitution is that of
+// k_ksolve3, so both forms give identical results.
+constexpr int KF_COLS = 16;
+
+template <int MAXM>
+__device__ __forceinline__ void ksolve_fused_body(const FluidDev& F, const VBlock& vb)
+{
+    static_assert(MAXM <= 32, "fused k-solve: chunks of at most 32 rows");
+    constexpr int MI = MAXM - 1;   // interior rows of the longest chunk
+    __shared__ double cf[6][MAXM * 64];   // S, a, inv, g, vp, cp
+    __shared__ double sY[3][64][KF_COLS + 1];
+    __shared__ double sX[64][KF_COLS + 1];
+    __shared__ int scs[FRIBO_KMAXCH + 1];
+    const DevScalars* S = F.S;
+    if (S->err) return;
+    const int f = vb.z;
+    const KSys& K = F.ks[f];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int jl = lane & (KF_COLS - 1), p = 4 * wave + (lane >> 4);
+    const int nch = K.nch;
+    for (int i = t; i <= nch; i += 1024) scs[i] = K.cs[i];
+    __syncthreads();
+    for (int i = t; i < MAXM * 64; i += 1024) {
+        const int pp = i & 63, tt = i >> 6;
+        if (pp < nch) {
+            const int r = scs[pp] + tt;
+            if (r < scs[pp + 1]) {
+                cf[0][i] = K.S[r];
+                cf[1][i] = K.a[r];
+                cf[2][i] = K.inv[r];
+                cf[3][i] = K.g[r];
+                cf[4][i] = K.vp[r];
+                cf[5][i] = K.cp[r];
+            }
+        }
+    }
+    __syncthreads();
+    const int j = vb.x * KF_COLS + jl + 1;
+    const bool act = j <= F.n2m;
+    const int jc = act ? j : F.n2m;
+    const size_t P = F.P;
+    const bool own = p < nch;
+    const int s = own ? scs[p] : 0, e = own ? scs[p + 1] - 1 : 0, m = e - s;
+    double b0 = 0.0, bn = 0.0;
+    double y[MI > 0 ? MI : 1];
+    if (own) {
+        const double* __restrict__ src = (f == 0 ? F.sA : F.sB) + fidx(F, jc, K.kofs);
+        kbound(F, K, f, jc, S, b0, bn);
+#pragma unroll
+        for (int tt = 0; tt < MI; ++tt) y[tt] = src[(size_t)(s + (tt < m ? tt : 0)) * P];
+        const double dsep = src[(size_t)e * P];
+        double yp = 0.0, acc = 0.0;
+#pragma unroll
+        for (int tt = 0; tt < MI; ++tt) {
+            if (tt < m) {
+                const int i = s + tt;
+                double d = y[tt] * cf[0][tt * 64 + p];
+                d = (i == 0) ? d + b0 : d;
+                yp = (d - cf[1][tt * 64 + p] * yp) * cf[2][tt * 64 + p];
+                acc = acc + cf[3][tt * 64 + p] * yp;
+                y[tt] = yp;
+            }
+        }
+        double de = dsep * cf[0][m * 64 + p];
+        de = (e == 0) ? de + b0 : de;
+        de = (e == K.n - 1) ? de + bn : de;
+        sY[0][p][jl] = acc;
+        sY[1][p][jl] = yp;
+        sY[2][p][jl] = de;
+    }
+    __syncthreads();
+    {
+        // separator system of column wave (k_ksolve2's operations); lane = chunk
+        const int c = wave, q = lane;
+        double delta = 0.0;
+        if (q < nch) {
+            const int eq = scs[q + 1] - 1;
+            const double ysn = (q + 1 < nch) ? sY[0][q + 1][c] : 0.0;
+            delta = sY[2][q][c] - K.a[eq] * sY[1][q][c] - K.c[eq] * ysn;
+        }
+        int li = 0;
+        for (int h = 1; h < 64; h <<= 1, ++li) {
+            double dm = __shfl_up(delta, h, 64), dp = __shfl_down(delta, h, 64);
+            if (q - h < 0) dm = 0.0;
+            if (q + h >= 64) dp = 0.0;
+            delta = delta - K.k1[li * 64 + q] * dm - K.k2[li * 64 + q] * dp;
+        }
+        if (q < nch) sX[q][c] = delta * K.ib[q];
+    }
+    __syncthreads();
+    if (!own) return;
+    const double X = sX[p][jl];
+    const double Xm = (p > 0) ? sX[p - 1][jl] : 0.0;
+    const double wlp = K.wl[p];
+    const bool zero_row = (f == 0 && j == 1);
+    double* __restrict__ q = (f == 0 ? F.q2 : F.q3) + fidx(F, jc, K.kofs);
+    double* qn2 = (f == 0 ? F.q2 : F.q3) + fidx(F, F.n2, K.kofs);
+    const double qe = q[(size_t)e * P];
+    double x = 0.0;
+#pragma unroll
+    for (int tt = MI - 1; tt >= 0; --tt) {
+        if (tt < m) {
+            const int i = s + tt;
+            const double qv = q[(size_t)i * P];
+            if (tt == m - 1) x = y[tt] + cf[4][tt * 64 + p] * Xm + wlp * X;
+            else x = (y[tt] + cf[4][tt * 64 + p] * Xm) - cf[5][tt * 64 + p] * x;
+            if (act) q[(size_t)i * P] = zero_row ? 0.0 : qv + x;
+            if (act && zero_row) qn2[(size_t)i * P] = 0.0;
+        }
+    }
+    if (!act) return;
+    q[(size_t)e * P] = zero_row ? 0.0 : qe + X;
+    if (zero_row) qn2[(size_t)e * P] = 0.0;
+    if (f == 1) {
+        double* qf = F.q3;
+        if (p == 0) {
+            const size_t c0 = fidx(F, j, 1);
+            qf[c0] = qf[c0] + F.dqb3s[j];
+        }
+        if (p == nch - 1) {
+            const size_t c1 = fidx(F, j, F.n3);
+            qf[c1] = qf[c1] + F.dqb3n[j];
+        }
+    }
+}
+template <int MAXM>
+__global__ __launch_bounds__(1024) void k_ksolve_fused(const FluidDev F)
+{
+    ksolve_fused_body<MAXM>(F, vb_hw());
+}
+template <int MAXM>
+__global__ __launch_bounds__(1024) void kb_ksolve_fused(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    ksolve_fused_body<MAXM>(*Fp, vb);
+}
+
+// Largest chunk (rows incl. the separator) of either k-system.
+static inline int kmaxm(const FluidDev& F) { return std::max(F.ks[0].maxm, F.ks[1].maxm); }
+
+bool ksolve_fused_fits(const FluidDev& F) { return kmaxm(F) <= 32; }
+
 void launch_ksolve(const FluidDev& F, hipStream_t st)
 {
     int nchmax = F.ks[0].nch > F.ks[1].nch ? F.ks[0].nch : F.ks[1].nch;
-    int maxm = F.ks[0].maxm > F.ks[1].maxm ? F.ks[0].maxm : F.ks[1].maxm;
+    int maxm = kmaxm(F);
+    if (F.kform && maxm <= 32) {
+        const dim3 gf((F.n2m + KF_COLS - 1) / KF_COLS, 1, 2);
+        if (maxm <= 8) hipLaunchKernelGGL(k_ksolve_fused<8>, gf, dim3(1024), 0, st, F);
+        else if (maxm <= 16) hipLaunchKernelGGL(k_ksolve_fused<16>, gf, dim3(1024), 0, st, F);
+        else hipLaunchKernelGGL(k_ksolve_fused<32>, gf, dim3(1024), 0, st, F);
+        return;
+    }
     dim3 g((F.n2m + 63) / 64, nchmax, 2);
     // maxm counts the separator row; the passes loop over the m-1 interior rows
     if (maxm - 1 <= 8) hipLaunchKernelGGL(k_ksolve1<8>, g, dim3(64), 0, st, F);
@@ -978,7 +1137,18 @@ void launch_ksolve(const FluidDev& F, hipStream_t st)
 void launch_ksolve_b(BatchSet& B, hipStream_t st)
 {
     int maxm = 0;
-    for (const auto& F : B.hF) maxm = std::max(maxm, std::max(F.ks[0].maxm, F.ks[1].maxm));
+    bool fused = true;
+    for (const auto& F : B.hF) {
+        maxm = std::max(maxm, kmaxm(F));
+        fused = fused && F.kform;
+    }
+    if (fused && maxm <= 32) {
+        const auto gf = bdims(B, [](const FluidDev& F) { return dim3((F.n2m + KF_COLS - 1) / KF_COLS, 1, 2); });
+        if (maxm <= 8) blaunch(B, BK_KF, gf, kb_ksolve_fused<8>, dim3(1024), 0, st);
+        else if (maxm <= 16) blaunch(B, BK_KF, gf, kb_ksolve_fused<16>, dim3(1024), 0, st);
+        else blaunch(B, BK_KF, gf, kb_ksolve_fused<32>, dim3(1024), 0, st);
+        return;
+    }
     const auto g = bdims(B, [](const FluidDev& F) {
         return dim3((F.n2m + 63) / 64, std::max(F.ks[0].nch, F.ks[1].nch), 2);
     });

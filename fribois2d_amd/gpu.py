@@ -23,10 +23,11 @@ ERR_NAMES = {1: "DT_TOO_SMALL", 2: "CFL_TOO_LARGE", 3: "MARKER_OUTSIDE", 4: "PAR
              5: "ARG", 6: "HIP", 7: "LIMIT"}
 
 EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state", "fribo_set_markers",
-           "fribo_fluid_tsch", "fribo_cfl_dt", "fribo_substep", "fribo_array_size", "fribo_get_array",
-           "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_test_gemm_nn", "fribo_set_profiling", "fribo_get_profile",
+           "fribo_fluid_tsch", "fribo_cfl_dt", "fribo_substep", "fribo_substep_folded", "fribo_array_size",
+           "fribo_get_array", "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_test_gemm_nn", "fribo_set_profiling", "fribo_get_profile",
            "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
-           "fribo_set_transform", "fribo_get_transform", "fribo_test_ytransform", "fribo_loads_transfer",
+           "fribo_set_transform", "fribo_get_transform", "fribo_set_ksolve", "fribo_get_ksolve",
+           "fribo_test_ytransform", "fribo_loads_transfer",
            "fribo_checkbcs", "fribo_diagnostics", "fribo_save_state", "fribo_restore_state",
            "fribo_phini", "fribo_phini_device", "fribo_loads_transfer_static",
            "fribo_batch_create", "fribo_batch_destroy", "fribo_batch_set_markers", "fribo_batch_step",
@@ -82,6 +83,7 @@ def load():
     L.fribo_fluid_tsch.argtypes = [vp, C.c_int32, dp, C.c_double, dp]
     L.fribo_cfl_dt.argtypes = [vp, C.c_int32, dp, C.c_double, dp]
     L.fribo_substep.argtypes = [vp, C.c_int32, C.c_int32]
+    L.fribo_substep_folded.argtypes = [vp, C.c_int32, C.c_int32]
     L.fribo_array_size.argtypes = [vp, C.c_char_p]
     L.fribo_array_size.restype = C.c_long
     L.fribo_get_array.argtypes = [vp, C.c_char_p, vp, C.c_long]
@@ -98,6 +100,8 @@ def load():
     L.fribo_test_copy.argtypes = [C.c_int64, C.c_int32, dp]
     L.fribo_set_transform.argtypes = [vp, C.c_int32]
     L.fribo_get_transform.argtypes = [vp]
+    L.fribo_set_ksolve.argtypes = [vp, C.c_int32]
+    L.fribo_get_ksolve.argtypes = [vp]
     L.fribo_test_ytransform.argtypes = [C.c_int32, dp, dp, dp, dp, dp, dp]
     L.fribo_loads_transfer.argtypes = [vp, C.c_double, dp, dp, dp, dp]
     L.fribo_loads_transfer_static.argtypes = [vp, C.c_double, dp, dp, dp, dp]
@@ -233,8 +237,11 @@ class FluidStep:
         _check(load().fribo_cfl_dt(self.h, ns, C.byref(d), float(dt if dt_o is None else dt_o), C.byref(cf)))
         return d.value, cf.value
 
-    def substep(self, n, ns):
-        _check(load().fribo_substep(self.h, n, ns))
+    def substep(self, n, ns, folded=False):
+        """Substep n; folded: the launch sequence of a whole step (the mean
+        of substep n-1 runs inside substep n's boundary kernel)."""
+        L = load()
+        _check((L.fribo_substep_folded if folded else L.fribo_substep)(self.h, n, ns))
 
     def sync(self):
         _check(load().fribo_sync(self.h))
@@ -293,6 +300,15 @@ class FluidStep:
     @property
     def transform(self):
         return "y" if load().fribo_get_transform(self.h) == 1 else "z"
+
+    def set_ksolve(self, form):
+        """k-line solve form: "fused" (one pass, lines of at most 2048 rows)
+        or "3pass"."""
+        _check(load().fribo_set_ksolve(self.h, {"3pass": 0, "fused": 1}[form]))
+
+    @property
+    def ksolve(self):
+        return "fused" if load().fribo_get_ksolve(self.h) == 1 else "3pass"
 
     def set_precision(self, refine):
         """Poisson transforms: refine < 0 FP64; n >= 0 FP32 MFMA plus n

@@ -96,6 +96,13 @@ int fribo_fluid_tsch(fribo_handle h, int32_t ns, double* dt, double dt_o, double
  * (fluid_tsch.F90:30-57), then one substep n=1..nsst (fluid_tsch.F90:62-166). */
 int fribo_cfl_dt(fribo_handle h, int32_t ns, double* dt, double dt_o, double* cflm);
 int fribo_substep(fribo_handle h, int32_t n, int32_t ns);
+/* fribo_substep in the launch sequence fribo_fluid_tsch runs: substep n's
+ * boundary kernel also applies substep n-1's pressure mean and boundary
+ * update (pr_submean + boucqt, updvp.F90:112-144, iobcs.F90:81-108), and only
+ * the last substep ends with the mean kernel.  Between two substeps "pr"
+ * still holds the pending mean (a constant) and qb2s..qb3n their pending
+ * increments dqb*; the other arrays are final. */
+int fribo_substep_folded(fribo_handle h, int32_t n, int32_t ns);
 
 /* Named arrays in the reference layout: "q2","q3","pr","dph","ru2","ru3",
  * "rhs2","rhs3" (rhs2/rhs3 include the forcing term f*dt at the IB
@@ -202,6 +209,17 @@ int fribo_test_gemm_nn(int32_t M, int32_t N, int32_t K, const double* A, int32_t
  * use. */
 int fribo_set_transform(fribo_handle h, int32_t axis);
 int fribo_get_transform(fribo_handle h);
+
+/* Execution option (no reference counterpart): the form of the k-direction
+ * line solves (solq2k / solq3k, solve_fluid.F90:107-255).  form 1: one
+ * pass per field that reads the right-hand side once (chunks of at most 32
+ * rows, i.e. k-lines of at most 2048 rows; the default where it fits);
+ * form 0: three passes (chunk elimination, separator system, chunk
+ * back-substitution), any length.  Both give identical results.
+ * FRIBO_ERR_LIMIT for form 1 on longer lines, FRIBO_ERR_ARG for an instance
+ * that is in a batch.  fribo_get_ksolve returns the form in use. */
+int fribo_set_ksolve(fribo_handle h, int32_t form);
+int fribo_get_ksolve(fribo_handle h);
 
 /* Execution option (no reference counterpart; the reference's DGEMMs are
  * FP64): precision of the Poisson eigen-transforms along y.  refine < 0:

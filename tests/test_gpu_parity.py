@@ -130,6 +130,74 @@ def test_substep_stages(name, axis):
 
 
 @pytest.mark.parametrize("axis", ["z", "y"])
+@pytest.mark.parametrize("name", ["tiny", "cyl64"])
+def test_folded_substep_stages(name, axis):
+    """The launch sequence of a whole step, stage by stage: substep n's
+    boundary kernel runs substep n-1's pressure mean and boundary update
+    (pr_submean + boucqt, updvp.F90:112-144, iobcs.F90:81-108) instead of a
+    kernel of their own.  Between substeps the pressure differs from the
+    oracle's by one constant, its area-weighted mean (updvp.F90:112-144), and
+    qb2s..qb3n lack their increments dqb*; every other array is final.  Two
+    steps, so the second step's first substep starts from a folded one."""
+    c = case(name)
+    dec = decomposition(c)
+    o = oracle_for(c, dec)
+    g = gpu_for(c, dec, axis=axis)
+    p = c.params
+    n2, n3m = p.n2, p.n3 - 1
+    n2m = n2 - 1
+    zz, rc = o.get("zz"), o.get("rc")
+    area = np.outer(rc[1:n2] - rc[:n2m], zz[1:n3m + 1] - zz[:n3m])
+    lm = c.markers(0.0)
+    o.set_markers(*lm)
+    g.set_markers(*lm)
+    dt = p.dt0
+    for ns in range(2):
+        rc_, dto, cfo = o.cfl_dt(ns, dt)
+        assert rc_ == 0
+        dtg, _ = g.cfl_dt(ns, dt)
+        assert dtg == dto
+        dt = dto
+        for n in range(1, p.nsst + 1):
+            print(f"{name} {axis}: step {ns} substep {n}", flush=True)
+            assert o.substep(n, ns) == 0
+            g.substep(n, ns, folded=True)
+            exact = ns == 0 and n == 1
+            tol = 1e-14 if exact else 1e-11
+            for f, sl in (("ru2", np.s_[1:n2m]), ("ru3", np.s_[:n2m, 1:])):
+                assert relmax(g.get(f)[sl], o.get(f)[sl]) <= tol, f
+            # the library's rhs carries the forcing term f*dt (helsolve.F90:22,47)
+            dts = o.get("scalars")[0]
+            fey, fez = o.get("fey"), o.get("fez")
+            exp2 = o.get("rhs2") + fey[:, 3:3 + n3m] * dts
+            exp3 = o.get("rhs3") + fez[:, 3:3 + n3m] * dts
+            assert relmax(g.get("rhs2")[1:n2m], exp2[1:n2m]) <= tol
+            assert relmax(g.get("rhs3")[:n2m, 1:], exp3[:n2m, 1:]) <= tol
+            # the outflow arrays are difference quotients of q2/q3 on the last
+            # planes (boucdq, iobcs.F90:5-77): their cancellation takes the fields'
+            # ~1e-14 rounding to a few 1e-11 of their own maximum over two
+            # steps, so they get the multi-step bar
+            for f in ("q2", "q3", "dph", "dqb2n", "dqb3n", "dq2x2o", "dq3x2o"):
+                e = relmax(g.get(f), o.get(f))
+                print(f"  {f}: {e:.2e}", flush=True)
+                assert e <= (1e-11 if f in ("q2", "q3", "dph") else 1e-10), f
+            pg, po = g.get("pr"), o.get("pr")
+            last = n == p.nsst
+            if last:
+                assert relmax(pg, po) <= 1e-11
+            else:
+                d = (pg - po)[:n2m, 3:3 + n3m]
+                scale = np.max(np.abs(po))
+                assert np.max(np.abs(d - d[0, 0])) <= 1e-11 * scale
+                # the constant is the pending mean of the raw pressure
+                mean = np.sum(pg[:n2m, 3:3 + n3m] * area) / (p.rext2 * p.alx3)
+                assert abs(d[0, 0] - mean) <= 1e-11 * scale
+            for f in ("qb2n", "qb3n", "qb2s", "qb3s"):
+                gv = g.get(f) + (0.0 if last else g.get("d" + f))
+                assert relmax(gv, o.get(f)) <= 1e-10, f
+
+
+@pytest.mark.parametrize("axis", ["z", "y"])
 @pytest.mark.parametrize("name,nsteps", [("cyl64", 40), ("cyl384", 4)])
 def test_multistep(name, nsteps, axis):
     c = case(name)
@@ -404,3 +472,35 @@ def test_library_phini_gives_the_same_step(axis):
         e = relmax(sb[f], sa[f])
         print(f"library vs host phini ({axis}) {f}: {e:.2e}", flush=True)
         assert e <= 1e-11, f
+
+
+@pytest.mark.parametrize("n3m,n2m", [(384, 96), (1024, 256), (2048, 192)])
+def test_ksolve_forms_agree(n3m, n2m):
+    """The one-pass k-line solve (chunks of 8, 16 and 32 rows here) and the
+    three-pass form run the same operations in the same order: q2, q3 and pr
+    after 3 steps with an immersed cylinder are identical."""
+    c = cases.ensemble_case(0, n3m, n2m, 150.0)
+    dec = decomposition(c)
+    g1 = gpu_for(c, dec)
+    g0 = gpu_for(c, dec)
+    assert g1.ksolve == "fused"
+    g0.set_ksolve("3pass")
+    assert g0.ksolve == "3pass"
+    run_steps(g1, c, 3)
+    run_steps(g0, c, 3)
+    s1, s0 = g1.get_state(), g0.get_state()
+    for f in ("q2", "q3", "pr"):
+        np.testing.assert_array_equal(s1[f], s0[f], err_msg=f)
+    print(f"{n3m}x{n2m}: forms identical, max |q3| {np.max(np.abs(s1['q3'])):.3e}", flush=True)
+
+
+def test_ksolve_fused_limit():
+    """k-lines of 4096 rows (chunks of 64) take the three-pass form; asking
+    for the one-pass form is a LIMIT error."""
+    from fribois2d_amd import gpu
+    c = cases.ensemble_case(0, 4096, 64, 150.0)
+    g = gpu_for(c)
+    assert g.ksolve == "3pass"
+    with pytest.raises(gpu.FriboError) as e:
+        g.set_ksolve("fused")
+    assert "LIMIT" in str(e.value)
