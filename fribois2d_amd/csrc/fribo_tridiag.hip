@@ -28,6 +28,7 @@
 #include "fribo_batch.h"
 
 #include <algorithm>
+#include <utility>
 
 // Row r of a j-line belongs to chunk owner(r) when chunk starts are
 // floor(p*n/nch); (r+1)*nch < 2^32 for every line this build accepts.
@@ -962,22 +963,67 @@ __global__ __launch_bounds__(64) void kb_ksolve3(const BGrid<FluidDev> G)
 // with jl = lane % 16, p = 4 * wave + lane / 16, so a wave's loads cover 16
 // columns (128 bytes) of 4 chunks.  The chunk's forward-eliminated rows stay
 // in registers across the separator solve, so the right-hand side is read
-// once: read src, read q, write q.  The coefficient rows of every chunk are
-// staged in LDS lane-interleaved ([t * 64 + p]); the chunk summaries go to
-// LDS, wave w solves the separator system of column w with the multipliers
-// and operation order of k_ksolve2, and the back-substitution is that of
-// k_ksolve3, so both forms give identical results.
+// once: read src, read q, write q.  The 16 lanes of a chunk hold its
+// coefficient rows (lane u rows u and u + 16) and each elimination step takes
+// its row by a DPP broadcast within the 16-lane row (no LDS traffic); the
+// chunk summaries go to LDS, wave w solves the separator system of column w
+// with the multipliers and operation order of k_ksolve2, and the
+// back-substitution is that of k_ksolve3, so both forms give identical
+// results.
 constexpr int KF_COLS = 16;
+
+// Lane n of each 16-lane row of the wave, broadcast to the row (DPP
+// row_newbcast), for a double.
+template <int N>
+__device__ inline double row_bcast(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + N, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + N, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Row T of the chunk held by the lanes of this 16-lane row (T < 16 * NG).
+template <int T, int NG>
+__device__ inline double krow(const double (&v)[NG])
+{
+    return row_bcast<T & 15>(v[T >> 4]);
+}
+
+template <int... I, class Fn>
+__device__ inline void kf_unroll(std::integer_sequence<int, I...>, Fn fn)
+{
+    (fn(std::integral_constant<int, I>{}), ...);
+}
+
+// Raw buffer access to a field: a uniform descriptor plus one 32-bit byte
+// offset per lane (a 64-bit address would take two registers per row).
+typedef unsigned int kf_u32x2 __attribute__((ext_vector_type(2)));
+__device__ inline __amdgpu_buffer_rsrc_t kf_rsrc(const double* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), 0, 0x7fffffff, 0x00020000);
+}
+// voff: per-lane byte offset; soff: uniform byte offset (a scalar register)
+__device__ inline double kf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff = 0)
+{
+    const kf_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    return __hiloint2double((int)v.y, (int)v.x);
+}
+__device__ inline void kf_store(__amdgpu_buffer_rsrc_t r, int voff, double d, int soff = 0)
+{
+    kf_u32x2 v;
+    v.x = (unsigned)__double2loint(d);
+    v.y = (unsigned)__double2hiint(d);
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+}
 
 template <int MAXM>
 __device__ __forceinline__ void ksolve_fused_body(const FluidDev& F, const VBlock& vb)
 {
     static_assert(MAXM <= 32, "fused k-solve: chunks of at most 32 rows");
-    constexpr int MI = MAXM - 1;   // interior rows of the longest chunk
-    __shared__ double cf[6][MAXM * 64];   // S, a, inv, g, vp, cp
+    constexpr int MI = MAXM - 1;                  // interior rows of the longest chunk
+    constexpr int NG = (MI + 15) / 16;            // coefficient rows per lane
     __shared__ double sY[3][64][KF_COLS + 1];
     __shared__ double sX[64][KF_COLS + 1];
-    __shared__ int scs[FRIBO_KMAXCH + 1];
     const DevScalars* S = F.S;
     if (S->err) return;
     const int f = vb.z;
@@ -985,55 +1031,70 @@ __device__ __forceinline__ void ksolve_fused_body(const FluidDev& F, const VBloc
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int jl = lane & (KF_COLS - 1), p = 4 * wave + (lane >> 4);
     const int nch = K.nch;
-    for (int i = t; i <= nch; i += 1024) scs[i] = K.cs[i];
-    __syncthreads();
-    for (int i = t; i < MAXM * 64; i += 1024) {
-        const int pp = i & 63, tt = i >> 6;
-        if (pp < nch) {
-            const int r = scs[pp] + tt;
-            if (r < scs[pp + 1]) {
-                cf[0][i] = K.S[r];
-                cf[1][i] = K.a[r];
-                cf[2][i] = K.inv[r];
-                cf[3][i] = K.g[r];
-                cf[4][i] = K.vp[r];
-                cf[5][i] = K.cp[r];
-            }
-        }
-    }
-    __syncthreads();
     const int j = vb.x * KF_COLS + jl + 1;
     const bool act = j <= F.n2m;
     const int jc = act ? j : F.n2m;
-    const size_t P = F.P;
+    // byte offsets of a field fit in 31 bits (the launcher checks), so every
+    // access is a uniform buffer descriptor plus a 32-bit lane offset
+    const int P8 = F.P * 8;
+    const int col = ((jc - 1) + (K.kofs + 2) * F.P) * 8;  // row 0 of this column's line
     const bool own = p < nch;
-    const int s = own ? scs[p] : 0, e = own ? scs[p + 1] - 1 : 0, m = e - s;
-    double b0 = 0.0, bn = 0.0;
-    double y[MI > 0 ? MI : 1];
-    if (own) {
-        const double* __restrict__ src = (f == 0 ? F.sA : F.sB) + fidx(F, jc, K.kofs);
-        kbound(F, K, f, jc, S, b0, bn);
+    const int pc = own ? p : nch - 1;             // idle rows of lanes work on the last chunk, store nothing
+    const int s = K.cs[pc], e = K.cs[pc + 1] - 1, m = e - s;
+    const __amdgpu_buffer_rsrc_t src = kf_rsrc(f == 0 ? F.sA : F.sB);
+    // row tt of the chunk: lane offset of row s plus the uniform tt * P8
+    const int v0 = col + s * P8;
+    double y[MI];
 #pragma unroll
-        for (int tt = 0; tt < MI; ++tt) y[tt] = src[(size_t)(s + (tt < m ? tt : 0)) * P];
-        const double dsep = src[(size_t)e * P];
-        double yp = 0.0, acc = 0.0;
+    for (int tt = 0; tt < MI; ++tt) y[tt] = tt < m ? kf_load(src, v0, tt * P8) : 0.0;
+    const double dsep = kf_load(src, col + e * P8);
+    // coefficient rows of the chunk: lane u of the row holds rows u + 16 g
+    double cS[NG], ca[NG], ci[NG], cg[NG];
 #pragma unroll
-        for (int tt = 0; tt < MI; ++tt) {
-            if (tt < m) {
-                const int i = s + tt;
-                double d = y[tt] * cf[0][tt * 64 + p];
-                d = (i == 0) ? d + b0 : d;
-                yp = (d - cf[1][tt * 64 + p] * yp) * cf[2][tt * 64 + p];
-                acc = acc + cf[3][tt * 64 + p] * yp;
-                y[tt] = yp;
-            }
+    for (int g = 0; g < NG; ++g) {
+        const int r = s + min(jl + 16 * g, m > 0 ? m - 1 : 0);
+        cS[g] = K.S[r];
+        ca[g] = K.a[r];
+        ci[g] = K.inv[r];
+        cg[g] = K.g[r];
+    }
+    const double Se = K.S[e];
+    double b0, bn;
+    kbound(F, K, f, jc, S, b0, bn);
+    double yp = 0.0, acc = 0.0;
+    kf_unroll(std::make_integer_sequence<int, MI>{}, [&](auto ic) {
+        constexpr int tt = decltype(ic)::value;
+        // one step at a time: hoisted, the broadcasts of all steps would be
+        // live at once and spill
+        __builtin_amdgcn_sched_barrier(0);
+        const double rS = krow<tt>(cS), ra = krow<tt>(ca), ri = krow<tt>(ci), rg = krow<tt>(cg);
+        if (tt < m) {
+            const int i = s + tt;
+            double d = y[tt] * rS;
+            d = (i == 0) ? d + b0 : d;
+            yp = (d - ra * yp) * ri;
+            acc = acc + rg * yp;
+            y[tt] = yp;
         }
-        double de = dsep * cf[0][m * 64 + p];
-        de = (e == 0) ? de + b0 : de;
-        de = (e == K.n - 1) ? de + bn : de;
+    });
+    double de = dsep * Se;
+    de = (e == 0) ? de + b0 : de;
+    de = (e == K.n - 1) ? de + bn : de;
+    if (own) {
         sY[0][p][jl] = acc;
         sY[1][p][jl] = yp;
         sY[2][p][jl] = de;
+    }
+    // back-substitution rows and the separator row of q, loaded while the
+    // separator systems are solved
+    const __amdgpu_buffer_rsrc_t q = kf_rsrc(f == 0 ? F.q2 : F.q3);
+    const double qe = kf_load(q, col + e * P8);
+    double cv[NG], cc[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int r = s + min(jl + 16 * g, m > 0 ? m - 1 : 0);
+        cv[g] = K.vp[r];
+        cc[g] = K.cp[r];
     }
     __syncthreads();
     {
@@ -1041,7 +1102,7 @@ __device__ __forceinline__ void ksolve_fused_body(const FluidDev& F, const VBloc
         const int c = wave, q = lane;
         double delta = 0.0;
         if (q < nch) {
-            const int eq = scs[q + 1] - 1;
+            const int eq = K.cs[q + 1] - 1;
             const double ysn = (q + 1 < nch) ? sY[0][q + 1][c] : 0.0;
             delta = sY[2][q][c] - K.a[eq] * sY[1][q][c] - K.c[eq] * ysn;
         }
@@ -1055,29 +1116,47 @@ __device__ __forceinline__ void ksolve_fused_body(const FluidDev& F, const VBloc
         if (q < nch) sX[q][c] = delta * K.ib[q];
     }
     __syncthreads();
-    if (!own) return;
-    const double X = sX[p][jl];
-    const double Xm = (p > 0) ? sX[p - 1][jl] : 0.0;
-    const double wlp = K.wl[p];
+    const double X = sX[pc][jl];
+    const double Xm = (pc > 0) ? sX[pc - 1][jl] : 0.0;
+    const double wlp = K.wl[pc];
+    const bool st = own && act;
     const bool zero_row = (f == 0 && j == 1);
-    double* __restrict__ q = (f == 0 ? F.q2 : F.q3) + fidx(F, jc, K.kofs);
-    double* qn2 = (f == 0 ? F.q2 : F.q3) + fidx(F, F.n2, K.kofs);
-    const double qe = q[(size_t)e * P];
+    // back-substitution into y (x of every interior row, no memory traffic)
     double x = 0.0;
-#pragma unroll
-    for (int tt = MI - 1; tt >= 0; --tt) {
+    kf_unroll(std::make_integer_sequence<int, MI>{}, [&](auto ic) {
+        constexpr int tt = MI - 1 - decltype(ic)::value;
+        __builtin_amdgcn_sched_barrier(0);
+        const double rv = krow<tt>(cv), rc = krow<tt>(cc);
         if (tt < m) {
-            const int i = s + tt;
-            const double qv = q[(size_t)i * P];
-            if (tt == m - 1) x = y[tt] + cf[4][tt * 64 + p] * Xm + wlp * X;
-            else x = (y[tt] + cf[4][tt * 64 + p] * Xm) - cf[5][tt * 64 + p] * x;
-            if (act) q[(size_t)i * P] = zero_row ? 0.0 : qv + x;
-            if (act && zero_row) qn2[(size_t)i * P] = 0.0;
+            if (tt == m - 1) x = y[tt] + rv * Xm + wlp * X;
+            else x = (y[tt] + rv * Xm) - rc * x;
+            y[tt] = x;
+        }
+    });
+    // q += x, rows in batches whose loads are issued together (the stores of
+    // a batch precede the next batch's loads, which the compiler keeps in
+    // order since both address the same field)
+    const int coln = ((F.n2 - 1) + (K.kofs + 2) * F.P) * 8;   // column n2
+    constexpr int QB = MI < 8 ? MI : 8;
+#pragma unroll
+    for (int b = 0; b < MI; b += QB) {
+        double qv[QB];
+#pragma unroll
+        for (int u = 0; u < QB; ++u) {
+            const int tt = b + u;
+            if (tt < MI) qv[u] = tt < m ? kf_load(q, v0, tt * P8) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < QB; ++u) {
+            const int tt = b + u;
+            if (tt < MI && tt < m && st) kf_store(q, v0, zero_row ? 0.0 : qv[u] + y[tt], tt * P8);
         }
     }
-    if (!act) return;
-    q[(size_t)e * P] = zero_row ? 0.0 : qe + X;
-    if (zero_row) qn2[(size_t)e * P] = 0.0;
+    if (st && zero_row)
+        for (int i = s; i < e; ++i) kf_store(q, coln + i * P8, 0.0);
+    if (!st) return;
+    kf_store(q, col + e * P8, zero_row ? 0.0 : qe + X);
+    if (zero_row) kf_store(q, coln + e * P8, 0.0);
     if (f == 1) {
         double* qf = F.q3;
         if (p == 0) {
@@ -1106,13 +1185,16 @@ __global__ __launch_bounds__(1024) void kb_ksolve_fused(const BGrid<FluidDev> G)
 // Largest chunk (rows incl. the separator) of either k-system.
 static inline int kmaxm(const FluidDev& F) { return std::max(F.ks[0].maxm, F.ks[1].maxm); }
 
-bool ksolve_fused_fits(const FluidDev& F) { return kmaxm(F) <= 32; }
+bool ksolve_fused_fits(const FluidDev& F)
+{
+    return kmaxm(F) <= 32 && (long)F.P * F.nk * 8 < (1L << 31);
+}
 
 void launch_ksolve(const FluidDev& F, hipStream_t st)
 {
     int nchmax = F.ks[0].nch > F.ks[1].nch ? F.ks[0].nch : F.ks[1].nch;
     int maxm = kmaxm(F);
-    if (F.kform && maxm <= 32) {
+    if (F.kform && ksolve_fused_fits(F)) {
         const dim3 gf((F.n2m + KF_COLS - 1) / KF_COLS, 1, 2);
         if (maxm <= 8) hipLaunchKernelGGL(k_ksolve_fused<8>, gf, dim3(1024), 0, st, F);
         else if (maxm <= 16) hipLaunchKernelGGL(k_ksolve_fused<16>, gf, dim3(1024), 0, st, F);
@@ -1140,9 +1222,9 @@ void launch_ksolve_b(BatchSet& B, hipStream_t st)
     bool fused = true;
     for (const auto& F : B.hF) {
         maxm = std::max(maxm, kmaxm(F));
-        fused = fused && F.kform;
+        fused = fused && F.kform && ksolve_fused_fits(F);
     }
-    if (fused && maxm <= 32) {
+    if (fused) {
         const auto gf = bdims(B, [](const FluidDev& F) { return dim3((F.n2m + KF_COLS - 1) / KF_COLS, 1, 2); });
         if (maxm <= 8) blaunch(B, BK_KF, gf, kb_ksolve_fused<8>, dim3(1024), 0, st);
         else if (maxm <= 16) blaunch(B, BK_KF, gf, kb_ksolve_fused<16>, dim3(1024), 0, st);
