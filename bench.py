@@ -164,7 +164,7 @@ def run_ensemble(args):
     flops = float(sum(2.0 * (c.params.n3 - 1) * (c.params.n2 - 1) ** 2 for c in cs))
     gemm_ms = float(tune_ms[best])
     achieved = flops / (gemm_ms * 1e-3) / 1e12
-    shapes = {0: "32x32", 1: "64x64", 2: "96x128", 3: "direct-to-LDS 96x64"}
+    shapes = {0: "32x32", 1: "64x64", 2: "96x128", 3: "direct-to-LDS 96x128"}
     out = {
         "metric": "Mcell-steps/s (batched ensemble, plain FP64)",
         "value": round(value, 3),
@@ -389,7 +389,7 @@ def main():
     achieved = gemm_flops / gemm_avg_s / 1e12
     # the transforms of this workload take the direct-to-LDS kernel (at least
     # 192 tiles of 96x128 in the library's size test, operands with padded
-    # leading dimensions; the kernel itself runs 96x64 tiles)
+    # leading dimensions)
     gemm_kernel = "k_gemm_dma" if -(-n2m // 96) * -(-n3m // 128) >= 192 else "k_gemm_nt"
     peak = PEAK_FP64_MFMA_TFLOPS
     if args.precision >= 0:

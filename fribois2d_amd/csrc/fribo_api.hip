@@ -1967,7 +1967,7 @@ extern "C" int fribo_batch_set_graph(fribo_batch b, int32_t on)
 }
 
 // Tile shape of the batch's transforms: 0 (32x32), 1 (64x64), 2 (96x128),
-// 3 (direct-to-LDS 96x64) or -1 for the default.
+// 3 (direct-to-LDS 96x128) or -1 for the default.
 extern "C" int fribo_batch_set_gemm_shape(fribo_batch b, int32_t cfg)
 {
     if (!b || cfg < -1 || cfg > 3) return set_err(FRIBO_ERR_ARG, "bad tile shape");

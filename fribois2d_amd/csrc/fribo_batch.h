@@ -81,7 +81,7 @@ struct BatchSet {
                                const std::vector<int>& dims) = nullptr;
     void* owner = nullptr;
     // choices shared by all instances (largest size class over the batch)
-    int gemm_cfg = 0;        // 0: 32x32 tiles, 1: 64x64, 2: 96x128, 3: direct-to-LDS 96x64
+    int gemm_cfg = 0;        // 0: 32x32 tiles, 1: 64x64, 2: 96x128, 3: direct-to-LDS 96x128
     bool ysolve_fused = false;
 
     BGrid<FluidDev> grid(int key, const std::vector<int>& dims) { return grid_fn(this, key, dims); }

@@ -23,10 +23,10 @@
 // read as 16-double runs of each column and stored with a row pitch of BN+1,
 // which spreads the transposing LDS stores over the banks.
 //
-// Large problems (k_gemm_dma): 96 x 64 tiles of 8 waves (48 x 16 each), two
-// workgroups per CU so that one's barrier waits overlap the other's MFMAs
-// (72 KB of LDS each), and the operand slices move from global memory straight
-// into LDS
+// Large problems (k_gemm_dma): 96 x 128 tiles of 16 waves (48 x 16 each, one
+// workgroup of 96 KB of LDS per CU; 96 x 64 tiles of 8 waves, two per CU,
+// measured 1-5% slower), and the operand slices move from global memory
+// straight into LDS
 // (global_load_lds_dwordx4, 16 bytes per lane, no register staging) through
 // three stages, so two K-slices of loads are in flight while one is
 // multiplied.  Each stage is its own LDS array, the slice loop is unrolled
@@ -45,7 +45,7 @@
 // the k-rows past K of the last slice, which are cleared before use; when
 // the load count of a stage does not divide evenly among the waves, the spare
 // loads land past the stage's B block.  The 1536 x 2048 transforms of the
-// 2048x1536-cell grid give 512 workgroups, two per CU; workgroups are
+// 2048x1536-cell grid give 256 workgroups, one per CU; workgroups are
 // dispatched to the 8 XCDs round-robin, so when the grid is a multiple of 256
 // tiles in 4 x 8 patches the tile indices are permuted to give each XCD
 // compact patches (its L2 then serves 4 A- and 8 B-tiles to 32 workgroups).
@@ -63,10 +63,10 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 #define GEMM_DMA_WM 2
 #endif
 #ifndef GEMM_DMA_WN
-#define GEMM_DMA_WN 4
+#define GEMM_DMA_WN 8
 #endif
 #ifndef GEMM_DMA_BN
-#define GEMM_DMA_BN 64
+#define GEMM_DMA_BN 128
 #endif
 
 // Gauge partial of the transform along y (zg != nullptr): the tile holding

@@ -596,6 +596,9 @@ void launch_bc_b(BatchSet& B, int n, bool mean, hipStream_t st)
 // measured gain holds for k_div only (k_upd runs faster with one wave per
 // workgroup, k_rhs is indifferent).
 #define SW_RHS 1
+#ifndef FRIBO_RHS_PF2
+#define FRIBO_RHS_PF2 0
+#endif
 #define SW_DIV 8
 #define SW_UPD 1
 static_assert(MK <= 64 && MKR <= 64, "one plane of a tile per lane");
@@ -630,6 +633,20 @@ __device__ __forceinline__ void rhs_body(const FluidDev& F, const VBlock& vb)
     double* __restrict__ rhs3 = F.rhs3;
     const size_t P = F.P;
     size_t c = inj ? fidx(F, j, k0) : 0;
+#if FRIBO_RHS_PF2
+    // window: planes k-1, k, k+1 of q2/q3; the loads run two planes ahead
+    // (q2/q3 of plane k+3, pr/ru2/ru3 of plane k+2 are issued in plane k)
+    const bool two = k0 + 1 <= k1;
+    double q2m = inj ? q2[c - P] : 0.0, q2c = inj ? q2[c] : 0.0;
+    double q3m = inj ? q3[c - P] : 0.0, q3c = inj ? q3[c] : 0.0;
+    double prm = inj ? pr[c - P] - pend : 0.0;
+    double q2n = inj ? q2[c + P] : 0.0, q3n = inj ? q3[c + P] : 0.0;
+    double prn = inj ? pr[c] - pend : 0.0;
+    double ru2n = out ? ru2[c] : 0.0, ru3n = out ? ru3[c] : 0.0;
+    double q2nn = (inj && two) ? q2[c + 2 * P] : 0.0, q3nn = (inj && two) ? q3[c + 2 * P] : 0.0;
+    double prnn = (inj && two) ? pr[c + P] - pend : 0.0;
+    double ru2nn = (out && two) ? ru2[c + P] : 0.0, ru3nn = (out && two) ? ru3[c + P] : 0.0;
+#else
     // window: planes k-1, k; plane k+1 (and ru2/ru3/pr of plane k) are
     // loaded one iteration ahead so two planes of loads are in flight
     double q2m = inj ? q2[c - P] : 0.0, q2c = inj ? q2[c] : 0.0;
@@ -638,6 +655,7 @@ __device__ __forceinline__ void rhs_body(const FluidDev& F, const VBlock& vb)
     double q2n = inj ? q2[c + P] : 0.0, q3n = inj ? q3[c + P] : 0.0;
     double prn = inj ? pr[c] - pend : 0.0;
     double ru2n = out ? ru2[c] : 0.0, ru3n = out ? ru3[c] : 0.0;
+#endif
     // per-column coefficients
     const int jj = out ? j : 1;
     const double udx2c = F.udx2c[jj], udx2m = F.udx2m[jj];
@@ -653,6 +671,20 @@ __device__ __forceinline__ void rhs_body(const FluidDev& F, const VBlock& vb)
     const double l_ap3ck = F.ap3ck[kl], l_ac3ck = F.ac3ck[kl], l_am3ck = F.am3ck[kl];
     for (int k = k0; k <= k1; ++k, c += P) {
         const double q2p = q2n, q3p = q3n, prc = prn, ru2c = ru2n, ru3c = ru3n;
+#if FRIBO_RHS_PF2
+        q2n = q2nn;
+        q3n = q3nn;
+        prn = prnn;
+        ru2n = ru2nn;
+        ru3n = ru3nn;
+        if (k + 1 < k1) {
+            q2nn = inj ? q2[c + 3 * P] : 0.0;
+            q3nn = inj ? q3[c + 3 * P] : 0.0;
+            prnn = inj ? pr[c + 2 * P] - pend : 0.0;
+            ru2nn = out ? ru2[c + 2 * P] : 0.0;
+            ru3nn = out ? ru3[c + 2 * P] : 0.0;
+        }
+#else
         if (k < k1) {
             q2n = inj ? q2[c + 2 * P] : 0.0;
             q3n = inj ? q3[c + 2 * P] : 0.0;
@@ -660,6 +692,7 @@ __device__ __forceinline__ void rhs_body(const FluidDev& F, const VBlock& vb)
             ru2n = out ? ru2[c + P] : 0.0;
             ru3n = out ? ru3[c + P] : 0.0;
         }
+#endif
         const int kk = k - k0;
         const double udx3m_k = lane_bcast(l_udx3m, kk), udx3c_k = lane_bcast(l_udx3c, kk);
         const double ap3sk_k = lane_bcast(l_ap3sk, kk), ac3sk_k = lane_bcast(l_ac3sk, kk);
