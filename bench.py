@@ -39,6 +39,33 @@ def pmc_traffic(name):
         return None
 
 
+def run_gemm_reference(reps=50):
+    """The library-independent yardstick of the transform GEMM: torch's FP64
+    matmul (hipBLASLt / rocBLAS) on the y transform of the 2048x1536 grid,
+    C (1536 x 2048) = A (1536 x 1536) * B (1536 x 2048) in the column-major
+    layout k_gemm_dma reads (A modes-contiguous, B k-contiguous), i.e. the
+    row-major product B^T (2048 x 1536) @ A^T (1536 x 1536)."""
+    import torch
+    M, N, K = 1536, 2048, 1536
+    g = torch.Generator(device="cuda").manual_seed(5)
+    bt = torch.randn(N, K, dtype=torch.float64, device="cuda", generator=g)
+    at = torch.randn(K, M, dtype=torch.float64, device="cuda", generator=g)
+    for _ in range(10):
+        ct = bt @ at
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ct = bt @ at
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    tf = 2.0 * M * N * K / (us * 1e-6) / 1e12
+    print(json.dumps({"kernel": "torch.matmul float64 (vendor BLAS)", "M": M, "N": N, "K": K,
+                      "us_per_gemm": round(us, 2), "TFLOP/s": round(tf, 2),
+                      "frac_of_fp64_mfma_peak": round(tf / PEAK_FP64_MFMA_TFLOPS, 4), "checksum": float(ct[0, 0])}))
+
+
 def ensemble_members(n, seed=20261202):
     """The batched ensemble: n cylinders, n3m uniform in 256..1024 (multiples
     of 8), n2m = 3/4 n3m (multiple of 8), Re uniform in [60, 200], reduced
@@ -164,7 +191,7 @@ def run_ensemble(args):
     flops = float(sum(2.0 * (c.params.n3 - 1) * (c.params.n2 - 1) ** 2 for c in cs))
     gemm_ms = float(tune_ms[best])
     achieved = flops / (gemm_ms * 1e-3) / 1e12
-    shapes = {0: "32x32", 1: "64x64", 2: "96x128", 3: "direct-to-LDS 96x128"}
+    shapes = {0: "32x32", 1: "64x64", 2: "96x128", 3: "direct-to-LDS 96x128", 4: "direct-to-LDS 96x64"}
     out = {
         "metric": "Mcell-steps/s (batched ensemble, plain FP64)",
         "value": round(value, 3),
@@ -301,6 +328,9 @@ def main():
                          "(steps with GEMM timing events always launch on the stream)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the library's plain 256 MiB copy kernel 3+1 times (HBM counter scale)")
+    ap.add_argument("--gemm-reference", action="store_true",
+                    help="only time the vendor FP64 GEMM (torch.matmul: hipBLASLt/rocBLAS) on the y transform "
+                         "of the 2048x1536 grid, the yardstick of k_gemm_dma")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -314,6 +344,10 @@ def main():
         print(json.dumps({"kernel": "k_plain_copy", "bytes_read": nbytes, "bytes_written": nbytes,
                           "dispatches": 4, "ms_per_copy": round(ms, 4),
                           "GB/s": round(2 * nbytes / ms / 1e6, 1)}))
+        return
+
+    if args.gemm_reference:
+        run_gemm_reference()
         return
 
     if args.workload.startswith("ensemble"):

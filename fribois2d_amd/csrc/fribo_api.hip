@@ -1248,9 +1248,12 @@ extern "C" int fribo_test_gemm_nn(int32_t M, int32_t N, int32_t K, const double*
 }
 
 // Streaming copy of a known byte count: the yardstick that turns the HBM
-// counters (FETCH_SIZE / WRITE_SIZE) of the step's kernels into bytes.
-__global__ void __launch_bounds__(256) k_plain_copy(const double2* __restrict__ a, double2* __restrict__ b,
-                                                    size_t n)
+// counters (FETCH_SIZE / WRITE_SIZE) of the step's kernels into bytes, and
+// the achievable-bandwidth reference.  One double per lane, 8192 workgroups
+// of 256 over a grid-stride loop: the fastest of the simple copies measured
+// on MI355X (5.9-6.05 TB/s on 256 MiB; two doubles per lane, unrolled or
+// non-temporal forms ran at 4.6-5.6).
+__global__ void __launch_bounds__(256) k_plain_copy(const double* __restrict__ a, double* __restrict__ b, size_t n)
 {
     const size_t s = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += s) b[i] = a[i];
@@ -1258,18 +1261,18 @@ __global__ void __launch_bounds__(256) k_plain_copy(const double2* __restrict__ 
 
 extern "C" int fribo_test_copy(int64_t bytes, int32_t reps, double* ms_per_copy)
 {
-    if (bytes <= 0 || bytes % 16 != 0 || reps <= 0) return set_err(FRIBO_ERR_ARG, "bad copy arguments");
-    double2 *a, *b;
+    if (bytes <= 0 || bytes % 8 != 0 || reps <= 0) return set_err(FRIBO_ERR_ARG, "bad copy arguments");
+    double *a, *b;
     HIPCHK(hipMalloc(&a, (size_t)bytes));
     HIPCHK(hipMalloc(&b, (size_t)bytes));
     HIPCHK(hipMemset(a, 0, (size_t)bytes));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    const size_t n = (size_t)bytes / 16;
-    hipLaunchKernelGGL(k_plain_copy, dim3(4096), dim3(256), 0, 0, a, b, n);   // warm-up
+    const size_t n = (size_t)bytes / 8;
+    hipLaunchKernelGGL(k_plain_copy, dim3(8192), dim3(256), 0, 0, a, b, n);   // warm-up
     HIPCHK(hipEventRecord(e0, 0));
-    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_plain_copy, dim3(4096), dim3(256), 0, 0, a, b, n);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_plain_copy, dim3(8192), dim3(256), 0, 0, a, b, n);
     HIPCHK(hipEventRecord(e1, 0));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1658,7 +1661,7 @@ static BGrid<GemmProb> batch_gemm(BatchSet* B, int key, const std::vector<GemmPr
 // instance would take on its own, among those every instance can run.
 static int batch_default_gemm(const fribo_batch_ctx* b)
 {
-    int mask = 15;
+    int mask = 31;
     long best = -1;
     const FluidDev* big = nullptr;
     for (auto h : b->inst) {
@@ -1685,8 +1688,8 @@ static int batch_refresh(fribo_batch_ctx* b)
 {
     BatchSet& B = b->B;
     B.gemm_cfg = b->gemm_fixed >= 0 ? b->gemm_fixed : batch_default_gemm(b);
-    bool fuse = B.gemm_cfg == 3;
-    for (auto h : b->inst) fuse = fuse && gemm_fuse_ok(h->F);
+    bool fuse = B.gemm_cfg == 3 || B.gemm_cfg == 4;
+    for (auto h : b->inst) fuse = fuse && gemm_fuse_ok(h->F, B.gemm_cfg);
     B.ysolve_fused = fuse;
     for (int i = 0; i < B.n; ++i) {
         FluidDev F = b->inst[i]->F;
@@ -1967,10 +1970,10 @@ extern "C" int fribo_batch_set_graph(fribo_batch b, int32_t on)
 }
 
 // Tile shape of the batch's transforms: 0 (32x32), 1 (64x64), 2 (96x128),
-// 3 (direct-to-LDS 96x128) or -1 for the default.
+// 3 (direct-to-LDS 96x128), 4 (direct-to-LDS 96x64) or -1 for the default.
 extern "C" int fribo_batch_set_gemm_shape(fribo_batch b, int32_t cfg)
 {
-    if (!b || cfg < -1 || cfg > 3) return set_err(FRIBO_ERR_ARG, "bad tile shape");
+    if (!b || cfg < -1 || cfg > 4) return set_err(FRIBO_ERR_ARG, "bad tile shape");
     if (cfg >= 0)
         for (auto h : b->inst)
             if (!((gemm_cfg_mask(h->F) >> cfg) & 1))
@@ -1988,12 +1991,12 @@ extern "C" int fribo_batch_get_gemm_shape(fribo_batch b)
 // every instance (its output hmn is scratch between steps; nothing of the
 // state changes) with each shape all instances can run, reps timed launches
 // after one untimed, on the batch's own problems; the fastest shape is kept.
-// ms[4] receives the mean time per grouped launch of shapes 0..3 (0 where a
+// ms[5] receives the mean time per grouped launch of shapes 0..4 (0 where a
 // shape is not available).
 extern "C" int fribo_batch_autotune(fribo_batch b, int32_t reps, double* ms, int32_t* best_out)
 {
     if (!b || reps < 1) return set_err(FRIBO_ERR_ARG, "bad autotune arguments");
-    int mask = 15;
+    int mask = 31;
     for (auto h : b->inst) mask &= gemm_cfg_mask(h->F);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(hipEventCreate(&e0));
@@ -2001,7 +2004,7 @@ extern "C" int fribo_batch_autotune(fribo_batch b, int32_t reps, double* ms, int
     int best = -1;
     float best_ms = 0.0f;
     const int keep = b->gemm_fixed;
-    for (int cfg = 0; cfg < 4; ++cfg) {
+    for (int cfg = 0; cfg < 5; ++cfg) {
         if (ms) ms[cfg] = 0.0;
         if (!((mask >> cfg) & 1)) continue;
         b->gemm_fixed = cfg;

@@ -81,7 +81,7 @@ struct BatchSet {
                                const std::vector<int>& dims) = nullptr;
     void* owner = nullptr;
     // choices shared by all instances (largest size class over the batch)
-    int gemm_cfg = 0;        // 0: 32x32 tiles, 1: 64x64, 2: 96x128, 3: direct-to-LDS 96x128
+    int gemm_cfg = 0;        // 0: 32x32 tiles, 1: 64x64, 2: 96x128, 3: direct-to-LDS 96x128, 4: direct-to-LDS 96x64
     bool ysolve_fused = false;
 
     BGrid<FluidDev> grid(int key, const std::vector<int>& dims) { return grid_fn(this, key, dims); }
@@ -149,7 +149,7 @@ bool launch_poisson_gemm_b(BatchSet& B, bool fwd, hipStream_t st);
 int jsolve_class_h(const FluidDev& F);
 int gemm_cfg_mask(const FluidDev& F);
 int gemm_cfg_parts(const FluidDev& F, int cfg);
-bool gemm_fuse_ok(const FluidDev& F);
+bool gemm_fuse_ok(const FluidDev& F, int cfg);
 int jsolve_stride_for(const FluidDev& F, int mh);
 // true when every instance can use the counting-sort marker order
 bool mls_batchable(const FluidDev& F);

@@ -516,9 +516,13 @@ int gemm_nn_tiles_n(int M, int N, int K, const double* A, int lda, const double*
 // output element over k in the same order, so the transform results do not
 // depend on the shape; the gauge partials are one per tile column of the
 // shape in use (gemm_cfg_tile_n).
+// Shapes 3 and 4 are the direct-to-LDS kernel with 96 x GEMM_DMA_BN tiles of
+// 16 waves (the single-instance transforms' shape) and 96 x 64 tiles of 8
+// waves, two workgroups per CU (faster when many small instances leave
+// partial tiles).
 int gemm_cfg_tile_n(int cfg)
 {
-    return cfg == 0 ? 32 : cfg == 1 ? 64 : cfg == 2 ? 128 : GEMM_DMA_BN;
+    return cfg == 0 ? 32 : cfg == 1 ? 64 : cfg == 2 ? 128 : cfg == 3 ? GEMM_DMA_BN : 64;
 }
 
 // The shapes every instance of the batch can run: bit c set for cfg c.
@@ -534,7 +538,7 @@ int gemm_cfg_mask(const FluidDev& F)
     const double* A2 = yax ? F.ymm : F.hmn;
     const double* B2 = yax ? F.hmn : F.zmm;
     const bool ok2 = yax ? dma_ok<true>(F.n2m, F.n3m, K, A2, lda, B2, ldb) : dma_ok<false>(F.n2m, F.n3m, K, A2, F.P, B2, ldb);
-    if (ok1 && ok2) m |= 8;
+    if (ok1 && ok2) m |= 8 | 16;
     return m;
 }
 
@@ -550,6 +554,7 @@ static void launch_gemm_b(BatchSet& B, int key, const std::vector<GemmProb>& pr,
     case 1: bm = 64; bn = 64; break;
     case 2: bm = 96; bn = 128; break;
     case 3: bm = 96; bn = GEMM_DMA_BN; break;
+    case 4: bm = 96; bn = 64; break;
     default: break;
     }
     for (int i = 0; i < B.n; ++i) {
@@ -565,6 +570,7 @@ static void launch_gemm_b(BatchSet& B, int key, const std::vector<GemmProb>& pr,
         hipLaunchKernelGGL((kb_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN, GEMM_DMA_BN>), dim3(tot),
                            dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), 0, st, G);
         break;
+    case 4: hipLaunchKernelGGL((kb_gemm_dma<GEMM_DMA_BK, NN, 2, 4, 64>), dim3(tot), dim3(512), 0, st, G); break;
     case 2: hipLaunchKernelGGL((kb_gemm_nt<96, 128, 2, 4, NN>), dim3(tot), dim3(512), 0, st, G); break;
     case 1: hipLaunchKernelGGL((kb_gemm_nt<64, 64, 2, 2, NN>), dim3(tot), dim3(256), 0, st, G); break;
     default: hipLaunchKernelGGL((kb_gemm_nt<32, 32, 2, 2, NN>), dim3(tot), dim3(256), 0, st, G); break;
@@ -615,14 +621,17 @@ bool launch_poisson_gemm_b(BatchSet& B, bool fwd, hipStream_t st)
 }
 
 // The fused chunk pass is possible for an instance when its line partition
-// has equal chunks that fill the direct-to-LDS kernel's output tiles.
-bool gemm_fuse_ok(const FluidDev& F)
+// has equal chunks that fill the direct-to-LDS kernel's output tiles of batch
+// shape cfg (3 or 4), and the tile has a group of (96 + 63) / 64 waves per
+// chunk.
+bool gemm_fuse_ok(const FluidDev& F, int cfg)
 {
-    constexpr int kChunkGroups = (GEMM_DMA_WM * GEMM_DMA_WN) / ((96 + 63) / 64);
+    if (cfg != 3 && cfg != 4) return false;
+    const int bn = gemm_cfg_tile_n(cfg), waves = cfg == 3 ? GEMM_DMA_WM * GEMM_DMA_WN : 8;
+    const int groups = waves / ((96 + 63) / 64);
     if (F.taxis != 1 || F.n3m % F.ks[0].nch != 0) return false;
     const int csz = F.n3m / F.ks[0].nch;
-    return csz > 1 && csz <= 33 && GEMM_DMA_BN % csz == 0 && F.n3m % GEMM_DMA_BN == 0 &&
-           GEMM_DMA_BN / csz <= kChunkGroups;
+    return csz > 1 && csz <= 33 && bn % csz == 0 && F.n3m % bn == 0 && bn / csz <= groups;
 }
 
 // ---------------------------------------------------------------------------

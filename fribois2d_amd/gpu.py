@@ -478,7 +478,7 @@ class FluidBatch:
 
     def autotune(self, reps=5):
         """Pick the transform tile shape by timing; returns (best, ms per shape)."""
-        ms = np.zeros(4)
+        ms = np.zeros(5)
         best = C.c_int32(-1)
         _check(load().fribo_batch_autotune(self.b, int(reps), _dp(ms), C.byref(best)))
         return best.value, ms

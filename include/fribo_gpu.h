@@ -259,7 +259,7 @@ int fribo_test_ytransform(int32_t n, const double* am, const double* ac, const d
                           double* wy);
 
 /* Diagnostic entry point (no reference counterpart): `reps` streaming copies
- * of `bytes` (a multiple of 16) between two device buffers after one warm-up
+ * of `bytes` (a multiple of 8) between two device buffers after one warm-up
  * copy; returns the mean device time per copy.  It is the known-size kernel
  * that scales the HBM traffic counters of the step's kernels. */
 int fribo_test_copy(int64_t bytes, int32_t reps, double* ms_per_copy);
@@ -294,7 +294,8 @@ const char* fribo_profile_name(int32_t cls);
  *     out (NULL to skip): the codes fribo_fluid_tsch would return;
  *   fribo_batch_set_graph: replay the batch's launches as one HIP graph;
  *   fribo_batch_set_gemm_shape: tile shape of the transforms (0: 32x32,
- *     1: 64x64, 2: 96x128, 3: direct-to-LDS 96x128, -1: default), and
+ *     1: 64x64, 2: 96x128, 3: direct-to-LDS 96x128, 4: direct-to-LDS
+ *     96x64, -1: default), and
  *     fribo_batch_get_gemm_shape the one in use.
  * Destroy the batch before finalizing its instances. */
 typedef struct fribo_batch_ctx* fribo_batch;
@@ -307,8 +308,8 @@ int fribo_batch_set_gemm_shape(fribo_batch b, int32_t cfg);
 int fribo_batch_get_gemm_shape(fribo_batch b);
 /* Autotuner (no reference counterpart): times the batch's forward transform
  * with every tile shape all its instances can run (reps launches each, no
- * change to any state) and keeps the fastest; ms[4] gets the time per
- * launch of shapes 0..3 (0: not available), best the shape kept. */
+ * change to any state) and keeps the fastest; ms[5] gets the time per
+ * launch of shapes 0..4 (0: not available), best the shape kept. */
 int fribo_batch_autotune(fribo_batch b, int32_t reps, double* ms, int32_t* best);
 
 /* Learned first iterate of the strong coupling loop (no counterpart in the

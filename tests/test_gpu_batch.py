@@ -118,8 +118,24 @@ def test_batch_equals_single_instances():
 def test_batch_with_other_transform_tiles():
     cs = _members()
     single = _run_single(cs, 3)
-    for shape in (1, 2):
+    for shape in (1, 2, 3, 4):
         batch = _run_batch(cs, 3, shape=shape)
+        assert batch[3] == shape
+        worst = _compare(single, batch, 1e-14)
+        print(f"tile shape {shape}: largest relative difference {worst:.2e}", flush=True)
+
+
+def test_batch_fused_chunk_pass_on_both_direct_tiles():
+    """Two 1024x768 members (chunks of 16 z-rows): the forward transform's
+    epilogue runs the first pass of the per-mode z-lines on the 96x128 tiles
+    of 16 waves (the single-instance shape) and on the 96x64 tiles of 8
+    waves; both batches match the single-instance runs to the other-tiles
+    bar."""
+    cs = [cases.ensemble_case(i, 1024, 768, re, amp=0.1, freq=0.2) for i, re in enumerate((90.0, 180.0))]
+    single = _run_single(cs, 3)
+    for shape in (3, 4):
+        batch = _run_batch(cs, 3, shape=shape)
+        assert batch[3] == shape
         worst = _compare(single, batch, 1e-14)
         print(f"tile shape {shape}: largest relative difference {worst:.2e}", flush=True)
 
