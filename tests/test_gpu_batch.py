@@ -140,6 +140,35 @@ def test_batch_fused_chunk_pass_on_both_direct_tiles():
         print(f"tile shape {shape}: largest relative difference {worst:.2e}", flush=True)
 
 
+def test_batch_three_pass_k_lines():
+    """Members set to the three-pass k-line solve (fribo_set_ksolve 0) take
+    the batched three-pass kernels; the forms are bit-identical, so the batch
+    equals the single-instance (one-pass) runs bit for bit."""
+    from fribois2d_amd.gpu import FluidBatch
+    cs = _members()[:3]
+    single = _run_single(cs, 3)
+    gs = [_step(c) for c in cs]
+    for g in gs:
+        g.set_ksolve("3pass")
+    b = FluidBatch(gs)
+    t = np.zeros(len(cs))
+    dt = np.array([c.params.dt0 for c in cs])
+    hist = [[] for _ in cs]
+    for ns in range(3):
+        b.set_markers([c.markers(t[i]) for i, c in enumerate(cs)])
+        dtn, cf, err = b.step(ns, dt, dt)
+        assert not err.any()
+        for i in range(len(cs)):
+            hist[i].append((dtn[i], cf[i]))
+        t += dtn
+        dt = dtn
+    states = [g.get_state() for g in gs]
+    b.close()
+    for g in gs:
+        g.close()
+    _compare(single, (states, [np.array(h) for h in hist], None, None), 0.0)
+
+
 def test_batch_graph_replay_equals_eager():
     cs = _members()[:3]
     eager = _run_batch(cs, 6)
