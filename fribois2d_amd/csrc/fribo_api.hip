@@ -489,7 +489,8 @@ extern "C" int fribo_init(const fribo_params* p, const double* rc, const double*
                              hipMemcpyHostToDevice);
         if (he != hipSuccess) e = set_err(FRIBO_ERR_HIP, std::string("hipMemcpy2D (zmm, zmmt): ") + hipGetErrorString(he));
     }
-    if (!e) e = dalloc(c, &F.cfl_part, 2 * FRIBO_NPART);
+    F.ncfl = fribo_ncfl(F.n2m, F.n3m);
+    if (!e) e = dalloc(c, &F.cfl_part, 2 * (size_t)F.ncfl);
     F.npr_part = fribo_npr_part(F.n2m, F.n3m);
     if (!e) e = dalloc(c, &F.pr_part, F.npr_part);
     if (!e) e = setup_ksys(c, F.ks[0], F.n3m, 1, 0, 0);

@@ -12,7 +12,10 @@
 
 #define FRIBO_DIMSD 9
 #define FRIBO_KMAXCH 64      // max chunks of a partitioned k-line (one wave)
-#define FRIBO_NPART 512      // partial sums of the CFL reduction
+// CFL reduction: one partial (max, q3 sum) per workgroup of k_cfl, which
+// covers 256 columns and FRIBO_CFL_KT planes
+#define FRIBO_CFL_KT 8
+inline int fribo_ncfl(int n2m, int n3m) { return ((n2m + 255) / 256) * ((n3m + FRIBO_CFL_KT - 1) / FRIBO_CFL_KT); }
 #define FRIBO_LWORK 20       // per-marker record of the load computation
 #define FRIBO_STATIC_LOADS_MAX_NLM 8192   // markers fribo_loads_transfer_static serves
 
@@ -121,7 +124,8 @@ struct FluidDev {
     double *amphj, *acphj, *apphj, *amphk, *acphk, *apphk, *wr;
     int *jmv, *jpv, *kmv, *kpv;
     // reductions
-    double *cfl_part;        // [2*FRIBO_NPART]
+    double *cfl_part;        // [2*ncfl]
+    int ncfl;                // CFL partials (workgroups of k_cfl)
     double *pr_part;         // [npr_part]
     int npr_part;
     // Helmholtz systems: ks[0] q2 along k, ks[1] q3 along k,

@@ -61,47 +61,51 @@ __device__ inline double block_max(double* sh, double v)
 // j = t+1, t+257, ... and carries q3 of plane k+1 into the next plane, so q2
 // and q3 are each read from HBM once.  Blocks without planes write the
 // neutral partials (max seed 1e-8, sum 0).
-#define CFL_NJ 16  // columns per thread: n2m <= 256*CFL_NJ
+// cfl.F90:19-38: workgroup (bx, by) takes columns 256*bx+1.. and planes
+// FRIBO_CFL_KT*by+1..; every load of its planes is issued before the
+// arithmetic (q3 of the plane above comes from the next plane's load, one
+// extra plane at the top), and it leaves one partial (max, q3 sum).
 __device__ __forceinline__ void cfl_body(const FluidDev& F, const VBlock& vb)
 {
+    constexpr int KT = FRIBO_CFL_KT;
     __shared__ double sh[256];
     const int t = threadIdx.x;
-    const int per = (F.n3m + FRIBO_NPART - 1) / FRIBO_NPART;
-    const int k0 = vb.x * per + 1;
-    const int k1 = min(k0 + per - 1, F.n3m);
+    const int j = (int)vb.x * 256 + t + 1;
+    const int k0 = (int)vb.y * KT + 1;
+    const int k1 = min(k0 + KT - 1, F.n3m);
+    const bool act = j <= F.n2m;
+    const int jc = act ? j : F.n2m;
     const double* __restrict__ q2 = F.q2;
     const double* __restrict__ q3 = F.q3;
-    double m = 1.0e-8, s = 0.0;
-    double u2[CFL_NJ], dy[CFL_NJ], q3c[CFL_NJ];
+    const size_t P = F.P, c0 = fidx(F, jc, k0);
+    double a2[KT], b2[KT], a3[KT + 1];
 #pragma unroll
-    for (int u = 0; u < CFL_NJ; ++u) {
-        const int j = t + 1 + 256 * u;
-        const bool ok = j <= F.n2m && k0 <= k1;
-        u2[u] = ok ? F.udx2m[j] : 0.0;
-        dy[u] = ok ? F.rc[j + 1] - F.rc[j] : 0.0;
-        q3c[u] = ok ? q3[fidx(F, j, k0)] : 0.0;
+    for (int u = 0; u < KT; ++u) {
+        const size_t c = c0 + (size_t)(min(k0 + u, k1) - k0) * P;
+        a2[u] = q2[c];
+        b2[u] = q2[c + 1];
+        a3[u] = q3[c];
     }
-    for (int k = k0; k <= k1; ++k) {
-        const double u3 = F.udx3m[k], dz = F.zz[k + 1] - F.zz[k];
+    a3[KT] = q3[c0 + (size_t)(k1 + 1 - k0) * P];
+    const double u2 = F.udx2m[jc], dy = F.rc[jc + 1] - F.rc[jc];
+    double m = 1.0e-8, s = 0.0;
 #pragma unroll
-        for (int u = 0; u < CFL_NJ; ++u) {
-            const int j = t + 1 + 256 * u;
-            if (j <= F.n2m) {
-                const size_t c = fidx(F, j, k);
-                const double q2a = q2[c], q2b = q2[c + 1];
-                const double q3a = q3c[u], q3b = q3[c + F.P];
-                const double qcf = fabs((q2a + q2b) * 0.5 * u2[u]) + fabs((q3a + q3b) * 0.5 * u3);
-                m = fmax(m, qcf);
-                s = s + q3a * dz * dy[u];
-                q3c[u] = q3b;
-            }
+    for (int u = 0; u < KT; ++u) {
+        const int k = k0 + u;
+        if (act && k <= k1) {
+            const double u3 = F.udx3m[k], dz = F.zz[k + 1] - F.zz[k];
+            const double q3b = (k < k1) ? a3[u + 1] : a3[KT];
+            const double qcf = fabs((a2[u] + b2[u]) * 0.5 * u2) + fabs((a3[u] + q3b) * 0.5 * u3);
+            m = fmax(m, qcf);
+            s = s + a3[u] * dz * dy;
         }
     }
     const double bm = block_max<256>(sh, m);
     const double bs = block_sum<256>(sh, s);
     if (t == 0) {
-        F.cfl_part[2 * vb.x] = bm;
-        F.cfl_part[2 * vb.x + 1] = bs;
+        const int part = (int)vb.y * (int)vb.gx + (int)vb.x;
+        F.cfl_part[2 * part] = bm;
+        F.cfl_part[2 * part + 1] = bs;
     }
 }
 __global__ __launch_bounds__(256) void k_cfl(const FluidDev F)
@@ -115,14 +119,19 @@ __global__ __launch_bounds__(256) void kb_cfl(const BGrid<FluidDev> G)
     cfl_body(*Fp, vb);
 }
 
+static dim3 grid_cfl(const FluidDev& F)
+{
+    return dim3((F.n2m + 255) / 256, (F.n3m + FRIBO_CFL_KT - 1) / FRIBO_CFL_KT);
+}
+
 void launch_cfl(const FluidDev& F, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cfl, dim3(FRIBO_NPART), dim3(256), 0, st, F);
+    hipLaunchKernelGGL(k_cfl, grid_cfl(F), dim3(256), 0, st, F);
 }
 
 void launch_cfl_b(BatchSet& B, hipStream_t st)
 {
-    blaunch(B, BK_CFL, bdims(B, [](const FluidDev&) { return dim3(FRIBO_NPART); }), kb_cfl, dim3(256), 0, st);
+    blaunch(B, BK_CFL, bdims(B, grid_cfl), kb_cfl, dim3(256), 0, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -142,7 +151,7 @@ __device__ StepScalars step_scalars(const FluidDev& F, int n, int ns, double dt_
     double cmax = 0.0, q3s = 0.0;
     if (n == 1) {
         double m = 1.0e-8, s = 0.0;
-        for (int i = t; i < FRIBO_NPART; i += 1024) {
+        for (int i = t; i < F.ncfl; i += 1024) {
             m = fmax(m, F.cfl_part[2 * i]);
             s = s + F.cfl_part[2 * i + 1];
         }
