@@ -612,6 +612,30 @@ void launch_bc_b(BatchSet& B, int n, bool mean, hipStream_t st)
 #define SW_UPD 1
 static_assert(MK <= 64 && MKR <= 64, "one plane of a tile per lane");
 
+// Single-instance launches of the marching kernels: the hardware hands
+// workgroups to the 8 XCDs round-robin by linear id, so the launch grid is
+// padded to a multiple of 8 tile rows and workgroup id i takes tile
+// (i % 8) * (n / 8) + i / 8 in row-major order.  Each XCD then works on a
+// contiguous band of planes with all its columns: tiles adjacent in j (which
+// share the cache lines at their edges) and most tiles adjacent in k (which
+// share their halo planes) meet in one L2 instead of being fetched by two.
+// Returns false for the padding workgroups.
+__device__ inline bool vb_xcd_band(VBlock& vb, int gy_real)
+{
+    const int n = vb.gx * vb.gy;
+    const int id = vb.x + vb.gx * vb.y;
+    const int t = (id & 7) * (n >> 3) + (id >> 3);
+    vb.x = t % vb.gx;
+    vb.y = t / vb.gx;
+    vb.gy = gy_real;
+    return vb.y < gy_real;
+}
+
+static inline dim3 xcd_pad(dim3 g)
+{
+    return dim3(g.x, (g.y + 7) / 8 * 8);
+}
+
 __device__ inline double up1(double v) { return __shfl_up(v, 1, 64); }
 __device__ inline double dn1(double v) { return __shfl_down(v, 1, 64); }
 
@@ -781,7 +805,8 @@ __device__ __forceinline__ void rhs_body(const FluidDev& F, const VBlock& vb)
 }
 __global__ __launch_bounds__(64 * SW_RHS) void k_rhs(const FluidDev F)
 {
-    rhs_body(F, vb_hw());
+    VBlock vb = vb_hw();
+    if (vb_xcd_band(vb, (F.n3m + MKR - 1) / MKR)) rhs_body(F, vb);
 }
 __global__ __launch_bounds__(64 * SW_RHS) void kb_rhs(const BGrid<FluidDev> G)
 {
@@ -798,7 +823,7 @@ static dim3 grid_rhs(const FluidDev& F)
 
 void launch_rhs(const FluidDev& F, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_rhs, grid_rhs(F), dim3(64 * SW_RHS), 0, st, F);
+    hipLaunchKernelGGL(k_rhs, xcd_pad(grid_rhs(F)), dim3(64 * SW_RHS), 0, st, F);
 }
 
 void launch_rhs_b(BatchSet& B, hipStream_t st)
@@ -848,7 +873,8 @@ __device__ __forceinline__ void div_body(const FluidDev& F, const VBlock& vb)
 }
 __global__ __launch_bounds__(64 * SW_DIV) void k_div(const FluidDev F)
 {
-    div_body(F, vb_hw());
+    VBlock vb = vb_hw();
+    if (vb_xcd_band(vb, (F.n3m + MK - 1) / MK)) div_body(F, vb);
 }
 __global__ __launch_bounds__(64 * SW_DIV) void kb_div(const BGrid<FluidDev> G)
 {
@@ -865,7 +891,7 @@ static dim3 grid_div(const FluidDev& F)
 
 void launch_div(const FluidDev& F, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_div, grid_div(F), dim3(64 * SW_DIV), 0, st, F);
+    hipLaunchKernelGGL(k_div, xcd_pad(grid_div(F)), dim3(64 * SW_DIV), 0, st, F);
 }
 
 void launch_div_b(BatchSet& B, hipStream_t st)
@@ -961,7 +987,8 @@ __device__ __forceinline__ void upd_body(const FluidDev& F, const VBlock& vb)
 }
 __global__ __launch_bounds__(64 * SW_UPD) void k_upd(const FluidDev F)
 {
-    upd_body(F, vb_hw());
+    VBlock vb = vb_hw();
+    if (vb_xcd_band(vb, (F.n3m + MK - 1) / MK)) upd_body(F, vb);
 }
 __global__ __launch_bounds__(64 * SW_UPD) void kb_upd(const BGrid<FluidDev> G)
 {
@@ -978,7 +1005,7 @@ static dim3 grid_upd(const FluidDev& F)
 
 void launch_upd(const FluidDev& F, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_upd, grid_upd(F), dim3(64 * SW_UPD), 0, st, F);
+    hipLaunchKernelGGL(k_upd, xcd_pad(grid_upd(F)), dim3(64 * SW_UPD), 0, st, F);
 }
 
 void launch_upd_b(BatchSet& B, hipStream_t st)
