@@ -56,6 +56,35 @@ __device__ __forceinline__ const T* bg_block(const BGrid<T>& B, VBlock& v)
     return B.item + lo;
 }
 
+// bg_block for a launch of bg_total rounded up to a multiple of 8 blocks:
+// the hardware hands blocks to the 8 XCDs round-robin, so block b takes
+// virtual block (b % 8) * (blocks / 8) + b / 8 and each XCD works through a
+// contiguous range of the instances' tiles (neighbouring tiles, which share
+// the cache lines at their edges, meet in one L2).  nullptr for the padding
+// blocks.
+template <class T>
+__device__ __forceinline__ const T* bg_block_xcd(const BGrid<T>& B, VBlock& v)
+{
+    const int total = B.off[B.n];
+    const int nb = (int)gridDim.x;
+    const int b = ((int)blockIdx.x & 7) * (nb >> 3) + ((int)blockIdx.x >> 3);
+    if (b >= total) return nullptr;
+    int lo = 0, hi = B.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (B.off[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const int l = b - B.off[lo];
+    v.gx = B.dims[3 * lo];
+    v.gy = B.dims[3 * lo + 1];
+    v.gz = B.dims[3 * lo + 2];
+    v.x = l % v.gx;
+    v.y = (l / v.gx) % v.gy;
+    v.z = l / (v.gx * v.gy);
+    return B.item + lo;
+}
+
 // GEMM problem of one instance (the arguments of a single-instance launch).
 struct GemmProb {
     int M, N, K, lda, ldb, ldc;
@@ -122,6 +151,16 @@ inline void blaunch(BatchSet& B, int key, const std::vector<int>& dims, Kern ker
     const int tot = bg_total(dims);
     if (tot <= 0) return;
     hipLaunchKernelGGL(kern, dim3(tot), block, lds, st, B.grid(key, dims), args...);
+}
+
+// blaunch for kernels that look their block up with bg_block_xcd.
+template <class Kern, class... Args>
+inline void blaunch_xcd(BatchSet& B, int key, const std::vector<int>& dims, Kern kern, dim3 block, size_t lds,
+                        hipStream_t st, Args... args)
+{
+    const int tot = bg_total(dims);
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(kern, dim3((tot + 7) / 8 * 8), block, lds, st, B.grid(key, dims), args...);
 }
 
 // Stage keys of the launch tables.

@@ -811,8 +811,8 @@ __global__ __launch_bounds__(64 * SW_RHS) void k_rhs(const FluidDev F)
 __global__ __launch_bounds__(64 * SW_RHS) void kb_rhs(const BGrid<FluidDev> G)
 {
     VBlock vb;
-    const FluidDev* Fp = bg_block(G, vb);
-    rhs_body(*Fp, vb);
+    const FluidDev* Fp = bg_block_xcd(G, vb);
+    if (Fp) rhs_body(*Fp, vb);
 }
 
 static dim3 grid_rhs(const FluidDev& F)
@@ -828,7 +828,7 @@ void launch_rhs(const FluidDev& F, hipStream_t st)
 
 void launch_rhs_b(BatchSet& B, hipStream_t st)
 {
-    blaunch(B, BK_RHS, bdims(B, grid_rhs), kb_rhs, dim3(64 * SW_RHS), 0, st);
+    blaunch_xcd(B, BK_RHS, bdims(B, grid_rhs), kb_rhs, dim3(64 * SW_RHS), 0, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -879,8 +879,8 @@ __global__ __launch_bounds__(64 * SW_DIV) void k_div(const FluidDev F)
 __global__ __launch_bounds__(64 * SW_DIV) void kb_div(const BGrid<FluidDev> G)
 {
     VBlock vb;
-    const FluidDev* Fp = bg_block(G, vb);
-    div_body(*Fp, vb);
+    const FluidDev* Fp = bg_block_xcd(G, vb);
+    if (Fp) div_body(*Fp, vb);
 }
 
 static dim3 grid_div(const FluidDev& F)
@@ -896,7 +896,7 @@ void launch_div(const FluidDev& F, hipStream_t st)
 
 void launch_div_b(BatchSet& B, hipStream_t st)
 {
-    blaunch(B, BK_DIV, bdims(B, grid_div), kb_div, dim3(64 * SW_DIV), 0, st);
+    blaunch_xcd(B, BK_DIV, bdims(B, grid_div), kb_div, dim3(64 * SW_DIV), 0, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -993,8 +993,8 @@ __global__ __launch_bounds__(64 * SW_UPD) void k_upd(const FluidDev F)
 __global__ __launch_bounds__(64 * SW_UPD) void kb_upd(const BGrid<FluidDev> G)
 {
     VBlock vb;
-    const FluidDev* Fp = bg_block(G, vb);
-    upd_body(*Fp, vb);
+    const FluidDev* Fp = bg_block_xcd(G, vb);
+    if (Fp) upd_body(*Fp, vb);
 }
 
 static dim3 grid_upd(const FluidDev& F)
@@ -1010,7 +1010,7 @@ void launch_upd(const FluidDev& F, hipStream_t st)
 
 void launch_upd_b(BatchSet& B, hipStream_t st)
 {
-    blaunch(B, BK_UPD, bdims(B, grid_upd), kb_upd, dim3(64 * SW_UPD), 0, st);
+    blaunch_xcd(B, BK_UPD, bdims(B, grid_upd), kb_upd, dim3(64 * SW_UPD), 0, st);
 }
 
 // ---------------------------------------------------------------------------
