@@ -227,17 +227,17 @@ __device__ __forceinline__ void jsolve_hp_body(const FluidDev& F, const VBlock& 
     const double ce = own ? K.c[ie] : 0.0;
     int slot[NR];
     line_map_pad<NR>(slot, K.cs, n, nch, stride, lane, MAXR);
-    double pre[NR];
     const int step = nwgf * JW;
     const int iters = (nlines + step - 1) / step;
     int line = wg * JW + wave;
-    line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (line < nlines ? line : 0)), n, lane);
-    __syncthreads();
     double* row = stage + p * stride;
-    for (int it = 0; it < iters; ++it, line += step) {
+    // one line: stage it, refill its buffer with the line `ahead` lines of
+    // this wave later (one line of loads in flight: a second buffer takes
+    // the kernel past 256 VGPRs), solve, write back
+    auto solve = [&](double (&pre)[NR], int ahead) {
         const bool act = line < nlines;
         line_put<NR>(stage, pre, slot);
-        const int nxt = line + step;
+        const int nxt = line + ahead * step;
         line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (nxt < nlines ? nxt : 0)), n, lane);
         wave_lds_sync();
         double y[MAXR];
@@ -277,7 +277,12 @@ __device__ __forceinline__ void jsolve_hp_body(const FluidDev& F, const VBlock& 
         wave_lds_sync();
         if (act) line_get<NR>(dstb + fidx(F, K.kofs, kfirst + line), stage, slot, lane);
         wave_lds_sync();
-    }
+        line += step;
+    };
+    double pre[NR];
+    line_fetch<NR>(pre, srcb + fidx(F, K.kofs, kfirst + (line < nlines ? line : 0)), n, lane);
+    __syncthreads();
+    for (int it = 0; it < iters; ++it) solve(pre, 1);
 }
 template <int MAXR, int JW>
 __global__ __launch_bounds__(64 * JW) void k_jsolve_hp(const FluidDev F)
