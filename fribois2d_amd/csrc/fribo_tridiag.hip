@@ -30,6 +30,11 @@
 #include <algorithm>
 #include <utility>
 
+// j-lines per wave of the batched Helmholtz j-line launch
+#ifndef FRIBO_JB_LPW
+#define FRIBO_JB_LPW 16
+#endif
+
 // Row r of a j-line belongs to chunk owner(r) when chunk starts are
 // floor(p*n/nch); (r+1)*nch < 2^32 for every line this build accepts.
 __device__ inline int chunk_owner(int r, int n, int nch)
@@ -653,7 +658,10 @@ static void launch_h_b(BatchSet& B, hipStream_t st)
     const size_t lim = (size_t)B.hF[0].lds_max;
     auto lines = [](const FluidDev& F) { return 2 * F.n3m - 1; };
     if (jsolve_h_lds_bytes(MAXR, 8, sd, true) <= lim) {
-        blaunch(B, BK_JH, bdims(B, [&](const FluidDev& F) { return dim3(std::max(2, std::min(256, (lines(F) + 7) / 8) & ~1)); }),
+        // FRIBO_JB_LPW lines per wave: each workgroup first stages the
+        // factors of its instance in LDS, which many short lines amortise
+        constexpr int LPW = FRIBO_JB_LPW;
+        blaunch(B, BK_JH, bdims(B, [&](const FluidDev& F) { return dim3(std::max(2, std::min(256, (lines(F) + 8 * LPW - 1) / (8 * LPW)) & ~1)); }),
                 kb_jsolve_hp<MAXR, 8>, dim3(512), jsolve_h_lds_bytes(MAXR, 8, sd, true), st);
     } else if (jsolve_h_lds_bytes(MAXR, 4, sd, true) <= lim) {
         blaunch(B, BK_JH, bdims(B, [&](const FluidDev& F) { return dim3(std::max(2, std::min(256, (lines(F) + 3) / 4) & ~1)); }),
