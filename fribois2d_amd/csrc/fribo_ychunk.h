@@ -78,18 +78,28 @@ __device__ inline void ychunk_rows(const double* amphk, const double* acphk, con
     ws = glast * wl;
 }
 
-// The six separator summaries of chunk p of y-mode j: e is the separator
-// row, de its right-hand side.
+// The six separator summaries of a chunk: e is the separator row, de its
+// right-hand side.
+__device__ inline void ysum_vals(double (&v)[6], const double* amphk, const double* acphk, int e, double w, double de,
+                                 double yl, double vl, double wl, double ys, double vs, double ws)
+{
+    const int k = e + 1;
+    const double ae = amphk[k];
+    v[0] = ae * vl;                        // alpha
+    v[1] = (acphk[k] + w) + ae * wl;       // beta before the coupling to chunk p+1
+    v[2] = de - ae * yl;                   // delta before the coupling
+    v[3] = ys;
+    v[4] = vs;
+    v[5] = ws;
+}
+
+// The six separator summaries of chunk p of y-mode j, stored.
 __device__ inline void ysum_store(double* Y, int n2m, int j, int p, const double* amphk, const double* acphk, int e,
                                   double w, double de, double yl, double vl, double wl, double ys, double vs,
                                   double ws)
 {
-    const int k = e + 1;
-    const double ae = amphk[k];
-    Y[ysum_idx(n2m, 0, j, p)] = ae * vl;                        // alpha
-    Y[ysum_idx(n2m, 1, j, p)] = (acphk[k] + w) + ae * wl;       // beta before the coupling to chunk p+1
-    Y[ysum_idx(n2m, 2, j, p)] = de - ae * yl;                   // delta before the coupling
-    Y[ysum_idx(n2m, 3, j, p)] = ys;
-    Y[ysum_idx(n2m, 4, j, p)] = vs;
-    Y[ysum_idx(n2m, 5, j, p)] = ws;
+    double v[6];
+    ysum_vals(v, amphk, acphk, e, w, de, yl, vl, wl, ys, vs, ws);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Y[ysum_idx(n2m, q, j, p)] = v[q];
 }

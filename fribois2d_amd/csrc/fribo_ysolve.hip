@@ -52,34 +52,51 @@ __device__ inline void ychunk(const FluidDev& F, const KSys& K, int p, double w,
     ychunk_rows<MAXM, STORE>(F.amphk, F.acphk, F.apphk, s, m, w, dv, cp, vp, y, yl, vl, ys, vs, ws, wl);
 }
 
-// Lanes past the last y-mode work on mode n2m and store nothing (every lane
-// stays active through the row broadcasts).
+// Y1C chunks of the same 64 y-modes per workgroup, one wave per chunk
+// (lanes past the last y-mode work on mode n2m and store nothing: every lane
+// stays active through the row broadcasts).  The summaries are chunk-fastest
+// in memory, so they are transposed in LDS and stored as runs of Y1C chunks
+// per mode instead of one 8-byte store per lane.
+// (16 chunks per workgroup for chunks of up to 32 rows; 4 for longer ones,
+// whose registers allow fewer waves per SIMD)
+__host__ __device__ constexpr int y1c(int maxm) { return maxm <= 32 ? 16 : 4; }
 template <int MAXM>
 __device__ __forceinline__ void ysolve1_body(const FluidDev& F, const VBlock& vb)
 {
+    constexpr int Y1C = y1c(MAXM);
+    __shared__ double sy[6][Y1C][65];
     const DevScalars* S = F.S;
     if (S->err) return;
     const KSys& K = F.ks[0];
-    const int p = vb.y;
-    if (p >= K.nch) return;
-    const int j = (vb.x * YW + (threadIdx.x >> 6)) * 64 + (threadIdx.x & 63) + 1;
-    const bool act = j <= F.n2m;
-    const int jc = act ? j : F.n2m;
-    const double* col = F.hmn + (jc - 1);
-    const double w = F.wy[jc - 1];
-    double yl, vl, ys, vs, ws, wl;
-    ychunk<MAXM, false>(F, K, p, w, col, nullptr, nullptr, nullptr, yl, vl, ys, vs, ws, wl);
-    if (!act) return;
-    const int e = K.cs[p + 1] - 1;
-    ysum_store(F.ysum, F.n2m, j, p, F.amphk, F.acphk, e, w, col[(size_t)e * F.P], yl, vl, wl, ys, vs, ws);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p = vb.y * Y1C + wave;
+    const int j0 = vb.x * 64;
+    const int jc = min(j0 + lane + 1, F.n2m);
+    if (p < K.nch) {
+        const double* col = F.hmn + (jc - 1);
+        const double w = F.wy[jc - 1];
+        double yl, vl, ys, vs, ws, wl;
+        ychunk<MAXM, false>(F, K, p, w, col, nullptr, nullptr, nullptr, yl, vl, ys, vs, ws, wl);
+        const int e = K.cs[p + 1] - 1;
+        double v[6];
+        ysum_vals(v, F.amphk, F.acphk, e, w, col[(size_t)e * F.P], yl, vl, wl, ys, vs, ws);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sy[q][wave][lane] = v[q];
+    }
+    __syncthreads();
+    const int mode = threadIdx.x / Y1C, c = threadIdx.x % Y1C;
+    const int pj = vb.y * Y1C + c, jj = j0 + mode + 1;
+    if (pj < K.nch && jj <= F.n2m)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) F.ysum[yidx(F, q, jj, pj)] = sy[q][c][mode];
 }
 template <int MAXM>
-__global__ __launch_bounds__(64 * YW) void k_ysolve1(const FluidDev F)
+__global__ __launch_bounds__(64 * y1c(MAXM)) void k_ysolve1(const FluidDev F)
 {
     ysolve1_body<MAXM>(F, vb_hw());
 }
 template <int MAXM>
-__global__ __launch_bounds__(64 * YW) void kb_ysolve1(const BGrid<FluidDev> G)
+__global__ __launch_bounds__(64 * y1c(MAXM)) void kb_ysolve1(const BGrid<FluidDev> G)
 {
     VBlock vb;
     const FluidDev* Fp = bg_block(G, vb);
@@ -192,10 +209,12 @@ void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done)
     const int mm = F.ks[0].maxm - 1;   // interior rows per chunk
     dim3 g((F.n2m + 64 * YW - 1) / (64 * YW), F.ks[0].nch), b(64 * YW);
     if (!first_done) {
-        if (mm <= 16) hipLaunchKernelGGL(k_ysolve1<16>, g, b, 0, st, F);
-        else if (mm <= 32) hipLaunchKernelGGL(k_ysolve1<32>, g, b, 0, st, F);
-        else if (mm <= 64) hipLaunchKernelGGL(k_ysolve1<64>, g, b, 0, st, F);
-        else hipLaunchKernelGGL(k_ysolve1<128>, g, b, 0, st, F);
+        const int c = y1c(mm <= 32 ? 32 : 64);
+        const dim3 g1((F.n2m + 63) / 64, (F.ks[0].nch + c - 1) / c), b1(64 * c);
+        if (mm <= 16) hipLaunchKernelGGL(k_ysolve1<16>, g1, b1, 0, st, F);
+        else if (mm <= 32) hipLaunchKernelGGL(k_ysolve1<32>, g1, b1, 0, st, F);
+        else if (mm <= 64) hipLaunchKernelGGL(k_ysolve1<64>, g1, b1, 0, st, F);
+        else hipLaunchKernelGGL(k_ysolve1<128>, g1, b1, 0, st, F);
     }
     hipLaunchKernelGGL(k_ysolve2, dim3((F.n2m + YW - 1) / YW), b, 0, st, F);
     if (mm <= 16) hipLaunchKernelGGL(k_ysolve3<16>, g, b, 0, st, F);
@@ -211,10 +230,13 @@ void launch_ysolve_b(BatchSet& B, hipStream_t st, bool first_done)
     const dim3 b(64 * YW);
     const auto g = bdims(B, [](const FluidDev& F) { return dim3((F.n2m + 64 * YW - 1) / (64 * YW), F.ks[0].nch); });
     if (!first_done) {
-        if (mm <= 16) blaunch(B, BK_Y1, g, kb_ysolve1<16>, b, 0, st);
-        else if (mm <= 32) blaunch(B, BK_Y1, g, kb_ysolve1<32>, b, 0, st);
-        else if (mm <= 64) blaunch(B, BK_Y1, g, kb_ysolve1<64>, b, 0, st);
-        else blaunch(B, BK_Y1, g, kb_ysolve1<128>, b, 0, st);
+        const int c = y1c(mm <= 32 ? 32 : 64);
+        const auto g1 = bdims(B, [c](const FluidDev& F) { return dim3((F.n2m + 63) / 64, (F.ks[0].nch + c - 1) / c); });
+        const dim3 b1(64 * c);
+        if (mm <= 16) blaunch(B, BK_Y1, g1, kb_ysolve1<16>, b1, 0, st);
+        else if (mm <= 32) blaunch(B, BK_Y1, g1, kb_ysolve1<32>, b1, 0, st);
+        else if (mm <= 64) blaunch(B, BK_Y1, g1, kb_ysolve1<64>, b1, 0, st);
+        else blaunch(B, BK_Y1, g1, kb_ysolve1<128>, b1, 0, st);
     }
     blaunch(B, BK_Y2, bdims(B, [](const FluidDev& F) { return dim3((F.n2m + YW - 1) / YW); }), kb_ysolve2, b, 0, st);
     if (mm <= 16) blaunch(B, BK_Y3, g, kb_ysolve3<16>, b, 0, st);
