@@ -604,12 +604,18 @@ void launch_bc_b(BatchSet& B, int n, bool mean, hipStream_t st)
 // cache lines, which then come from the one L2 of the workgroup's XCD; the
 // measured gain holds for k_div only (k_upd runs faster with one wave per
 // workgroup, k_rhs is indifferent).
+#ifndef SW_RHS
 #define SW_RHS 1
+#endif
 #ifndef FRIBO_RHS_PF2
 #define FRIBO_RHS_PF2 0
 #endif
+#ifndef SW_DIV
 #define SW_DIV 8
+#endif
+#ifndef SW_UPD
 #define SW_UPD 1
+#endif
 static_assert(MK <= 64 && MKR <= 64, "one plane of a tile per lane");
 
 // Single-instance launches of the marching kernels: the hardware hands
