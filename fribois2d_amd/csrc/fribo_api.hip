@@ -1430,6 +1430,7 @@ extern "C" int fribo_set_transform(fribo_handle h, int32_t axis)
     if (axis == 1 && !h->yavail)
         return set_err(FRIBO_ERR_LIMIT, "no transform along y for this grid (needs n2-1 < n3-1 and Neumann null modes)");
     const int want = axis == -1 ? (h->yavail ? 1 : 0) : axis;
+    if (want != h->F.taxis && h->batch) return set_err(FRIBO_ERR_ARG, "instance belongs to a batch");
     if (want != h->F.taxis) {
         HIPCHK(hipStreamSynchronize(h->st));
         h->F.taxis = want;
@@ -1454,7 +1455,7 @@ extern "C" int fribo_get_transform(fribo_handle h)
 extern "C" int fribo_set_ksolve(fribo_handle h, int32_t form)
 {
     if (!h || form < 0 || form > 1) return set_err(FRIBO_ERR_ARG, "bad k-line solve form");
-    if (h->batch) return set_err(FRIBO_ERR_ARG, "instance is in a batch");
+    if (h->batch) return set_err(FRIBO_ERR_ARG, "instance belongs to a batch");
     if (form == 1 && !ksolve_fused_fits(h->F))
         return set_err(FRIBO_ERR_LIMIT, "k-lines too long for the one-pass solve (chunks of more than 32 rows)");
     if (form != h->F.kform) {

@@ -202,7 +202,8 @@ int fribo_test_gemm_nn(int32_t M, int32_t N, int32_t K, const double* A, int32_t
  * eigen-system of the y pressure operator built by fribo_init (one
  * tridiagonal system along z per y-mode), available when n2-1 < n3-1 and
  * both 1-D operators have their Neumann null mode; axis -1: y when
- * available, else z (the default).  Both orders solve the same discrete
+ * available, else z (the default).  FRIBO_ERR_ARG when it would change the
+ * axis of an instance that is in a batch.  Both orders solve the same discrete
  * system with the same gauge (the constant fixed by the z-order's pinned
  * null mode); the y order costs 2*(n3-1)*(n2-1)^2 flops per transform
  * instead of 2*(n2-1)*(n3-1)^2.  fribo_get_transform returns the axis in

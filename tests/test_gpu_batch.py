@@ -169,6 +169,23 @@ def test_batch_three_pass_k_lines():
     _compare(single, (states, [np.array(h) for h in hist], None, None), 0.0)
 
 
+def test_ksolve_form_is_fixed_while_in_a_batch():
+    """fribo_set_ksolve refuses an instance that is in a batch (the batch
+    holds a copy of its settings) and works again once the batch is gone."""
+    from fribois2d_amd.gpu import FluidBatch, FriboError
+    cs = _members()[:2]
+    gs = [_step(c) for c in cs]
+    b = FluidBatch(gs)
+    with pytest.raises(FriboError) as e:
+        gs[0].set_ksolve("3pass")
+    assert "ARG" in str(e.value)
+    b.close()
+    gs[0].set_ksolve("3pass")
+    assert gs[0].ksolve == "3pass"
+    for g in gs:
+        g.close()
+
+
 def test_batch_graph_replay_equals_eager():
     cs = _members()[:3]
     eager = _run_batch(cs, 6)
