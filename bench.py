@@ -145,6 +145,7 @@ def run_ensemble(args):
         gs.append(g)
     t_init = time.perf_counter() - t0
     cells = np.array([(c.params.n2 - 1) * (c.params.n3 - 1) for c in cs], dtype=np.float64)
+    mirror = [g.ymirror for g in gs]
     mk = EnsembleMarkers(cs)
     b = FluidBatch(gs)
     best, tune_ms = b.autotune(5)
@@ -187,8 +188,10 @@ def run_ensemble(args):
     for g in gs:
         g.close()
     # transform GEMM of the batch (forward, the autotuned shape): flops of
-    # every member's 2*n3m*n2m^2 (along y) per grouped launch
-    flops = float(sum(2.0 * (c.params.n3 - 1) * (c.params.n2 - 1) ** 2 for c in cs))
+    # every member's 2*n3m*n2m^2 (along y) per grouped launch, half that for
+    # the members in the mirror form (two n2m/2 blocks)
+    flops = float(sum((1.0 if m else 2.0) * (c.params.n3 - 1) * (c.params.n2 - 1) ** 2
+                      for c, m in zip(cs, mirror)))
     gemm_ms = float(tune_ms[best])
     achieved = flops / (gemm_ms * 1e-3) / 1e12
     shapes = {0: "32x32", 1: "64x64", 2: "96x128", 3: "direct-to-LDS 96x128", 4: "direct-to-LDS 96x64"}
@@ -213,7 +216,8 @@ def run_ensemble(args):
         "batched_over_unbatched": round(value / single, 2),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_MFMA_TFLOPS, 4), "traffic": None,
-                     "kernel": "kb_gemm (grouped forward transform)", "avg_ms": round(gemm_ms, 4)},
+                     "kernel": "kb_gemm (grouped forward transform)", "avg_ms": round(gemm_ms, 4),
+                     "members_in_mirror_form": int(sum(mirror))},
         "autotune": {"shape": shapes.get(best, best),
                      "ms_per_launch": {shapes[k]: round(float(v), 4) for k, v in enumerate(tune_ms) if v > 0}},
         "init_s": round(t_init, 1),
@@ -419,7 +423,10 @@ def main():
     # flops of one transform GEMM: along y (Yinv*dph, Y*V) 2*n3m*n2m^2, along
     # z (dph*zmmt^T, V*zmm^T, the reference's order) 2*n2m*n3m^2
     axis = g.transform
-    gemm_flops = 2.0 * n3m * n2m * n2m if axis == "y" else 2.0 * n2m * n3m * n3m
+    # the mirror form (y grids symmetric about the centre) multiplies by two
+    # n2m/2 x n2m/2 blocks: n3m*n2m^2
+    mirror = axis == "y" and g.ymirror and args.precision < 0
+    gemm_flops = (1.0 if mirror else 2.0) * n3m * n2m * n2m if axis == "y" else 2.0 * n2m * n3m * n3m
     achieved = gemm_flops / gemm_avg_s / 1e12
     # the transforms of this workload take the direct-to-LDS kernel (at least
     # 192 tiles of 96x128 in the library's size test, operands with padded
@@ -431,7 +438,7 @@ def main():
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": pmc_traffic(gemm_kernel),
             "kernel": gemm_kernel, "avg_ms": round(gemm_avg_s * 1e3, 4), "transform_axis": axis,
-            "flops_per_launch": gemm_flops}
+            "mirror_form": mirror, "flops_per_launch": gemm_flops}
     kernels = {k: {"ms_per_step": round(v[0] / nb, 4), "launches_per_step": v[1] / nb}
                for k, v in prof_all.items() if v[1]}
     # HBM fraction of the memory-bound classes: compulsory FP64 field values

@@ -84,7 +84,8 @@ struct fribo_ctx {
     hipGraphExec_t gexec = nullptr;
     int graph_nlm = -1;
     int mk_gen = 0;              // bumped whenever the marker buffers are reallocated
-    int ngpart64 = 0;            // gauge partials of the FP64 backward transform along y
+    int ngpart64 = 0;            // tile columns of the FP64 backward transform along y
+    int ysh_avail = 0;           // n2m / 2 when the mirror form of the y transform is set up
     struct fribo_batch_ctx* batch = nullptr;   // batch this instance belongs to (shares its stream)
     int dev = 0;                 // device of the handle
     double* scratch = nullptr;   // device address of the static loads scratch on that device
@@ -306,6 +307,14 @@ static int upload1i(fribo_ctx* c, int** dst, const std::vector<int>& h)
     return FRIBO_OK;
 }
 
+// Gauge partials the backward transform along y writes: one per tile column
+// (two in the mirror form), or those of the FP32 transform.
+static int ngpart_for(const fribo_ctx* c)
+{
+    if (c->F.refine >= 0) return gemm32_tiles_n(c->F.n3m);
+    return c->F.ysh > 0 ? 2 * c->ngpart64 : c->ngpart64;
+}
+
 // Eigen-system of the y pressure operator (fribo_eig.hip) and the gauge
 // weights zg = zmmt(null_mode, :); leaves taxis = 1 when it succeeds and the
 // y operator has a single null mode, otherwise keeps the transform along z.
@@ -339,10 +348,10 @@ static int setup_ytransform(fribo_ctx* c, const double* zmmt, int ldz)
     if (!e) e = dalloc(c, &F.wy, n);
     if (!e) e = dalloc(c, &F.zg, F.n3m);
     if (!e) e = dalloc(c, &F.ysum, (size_t)7 * n * FRIBO_KMAXCH);
-    F.ngpart = gemm_nn_tiles_n(n, F.n3m, n, F.ymm, F.ldy, F.hmn, F.P);
-    c->ngpart64 = F.ngpart;
-    // room for the gauge partials of any tile shape (a batch may use 32-column tiles)
-    if (!e) e = dalloc(c, &F.gpart, std::max(F.ngpart, (F.n3m + 31) / 32));
+    c->ngpart64 = gemm_nn_tiles_n(n, F.n3m, n, F.ymm, F.ldy, F.hmn, F.P);
+    // room for the gauge partials of any tile shape (a batch may use
+    // 32-column tiles) and of the mirror form (two per tile column)
+    if (!e) e = dalloc(c, &F.gpart, 2 * std::max(c->ngpart64, (F.n3m + 31) / 32));
     if (e) return e;
     HIPCHK(hipMemcpy(F.ymm, Y.data(), Y.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(F.ymmi, Yi.data(), Yi.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -352,6 +361,28 @@ static int setup_ytransform(fribo_ctx* c, const double* zmmt, int ldz)
     F.zginv = 1.0 / zs;
     F.taxis = 1;
     c->yavail = true;
+    // the mirror form, when the grid is symmetric about the channel centre
+    // (coefficients equal to their mirror image to 1e-10 of the largest)
+    if (ytransform_mirror(n, c->h1[A_AMPHJ].data() + 1, c->h1[A_ACPHJ].data() + 1, c->h1[A_APPHJ].data() + 1, F.ldy,
+                          Y.data(), Yi.data(), wy.data(), 1e-10) == 0) {
+        double smin = 1e300;
+        int is = 0;
+        for (int m = 0; m < n; ++m)
+            if (fabs(wy[m]) < smin) { smin = fabs(wy[m]); is = m; }
+        if (smin <= 1e-9 * wmax) {
+            if (!e) e = dalloc(c, &F.ysm, (size_t)F.ldy * n);
+            if (!e) e = dalloc(c, &F.ysmi, (size_t)F.ldy * n);
+            if (!e) e = dalloc(c, &F.wys, n);
+            if (e) return e;
+            HIPCHK(hipMemcpy(F.ysm, Y.data(), Y.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(F.ysmi, Yi.data(), Yi.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(F.wys, wy.data(), n * sizeof(double), hipMemcpyHostToDevice));
+            F.ynulls = is + 1;
+            c->ysh_avail = n / 2;
+            F.ysh = c->ysh_avail;
+        }
+    }
+    F.ngpart = ngpart_for(c);
     return FRIBO_OK;
 }
 
@@ -905,6 +936,31 @@ static void enqueue_poisson(fribo_ctx* c)
         }
         return;
     }
+    if (F.taxis == 1 && F.ysh > 0) {
+        // mirror form: pair butterfly, block-diagonal transforms with half
+        // the flops, the per-mode z-lines in the mirror form's mode order,
+        // pair butterfly of the result (the gauge partials come from rows 1
+        // and n2m of the backward transform's output)
+        double* d = F.dph + fidx(F, 1, 1);
+        FluidDev Fm = F;
+        Fm.wy = F.wys;
+        Fm.ynull = F.ynulls;
+        YFuse yf{F.wys, F.amphk, F.acphk, F.apphk, F.ysum, F.n2m, 0};
+        if (F.n3m % F.ks[0].nch == 0) yf.csz = F.n3m / F.ks[0].nch;
+        bool fused = false;
+        run(c, PC_JP, [&] { launch_ymirror(F, d, st); });
+        run_ev(c, PC_GEMM, [&](hipEvent_t a, hipEvent_t b) {
+            fused = launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ysmi, F.ldy, d, F.P, F.hmn, F.P, F.S, st, nullptr, nullptr,
+                                   a, b, &yf, F.ysh);
+        });
+        run(c, PC_JP, [&] { launch_ysolve(Fm, st, fused); });
+        run_ev(c, PC_GEMM, [&](hipEvent_t a, hipEvent_t b) {
+            launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ysm, F.ldy, F.hmn, F.P, d, F.P, F.S, st, F.zg, F.gpart, a, b,
+                           nullptr, F.ysh);
+        });
+        run(c, PC_JP, [&] { launch_ymirror(F, d, st); });
+        return;
+    }
     if (F.taxis == 1) {
         // the chunk pass of the per-mode z-lines runs in the forward
         // transform's epilogue when the line partition has equal chunks
@@ -1435,7 +1491,7 @@ extern "C" int fribo_set_transform(fribo_handle h, int32_t axis)
         HIPCHK(hipStreamSynchronize(h->st));
         h->F.taxis = want;
         if (want != 1) h->F.refine = -1;
-        if (want == 1 && h->F.refine < 0) h->F.ngpart = h->ngpart64;
+        if (want == 1) h->F.ngpart = ngpart_for(h);
         const double zero = 0.0;
         HIPCHK(hipMemcpy(&h->F.S->dphoff, &zero, sizeof(double), hipMemcpyHostToDevice));
         if (h->gexec) {
@@ -1445,6 +1501,35 @@ extern "C" int fribo_set_transform(fribo_handle h, int32_t axis)
         }
     }
     return FRIBO_OK;
+}
+
+// Mirror form of the transform along y (no counterpart in the reference):
+// on = 1 selects it (the default when the grid is symmetric about the
+// channel centre), 0 the plain dense form.  It applies to the FP64
+// transforms along y (the FP32 ones keep the dense form).
+extern "C" int fribo_set_ymirror(fribo_handle h, int32_t on)
+{
+    if (!h || on < 0 || on > 1) return set_err(FRIBO_ERR_ARG, "on must be 0 or 1");
+    if (on && !h->ysh_avail)
+        return set_err(FRIBO_ERR_LIMIT, "no mirror form: the y grid is not symmetric about the centre, or n2-1 is odd");
+    if (h->batch) return set_err(FRIBO_ERR_ARG, "instance belongs to a batch");
+    const int want = on ? h->ysh_avail : 0;
+    if (want != h->F.ysh) {
+        HIPCHK(hipStreamSynchronize(h->st));
+        h->F.ysh = want;
+        if (h->F.taxis == 1) h->F.ngpart = ngpart_for(h);
+        if (h->gexec) {
+            hipGraphExecDestroy(h->gexec);
+            h->gexec = nullptr;
+            h->graph_nlm = -1;
+        }
+    }
+    return FRIBO_OK;
+}
+
+extern "C" int fribo_get_ymirror(fribo_handle h)
+{
+    return h ? (h->F.ysh > 0 ? 1 : 0) : -1;
 }
 
 extern "C" int fribo_get_transform(fribo_handle h)
@@ -1616,7 +1701,7 @@ static bool same_prob(const GemmProb& a, const GemmProb& b)
     return a.M == b.M && a.N == b.N && a.K == b.K && a.lda == b.lda && a.ldb == b.ldb && a.ldc == b.ldc &&
            a.A == b.A && a.B == b.B && a.C == b.C && a.S == b.S && a.zg == b.zg && a.gpart == b.gpart &&
            a.yf.wy == b.yf.wy && a.yf.amphk == b.yf.amphk && a.yf.acphk == b.yf.acphk && a.yf.apphk == b.yf.apphk &&
-           a.yf.ysum == b.yf.ysum && a.yf.n2m == b.yf.n2m && a.yf.csz == b.yf.csz;
+           a.yf.ysum == b.yf.ysum && a.yf.n2m == b.yf.n2m && a.yf.csz == b.yf.csz && a.ks == b.ks;
 }
 
 static bool same_probs(const std::vector<GemmProb>& a, const std::vector<GemmProb>& b)
@@ -1696,7 +1781,13 @@ static int batch_refresh(fribo_batch_ctx* b)
     for (int i = 0; i < B.n; ++i) {
         FluidDev F = b->inst[i]->F;
         F.jstride = jsolve_stride_for(F, b->jclass);
-        if (F.taxis == 1) F.ngpart = gemm_cfg_parts(F, B.gemm_cfg);
+        if (F.taxis != 1) F.ysh = 0;
+        if (F.ysh > 0) {
+            // the mirror form's mode order for the per-mode z-lines
+            F.wy = F.wys;
+            F.ynull = F.ynulls;
+        }
+        if (F.taxis == 1) F.ngpart = gemm_cfg_parts(F, B.gemm_cfg) * (F.ysh > 0 ? 2 : 1);
         B.hF[i] = F;
         b->gen[i] = b->inst[i]->mk_gen;
     }
@@ -1892,10 +1983,12 @@ static void batch_enqueue(fribo_batch_ctx* b)
         launch_jsolve_helm_b(B, st);
         launch_ksolve_b(B, st);
         launch_div_b(B, st);
+        if (B.hF[0].taxis == 1) launch_ymirror_b(B, st);
         const bool fused = launch_poisson_gemm_b(B, true, st);
         if (B.hF[0].taxis == 1) launch_ysolve_b(B, st, fused);
         else launch_jsolve_poisson_b(B, st);
         launch_poisson_gemm_b(B, false, st);
+        if (B.hF[0].taxis == 1) launch_ymirror_b(B, st);
         launch_upd_b(B, st);
         if (n == nsst) launch_mean_b(B, st);
     }
@@ -2049,7 +2142,7 @@ extern "C" int fribo_set_precision(fribo_handle h, int32_t refine)
     if (h->batch) return set_err(FRIBO_ERR_ARG, "instance belongs to a batch");
     HIPCHK(hipStreamSynchronize(h->st));
     h->F.refine = refine < 0 ? -1 : refine;
-    if (h->F.taxis == 1) h->F.ngpart = refine < 0 ? h->ngpart64 : gemm32_tiles_n(h->F.n3m);
+    if (h->F.taxis == 1) h->F.ngpart = ngpart_for(h);
     if (h->gexec) {
         hipGraphExecDestroy(h->gexec);
         h->gexec = nullptr;

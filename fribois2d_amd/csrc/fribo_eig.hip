@@ -208,6 +208,78 @@ int ytransform_build(int n, const double* am, const double* ac, const double* ap
     return 0;
 }
 
+// Mirror form of the y eigen-system.  When the grid is mirror-symmetric
+// about the channel centre (am[j] = ap[n-1-j], ac[j] = ac[n-1-j] to within
+// tol * max|ac|) and n = 2h is even, the operator commutes with the
+// reflection J (j -> n-1-j), so every eigenvector is even ([u; Ju]) or odd
+// ([w; -Ju]).  Even modes are the eigenvectors u of the top half with the
+// centre coupling added to its last diagonal entry, odd modes those with it
+// subtracted: two h x h problems built from the top half's coefficients
+// (the bottom half's are their mirror image).  With the pair butterfly
+// (x_j, x_{n-1-j}) -> (x_j + x_{n-1-j}, x_j - x_{n-1-j}), j < h, applied to
+// the field before the forward transform and to the result of the backward
+// one, both transforms become block-diagonal with two h x h blocks, half
+// the flops of the dense form:
+//   forward  (modes i, positions kk): Ysi(i, kk) = u_i(kk) / (sqrt2 d_kk)
+//            for i, kk < h; Ysi(h + o, n-1-j) = w_o(j) / (sqrt2 d_j);
+//   backward (positions i, modes kk): Ys(i, kk) = d_i u_kk(i) / sqrt2 for
+//            i, kk < h; Ys(n-1-j, h + o) = d_j w_o(j) / sqrt2;
+// every other entry 0.  wys holds the even block's eigenvalues, then the
+// odd block's.  Returns -1 when the operator is not mirror-symmetric.
+int ytransform_mirror(int n, const double* am, const double* ac, const double* ap, int ld, double* Ys, double* Ysi,
+                      double* wys, double tol)
+{
+    if (n < 4 || (n & 1)) return -1;
+    const int h = n / 2;
+    double amax = 0.0;
+    for (int j = 0; j < n; ++j) amax = std::max(amax, std::fabs(ac[j]));
+    if (!(amax > 0.0)) return -1;
+    for (int j = 0; j < n; ++j)
+        if (!(std::fabs(am[j] - ap[n - 1 - j]) <= tol * amax) || !(std::fabs(ac[j] - ac[n - 1 - j]) <= tol * amax))
+            return -1;
+    std::vector<double> dg(h), e(h), de(h), doo(h), ee(h), eo(h);
+    dg[0] = 1.0;
+    for (int j = 0; j < h; ++j) {
+        const double up = ap[j], lo = am[j + 1];   // j = h-1: the coupling across the centre
+        if (!(up > 0.0) || !(lo > 0.0)) return -1;
+        e[j] = std::sqrt(up * lo);
+        if (j + 1 < h) dg[j + 1] = dg[j] * std::sqrt(lo / up);
+    }
+    for (int j = 0; j < h; ++j) {
+        de[j] = ac[j];
+        doo[j] = ac[j];
+        ee[j] = e[j];
+        eo[j] = e[j];
+    }
+    de[h - 1] = de[h - 1] + e[h - 1];
+    doo[h - 1] = doo[h - 1] - e[h - 1];
+    std::vector<double> qe((size_t)h * h, 0.0), qo((size_t)h * h, 0.0);
+    for (int j = 0; j < h; ++j) qe[(size_t)j * h + j] = qo[(size_t)j * h + j] = 1.0;
+    if (tridiag_ql(h, de.data(), ee.data(), qe.data()) || tridiag_ql(h, doo.data(), eo.data(), qo.data())) return -1;
+    const double r2 = std::sqrt(0.5);
+    std::fill(Ys, Ys + (size_t)ld * n, 0.0);
+    std::fill(Ysi, Ysi + (size_t)ld * n, 0.0);
+    for (int m = 0; m < h; ++m) {
+        wys[m] = de[m];
+        wys[h + m] = doo[m];
+        for (int j = 0; j < h; ++j) {
+            const double u = qe[(size_t)m * h + j], w = qo[(size_t)m * h + j];   // u_m(j), w_m(j)
+            Ysi[(size_t)j * ld + m] = u * r2 / dg[j];
+            Ysi[(size_t)(n - 1 - j) * ld + h + m] = w * r2 / dg[j];
+            Ys[(size_t)m * ld + j] = dg[j] * u * r2;
+            Ys[(size_t)(h + m) * ld + n - 1 - j] = dg[j] * w * r2;
+        }
+    }
+    return 0;
+}
+
+extern "C" int fribo_test_ymirror(int32_t n, const double* am, const double* ac, const double* ap, double tol,
+                                  double* Ys, double* Ysi, double* wys)
+{
+    if (n <= 0 || !am || !ac || !ap || !Ys || !Ysi || !wys) return 5;
+    return ytransform_mirror(n, am, ac, ap, n, Ys, Ysi, wys, tol) == 0 ? 0 : 7;
+}
+
 extern "C" int fribo_test_ytransform(int32_t n, const double* am, const double* ac, const double* ap, double* Y,
                                      double* Yinv, double* wy)
 {

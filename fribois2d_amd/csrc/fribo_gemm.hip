@@ -70,12 +70,13 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 #endif
 
 // Gauge partial of the transform along y (zg != nullptr): the tile holding
-// row M-1 of C adds zg(col) * C(M-1, col) over its columns, in a fixed order
-// (columns of a lane, lanes by butterfly, waves in index order), and writes
-// it to part[tile column].  The step then sums the partials in tile order.
+// row grow of C adds zg(col) * C(grow, col) over its columns, in a fixed
+// order (columns of a lane, lanes by butterfly, waves in index order), and
+// writes sgn times the sum to *part.  The step then sums the partials in
+// tile order.
 template <int RM, int RN, int TM, int TN, int NW>
-__device__ inline void gauge_partial(const f64x4 (&acc)[RM][RN], int M, int N, int i0, int n0, int wm, int wn,
-                                     int lane, int wave, int by, const double* __restrict__ zg, double* part,
+__device__ inline void gauge_partial(const f64x4 (&acc)[RM][RN], int grow, double sgn, int N, int i0, int n0, int wm,
+                                     int wn, int lane, int wave, const double* __restrict__ zg, double* part,
                                      double* sh)
 {
     double v = 0.0;
@@ -87,7 +88,7 @@ __device__ inline void gauge_partial(const f64x4 (&acc)[RM][RN], int M, int N, i
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = i0 + wm * TM + r * 16 + (lane >> 4) + 4 * q;
-                if (row == M - 1 && col < N) v = v + zg[col] * acc[r][c][q];
+                if (row == grow && col < N) v = v + zg[col] * acc[r][c][q];
             }
     }
 #pragma unroll
@@ -97,12 +98,46 @@ __device__ inline void gauge_partial(const f64x4 (&acc)[RM][RN], int M, int N, i
     if (threadIdx.x == 0) {
         double t = sh[0];
         for (int w = 1; w < NW; ++w) t = t + sh[w];
-        part[by] = t;
+        *part = sgn * t;
     }
 }
 
+// Gauge partials of a tile with rows [i0, i0 + BM), tile column by of gy.
+// Plain form: C(M-1, :) into part[by].  Mirror form (ks > 0, C before the
+// pair butterfly, whose row M-1 is C(0, :) - C(M-1, :)): C(0, :) into
+// part[by] and -C(M-1, :) into part[gy + by].
+template <int RM, int RN, int TM, int TN, int NW, int BM>
+__device__ inline void gauge_partials(const f64x4 (&acc)[RM][RN], int ks, int M, int N, int i0, int n0, int wm,
+                                      int wn, int lane, int wave, int by, int gy, const double* __restrict__ zg,
+                                      double* part, double* sh)
+{
+    const bool last = M - 1 >= i0 && M - 1 < i0 + BM;
+    if (ks > 0 && i0 == 0) {
+        gauge_partial<RM, RN, TM, TN, NW>(acc, 0, 1.0, N, i0, n0, wm, wn, lane, wave, zg, part + by, sh);
+        if (last) __syncthreads();
+    }
+    if (last)
+        gauge_partial<RM, RN, TM, TN, NW>(acc, M - 1, ks > 0 ? -1.0 : 1.0, N, i0, n0, wm, wn, lane, wave, zg,
+                                          part + (ks > 0 ? gy + by : by), sh);
+}
+
+// K range of an output tile with rows [i0, i0 + BM).  Plain form (ks = 0):
+// all of K.  Mirror form (ks > 0, A block-diagonal with blocks [0, ks) and
+// [ks, K) in both indices): rows below ks take k in [0, ks), rows from ks on
+// take [ks, K), a tile holding both takes all of K (the zero blocks add
+// exact zeros).  The start is rounded down to a multiple of BK: the k-rows
+// it adds meet zeros of A.
+__device__ inline void tile_krange(int ks, int K, int i0, int BM, int BK, int& kb, int& ke)
+{
+    kb = 0;
+    ke = K;
+    if (ks <= 0) return;
+    if (i0 >= ks) kb = ks / BK * BK;
+    if (i0 + BM <= ks) ke = ks;
+}
+
 template <int BM, int BN, int WM, int WN, bool NN>
-__device__ __forceinline__ void gemm_nt_body(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, const VBlock& vb)
+__device__ __forceinline__ void gemm_nt_body(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, int ks, const VBlock& vb)
 {
     constexpr int NT = 64 * WM * WN;
     constexpr int TM = BM / WM, TN = BN / WN;     // wave tile
@@ -119,6 +154,8 @@ __device__ __forceinline__ void gemm_nt_body(int M, int N, int K, const double* 
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave % WM, wn = wave / WM;
     const int i0 = vb.x * BM, n0 = vb.y * BN;
+    int kb, ke;
+    tile_krange(ks, K, i0, BM, GBK, kb, ke);
     double ra[NA], rb[NB];
 
     auto load = [&](int k0) {
@@ -126,14 +163,14 @@ __device__ __forceinline__ void gemm_nt_body(int M, int N, int K, const double* 
         for (int u = 0; u < NA; ++u) {
             int e = t + u * NT;
             int kk = e / BM, r = e % BM;
-            bool ok = (e < BM * GBK) && (k0 + kk < K) && (i0 + r < M);
+            bool ok = (e < BM * GBK) && (k0 + kk < ke) && (i0 + r < M);
             ra[u] = ok ? A[(size_t)(k0 + kk) * lda + i0 + r] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             int e = t + u * NT;
             int kk = NN ? e % GBK : e / BN, r = NN ? e / GBK : e % BN;
-            bool ok = (e < BN * GBK) && (k0 + kk < K) && (n0 + r < N);
+            bool ok = (e < BN * GBK) && (k0 + kk < ke) && (n0 + r < N);
             rb[u] = ok ? (NN ? B[(size_t)(n0 + r) * ldb + k0 + kk] : B[(size_t)(k0 + kk) * ldb + n0 + r]) : 0.0;
         }
     };
@@ -159,12 +196,12 @@ __device__ __forceinline__ void gemm_nt_body(int M, int N, int K, const double* 
 #pragma unroll
         for (int c = 0; c < RN; ++c) acc[r][c] = (f64x4){0.0, 0.0, 0.0, 0.0};
 
-    load(0);
+    load(kb);
     store(0);
     __syncthreads();
     int buf = 0;
-    for (int k0 = 0; k0 < K; k0 += GBK) {
-        const bool more = k0 + GBK < K;
+    for (int k0 = kb; k0 < ke; k0 += GBK) {
+        const bool more = k0 + GBK < ke;
         if (more) load(k0 + GBK);
 #pragma unroll
         for (int k4 = 0; k4 < GBK; k4 += 4) {
@@ -195,22 +232,23 @@ __device__ __forceinline__ void gemm_nt_body(int M, int N, int K, const double* 
                 if (row < M && col < N) C[(size_t)col * ldc + row] = acc[r][c][v];
             }
         }
-    if (zg && M - 1 >= i0 && M - 1 < i0 + BM) {
+    if (zg) {
         __shared__ double gsh[WM * WN];
-        gauge_partial<RM, RN, TM, TN, WM * WN>(acc, M, N, i0, n0, wm, wn, lane, wave, vb.y, zg, gpart, gsh);
+        gauge_partials<RM, RN, TM, TN, WM * WN, BM>(acc, ks, M, N, i0, n0, wm, wn, lane, wave, vb.y, vb.gy, zg,
+                                                    gpart, gsh);
     }
 }
 template <int BM, int BN, int WM, int WN, bool NN>
-__global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart)
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_nt(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, int ks)
 {
-    gemm_nt_body<BM, BN, WM, WN, NN>(M, N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, vb_hw());
+    gemm_nt_body<BM, BN, WM, WN, NN>(M, N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, ks, vb_hw());
 }
 template <int BM, int BN, int WM, int WN, bool NN>
 __global__ __launch_bounds__(64 * WM * WN) void kb_gemm_nt(const BGrid<GemmProb> G)
 {
     VBlock vb;
     const GemmProb& P = *bg_block(G, vb);
-    gemm_nt_body<BM, BN, WM, WN, NN>(P.M, P.N, P.K, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, P.S, P.zg, P.gpart, vb);
+    gemm_nt_body<BM, BN, WM, WN, NN>(P.M, P.N, P.K, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, P.S, P.zg, P.gpart, P.ks, vb);
 }
 
 // s_barrier without the memory fence of __syncthreads (whose release would
@@ -246,7 +284,7 @@ __device__ inline void dma16(const double* src, double* dst)
 }
 
 template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128, int BM = 96>
-__device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, YFuse yf, const VBlock& vb)
+__device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, YFuse yf, int ks, const VBlock& vb)
 {
     constexpr int NW = WM * WN;
     constexpr int TM = BM / WM, TN = BN / WN, RM = TM / 16, RN = TN / 16;
@@ -282,6 +320,8 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
         by = (patch / npm) * 8 + ll / 4;
     }
     const int i0 = bx * BM, n0 = by * BN;
+    int kb, ke;
+    tile_krange(ks, K, i0, BM, BK, kb, ke);
     const int mlast = M - 2 + (M & 1), klast = K - 2 + (K & 1), nlast = N - 2 + (N & 1);
 
     auto issue = [&](int k0, double* As, double* Bs) {
@@ -311,13 +351,13 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
 #pragma unroll
         for (int c = 0; c < RN; ++c) acc[r][c] = (f64x4){0.0, 0.0, 0.0, 0.0};
 
-    const int nk = (K + BK - 1) / BK;
+    const int nk = (ke - kb + BK - 1) / BK;
     // slice s is in stage (As, Bs); slice s+2 goes into (An, Bn), the stage of slice s-1
     auto step = [&](int s, double* As, double* Bs, double* An, double* Bn) {
         if (s + 1 < nk) __builtin_amdgcn_s_waitcnt(0x0F70 | NI);   // vmcnt(NI): only slice s+1 in flight
         else __builtin_amdgcn_s_waitcnt(0x0F70);                    // vmcnt(0)
         raw_barrier();
-        const int kv = K - s * BK;            // valid k-rows of this slice
+        const int kv = ke - kb - s * BK;      // valid k-rows of this slice
         if (kv < BK) {
             for (int e = t; e < (BK - kv) * LA; e += 64 * NW) As[kv * LA + e] = 0.0;
             if (NN) {
@@ -331,7 +371,7 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
 #if GEMM_PRIO == 2
         __builtin_amdgcn_s_setprio(1);
 #endif
-        if (s + 2 < nk) issue((s + 2) * BK, An, Bn);
+        if (s + 2 < nk) issue(kb + (s + 2) * BK, An, Bn);
 #if GEMM_PRIO == 2
         __builtin_amdgcn_s_setprio(0);
 #endif
@@ -357,8 +397,8 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
         __builtin_amdgcn_s_setprio(0);
 #endif
     };
-    issue(0, S0, S0 + SA);
-    if (nk > 1) issue(BK, S1, S1 + SA);
+    issue(kb, S0, S0 + SA);
+    if (nk > 1) issue(kb + BK, S1, S1 + SA);
     for (int s = 0; s < nk; s += 3) {
         step(s, S0, S0 + SA, S2, S2 + SA);
         if (s + 1 < nk) step(s + 1, S1, S1 + SA, S0, S0 + SA);
@@ -375,9 +415,9 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
                 if (row < M && col < N) C[(size_t)col * ldc + row] = acc[r][c][v];
             }
         }
-    if (zg && M - 1 >= i0 && M - 1 < i0 + BM) {
+    if (zg) {
         __shared__ double gsh[NW];
-        gauge_partial<RM, RN, TM, TN, NW>(acc, M, N, i0, n0, wm, wn, lane, wave, by, zg, gpart, gsh);
+        gauge_partials<RM, RN, TM, TN, NW, BM>(acc, ks, M, N, i0, n0, wm, wn, lane, wave, by, vb.gy, zg, gpart, gsh);
     }
     if (NN && yf.csz > 0) {
         // chunk elimination of the y-mode z-lines on this tile: rows i are
@@ -413,16 +453,16 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
     }
 }
 template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128, int BM = 96>
-__global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, YFuse yf)
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_dma(int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, const DevScalars* S, const double* __restrict__ zg, double* gpart, YFuse yf, int ks)
 {
-    gemm_dma_body<BK, NN, WM, WN, BN, BM>(M, N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, yf, vb_hw());
+    gemm_dma_body<BK, NN, WM, WN, BN, BM>(M, N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, yf, ks, vb_hw());
 }
 template <int BK, bool NN, int WM = 2, int WN = 8, int BN = 128, int BM = 96>
 __global__ __launch_bounds__(64 * WM * WN) void kb_gemm_dma(const BGrid<GemmProb> G)
 {
     VBlock vb;
     const GemmProb& P = *bg_block(G, vb);
-    gemm_dma_body<BK, NN, WM, WN, BN, BM>(P.M, P.N, P.K, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, P.S, P.zg, P.gpart, P.yf, vb);
+    gemm_dma_body<BK, NN, WM, WN, BN, BM>(P.M, P.N, P.K, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, P.S, P.zg, P.gpart, P.yf, P.ks, vb);
 }
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -451,7 +491,7 @@ template <bool NN>
 static int launch_gemm(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                        const DevScalars* S, hipStream_t st, const double* zg, double* gpart, bool plan,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const YFuse* yfp = nullptr,
-                       bool* fused = nullptr)
+                       bool* fused = nullptr, int ks = 0)
 {
     YFuse yf{};
     // the fused chunk pass needs whole chunks of at most 33 rows in every
@@ -467,23 +507,23 @@ static int launch_gemm(int M, int N, int K, const double* A, int lda, const doub
         if (!plan)
             launch_timed(k_gemm_dma<GEMM_DMA_BK, NN, GEMM_DMA_WM, GEMM_DMA_WN, GEMM_DMA_BN>,
                          dim3(cdiv(M, 96), cdiv(N, GEMM_DMA_BN)), dim3(64 * GEMM_DMA_WM * GEMM_DMA_WN), st, e0, e1, M,
-                         N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, yf);
+                         N, K, A, lda, B, ldb, C, ldc, S, zg, gpart, yf, ks);
         if (fused) *fused = yf.csz > 0;
         return cdiv(N, GEMM_DMA_BN);
     } else if ((long)cdiv(M, 96) * cdiv(N, 128) >= 192) {
         if (!plan)
             launch_timed(k_gemm_nt<96, 128, 2, 4, NN>, dim3(cdiv(M, 96), cdiv(N, 128)), dim3(512), st, e0, e1, M, N,
-                         K, A, lda, B, ldb, C, ldc, S, zg, gpart);
+                         K, A, lda, B, ldb, C, ldc, S, zg, gpart, ks);
         return cdiv(N, 128);
     } else if ((long)cdiv(M, 64) * cdiv(N, 64) >= 96) {
         if (!plan)
             launch_timed(k_gemm_nt<64, 64, 2, 2, NN>, dim3(cdiv(M, 64), cdiv(N, 64)), dim3(256), st, e0, e1, M, N,
-                         K, A, lda, B, ldb, C, ldc, S, zg, gpart);
+                         K, A, lda, B, ldb, C, ldc, S, zg, gpart, ks);
         return cdiv(N, 64);
     }
     if (!plan)
         launch_timed(k_gemm_nt<32, 32, 2, 2, NN>, dim3(cdiv(M, 32), cdiv(N, 32)), dim3(256), st, e0, e1, M, N, K,
-                     A, lda, B, ldb, C, ldc, S, zg, gpart);
+                     A, lda, B, ldb, C, ldc, S, zg, gpart, ks);
     return cdiv(N, 32);
 }
 
@@ -495,10 +535,10 @@ void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double*
 
 bool launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C, int ldc,
                     const DevScalars* S, hipStream_t st, const double* zg, double* gpart, hipEvent_t e0,
-                    hipEvent_t e1, const YFuse* yf)
+                    hipEvent_t e1, const YFuse* yf, int ks)
 {
     bool fused = false;
-    launch_gemm<true>(M, N, K, A, lda, B, ldb, C, ldc, S, st, zg, gpart, false, e0, e1, yf, &fused);
+    launch_gemm<true>(M, N, K, A, lda, B, ldb, C, ldc, S, st, zg, gpart, false, e0, e1, yf, &fused, ks);
     return fused;
 }
 
@@ -595,7 +635,8 @@ bool launch_poisson_gemm_b(BatchSet& B, bool fwd, hipStream_t st)
             q.lda = F.ldy;
             q.ldb = F.P;
             q.ldc = F.P;
-            q.A = fwd ? F.ymmi : F.ymm;
+            q.A = F.ysh > 0 ? (fwd ? F.ysmi : F.ysm) : (fwd ? F.ymmi : F.ymm);
+            q.ks = F.ysh;
             q.B = fwd ? d : F.hmn;
             q.C = fwd ? F.hmn : d;
             if (!fwd) {

@@ -113,6 +113,11 @@ struct FluidDev {
     double *ymm, *ymmi, *wy, *zg, *ysum;
     int ldy, ynull;
     double zginv;
+    // mirror form of the transform along y (grids symmetric about the channel
+    // centre): block-diagonal Ys (backward), Ysi (forward), the eigenvalues of
+    // its mode order and its null mode; ysh = n2m / 2 when in use, 0 otherwise
+    double *ysm, *ysmi, *wys;
+    int ysh, ynulls;
     double* gpart;           // gauge partials of the backward transform along y, one per tile column
     int ngpart;
     // outflow arrays (1-based, n2+2)
@@ -230,10 +235,12 @@ void launch_gemm_nt(int M, int N, int K, const double* A, int lda, const double*
 bool launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc, const DevScalars* S, hipStream_t st, const double* zg = nullptr,
                     double* gpart = nullptr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
-                    const YFuse* yf = nullptr);
+                    const YFuse* yf = nullptr, int ks = 0);
 int gemm_nn_tiles_n(int M, int N, int K, const double* A, int lda, const double* B, int ldb);
 // first_done: the chunk pass already ran in the forward transform's epilogue
 void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done = false);
+// pair butterfly of the mirror form on x (n2m x n3m, leading dimension P)
+void launch_ymirror(const FluidDev& F, double* x, hipStream_t st);
 void launch_loads(const FluidDev& F, double hpr, hipStream_t st);
 // mixed-precision Poisson solve: r = rhs - (A_y + A_z) x (r may alias rhs),
 // and the FP32 transform C = X0 + A * B (X0 optional, may alias C)
@@ -248,6 +255,10 @@ void launch_diag(const FluidDev& F, double* part, int kv, double* out, hipStream
 // of the host QL
 int ytransform_build(int n, const double* am, const double* ac, const double* ap, int ld, double* Y, double* Yinv,
                      double* wy, bool device = false);
+// mirror form (block-diagonal halves) when the operator is mirror-symmetric
+// to within tol * max|ac|; -1 otherwise
+int ytransform_mirror(int n, const double* am, const double* ac, const double* ap, int ld, double* Ys, double* Ysi,
+                      double* wys, double tol);
 int phini_sym(int n3, const double* zz, double alx3, int ld, double* zmm, double* zmmt, double* wr,
               bool device = false);
 int jsolve_stride(const FluidDev& F);

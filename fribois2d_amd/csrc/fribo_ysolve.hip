@@ -223,6 +223,63 @@ void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done)
     else hipLaunchKernelGGL(k_ysolve3<128>, g, b, 0, st, F);
 }
 
+// Pair butterfly of the mirror form, in place on every column of x
+// (n x ncol, leading dimension P): (x_j, x_{n-1-j}) -> (x_j + x_{n-1-j},
+// x_j - x_{n-1-j}) for j < h = n / 2.  Lanes take consecutive pairs of j
+// (for even h one 16-byte load from the top half and one from the mirrored
+// rows of the bottom half, whose two values swap; for odd h one value each),
+// blocks of 64 x 4: vb.x the slot range, vb.y a group of 4 columns.
+__device__ inline void ymirror_body(int n, int ncol, int P, double* __restrict__ x, const VBlock& vb)
+{
+    const int h = n / 2, u = vb.x * 64 + threadIdx.x, k = vb.y * 4 + threadIdx.y;
+    if (k >= ncol) return;
+    double* col = x + (size_t)k * P;
+    if (h % 2 == 0) {
+        if (2 * u >= h) return;
+        double2* top = reinterpret_cast<double2*>(col + 2 * u);
+        double2* bot = reinterpret_cast<double2*>(col + n - 2 - 2 * u);
+        const double2 a = *top, b = *bot;   // b.y mirrors a.x, b.x mirrors a.y
+        *top = make_double2(a.x + b.y, a.y + b.x);
+        *bot = make_double2(a.y - b.x, a.x - b.y);
+    } else {
+        if (u >= h) return;
+        const double a = col[u], b = col[n - 1 - u];
+        col[u] = a + b;
+        col[n - 1 - u] = a - b;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ymirror(int n, int ncol, int P, double* __restrict__ x)
+{
+    ymirror_body(n, ncol, P, x, vb_hw());
+}
+
+__global__ __launch_bounds__(256) void kb_ymirror(const BGrid<FluidDev> G)
+{
+    VBlock vb;
+    const FluidDev* Fp = bg_block(G, vb);
+    ymirror_body(Fp->n2m, Fp->n3m, (int)Fp->P, Fp->dph + fidx(*Fp, 1, 1), vb);
+}
+
+static dim3 ymirror_grid(const FluidDev& F)
+{
+    const int h = F.n2m / 2, slots = h % 2 == 0 ? h / 2 : h;
+    return dim3((slots + 63) / 64, (F.n3m + 3) / 4);
+}
+
+void launch_ymirror(const FluidDev& F, double* x, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ymirror, ymirror_grid(F), dim3(64, 4), 0, st, F.n2m, F.n3m, (int)F.P, x);
+}
+
+// The butterfly on dph of every instance in the mirror form (no blocks for
+// the others).
+void launch_ymirror_b(BatchSet& B, hipStream_t st)
+{
+    const auto g = bdims(B, [](const FluidDev& F) { return F.ysh > 0 ? ymirror_grid(F) : dim3(0, 0, 0); });
+    blaunch(B, BK_YM, g, kb_ymirror, dim3(64, 4), 0, st);
+}
+
 void launch_ysolve_b(BatchSet& B, hipStream_t st, bool first_done)
 {
     int mm = 0;

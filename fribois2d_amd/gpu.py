@@ -27,7 +27,8 @@ EXPORTS = ["fribo_init", "fribo_finalize", "fribo_set_state", "fribo_get_state",
            "fribo_get_array", "fribo_sync", "fribo_last_error", "fribo_test_gemm", "fribo_test_gemm_nn", "fribo_set_profiling", "fribo_get_profile",
            "fribo_reset_profile", "fribo_profile_name", "fribo_set_graph", "fribo_test_copy",
            "fribo_set_transform", "fribo_get_transform", "fribo_set_ksolve", "fribo_get_ksolve",
-           "fribo_test_ytransform", "fribo_loads_transfer",
+           "fribo_set_ymirror", "fribo_get_ymirror", "fribo_test_ytransform", "fribo_test_ymirror",
+           "fribo_loads_transfer",
            "fribo_checkbcs", "fribo_diagnostics", "fribo_save_state", "fribo_restore_state",
            "fribo_phini", "fribo_phini_device", "fribo_loads_transfer_static",
            "fribo_batch_create", "fribo_batch_destroy", "fribo_batch_set_markers", "fribo_batch_step",
@@ -103,6 +104,9 @@ def load():
     L.fribo_set_ksolve.argtypes = [vp, C.c_int32]
     L.fribo_get_ksolve.argtypes = [vp]
     L.fribo_test_ytransform.argtypes = [C.c_int32, dp, dp, dp, dp, dp, dp]
+    L.fribo_set_ymirror.argtypes = [vp, C.c_int32]
+    L.fribo_get_ymirror.argtypes = [vp]
+    L.fribo_test_ymirror.argtypes = [C.c_int32, dp, dp, dp, C.c_double, dp, dp, dp]
     L.fribo_loads_transfer.argtypes = [vp, C.c_double, dp, dp, dp, dp]
     L.fribo_loads_transfer_static.argtypes = [vp, C.c_double, dp, dp, dp, dp]
     L.fribo_checkbcs.argtypes = [vp, dp, dp, dp, dp]
@@ -310,6 +314,16 @@ class FluidStep:
     def ksolve(self):
         return "fused" if load().fribo_get_ksolve(self.h) == 1 else "3pass"
 
+    def set_ymirror(self, on):
+        """Mirror form of the transform along y (grids symmetric about the
+        channel centre): True halves the transforms' flops, False takes the
+        dense form."""
+        _check(load().fribo_set_ymirror(self.h, 1 if on else 0))
+
+    @property
+    def ymirror(self):
+        return load().fribo_get_ymirror(self.h) == 1
+
     def set_precision(self, refine):
         """Poisson transforms: refine < 0 FP64; n >= 0 FP32 MFMA plus n
         FP64-residual refinement sweeps."""
@@ -406,6 +420,23 @@ def ytransform(am, ac, ap):
     arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (am, ac, ap)]
     _check(load().fribo_test_ytransform(n, *[_dp(x) for x in arr], _dp(Y), _dp(Yi), _dp(w)))
     return Y, Yi, w
+
+
+def ymirror(am, ac, ap, tol=1e-10):
+    """Mirror form of the decomposition of an operator symmetric about its
+    centre: (Ys, Ysi, wys), block-diagonal backward and forward transforms
+    on the pair sums and differences and the eigenvalues in their mode
+    order; None when the operator is not symmetric to within tol."""
+    n = len(ac)
+    Ys = np.zeros((n, n), order="F")
+    Ysi = np.zeros((n, n), order="F")
+    w = np.zeros(n)
+    arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (am, ac, ap)]
+    rc = load().fribo_test_ymirror(n, *[_dp(x) for x in arr], float(tol), _dp(Ys), _dp(Ysi), _dp(w))
+    if rc == 7:
+        return None
+    _check(rc)
+    return Ys, Ysi, w
 
 
 def phini(zz, alx3, device=False):

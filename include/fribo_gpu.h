@@ -211,6 +211,24 @@ int fribo_test_gemm_nn(int32_t M, int32_t N, int32_t K, const double* A, int32_t
 int fribo_set_transform(fribo_handle h, int32_t axis);
 int fribo_get_transform(fribo_handle h);
 
+/* Execution option (no reference counterpart): the mirror form of the
+ * transform along y.  When the y grid is symmetric about the channel
+ * centre (the y pressure operator equals its mirror image to 1e-10 of its
+ * largest coefficient) and n2-1 is even, every eigenvector of the operator
+ * is even or odd about the centre; with the pairs (y_j, y_{n-1-j}) replaced
+ * by their sum and difference before the forward transform and after the
+ * backward one, both transforms are block-diagonal with two halves, half
+ * the flops of the dense form.  The two halves are the eigen-systems of the
+ * top half's operator with the centre coupling added or subtracted.
+ * on = 1 selects it (the default where it is available), 0 the dense form;
+ * the results agree to rounding.  It applies to the FP64 transforms along y
+ * (single-instance and batched steps; the FP32 transforms of
+ * fribo_set_precision keep the dense form).  FRIBO_ERR_LIMIT for on = 1
+ * without a symmetric grid, FRIBO_ERR_ARG for an instance that is in a
+ * batch.  fribo_get_ymirror returns 1 when the mirror form is selected. */
+int fribo_set_ymirror(fribo_handle h, int32_t on);
+int fribo_get_ymirror(fribo_handle h);
+
 /* Execution option (no reference counterpart): the form of the k-direction
  * line solves (solq2k / solq3k, solve_fluid.F90:107-255).  form 1: one
  * pass per field that reads the right-hand side once (chunks of at most 32
@@ -258,6 +276,15 @@ int fribo_phini_device(int32_t n3, const double* zz, double alx3, int32_t ld, do
  * columns), Yinv = Y^-1 (both n x n, column-major) and the eigenvalues wy. */
 int fribo_test_ytransform(int32_t n, const double* am, const double* ac, const double* ap, double* Y, double* Yinv,
                           double* wy);
+
+/* Diagnostic entry point (no reference counterpart): the mirror form of the
+ * same decomposition for an operator symmetric about its centre (n even):
+ * Ys (the backward transform on the pair sums and differences), Ysi (the
+ * forward one), both block-diagonal with two n/2 x n/2 blocks, and the
+ * eigenvalues wys in their mode order.  Returns 7 when the operator is not
+ * symmetric to within tol times its largest diagonal entry. */
+int fribo_test_ymirror(int32_t n, const double* am, const double* ac, const double* ap, double tol, double* Ys,
+                       double* Ysi, double* wys);
 
 /* Diagnostic entry point (no reference counterpart): `reps` streaming copies
  * of `bytes` (a multiple of 8) between two device buffers after one warm-up

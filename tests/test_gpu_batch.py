@@ -179,6 +179,10 @@ def test_ksolve_form_is_fixed_while_in_a_batch():
     with pytest.raises(FriboError) as e:
         gs[0].set_ksolve("3pass")
     assert "ARG" in str(e.value)
+    assert gs[1].ymirror   # 128 x 96: symmetric y grid
+    with pytest.raises(FriboError) as e:
+        gs[1].set_ymirror(False)
+    assert "ARG" in str(e.value)
     b.close()
     gs[0].set_ksolve("3pass")
     assert gs[0].ksolve == "3pass"

@@ -454,6 +454,59 @@ def test_transform_axes_agree_large():
     assert e <= pbar
 
 
+@pytest.mark.parametrize("name,nsteps", [("tiny", 4), ("cyl64", 5), ("cyl384", 5), ("rect2048", 2)])
+def test_ymirror_matches_dense(name, nsteps):
+    """The mirror form of the transform along y (the default on these grids,
+    which are symmetric about the channel centre: pair butterfly, then
+    block-diagonal transforms of half the flops) steps like the dense form.
+    Both solve the same discrete system; the mirror form's eigenvectors come
+    from the two half problems, so the transforms round differently:
+    velocities 1e-11 (1e-10 on rect2048), pressure and dph the first-order
+    bound kappa * eps of the Poisson solve.  tiny has an odd half (9 pairs
+    per column: the one-value butterfly), the others even halves."""
+    c = case(name)
+    dec = decomposition(c)
+    gm = gpu_for(c, dec, axis="y")
+    gd = gpu_for(c, dec, axis="y")
+    assert gm.ymirror
+    gd.set_ymirror(False)
+    assert not gd.ymirror
+    run_steps(gm, c, nsteps)
+    run_steps(gd, c, nsteps)
+    sm, sd = gm.get_state(), gd.get_state()
+    pbar = max(1e-10, poisson_kappa(c) * np.finfo(np.float64).eps)
+    vbar = 1e-10 if name == "rect2048" else 1e-11
+    for f in ("q2", "q3", "pr", "ru2", "ru3"):
+        e = relmax(sm[f], sd[f])
+        print(f"{name} mirror vs dense after {nsteps} steps, {f}: {e:.2e}", flush=True)
+        assert e <= (pbar if f == "pr" else vbar), f
+    e = relmax(gm.get("dph"), gd.get("dph"))
+    print(f"{name} mirror vs dense dph: {e:.2e}", flush=True)
+    assert e <= pbar
+
+
+def test_ymirror_needs_a_symmetric_grid():
+    """An ensemble grid whose y stretching is not symmetric about the centre
+    (256 x 192 cells: 43 / 106 / 44 nodes) has no mirror form: the dense form
+    runs and asking for the mirror form is a LIMIT error.  A symmetric one
+    (256 x 96) selects it by default and can be switched off and on."""
+    from fribois2d_amd import gpu
+    c = cases.ensemble_case(0, 256, 192, 120.0)
+    assert not np.allclose(np.diff(c.rc), np.diff(c.rc)[::-1])
+    g = gpu_for(c)
+    assert g.transform == "y" and not g.ymirror
+    with pytest.raises(gpu.FriboError) as e:
+        g.set_ymirror(True)
+    assert "LIMIT" in str(e.value)
+    c2 = cases.ensemble_case(1, 256, 96, 120.0)
+    g2 = gpu_for(c2)
+    assert g2.transform == "y" and g2.ymirror
+    g2.set_ymirror(False)
+    assert not g2.ymirror
+    g2.set_ymirror(True)
+    assert g2.ymirror
+
+
 @pytest.mark.parametrize("axis", ["z", "y"])
 def test_library_phini_gives_the_same_step(axis):
     """fribo_init without phini inputs (the library's own symmetrised QL

@@ -59,3 +59,81 @@ def test_library_phini_matches_the_operator():
     assert np.abs(zmmt @ zmm - np.eye(ac.size)).max() <= 1e-13
     _, _, wref = cases.phini(c.zz, c.params.alx3, method="dgeev")
     np.testing.assert_allclose(np.sort(wr), np.sort(wref), rtol=0, atol=1e-10 * scale)
+
+
+def _butterfly(n):
+    """The pair butterfly (x_j, x_{n-1-j}) -> (x_j + x_{n-1-j}, x_j - x_{n-1-j})."""
+    h = n // 2
+    B = np.zeros((n, n))
+    for j in range(h):
+        B[j, j] = B[j, n - 1 - j] = 1.0
+        B[n - 1 - j, j] = 1.0
+        B[n - 1 - j, n - 1 - j] = -1.0
+    return B
+
+
+def test_mirror_form_uniform_grid_closed_form():
+    """Uniform spacing, n = 40: the even block holds the eigenvalues
+    -4 dx2q sin^2(pi m / (2n)) of even m (cosines symmetric about the
+    centre), the odd block those of odd m, and the block transforms with the
+    butterfly invert each other and diagonalise the operator."""
+    from fribois2d_amd import gpu
+    n, L = 40, 3.0
+    rc = np.linspace(0.0, L, n + 1)
+    am, ac, ap = cases.tridiag_z(rc, L)
+    Ys, Ysi, w = gpu.ymirror(am, ac, ap)
+    h = n // 2
+    dx2q = (n / L) ** 2
+    lam = -4.0 * dx2q * np.sin(np.pi * np.arange(n) / (2 * n)) ** 2
+    tol = 1e-11 * np.abs(lam).max()
+    np.testing.assert_allclose(np.sort(w[:h]), np.sort(lam[0::2]), rtol=0, atol=tol)
+    np.testing.assert_allclose(np.sort(w[h:]), np.sort(lam[1::2]), rtol=0, atol=tol)
+    B = _butterfly(n)
+    Tf, Tb = Ysi @ B, B @ Ys
+    A = np.diag(ac) + np.diag(ap[:-1], 1) + np.diag(am[1:], -1)
+    assert np.abs(Tf @ Tb - np.eye(n)).max() <= 1e-13
+    assert np.abs(Tb @ np.diag(w) @ Tf - A).max() <= 1e-12 * np.abs(A).max()
+
+
+@pytest.mark.parametrize("name", ["cyl64", "cyl384"])
+def test_mirror_form_stretched_grid(name):
+    """Symmetric stretched grids: block-diagonal halves (the off-diagonal
+    blocks exactly zero), the same spectrum as the dense decomposition, and
+    with the butterfly an inverse pair that diagonalises the operator."""
+    from fribois2d_amd import gpu
+    c = cases.make_case(name)
+    am, ac, ap = _ops(c)
+    n = ac.size
+    h = n // 2
+    Ys, Ysi, w = gpu.ymirror(am, ac, ap)
+    for M in (Ys, Ysi):
+        assert not M[:h, h:].any() and not M[h:, :h].any()
+    _, _, wd = gpu.ytransform(am, ac, ap)
+    scale = np.abs(ac).max()
+    np.testing.assert_allclose(np.sort(w), np.sort(wd), rtol=0, atol=1e-11 * scale)
+    B = _butterfly(n)
+    Tf, Tb = Ysi @ B, B @ Ys
+    A = np.diag(ac) + np.diag(ap[:-1], 1) + np.diag(am[1:], -1)
+    assert np.abs(Tf @ Tb - np.eye(n)).max() <= 1e-12
+    assert np.abs(A @ Tb - Tb * w).max() <= 1e-12 * scale
+    # exactly one null mode, in the even block
+    k = int(np.argmin(np.abs(w)))
+    assert k < h and abs(w[k]) <= 1e-9 * scale and np.sort(np.abs(w))[1] > 1e-9 * scale
+
+
+def test_mirror_form_refuses_asymmetric_operators():
+    """A grid that is not symmetric about the centre, or an odd size, has no
+    mirror form."""
+    from fribois2d_amd import gpu
+    c = cases.make_case("cyl64")
+    am, ac, ap = _ops(c)
+    ac2 = ac.copy()
+    ac2[3] *= 1.0 + 1e-8
+    assert gpu.ymirror(am, ac2, ap) is None
+    assert gpu.ymirror(am, ac2, ap, tol=1e-6) is not None
+    rc = np.linspace(0.0, 3.0, 40) ** 1.2
+    am3, ac3, ap3 = cases.tridiag_z(rc, float(rc[-1]))
+    assert gpu.ymirror(am3, ac3, ap3) is None
+    rc = np.linspace(0.0, 3.0, 42)   # 41 cells
+    am4, ac4, ap4 = cases.tridiag_z(rc, 3.0)
+    assert gpu.ymirror(am4, ac4, ap4) is None
