@@ -86,6 +86,7 @@ struct fribo_ctx {
     int mk_gen = 0;              // bumped whenever the marker buffers are reallocated
     int ngpart64 = 0;            // tile columns of the FP64 backward transform along y
     int ysh_avail = 0;           // n2m / 2 when the mirror form of the y transform is set up
+    bool ymirror_on = true;      // the mirror form selected (used while the transform is along y in FP64)
     struct fribo_batch_ctx* batch = nullptr;   // batch this instance belongs to (shares its stream)
     int dev = 0;                 // device of the handle
     double* scratch = nullptr;   // device address of the static loads scratch on that device
@@ -315,6 +316,15 @@ static int ngpart_for(const fribo_ctx* c)
     return c->F.ysh > 0 ? 2 * c->ngpart64 : c->ngpart64;
 }
 
+// F.ysh from the options: n2m / 2 while the mirror form runs (selected,
+// available, transform along y in FP64), else 0; the kernels read only F.ysh.
+static void apply_ysh(fribo_ctx* c)
+{
+    FluidDev& F = c->F;
+    F.ysh = (c->ymirror_on && c->ysh_avail > 0 && F.taxis == 1 && F.refine < 0) ? c->ysh_avail : 0;
+    if (F.taxis == 1) F.ngpart = ngpart_for(c);
+}
+
 // Eigen-system of the y pressure operator (fribo_eig.hip) and the gauge
 // weights zg = zmmt(null_mode, :); leaves taxis = 1 when it succeeds and the
 // y operator has a single null mode, otherwise keeps the transform along z.
@@ -379,10 +389,9 @@ static int setup_ytransform(fribo_ctx* c, const double* zmmt, int ldz)
             HIPCHK(hipMemcpy(F.wys, wy.data(), n * sizeof(double), hipMemcpyHostToDevice));
             F.ynulls = is + 1;
             c->ysh_avail = n / 2;
-            F.ysh = c->ysh_avail;
         }
     }
-    F.ngpart = ngpart_for(c);
+    apply_ysh(c);
     return FRIBO_OK;
 }
 
@@ -937,10 +946,11 @@ static void enqueue_poisson(fribo_ctx* c)
         return;
     }
     if (F.taxis == 1 && F.ysh > 0) {
-        // mirror form: pair butterfly, block-diagonal transforms with half
-        // the flops, the per-mode z-lines in the mirror form's mode order,
-        // pair butterfly of the result (the gauge partials come from rows 1
-        // and n2m of the backward transform's output)
+        // mirror form: k_div left the pair butterfly of the divergence;
+        // block-diagonal transforms with half the flops, the per-mode
+        // z-lines in the mirror form's mode order, pair butterfly of the
+        // result (the gauge partials come from rows 1 and n2m of the
+        // backward transform's output)
         double* d = F.dph + fidx(F, 1, 1);
         FluidDev Fm = F;
         Fm.wy = F.wys;
@@ -948,7 +958,6 @@ static void enqueue_poisson(fribo_ctx* c)
         YFuse yf{F.wys, F.amphk, F.acphk, F.apphk, F.ysum, F.n2m, 0};
         if (F.n3m % F.ks[0].nch == 0) yf.csz = F.n3m / F.ks[0].nch;
         bool fused = false;
-        run(c, PC_JP, [&] { launch_ymirror(F, d, st); });
         run_ev(c, PC_GEMM, [&](hipEvent_t a, hipEvent_t b) {
             fused = launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ysmi, F.ldy, d, F.P, F.hmn, F.P, F.S, st, nullptr, nullptr,
                                    a, b, &yf, F.ysh);
@@ -1491,7 +1500,7 @@ extern "C" int fribo_set_transform(fribo_handle h, int32_t axis)
         HIPCHK(hipStreamSynchronize(h->st));
         h->F.taxis = want;
         if (want != 1) h->F.refine = -1;
-        if (want == 1) h->F.ngpart = ngpart_for(h);
+        apply_ysh(h);
         const double zero = 0.0;
         HIPCHK(hipMemcpy(&h->F.S->dphoff, &zero, sizeof(double), hipMemcpyHostToDevice));
         if (h->gexec) {
@@ -1513,11 +1522,10 @@ extern "C" int fribo_set_ymirror(fribo_handle h, int32_t on)
     if (on && !h->ysh_avail)
         return set_err(FRIBO_ERR_LIMIT, "no mirror form: the y grid is not symmetric about the centre, or n2-1 is odd");
     if (h->batch) return set_err(FRIBO_ERR_ARG, "instance belongs to a batch");
-    const int want = on ? h->ysh_avail : 0;
-    if (want != h->F.ysh) {
+    if ((on != 0) != h->ymirror_on) {
         HIPCHK(hipStreamSynchronize(h->st));
-        h->F.ysh = want;
-        if (h->F.taxis == 1) h->F.ngpart = ngpart_for(h);
+        h->ymirror_on = on != 0;
+        apply_ysh(h);
         if (h->gexec) {
             hipGraphExecDestroy(h->gexec);
             h->gexec = nullptr;
@@ -1529,7 +1537,7 @@ extern "C" int fribo_set_ymirror(fribo_handle h, int32_t on)
 
 extern "C" int fribo_get_ymirror(fribo_handle h)
 {
-    return h ? (h->F.ysh > 0 ? 1 : 0) : -1;
+    return h ? (h->ymirror_on && h->ysh_avail > 0 ? 1 : 0) : -1;
 }
 
 extern "C" int fribo_get_transform(fribo_handle h)
@@ -1983,7 +1991,6 @@ static void batch_enqueue(fribo_batch_ctx* b)
         launch_jsolve_helm_b(B, st);
         launch_ksolve_b(B, st);
         launch_div_b(B, st);
-        if (B.hF[0].taxis == 1) launch_ymirror_b(B, st);
         const bool fused = launch_poisson_gemm_b(B, true, st);
         if (B.hF[0].taxis == 1) launch_ysolve_b(B, st, fused);
         else launch_jsolve_poisson_b(B, st);
@@ -2142,7 +2149,7 @@ extern "C" int fribo_set_precision(fribo_handle h, int32_t refine)
     if (h->batch) return set_err(FRIBO_ERR_ARG, "instance belongs to a batch");
     HIPCHK(hipStreamSynchronize(h->st));
     h->F.refine = refine < 0 ? -1 : refine;
-    if (h->F.taxis == 1) h->F.ngpart = ngpart_for(h);
+    apply_ysh(h);
     if (h->gexec) {
         hipGraphExecDestroy(h->gexec);
         h->gexec = nullptr;

@@ -839,19 +839,27 @@ void launch_rhs_b(BatchSet& B, hipStream_t st)
 
 // ---------------------------------------------------------------------------
 // divergence.F90:26-39
+// MIRROR (the mirror form of the transform along y, n2m = 2h): the wave
+// owns tile bx of the top half (rows up to h) and the mirrored tile of the
+// bottom half, lane l on row n2m + 1 - j (descending, so the j+1 neighbour
+// is in lane l-1), and stores the pair butterfly of the two divergences,
+// dph(j) = d(j) + d(n2m+1-j), dph(n2m+1-j) = d(j) - d(n2m+1-j), instead of
+// the divergences themselves.
+template <bool MIRROR>
 __device__ __forceinline__ void div_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const int lane = threadIdx.x & 63;
-    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)vb.x * SW_DIV + (int)(threadIdx.x >> 6);
+    const int jtop = MIRROR ? F.n2m / 2 : F.n2m;   // last output row of the tiles
+    const int ntj = (jtop + WJ - 1) / WJ, bx = (int)vb.x * SW_DIV + (int)(threadIdx.x >> 6);
     if (bx >= ntj) return;
     const int j = WJ * bx + lane;
     const int k0 = MK * vb.y + 1;
     if (k0 > F.n3m) return;
     const int k1 = min(k0 + MK - 1, F.n3m);
     const bool inj = (j >= 1 && j <= F.n2);
-    const bool out = (lane >= 1 && lane <= WJ && j <= F.n2m);
+    const bool out = (lane >= 1 && lane <= WJ && j <= jtop);
     const double usdtal = 1.0 / (S->dt * S->al);
     const size_t P = F.P;
     size_t c = inj ? fidx(F, j, k0) : 0;
@@ -861,43 +869,70 @@ __device__ __forceinline__ void div_body(const FluidDev& F, const VBlock& vb)
     double* __restrict__ dph = F.dph;
     double q3c = inj ? q3[c] : 0.0;
     double q2n = inj ? q2[c] : 0.0, q3n = inj ? q3[c + P] : 0.0;
+    // the mirrored row (MIRROR only)
+    const int jm = F.n2m + 1 - j;
+    const bool injm = MIRROR && jm >= 1 && jm <= F.n2;
+    size_t cm = injm ? fidx(F, jm, k0) : 0;
+    const double udx2mm = MIRROR ? F.udx2m[out ? jm : 1] : 0.0;
+    double q3cm = injm ? q3[cm] : 0.0;
+    double q2nm = injm ? q2[cm] : 0.0, q3nm = injm ? q3[cm + P] : 0.0;
     const double l_udx3m = F.udx3m[min(k0 + lane, F.n3m)];   // lane l: plane k0 + l
-    for (int k = k0; k <= k1; ++k, c += P) {
+    for (int k = k0; k <= k1; ++k, c += P, cm += P) {
         const double q2c = q2n, q3p = q3n;
+        const double q2cm = q2nm, q3pm = q3nm;
         if (k < k1) {
             q2n = inj ? q2[c + P] : 0.0;
             q3n = inj ? q3[c + 2 * P] : 0.0;
+            if (MIRROR) {
+                q2nm = injm ? q2[cm + P] : 0.0;
+                q3nm = injm ? q3[cm + 2 * P] : 0.0;
+            }
         }
         const double q2E = dn1(q2c);
+        const double q2Em = MIRROR ? up1(q2cm) : 0.0;
         const double udx3m_k = lane_bcast(l_udx3m, k - k0);
         if (out) {
             double dq = (q2E - q2c) * udx2m + (q3p - q3c) * udx3m_k;
-            dph[c] = dq * usdtal;
+            if (MIRROR) {
+                const double dqm = (q2Em - q2cm) * udx2mm + (q3pm - q3cm) * udx3m_k;
+                const double vt = dq * usdtal, vm = dqm * usdtal;
+                dph[c] = vt + vm;
+                dph[cm] = vt - vm;
+            } else {
+                dph[c] = dq * usdtal;
+            }
         }
         q3c = q3p;
+        q3cm = q3pm;
     }
 }
+template <bool MIRROR>
 __global__ __launch_bounds__(64 * SW_DIV) void k_div(const FluidDev F)
 {
     VBlock vb = vb_hw();
-    if (vb_xcd_band(vb, (F.n3m + MK - 1) / MK)) div_body(F, vb);
+    if (vb_xcd_band(vb, (F.n3m + MK - 1) / MK)) div_body<MIRROR>(F, vb);
 }
 __global__ __launch_bounds__(64 * SW_DIV) void kb_div(const BGrid<FluidDev> G)
 {
     VBlock vb;
     const FluidDev* Fp = bg_block_xcd(G, vb);
-    if (Fp) div_body(*Fp, vb);
+    if (!Fp) return;
+    if (Fp->ysh > 0) div_body<true>(*Fp, vb);
+    else div_body<false>(*Fp, vb);
 }
 
 static dim3 grid_div(const FluidDev& F)
 {
-    const int ntj = (F.n2m + WJ - 1) / WJ;
+    const int ntj = ((F.ysh > 0 ? F.n2m / 2 : F.n2m) + WJ - 1) / WJ;
     return dim3((ntj + SW_DIV - 1) / SW_DIV, (F.n3m + MK - 1) / MK);
 }
 
+// The single-instance step passes F.ysh = 0 unless the mirror form runs
+// (transform along y, FP64); batches clear it for instances without it.
 void launch_div(const FluidDev& F, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_div, xcd_pad(grid_div(F)), dim3(64 * SW_DIV), 0, st, F);
+    if (F.ysh > 0) hipLaunchKernelGGL(k_div<true>, xcd_pad(grid_div(F)), dim3(64 * SW_DIV), 0, st, F);
+    else hipLaunchKernelGGL(k_div<false>, xcd_pad(grid_div(F)), dim3(64 * SW_DIV), 0, st, F);
 }
 
 void launch_div_b(BatchSet& B, hipStream_t st)

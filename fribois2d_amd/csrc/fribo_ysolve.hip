@@ -227,25 +227,50 @@ void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done)
 // (n x ncol, leading dimension P): (x_j, x_{n-1-j}) -> (x_j + x_{n-1-j},
 // x_j - x_{n-1-j}) for j < h = n / 2.  Lanes take consecutive pairs of j
 // (for even h one 16-byte load from the top half and one from the mirrored
-// rows of the bottom half, whose two values swap; for odd h one value each),
-// blocks of 64 x 4: vb.x the slot range, vb.y a group of 4 columns.
+// rows of the bottom half, whose two values swap; for odd h one value each).
+// Blocks of 64 x 4 threads: vb.x the slot range, vb.y a group of
+// 4 * YM_CPT columns, each thread YM_CPT of them (columns ty, ty + 4, ...),
+// all loads issued before the stores.
+#ifndef YM_CPT
+#define YM_CPT 4
+#endif
 __device__ inline void ymirror_body(int n, int ncol, int P, double* __restrict__ x, const VBlock& vb)
 {
-    const int h = n / 2, u = vb.x * 64 + threadIdx.x, k = vb.y * 4 + threadIdx.y;
-    if (k >= ncol) return;
-    double* col = x + (size_t)k * P;
+    const int h = n / 2, u = vb.x * 64 + threadIdx.x, k0 = vb.y * 4 * YM_CPT + threadIdx.y;
     if (h % 2 == 0) {
         if (2 * u >= h) return;
-        double2* top = reinterpret_cast<double2*>(col + 2 * u);
-        double2* bot = reinterpret_cast<double2*>(col + n - 2 - 2 * u);
-        const double2 a = *top, b = *bot;   // b.y mirrors a.x, b.x mirrors a.y
-        *top = make_double2(a.x + b.y, a.y + b.x);
-        *bot = make_double2(a.y - b.x, a.x - b.y);
+        double2 a[YM_CPT], b[YM_CPT];
+#pragma unroll
+        for (int c = 0; c < YM_CPT; ++c) {
+            const int k = min(k0 + 4 * c, ncol - 1);
+            a[c] = *reinterpret_cast<const double2*>(x + (size_t)k * P + 2 * u);
+            b[c] = *reinterpret_cast<const double2*>(x + (size_t)k * P + n - 2 - 2 * u);
+        }
+#pragma unroll
+        for (int c = 0; c < YM_CPT; ++c) {
+            const int k = k0 + 4 * c;
+            if (k >= ncol) break;
+            // b.y mirrors a.x, b.x mirrors a.y
+            *reinterpret_cast<double2*>(x + (size_t)k * P + 2 * u) = make_double2(a[c].x + b[c].y, a[c].y + b[c].x);
+            *reinterpret_cast<double2*>(x + (size_t)k * P + n - 2 - 2 * u) =
+                make_double2(a[c].y - b[c].x, a[c].x - b[c].y);
+        }
     } else {
         if (u >= h) return;
-        const double a = col[u], b = col[n - 1 - u];
-        col[u] = a + b;
-        col[n - 1 - u] = a - b;
+        double a[YM_CPT], b[YM_CPT];
+#pragma unroll
+        for (int c = 0; c < YM_CPT; ++c) {
+            const int k = min(k0 + 4 * c, ncol - 1);
+            a[c] = x[(size_t)k * P + u];
+            b[c] = x[(size_t)k * P + n - 1 - u];
+        }
+#pragma unroll
+        for (int c = 0; c < YM_CPT; ++c) {
+            const int k = k0 + 4 * c;
+            if (k >= ncol) break;
+            x[(size_t)k * P + u] = a[c] + b[c];
+            x[(size_t)k * P + n - 1 - u] = a[c] - b[c];
+        }
     }
 }
 
@@ -264,7 +289,7 @@ __global__ __launch_bounds__(256) void kb_ymirror(const BGrid<FluidDev> G)
 static dim3 ymirror_grid(const FluidDev& F)
 {
     const int h = F.n2m / 2, slots = h % 2 == 0 ? h / 2 : h;
-    return dim3((slots + 63) / 64, (F.n3m + 3) / 4);
+    return dim3((slots + 63) / 64, (F.n3m + 4 * YM_CPT - 1) / (4 * YM_CPT));
 }
 
 void launch_ymirror(const FluidDev& F, double* x, hipStream_t st)
