@@ -24,9 +24,11 @@
 // which spreads the transposing LDS stores over the banks.
 //
 // Large problems (k_gemm_dma): 96 x 128 tiles of 16 waves (48 x 16 each, one
-// workgroup of 96 KB of LDS per CU; 96 x 64 tiles of 8 waves, two per CU,
-// measured 1-5% slower), and the operand slices move from global memory
-// straight into LDS
+// workgroup per CU; 96 x 64 tiles of 8 waves, two per CU, measured 1-5%
+// slower), K-slices of 24 (144 KB of LDS for the three stages; 16-deep
+// slices, one more barrier per 48 k, measured 3.5% slower on the 768-deep
+// products of the mirror form), and the operand slices move from global
+// memory straight into LDS
 // (global_load_lds_dwordx4, 16 bytes per lane, no register staging) through
 // three stages, so two K-slices of loads are in flight while one is
 // multiplied.  Each stage is its own LDS array, the slice loop is unrolled
@@ -57,7 +59,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 #define GBK 16
 #ifndef GEMM_DMA_BK
-#define GEMM_DMA_BK 16
+#define GEMM_DMA_BK 24
 #endif
 #ifndef GEMM_DMA_WM
 #define GEMM_DMA_WM 2
