@@ -71,6 +71,21 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 #define GEMM_DMA_BN 128
 #endif
 
+// s_barrier without the memory fence of __syncthreads (whose release would
+// wait for the LDS loads still in flight for later stages); the asm memory
+// clobber keeps the compiler from moving memory accesses across it.
+__device__ inline void raw_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+// Workgroup barrier that orders LDS only: waits for this wave's LDS
+// accesses (lgkmcnt(0)), not for its global stores, which __syncthreads
+// would drain first (the tile's results then leave while the epilogue's
+// LDS work proceeds).
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+    raw_barrier();
+}
+
 // Gauge partial of the transform along y (zg != nullptr): the tile holding
 // row grow of C adds zg(col) * C(grow, col) over its columns, in a fixed
 // order (columns of a lane, lanes by butterfly, waves in index order), and
@@ -96,7 +111,7 @@ __device__ inline void gauge_partial(const f64x4 (&acc)[RM][RN], int grow, doubl
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
     if (lane == 0) sh[wave] = v;
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
         double t = sh[0];
         for (int w = 1; w < NW; ++w) t = t + sh[w];
@@ -116,7 +131,7 @@ __device__ inline void gauge_partials(const f64x4 (&acc)[RM][RN], int ks, int M,
     const bool last = M - 1 >= i0 && M - 1 < i0 + BM;
     if (ks > 0 && i0 == 0) {
         gauge_partial<RM, RN, TM, TN, NW>(acc, 0, 1.0, N, i0, n0, wm, wn, lane, wave, zg, part + by, sh);
-        if (last) __syncthreads();
+        if (last) lds_barrier();
     }
     if (last)
         gauge_partial<RM, RN, TM, TN, NW>(acc, M - 1, ks > 0 ? -1.0 : 1.0, N, i0, n0, wm, wn, lane, wave, zg,
@@ -253,10 +268,6 @@ __global__ __launch_bounds__(64 * WM * WN) void kb_gemm_nt(const BGrid<GemmProb>
     gemm_nt_body<BM, BN, WM, WN, NN>(P.M, P.N, P.K, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, P.S, P.zg, P.gpart, P.ks, vb);
 }
 
-// s_barrier without the memory fence of __syncthreads (whose release would
-// wait for the LDS loads still in flight for later stages); the asm memory
-// clobber keeps the compiler from moving memory accesses across it.
-__device__ inline void raw_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
 // Wave priority around the issue of the next slice's loads (2; 1: around
 // the MFMA loop instead, 0: none): with two workgroups per CU, a wave that
@@ -425,7 +436,7 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
         // chunk elimination of the y-mode z-lines on this tile: rows i are
         // y-modes, columns n the z rows; the tile holds BN / csz whole chunks
         constexpr int LT = BM + 1;   // LDS row pitch of the staged tile (one z row per LDS row)
-        __syncthreads();
+        lds_barrier();   // no DMA is in flight after the last slice
 #pragma unroll
         for (int r = 0; r < RM; ++r)
 #pragma unroll
@@ -433,7 +444,7 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
                     SM[(wn * TN + c * 16 + (lane & 15)) * LT + wm * TM + r * 16 + (lane >> 4) + 4 * v] = acc[r][c][v];
-        __syncthreads();
+        lds_barrier();
         constexpr int WPC = (BM + 63) / 64;   // waves per chunk
         const int csz = yf.csz, cc = wave / WPC, iw = (wave % WPC) * 64 + lane;
         if (cc < BN / csz) {                  // wave-uniform
