@@ -86,6 +86,7 @@ struct fribo_ctx {
     int mk_gen = 0;              // bumped whenever the marker buffers are reallocated
     int ngpart64 = 0;            // tile columns of the FP64 backward transform along y
     int ysh_avail = 0;           // n2m / 2 when the mirror form of the y transform is set up
+    bool dph_mirror = false;     // dph holds the mirror form's result before its pair butterfly
     bool ymirror_on = true;      // the mirror form selected (used while the transform is along y in FP64)
     struct fribo_batch_ctx* batch = nullptr;   // batch this instance belongs to (shares its stream)
     int dev = 0;                 // device of the handle
@@ -919,6 +920,7 @@ static void enqueue_poisson(fribo_ctx* c)
     FluidDev& F = c->F;
     hipStream_t st = c->st;
     if (F.taxis == 1 && F.refine >= 0) {
+        c->dph_mirror = false;
         // mixed precision: x0 = P32(rhs), then F.refine sweeps of
         // x += P32(rhs - L x) with the residual in FP64; P32 is the solve with
         // FP32 transforms and the FP64 per-mode z-lines.  The iterate lives
@@ -948,9 +950,9 @@ static void enqueue_poisson(fribo_ctx* c)
     if (F.taxis == 1 && F.ysh > 0) {
         // mirror form: k_div left the pair butterfly of the divergence;
         // block-diagonal transforms with half the flops, the per-mode
-        // z-lines in the mirror form's mode order, pair butterfly of the
-        // result (the gauge partials come from rows 1 and n2m of the
-        // backward transform's output)
+        // z-lines in the mirror form's mode order; dph is left as the
+        // backward transform's result, whose pair butterfly k_upd takes as
+        // it reads it (the gauge partials come from rows 1 and n2m)
         double* d = F.dph + fidx(F, 1, 1);
         FluidDev Fm = F;
         Fm.wy = F.wys;
@@ -967,10 +969,11 @@ static void enqueue_poisson(fribo_ctx* c)
             launch_gemm_nn(F.n2m, F.n3m, F.n2m, F.ysm, F.ldy, F.hmn, F.P, d, F.P, F.S, st, F.zg, F.gpart, a, b,
                            nullptr, F.ysh);
         });
-        run(c, PC_JP, [&] { launch_ymirror(F, d, st); });
+        c->dph_mirror = true;
         return;
     }
     if (F.taxis == 1) {
+        c->dph_mirror = false;
         // the chunk pass of the per-mode z-lines runs in the forward
         // transform's epilogue when the line partition has equal chunks
         YFuse yf{F.wy, F.amphk, F.acphk, F.apphk, F.ysum, F.n2m, 0};
@@ -987,6 +990,7 @@ static void enqueue_poisson(fribo_ctx* c)
         });
         return;
     }
+    c->dph_mirror = false;
     run_ev(c, PC_GEMM, [&](hipEvent_t a, hipEvent_t b) {
         launch_gemm_nt(F.n2m, F.n3m, F.n3m, F.dph + fidx(F, 1, 1), F.P, F.zmmt, F.ldz, F.hmn, F.P, F.S, st, a, b);
     });
@@ -1203,6 +1207,18 @@ extern "C" int fribo_get_array(fribo_handle h, const char* name, void* out, long
     if (s == "q2" || s == "q3" || s == "dph" || s == "sA" || s == "sB") {
         double* src = s == "q2" ? F.q2 : (s == "q3" ? F.q3 : (s == "dph" ? F.dph : (s == "sA" ? F.sA : F.sB)));
         e = get_field(h, (double*)out, src, -2, F.nk);
+        if (!e && s == "dph" && h->dph_mirror) {
+            // the pair butterfly of the mirror form, which k_upd applies as it reads dph
+            double* o = (double*)out;
+            for (int k = 1; k <= F.n3m; ++k) {
+                double* col = o + (size_t)(k + 2) * F.n2;
+                for (int j = 0; j < F.n2m / 2; ++j) {
+                    const double a = col[j], b = col[F.n2m - 1 - j];
+                    col[j] = a + b;
+                    col[F.n2m - 1 - j] = a - b;
+                }
+            }
+        }
         if (!e && s == "dph" && F.taxis == 1) {
             if ((e = read_scalars(h))) return e;
             const double off = h->hs.dphoff;
@@ -1995,7 +2011,6 @@ static void batch_enqueue(fribo_batch_ctx* b)
         if (B.hF[0].taxis == 1) launch_ysolve_b(B, st, fused);
         else launch_jsolve_poisson_b(B, st);
         launch_poisson_gemm_b(B, false, st);
-        if (B.hF[0].taxis == 1) launch_ymirror_b(B, st);
         launch_upd_b(B, st);
         if (n == nsst) launch_mean_b(B, st);
     }
@@ -2053,7 +2068,10 @@ extern "C" int fribo_batch_step(fribo_batch b, int32_t ns, double* dt, const dou
     for (int i = 0; i < n; ++i) {
         const long long w = *(const long long*)&b->in_h[3 * i + 2];
         const bool outside = (w >> 32) != 0;   // not stepped: dt stays as given
-        if (!outside) dt[i] = b->out_h[i].dt;
+        if (!outside) {
+            dt[i] = b->out_h[i].dt;
+            b->inst[i]->dph_mirror = b->B.hF[i].ysh > 0;
+        }
         if (cflm) cflm[i] = b->out_h[i].cflm;
         if (err) err[i] = b->out_h[i].err;
     }

@@ -167,7 +167,7 @@ inline void blaunch_xcd(BatchSet& B, int key, const std::vector<int>& dims, Kern
 // Stage keys of the launch tables.
 enum {
     BK_CFL, BK_BC, BK_RHS, BK_DIV, BK_UPD, BK_MEAN, BK_MLS_IDX, BK_MLS_SUP, BK_MLS_ORDER, BK_MLS_INTERP,
-    BK_MLS_SPREAD, BK_JH, BK_JP, BK_K1, BK_K2, BK_K3, BK_KF, BK_Y1, BK_Y2, BK_Y3, BK_YM, BK_GEMM_F, BK_GEMM_B, BK_IO,
+    BK_MLS_SPREAD, BK_JH, BK_JP, BK_K1, BK_K2, BK_K3, BK_KF, BK_Y1, BK_Y2, BK_Y3, BK_GEMM_F, BK_GEMM_B, BK_IO,
     BK_COUNT
 };
 
@@ -183,8 +183,6 @@ void launch_jsolve_helm_b(BatchSet& B, hipStream_t st);
 void launch_jsolve_poisson_b(BatchSet& B, hipStream_t st);
 void launch_ksolve_b(BatchSet& B, hipStream_t st);
 void launch_ysolve_b(BatchSet& B, hipStream_t st, bool first_done);
-// pair butterfly of the instances in the mirror form of the y transform
-void launch_ymirror_b(BatchSet& B, hipStream_t st);
 // transforms of the Poisson solve: forward (fwd) or backward, the axis of
 // the instances (all alike); returns true when the fused chunk pass ran
 bool launch_poisson_gemm_b(BatchSet& B, bool fwd, hipStream_t st);

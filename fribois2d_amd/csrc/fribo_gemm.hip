@@ -71,21 +71,6 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 #define GEMM_DMA_BN 128
 #endif
 
-// s_barrier without the memory fence of __syncthreads (whose release would
-// wait for the LDS loads still in flight for later stages); the asm memory
-// clobber keeps the compiler from moving memory accesses across it.
-__device__ inline void raw_barrier() { asm volatile("s_barrier" ::: "memory"); }
-
-// Workgroup barrier that orders LDS only: waits for this wave's LDS
-// accesses (lgkmcnt(0)), not for its global stores, which __syncthreads
-// would drain first (the tile's results then leave while the epilogue's
-// LDS work proceeds).
-__device__ inline void lds_barrier()
-{
-    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
-    raw_barrier();
-}
-
 // Gauge partial of the transform along y (zg != nullptr): the tile holding
 // row grow of C adds zg(col) * C(grow, col) over its columns, in a fixed
 // order (columns of a lane, lanes by butterfly, waves in index order), and

@@ -31,7 +31,7 @@ inline int fribo_ncfl(int n2m, int n3m) { return ((n2m + 255) / 256) * ((n3m + F
 #define FRIBO_MK_RHS 16
 #endif
 #define FRIBO_WJ 62
-inline int fribo_npr_part(int n2m, int n3m)
+__host__ __device__ inline int fribo_npr_part(int n2m, int n3m)
 {
     return ((n2m + FRIBO_WJ - 1) / FRIBO_WJ) * ((n3m + FRIBO_MK - 1) / FRIBO_MK);
 }
@@ -132,7 +132,7 @@ struct FluidDev {
     double *cfl_part;        // [2*ncfl]
     int ncfl;                // CFL partials (workgroups of k_cfl)
     double *pr_part;         // [npr_part]
-    int npr_part;
+    int npr_part;            // partials of k_upd (those of the mirror form's k_upd: npr_count)
     // Helmholtz systems: ks[0] q2 along k, ks[1] q3 along k,
     // js[0] q2 along j, js[1] q3 along j; Poisson j-lines
     KSys ks[2];
@@ -159,6 +159,22 @@ struct FluidDev {
     double *bcerr;                 // (nlm,2)
     DevScalars* S;
 };
+
+// s_barrier without the memory fence of __syncthreads (whose release waits
+// for every outstanding global access of the wave, loads into LDS and
+// stores included); the asm memory clobber keeps the compiler from moving
+// memory accesses across it.
+__device__ inline void raw_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+// Workgroup barrier that orders LDS only: waits for this wave's LDS
+// accesses (lgkmcnt(0)), not for its global stores, which __syncthreads
+// would drain first.  For exchanges through LDS whose global stores no
+// other thread of the workgroup reads back.
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+    raw_barrier();
+}
 
 // Value of v in lane l of the wave (l wave-uniform): a matrix row that one
 // lane loaded and every lane of the wave uses, moved through scalar registers
@@ -239,8 +255,6 @@ bool launch_gemm_nn(int M, int N, int K, const double* A, int lda, const double*
 int gemm_nn_tiles_n(int M, int N, int K, const double* A, int lda, const double* B, int ldb);
 // first_done: the chunk pass already ran in the forward transform's epilogue
 void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done = false);
-// pair butterfly of the mirror form on x (n2m x n3m, leading dimension P)
-void launch_ymirror(const FluidDev& F, double* x, hipStream_t st);
 void launch_loads(const FluidDev& F, double hpr, hipStream_t st);
 // mixed-precision Poisson solve: r = rhs - (A_y + A_z) x (r may alias rhs),
 // and the FP32 transform C = X0 + A * B (X0 optional, may alias C)

@@ -31,9 +31,9 @@ __device__ inline double block_sum(double* sh, double v)
     const int t = threadIdx.x;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
-    __syncthreads();
+    lds_barrier();
     if ((t & 63) == 0) sh[t >> 6] = v;
-    __syncthreads();
+    lds_barrier();
     double r = sh[0];
 #pragma unroll
     for (int w = 1; w < NT / 64; ++w) r = r + sh[w];
@@ -46,9 +46,9 @@ __device__ inline double block_max(double* sh, double v)
     const int t = threadIdx.x;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    __syncthreads();
+    lds_barrier();
     if ((t & 63) == 0) sh[t >> 6] = v;
-    __syncthreads();
+    lds_barrier();
     double r = sh[0];
 #pragma unroll
     for (int w = 1; w < NT / 64; ++w) r = fmax(r, sh[w]);
@@ -210,7 +210,7 @@ __device__ StepScalars step_scalars(const FluidDev& F, int n, int ns, double dt_
         r.betadx = r.beta * al;
         out = r;
     }
-    __syncthreads();
+    lds_barrier();
     return out;
 }
 
@@ -247,7 +247,7 @@ __device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, dou
             K.c[r] = lc[r];
         }
     }
-    __syncthreads();
+    lds_barrier();
     if (t >= 64) return;
     const int p = t;
     double alpha = 0.0, beta = 1.0, gamma = 0.0, vs = 0.0, ws = 0.0, ce = 0.0;
@@ -328,9 +328,16 @@ size_t sysprep_lds_bytes(const FluidDev& F)
 
 // First two pressure partials of thread t (entries t and t+1024, clamped to
 // the last), loaded ahead so that they travel with the kernel's other loads.
+// Pressure partials k_upd leaves: one per tile (the mirror form's tiles
+// cover the top half with its mirror image, half as many).
+__device__ inline int npr_count(const FluidDev& F)
+{
+    return F.ysh > 0 ? fribo_npr_part(F.n2m / 2, F.n3m) : F.npr_part;
+}
+
 __device__ inline void pr_load(const FluidDev& F, double* v)
 {
-    const int t = threadIdx.x, n = F.npr_part;
+    const int t = threadIdx.x, n = npr_count(F);
     v[0] = F.pr_part[min(t, n - 1)];
     v[1] = F.pr_part[min(t + 1024, n - 1)];
 }
@@ -341,7 +348,7 @@ __device__ inline void pr_load(const FluidDev& F, double* v)
 // then block_sum.  Fixed order.
 __device__ inline double pr_mean(const FluidDev& F, double* sh, const double* v0)
 {
-    const int t = threadIdx.x, n = F.npr_part;
+    const int t = threadIdx.x, n = npr_count(F);
     double s = 0.0;
     if (t < n) s = s + v0[0];
     if (t + 1024 < n) s = s + v0[1];
@@ -944,12 +951,22 @@ void launch_div_b(BatchSet& B, hipStream_t st)
 // updvp.F90:27-56 (velocity correction) + 83-102 (prcalc) + partial sums of
 // the pressure mean (updvp.F90:127-133).  The stored pressure is "raw": the
 // previous mean (S->pend) is subtracted on read.
+// MIRROR (the mirror form of the transform along y, n2m = 2h): dph holds the
+// backward transform's result before its pair butterfly, C; the wave owns
+// tile bx of the top half (rows up to h) and the mirrored tile of the bottom
+// half, lane l on row n2m + 1 - j (descending: its j-1 neighbour is in lane
+// l+1, its j+1 neighbour in lane l-1), loads C on both rows and takes
+// dph(j) = C(j) + C(n2m+1-j), dph(n2m+1-j) = C(j) - C(n2m+1-j) (for a halo
+// row j past h the roles swap).
+template <bool MIRROR>
 __device__ __forceinline__ void upd_body(const FluidDev& F, const VBlock& vb)
 {
     const DevScalars* S = F.S;
     if (S->err) return;
     const int lane = threadIdx.x & 63;
-    const int ntj = (F.n2m + WJ - 1) / WJ, bx = (int)vb.x * SW_UPD + (int)(threadIdx.x >> 6);
+    const int h = F.n2m / 2;
+    const int jtop = MIRROR ? h : F.n2m;   // last output row of the tiles
+    const int ntj = (jtop + WJ - 1) / WJ, bx = (int)vb.x * SW_UPD + (int)(threadIdx.x >> 6);
     if (bx >= ntj) return;
     const int j = WJ * bx + lane;
     const int k0 = MK * vb.y + 1;
@@ -957,7 +974,7 @@ __device__ __forceinline__ void upd_body(const FluidDev& F, const VBlock& vb)
     if (k0 <= F.n3m) {
         const int k1 = min(k0 + MK - 1, F.n3m);
         const bool inj = (j >= 1 && j <= F.n2);
-        const bool out = (lane >= 1 && lane <= WJ && j <= F.n2m);
+        const bool out = (lane >= 1 && lane <= WJ && j <= jtop);
         const double al = S->al, dt = S->dt, pend = S->pend, be = al * S->beta;
         const size_t P = F.P;
         const int jj = out ? j : 1;
@@ -966,6 +983,17 @@ __device__ __forceinline__ void upd_body(const FluidDev& F, const VBlock& vb)
         const double dy = F.rc[jj + 1] - F.rc[jj];
         const bool jlo = (j == 1), jhi = (j == F.n2m);
         size_t c = inj ? fidx(F, j, k0) : 0;
+        // the mirrored row (MIRROR only)
+        const int jm = F.n2m + 1 - j;
+        const bool injm = MIRROR && jm >= 1 && jm <= F.n2;
+        const int jjm = (MIRROR && out) ? jm : 1;
+        const double usujm_m = MIRROR ? al * dt * F.udx2c[jjm] : 0.0;
+        const double apj_m = MIRROR ? F.apphj[jjm] : 0.0, acj_m = MIRROR ? F.acphj[jjm] : 0.0;
+        const double amj_m = MIRROR ? F.amphj[jjm] : 0.0;
+        const double dy_m = MIRROR ? F.rc[jjm + 1] - F.rc[jjm] : 0.0;
+        const bool jlo_m = (jm == 1), jhi_m = (jm == F.n2m);
+        size_t cm = injm ? fidx(F, jm, k0) : 0;
+        const bool lower = j <= h;   // row j in the top half: dph(j) = a + b
         const double* __restrict__ dph = F.dph;
         double* __restrict__ q2 = F.q2;
         double* __restrict__ q3 = F.q3;
@@ -980,73 +1008,116 @@ __device__ __forceinline__ void upd_body(const FluidDev& F, const VBlock& vb)
             doff = g * F.zginv;
         }
         if (vb.x == 0 && vb.y == 0 && lane == 0) F.S->dphoff = doff;
-        double dm = (inj && k0 > 1) ? dph[c - P] - doff : 0.0;
-        double dc = inj ? dph[c] - doff : 0.0;
+        // dph of row j (t) and of the mirrored row (m) at plane offset o
+        auto ld = [&](size_t o, bool ok, double& t, double& m) {
+            if (!MIRROR) {
+                t = (inj && ok) ? dph[c + o] - doff : 0.0;
+                return;
+            }
+            const double a = (inj && ok) ? dph[c + o] : 0.0, b = (injm && ok) ? dph[cm + o] : 0.0;
+            t = (lower ? a + b : b - a) - doff;
+            m = (lower ? a - b : a + b) - doff;
+        };
+        double dm, dc, dn, dm_m = 0.0, dc_m = 0.0, dn_m = 0.0;
+        ld(-P, k0 > 1, dm, dm_m);
+        ld(0, true, dc, dc_m);
         // plane k+1 of dph and plane k of q2/q3/pr are loaded one iteration ahead
-        double dn = (inj && k0 < n3m) ? dph[c + P] - doff : 0.0;
+        ld(P, k0 < n3m, dn, dn_m);
         double q2n = out ? q2[c] : 0.0, q3n = out ? q3[c] : 0.0, prn = out ? pr[c] : 0.0;
+        double q2n_m = 0.0, q3n_m = 0.0, prn_m = 0.0;
+        if (MIRROR && out) {
+            q2n_m = q2[cm];
+            q3n_m = q3[cm];
+            prn_m = pr[cm];
+        }
         // per-plane coefficients of the tile: lane l holds plane k0 + l
         const int kl = min(k0 + lane, n3m);
         const double l_udx3c = F.udx3c[kl], l_apphk = F.apphk[kl], l_acphk = F.acphk[kl];
         const double l_amphk = F.amphk[kl], l_dz = F.zz[kl + 1] - F.zz[kl];
-        for (int k = k0; k <= k1; ++k, c += P) {
+        for (int k = k0; k <= k1; ++k, c += P, cm += P) {
             const int kk = k - k0;
             const double udx3c_k = lane_bcast(l_udx3c, kk), apphk_k = lane_bcast(l_apphk, kk);
             const double acphk_k = lane_bcast(l_acphk, kk), amphk_k = lane_bcast(l_amphk, kk);
             const double dz_k = lane_bcast(l_dz, kk);
             const double dp = dn, q2v = q2n, q3v = q3n, prv0 = prn;
+            const double dp_m = dn_m, q2v_m = q2n_m, q3v_m = q3n_m, prv0_m = prn_m;
             if (k < k1) {
-                dn = (inj && k + 1 < n3m) ? dph[c + 2 * P] - doff : 0.0;
+                ld(2 * P, k + 1 < n3m, dn, dn_m);
                 q2n = out ? q2[c + P] : 0.0;
                 q3n = out ? q3[c + P] : 0.0;
                 prn = out ? pr[c + P] : 0.0;
+                if (MIRROR && out) {
+                    q2n_m = q2[cm + P];
+                    q3n_m = q3[cm + P];
+                    prn_m = pr[cm + P];
+                }
             }
             const double dW = up1(dc), dE = dn1(dc);
+            const double dW_m = MIRROR ? dn1(dc_m) : 0.0, dE_m = MIRROR ? up1(dc_m) : 0.0;
             if (out) {
-                const double dJm = jlo ? dc : dW;     // dph(jmv(j))
-                const double dJp = jhi ? dc : dE;     // dph(jpv(j))
-                const double dKm = (k == 1) ? dc : dm; // dph(kmv(k))
-                const double dKp = (k == n3m) ? dc : dp;
-                q2[c] = q2v - (dc - dJm) * usujm;
-                if (k >= 2) {
-                    const double usukm = al * dt * udx3c_k;
-                    q3[c] = q3v - (dc - dm) * usukm;
+                const double usukm = al * dt * udx3c_k;
+                {
+                    const double dJm = jlo ? dc : dW;     // dph(jmv(j))
+                    const double dJp = jhi ? dc : dE;     // dph(jpv(j))
+                    const double dKm = (k == 1) ? dc : dm; // dph(kmv(k))
+                    const double dKp = (k == n3m) ? dc : dp;
+                    q2[c] = q2v - (dc - dJm) * usujm;
+                    if (k >= 2) q3[c] = q3v - (dc - dm) * usukm;
+                    double lap = (dJp * apj + dc * acj + dJm * amj) +
+                                 (dKp * apphk_k + dc * acphk_k + dKm * amphk_k);
+                    double prv = (prv0 - pend) + dc - be * lap;
+                    pr[c] = prv;
+                    part = part + prv * dz_k * dy;
                 }
-                double lap = (dJp * apj + dc * acj + dJm * amj) +
-                             (dKp * apphk_k + dc * acphk_k + dKm * amphk_k);
-                double prv = (prv0 - pend) + dc - be * lap;
-                pr[c] = prv;
-                part = part + prv * dz_k * dy;
+                if (MIRROR) {
+                    const double dJm = jlo_m ? dc_m : dW_m;
+                    const double dJp = jhi_m ? dc_m : dE_m;
+                    const double dKm = (k == 1) ? dc_m : dm_m;
+                    const double dKp = (k == n3m) ? dc_m : dp_m;
+                    q2[cm] = q2v_m - (dc_m - dJm) * usujm_m;
+                    if (k >= 2) q3[cm] = q3v_m - (dc_m - dm_m) * usukm;
+                    double lap = (dJp * apj_m + dc_m * acj_m + dJm * amj_m) +
+                                 (dKp * apphk_k + dc_m * acphk_k + dKm * amphk_k);
+                    double prv = (prv0_m - pend) + dc_m - be * lap;
+                    pr[cm] = prv;
+                    part = part + prv * dz_k * dy_m;
+                }
             }
             dm = dc;
             dc = dp;
+            dm_m = dc_m;
+            dc_m = dp_m;
         }
     }
     // wave reduction (fixed order) -> one partial per workgroup
     for (int off = 32; off > 0; off >>= 1) part = part + __shfl_xor(part, off, 64);
     if (lane == 0) F.pr_part[vb.y * ntj + bx] = part;
 }
+template <bool MIRROR>
 __global__ __launch_bounds__(64 * SW_UPD) void k_upd(const FluidDev F)
 {
     VBlock vb = vb_hw();
-    if (vb_xcd_band(vb, (F.n3m + MK - 1) / MK)) upd_body(F, vb);
+    if (vb_xcd_band(vb, (F.n3m + MK - 1) / MK)) upd_body<MIRROR>(F, vb);
 }
 __global__ __launch_bounds__(64 * SW_UPD) void kb_upd(const BGrid<FluidDev> G)
 {
     VBlock vb;
     const FluidDev* Fp = bg_block_xcd(G, vb);
-    if (Fp) upd_body(*Fp, vb);
+    if (!Fp) return;
+    if (Fp->ysh > 0) upd_body<true>(*Fp, vb);
+    else upd_body<false>(*Fp, vb);
 }
 
 static dim3 grid_upd(const FluidDev& F)
 {
-    const int ntj = (F.n2m + WJ - 1) / WJ;
+    const int ntj = ((F.ysh > 0 ? F.n2m / 2 : F.n2m) + WJ - 1) / WJ;
     return dim3((ntj + SW_UPD - 1) / SW_UPD, (F.n3m + MK - 1) / MK);
 }
 
 void launch_upd(const FluidDev& F, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_upd, xcd_pad(grid_upd(F)), dim3(64 * SW_UPD), 0, st, F);
+    if (F.ysh > 0) hipLaunchKernelGGL(k_upd<true>, xcd_pad(grid_upd(F)), dim3(64 * SW_UPD), 0, st, F);
+    else hipLaunchKernelGGL(k_upd<false>, xcd_pad(grid_upd(F)), dim3(64 * SW_UPD), 0, st, F);
 }
 
 void launch_upd_b(BatchSet& B, hipStream_t st)

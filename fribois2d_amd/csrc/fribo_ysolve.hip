@@ -223,88 +223,6 @@ void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done)
     else hipLaunchKernelGGL(k_ysolve3<128>, g, b, 0, st, F);
 }
 
-// Pair butterfly of the mirror form, in place on every column of x
-// (n x ncol, leading dimension P): (x_j, x_{n-1-j}) -> (x_j + x_{n-1-j},
-// x_j - x_{n-1-j}) for j < h = n / 2.  Lanes take consecutive pairs of j
-// (for even h one 16-byte load from the top half and one from the mirrored
-// rows of the bottom half, whose two values swap; for odd h one value each).
-// Blocks of 64 x 4 threads: vb.x the slot range, vb.y a group of
-// 4 * YM_CPT columns, each thread YM_CPT of them (columns ty, ty + 4, ...),
-// all loads issued before the stores.
-#ifndef YM_CPT
-#define YM_CPT 4
-#endif
-__device__ inline void ymirror_body(int n, int ncol, int P, double* __restrict__ x, const VBlock& vb)
-{
-    const int h = n / 2, u = vb.x * 64 + threadIdx.x, k0 = vb.y * 4 * YM_CPT + threadIdx.y;
-    if (h % 2 == 0) {
-        if (2 * u >= h) return;
-        double2 a[YM_CPT], b[YM_CPT];
-#pragma unroll
-        for (int c = 0; c < YM_CPT; ++c) {
-            const int k = min(k0 + 4 * c, ncol - 1);
-            a[c] = *reinterpret_cast<const double2*>(x + (size_t)k * P + 2 * u);
-            b[c] = *reinterpret_cast<const double2*>(x + (size_t)k * P + n - 2 - 2 * u);
-        }
-#pragma unroll
-        for (int c = 0; c < YM_CPT; ++c) {
-            const int k = k0 + 4 * c;
-            if (k >= ncol) break;
-            // b.y mirrors a.x, b.x mirrors a.y
-            *reinterpret_cast<double2*>(x + (size_t)k * P + 2 * u) = make_double2(a[c].x + b[c].y, a[c].y + b[c].x);
-            *reinterpret_cast<double2*>(x + (size_t)k * P + n - 2 - 2 * u) =
-                make_double2(a[c].y - b[c].x, a[c].x - b[c].y);
-        }
-    } else {
-        if (u >= h) return;
-        double a[YM_CPT], b[YM_CPT];
-#pragma unroll
-        for (int c = 0; c < YM_CPT; ++c) {
-            const int k = min(k0 + 4 * c, ncol - 1);
-            a[c] = x[(size_t)k * P + u];
-            b[c] = x[(size_t)k * P + n - 1 - u];
-        }
-#pragma unroll
-        for (int c = 0; c < YM_CPT; ++c) {
-            const int k = k0 + 4 * c;
-            if (k >= ncol) break;
-            x[(size_t)k * P + u] = a[c] + b[c];
-            x[(size_t)k * P + n - 1 - u] = a[c] - b[c];
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_ymirror(int n, int ncol, int P, double* __restrict__ x)
-{
-    ymirror_body(n, ncol, P, x, vb_hw());
-}
-
-__global__ __launch_bounds__(256) void kb_ymirror(const BGrid<FluidDev> G)
-{
-    VBlock vb;
-    const FluidDev* Fp = bg_block(G, vb);
-    ymirror_body(Fp->n2m, Fp->n3m, (int)Fp->P, Fp->dph + fidx(*Fp, 1, 1), vb);
-}
-
-static dim3 ymirror_grid(const FluidDev& F)
-{
-    const int h = F.n2m / 2, slots = h % 2 == 0 ? h / 2 : h;
-    return dim3((slots + 63) / 64, (F.n3m + 4 * YM_CPT - 1) / (4 * YM_CPT));
-}
-
-void launch_ymirror(const FluidDev& F, double* x, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_ymirror, ymirror_grid(F), dim3(64, 4), 0, st, F.n2m, F.n3m, (int)F.P, x);
-}
-
-// The butterfly on dph of every instance in the mirror form (no blocks for
-// the others).
-void launch_ymirror_b(BatchSet& B, hipStream_t st)
-{
-    const auto g = bdims(B, [](const FluidDev& F) { return F.ysh > 0 ? ymirror_grid(F) : dim3(0, 0, 0); });
-    blaunch(B, BK_YM, g, kb_ymirror, dim3(64, 4), 0, st);
-}
-
 void launch_ysolve_b(BatchSet& B, hipStream_t st, bool first_done)
 {
     int mm = 0;
