@@ -536,8 +536,15 @@ __device__ __forceinline__ void mls_order_body(const FluidDev& F, const VBlock& 
         if (lc[u] >= 0) {
             const unsigned int e = cnt[lc[u]], s0 = lc[u] > 0 ? cnt[lc[u] - 1] : 0u;
             const unsigned int me = (unsigned int)(key[u] & ENT_MASK);
-            unsigned int rank = 0u;
-            for (unsigned int q = s0; q < e; ++q) rank += ent[q] < me ? 1u : 0u;
+            // eight reads in flight per step (the range holds a few dozen entries)
+            unsigned int r8[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+            unsigned int q = s0;
+            for (; q + 8 <= e; q += 8) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) r8[i] += ent[q + i] < me ? 1u : 0u;
+            }
+            for (; q < e; ++q) r8[0] += ent[q] < me ? 1u : 0u;
+            const unsigned int rank = ((r8[0] + r8[1]) + (r8[2] + r8[3])) + ((r8[4] + r8[5]) + (r8[6] + r8[7]));
             gk[s0 + rank] = key[u];
         }
     for (int i = (int)nvalid + t; i < n; i += 1024) gk[i] = KEY_NONE;
