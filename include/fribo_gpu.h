@@ -323,7 +323,7 @@ const char* fribo_profile_name(int32_t cls);
  *   fribo_batch_set_graph: replay the batch's launches as one HIP graph;
  *   fribo_batch_set_gemm_shape: tile shape of the transforms (0: 32x32,
  *     1: 64x64, 2: 96x128, 3: direct-to-LDS 96x128, 4: direct-to-LDS
- *     96x64, -1: default), and
+ *     96x64 with 16-deep K-slices, -1: default), and
  *     fribo_batch_get_gemm_shape the one in use.
  * Destroy the batch before finalizing its instances. */
 typedef struct fribo_batch_ctx* fribo_batch;
