@@ -148,7 +148,7 @@ def run_ensemble(args):
     mirror = [g.ymirror for g in gs]
     mk = EnsembleMarkers(cs)
     b = FluidBatch(gs)
-    best, tune_ms = b.autotune(5)
+    best, tune_ms, tune_total = b.autotune(5)
     b.set_graph(args.launch == "graph")
     t = np.zeros(n)
     dt = np.array([c.params.dt0 for c in cs])
@@ -219,7 +219,9 @@ def run_ensemble(args):
                      "kernel": "kb_gemm (grouped forward transform)", "avg_ms": round(gemm_ms, 4),
                      "members_in_mirror_form": int(sum(mirror))},
         "autotune": {"shape": shapes.get(best, best),
-                     "ms_per_launch": {shapes[k]: round(float(v), 4) for k, v in enumerate(tune_ms) if v > 0}},
+                     "ms_per_launch": {shapes[k]: round(float(v), 4) for k, v in enumerate(tune_ms) if v > 0},
+                     "ms_with_unfused_chunk_pass": {shapes[k]: round(float(v), 4)
+                                                    for k, v in enumerate(tune_total) if v > 0}},
         "init_s": round(t_init, 1),
     }
     print(json.dumps(out))

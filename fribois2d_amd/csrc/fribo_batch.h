@@ -183,6 +183,8 @@ void launch_jsolve_helm_b(BatchSet& B, hipStream_t st);
 void launch_jsolve_poisson_b(BatchSet& B, hipStream_t st);
 void launch_ksolve_b(BatchSet& B, hipStream_t st);
 void launch_ysolve_b(BatchSet& B, hipStream_t st, bool first_done);
+// the first (chunk) pass of the batched per-mode z-lines alone
+void launch_ysolve1_b(BatchSet& B, hipStream_t st);
 // transforms of the Poisson solve: forward (fwd) or backward, the axis of
 // the instances (all alike); returns true when the fused chunk pass ran
 bool launch_poisson_gemm_b(BatchSet& B, bool fwd, hipStream_t st);

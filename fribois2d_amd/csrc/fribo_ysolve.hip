@@ -223,21 +223,31 @@ void launch_ysolve(const FluidDev& F, hipStream_t st, bool first_done)
     else hipLaunchKernelGGL(k_ysolve3<128>, g, b, 0, st, F);
 }
 
-void launch_ysolve_b(BatchSet& B, hipStream_t st, bool first_done)
+static int ysolve_maxm_b(const BatchSet& B)
 {
     int mm = 0;
     for (const auto& F : B.hF) mm = std::max(mm, F.ks[0].maxm - 1);
+    return mm;
+}
+
+void launch_ysolve1_b(BatchSet& B, hipStream_t st)
+{
+    const int mm = ysolve_maxm_b(B);
+    const int c = y1c(mm <= 32 ? 32 : 64);
+    const auto g1 = bdims(B, [c](const FluidDev& F) { return dim3((F.n2m + 63) / 64, (F.ks[0].nch + c - 1) / c); });
+    const dim3 b1(64 * c);
+    if (mm <= 16) blaunch(B, BK_Y1, g1, kb_ysolve1<16>, b1, 0, st);
+    else if (mm <= 32) blaunch(B, BK_Y1, g1, kb_ysolve1<32>, b1, 0, st);
+    else if (mm <= 64) blaunch(B, BK_Y1, g1, kb_ysolve1<64>, b1, 0, st);
+    else blaunch(B, BK_Y1, g1, kb_ysolve1<128>, b1, 0, st);
+}
+
+void launch_ysolve_b(BatchSet& B, hipStream_t st, bool first_done)
+{
+    const int mm = ysolve_maxm_b(B);
     const dim3 b(64 * YW);
     const auto g = bdims(B, [](const FluidDev& F) { return dim3((F.n2m + 64 * YW - 1) / (64 * YW), F.ks[0].nch); });
-    if (!first_done) {
-        const int c = y1c(mm <= 32 ? 32 : 64);
-        const auto g1 = bdims(B, [c](const FluidDev& F) { return dim3((F.n2m + 63) / 64, (F.ks[0].nch + c - 1) / c); });
-        const dim3 b1(64 * c);
-        if (mm <= 16) blaunch(B, BK_Y1, g1, kb_ysolve1<16>, b1, 0, st);
-        else if (mm <= 32) blaunch(B, BK_Y1, g1, kb_ysolve1<32>, b1, 0, st);
-        else if (mm <= 64) blaunch(B, BK_Y1, g1, kb_ysolve1<64>, b1, 0, st);
-        else blaunch(B, BK_Y1, g1, kb_ysolve1<128>, b1, 0, st);
-    }
+    if (!first_done) launch_ysolve1_b(B, st);
     blaunch(B, BK_Y2, bdims(B, [](const FluidDev& F) { return dim3((F.n2m + YW - 1) / YW); }), kb_ysolve2, b, 0, st);
     if (mm <= 16) blaunch(B, BK_Y3, g, kb_ysolve3<16>, b, 0, st);
     else if (mm <= 32) blaunch(B, BK_Y3, g, kb_ysolve3<32>, b, 0, st);

@@ -508,11 +508,13 @@ class FluidBatch:
         return load().fribo_batch_get_gemm_shape(self.b)
 
     def autotune(self, reps=5):
-        """Pick the transform tile shape by timing; returns (best, ms per shape)."""
-        ms = np.zeros(5)
+        """Pick the transform tile shape by timing; returns (best, ms per
+        shape of the transform alone, ms per shape with the chunk pass of
+        the per-mode z-lines that the shape does not fuse, which decides)."""
+        ms = np.zeros(10)
         best = C.c_int32(-1)
         _check(load().fribo_batch_autotune(self.b, int(reps), _dp(ms), C.byref(best)))
-        return best.value, ms
+        return best.value, ms[:5], ms[5:]
 
     def set_markers_flat(self, buf):
         """Marker kinematics of all instances, already laid out as

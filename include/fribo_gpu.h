@@ -336,8 +336,11 @@ int fribo_batch_set_gemm_shape(fribo_batch b, int32_t cfg);
 int fribo_batch_get_gemm_shape(fribo_batch b);
 /* Autotuner (no reference counterpart): times the batch's forward transform
  * with every tile shape all its instances can run (reps launches each, no
- * change to any state) and keeps the fastest; ms[5] gets the time per
- * launch of shapes 0..4 (0: not available), best the shape kept. */
+ * change to any state) and keeps the fastest, counting with the transform
+ * along y the chunk pass of the per-mode z-lines where the shape leaves it
+ * to its own kernel instead of the transform's epilogue; ms[10] gets the
+ * time per launch of shapes 0..4 without (ms[0..4]) and with (ms[5..9])
+ * that pass (0: not available), best the shape kept. */
 int fribo_batch_autotune(fribo_batch b, int32_t reps, double* ms, int32_t* best);
 
 /* Learned first iterate of the strong coupling loop (no counterpart in the

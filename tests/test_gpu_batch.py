@@ -52,7 +52,7 @@ def _run_single(cs, nsteps):
     return out
 
 
-def _run_batch(cs, nsteps, graph=False, shape=None, bad=None):
+def _run_batch(cs, nsteps, graph=False, shape=None, bad=None, tune=None):
     from fribois2d_amd.gpu import FluidBatch
     gs = [_step(c) for c in cs]
     b = FluidBatch(gs)
@@ -60,6 +60,8 @@ def _run_batch(cs, nsteps, graph=False, shape=None, bad=None):
         b.set_graph(True)
     if shape is not None:
         b.set_gemm_shape(shape)
+    if tune is not None:
+        tune.append(b.autotune(3))
     n = len(cs)
     t = np.zeros(n)
     dt = np.array([c.params.dt0 for c in cs])
@@ -115,6 +117,25 @@ def test_batch_equals_single_instances():
     _compare(single, batch, 0.0)
 
 
+def test_autotune_keeps_the_fastest_shape_and_no_state():
+    """fribo_batch_autotune times every tile shape the members can run (the
+    forward transform alone, and with the chunk pass of the per-mode z-lines
+    where the shape does not fuse it) and keeps the fastest by the second
+    figure; it changes no state, so the batch then steps like the single
+    instances (other-tiles bar)."""
+    cs = _members()
+    single = _run_single(cs, 3)
+    tune = []
+    batch = _run_batch(cs, 3, tune=tune)
+    best, ms, tot = tune[0]
+    avail = tot > 0
+    print(f"autotune: shape {best}, transform ms {ms}, with unfused chunk pass {tot}", flush=True)
+    assert batch[3] == best and avail[best]
+    assert np.array_equal(avail, ms > 0)
+    assert tot[best] == tot[avail].min()
+    _compare(single, batch, 1e-14)
+
+
 def test_batch_with_other_transform_tiles():
     cs = _members()
     single = _run_single(cs, 3)
@@ -125,13 +146,16 @@ def test_batch_with_other_transform_tiles():
         print(f"tile shape {shape}: largest relative difference {worst:.2e}", flush=True)
 
 
-def test_batch_fused_chunk_pass_on_both_direct_tiles():
-    """Two 1024x768 members (chunks of 16 z-rows): the forward transform's
-    epilogue runs the first pass of the per-mode z-lines on the 96x128 tiles
-    of 16 waves (the single-instance shape) and on the 96x64 tiles of 8
-    waves; both batches match the single-instance runs to the other-tiles
-    bar."""
-    cs = [cases.ensemble_case(i, 1024, 768, re, amp=0.1, freq=0.2) for i, re in enumerate((90.0, 180.0))]
+@pytest.mark.parametrize("n3m,n2ms", [(1024, (768, 768)), (256, (192, 96))])
+def test_batch_fused_chunk_pass_on_both_direct_tiles(n3m, n2ms):
+    """Both direct-to-LDS tile shapes (96x128 of 16 waves, the
+    single-instance shape, and 96x64 of 8 waves with 16-deep K-slices) match
+    the single-instance runs to the other-tiles bar.  1024 z-rows: chunks of
+    16, one per wave group of the tile, so the forward transform's epilogue
+    runs the first pass of the per-mode z-lines; 256 z-rows: chunks of 4,
+    more than a tile has wave groups, so that pass keeps its own kernel (one
+    member with a symmetric y grid, in the mirror form, one without)."""
+    cs = [cases.ensemble_case(i, n3m, n2m, re, amp=0.1, freq=0.2) for i, (n2m, re) in enumerate(zip(n2ms, (90.0, 180.0)))]
     single = _run_single(cs, 3)
     for shape in (3, 4):
         batch = _run_batch(cs, 3, shape=shape)
