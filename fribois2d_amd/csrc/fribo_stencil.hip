@@ -326,8 +326,6 @@ size_t sysprep_lds_bytes(const FluidDev& F)
     return (size_t)3 * n * sizeof(double);
 }
 
-// First two pressure partials of thread t (entries t and t+1024, clamped to
-// the last), loaded ahead so that they travel with the kernel's other loads.
 // Pressure partials k_upd leaves: one per tile (the mirror form's tiles
 // cover the top half with its mirror image, half as many).
 __device__ inline int npr_count(const FluidDev& F)
@@ -335,24 +333,30 @@ __device__ inline int npr_count(const FluidDev& F)
     return F.ysh > 0 ? fribo_npr_part(F.n2m / 2, F.n3m) : F.npr_part;
 }
 
+// The first PR_PRE pressure partials of thread t (entries t, t+1024, ...,
+// clamped to the last), loaded ahead so that they travel with the kernel's
+// other loads (8192 partials: the 2048x1536 grid has 6400, 3328 in the
+// mirror form).
+#define PR_PRE 8
 __device__ inline void pr_load(const FluidDev& F, double* v)
 {
     const int t = threadIdx.x, n = npr_count(F);
-    v[0] = F.pr_part[min(t, n - 1)];
-    v[1] = F.pr_part[min(t + 1024, n - 1)];
+#pragma unroll
+    for (int u = 0; u < PR_PRE; ++u) v[u] = F.pr_part[min(t + 1024 * u, n - 1)];
 }
 
 // Pressure mean of the previous substep (pr_submean, updvp.F90:125-141) from
 // the npr_part partials of k_upd: thread t adds entries t, t+1024, ... in
-// turn (the first two from pr_load, then eight loads in flight at a time),
-// then block_sum.  Fixed order.
+// turn (the first PR_PRE from pr_load, then eight loads in flight at a
+// time), then block_sum.  Fixed order.
 __device__ inline double pr_mean(const FluidDev& F, double* sh, const double* v0)
 {
     const int t = threadIdx.x, n = npr_count(F);
     double s = 0.0;
-    if (t < n) s = s + v0[0];
-    if (t + 1024 < n) s = s + v0[1];
-    for (int b = t + 2048; b < n; b += 8 * 1024) {
+#pragma unroll
+    for (int u = 0; u < PR_PRE; ++u)
+        if (t + 1024 * u < n) s = s + v0[u];
+    for (int b = t + 1024 * PR_PRE; b < n; b += 8 * 1024) {
         double v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) v[u] = F.pr_part[min(b + u * 1024, n - 1)];
@@ -373,7 +377,7 @@ __device__ void mean_block(const FluidDev& F, double* sh)
 {
     DevScalars* S = F.S;
     const int t = threadIdx.x, n2m = F.n2m;
-    double prv[2];
+    double prv[PR_PRE];
     pr_load(F, prv);
     const double pravg = pr_mean(F, sh, prv);
     for (int j0 = 1 + t; j0 <= n2m; j0 += 2048) {
@@ -530,7 +534,7 @@ __device__ __forceinline__ void bc_body(const FluidDev& F, const VBlock& vb, int
     const int s_err = S->err;
     // workgroup 0: its row values and pressure partials, in flight with the above
     BcRows pre;
-    double prv[2] = {0.0, 0.0};
+    double prv[PR_PRE] = {};
     if (vb.x == 0) {
         bc_load(F, mean, 1 + t, pre);
         if (mean) pr_load(F, prv);

@@ -485,6 +485,41 @@ def test_ymirror_matches_dense(name, nsteps):
     assert e <= pbar
 
 
+def test_ymirror_toggles_between_steps():
+    """Switching the mirror form off and on between steps: dph read back
+    right after a switch is the last step's (the read-back applies the pair
+    butterfly only to a mirror-form result), and the run with switches
+    follows the dense run to the mirror-vs-dense bar."""
+    c = case("cyl64")
+    dec = decomposition(c)
+    gs = gpu_for(c, dec, axis="y")
+    gd = gpu_for(c, dec, axis="y")
+    gd.set_ymirror(False)
+    pbar = max(1e-10, poisson_kappa(c) * np.finfo(np.float64).eps)
+    t, dt = 0.0, c.params.dt0
+    tdd, dtd = 0.0, c.params.dt0
+    for ns in range(6):
+        if ns in (2, 4):
+            before = gs.get("dph")
+            gs.set_ymirror(ns == 4)
+            assert gs.ymirror == (ns == 4)
+            np.testing.assert_array_equal(gs.get("dph"), before)
+        gs.set_markers(*c.markers(t))
+        dt, _ = gs.fluid_tsch(ns, dt, dt)
+        t += dt
+        gd.set_markers(*c.markers(tdd))
+        dtd, _ = gd.fluid_tsch(ns, dtd, dtd)
+        tdd += dtd
+    ss, sd = gs.get_state(), gd.get_state()
+    for f in ("q2", "q3", "pr"):
+        e = relmax(ss[f], sd[f])
+        print(f"toggled vs dense, {f}: {e:.2e}", flush=True)
+        assert e <= (pbar if f == "pr" else 1e-11), f
+    e = relmax(gs.get("dph"), gd.get("dph"))
+    print(f"toggled vs dense dph: {e:.2e}", flush=True)
+    assert e <= pbar
+
+
 def test_ymirror_needs_a_symmetric_grid():
     """An ensemble grid whose y stretching is not symmetric about the centre
     (256 x 192 cells: 43 / 106 / 44 nodes) has no mirror form: the dense form
