@@ -2,7 +2,7 @@
 // MLP (no counterpart in the reference, whose coupling is loose; the
 // iterated coupling the reference prepares for is declarations.F90:250,
 // 271-278) that maps the recent history of the body (displacement, velocity,
-// loads) to the next step's loads.  The coupling iteration stays the judge:
+// loads) to the next step's loads.  The coupling iteration still decides the result:
 // the network only proposes where it starts.
 //
 // Network: NI inputs -> H tanh -> H tanh -> NO linear outputs (H = 64), FP32
