@@ -140,6 +140,14 @@
           integer(c_int32_t), value :: axis
         end function fribo_set_transform
 
+        ! mirror form of the transform along y on grids symmetric about the
+        ! channel centre: 1 (the default where available), 0 the dense form
+        integer(c_int) function fribo_set_ymirror(h, on) bind(c, name="fribo_set_ymirror")
+          import :: c_int, c_ptr, c_int32_t
+          type(c_ptr), value :: h
+          integer(c_int32_t), value :: on
+        end function fribo_set_ymirror
+
         integer(c_int) function fribo_diagnostics(h, out) bind(c, name="fribo_diagnostics")
           import :: c_int, c_double, c_ptr
           type(c_ptr), value :: h
