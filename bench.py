@@ -28,14 +28,18 @@ PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
-def pmc_traffic(name):
-    """HBM bytes per dispatch of `name` from the committed rocprofv3 --pmc summary."""
+def pmc_traffic(name, mirror=None):
+    """HBM bytes per dispatch of `name` from the committed rocprofv3 --pmc
+    summary (for the transform GEMM only when the summary's run took the same
+    form of the transform, mirror or dense)."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(path):
         return None
     try:
-        d = json.load(open(path))
-        return d.get(name, {}).get("bytes_per_dispatch")
+        e = json.load(open(path)).get(name, {})
+        if mirror is not None and e.get("mirror_form", False) != mirror:
+            return None
+        return e.get("bytes_per_dispatch")
     except Exception:
         return None
 
@@ -363,6 +367,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=300, help="coupled512: training epochs")
     ap.add_argument("--transform", default="auto", choices=["auto", "y", "z"],
                     help="axis of the Poisson eigen-transform (auto: the shorter one)")
+    ap.add_argument("--dense-y", action="store_true",
+                    help="transform along y in the dense form even where the grid has the mirror form")
     ap.add_argument("--precision", type=int, default=-1,
                     help="Poisson transforms: -1 FP64; n >= 0 FP32 MFMA + n FP64-residual refinement sweeps")
     ap.add_argument("--launch", default="stream", choices=["stream", "graph"],
@@ -411,6 +417,8 @@ def main():
     st0 = cases.initial_state(p, c.seed)
     g = FluidStep(p, c.rc, c.zz, zmm, zmmt, wr)
     g.set_transform(args.transform)
+    if args.dense_y and g.ymirror:
+        g.set_ymirror(False)
     if args.precision >= 0:
         g.set_precision(args.precision)
     g.set_graph(args.launch == "graph")
@@ -475,7 +483,7 @@ def main():
     if args.precision >= 0:
         gemm_kernel, peak = "k_gemm32", PEAK_FP32_MFMA_TFLOPS
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": pmc_traffic(gemm_kernel),
+            "frac": round(achieved / peak, 4), "traffic": pmc_traffic(gemm_kernel, mirror),
             "kernel": gemm_kernel, "avg_ms": round(gemm_avg_s * 1e3, 4), "transform_axis": axis,
             "mirror_form": mirror, "flops_per_launch": gemm_flops}
     kernels = {k: {"ms_per_step": round(v[0] / nb, 4), "launches_per_step": v[1] / nb}
@@ -568,7 +576,8 @@ def main():
         "config": {"workload": f"{args.workload}: {n3m}x{n2m} cells, {c.nlm} markers, Re={1.0 / p.visc:g}, CFL {p.cflmax}",
                    "precision": "plain FP64" if args.precision < 0 else
                    f"FP32 MFMA transforms + {args.precision} FP64-residual refinement sweeps",
-                   "poisson_transform": f"along {axis}", "launch": args.launch},
+                   "poisson_transform": f"along {axis}" + (" (mirror form)" if mirror else ""),
+                   "launch": args.launch},
         "roofline": roof,
         "cpu_baseline": cpu,
         "drop_in": drop_in,
