@@ -49,8 +49,8 @@
 // loads land past the stage's B block.  The 1536 x 2048 transforms of the
 // 2048x1536-cell grid give 256 workgroups, one per CU; workgroups are
 // dispatched to the 8 XCDs round-robin, so when the grid is a multiple of 256
-// tiles in 4 x 8 patches the tile indices are permuted to give each XCD
-// compact patches (its L2 then serves 4 A- and 8 B-tiles to 32 workgroups).
+// tiles in 8 x 4 patches the tile indices are permuted to give each XCD
+// compact patches (its L2 then serves 8 A- and 4 B-tiles to 32 workgroups).
 #include "fribo_ychunk.h"
 #include "fribo_batch.h"
 #include <hip/hip_ext.h>
@@ -69,6 +69,12 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 #endif
 #ifndef GEMM_DMA_BN
 #define GEMM_DMA_BN 128
+#endif
+// tile rows (M) of the 32-tile patch each XCD works on (32 / GEMM_PATCH_M tile
+// columns): 8 x 4 patches read 31.3 MiB per mirror-form transform of the
+// 2048x1536 grid against 33.8 MiB for 4 x 8 and 53.5 MiB for 2 x 16
+#ifndef GEMM_PATCH_M
+#define GEMM_PATCH_M 8
 #endif
 
 // Gauge partial of the transform along y (zg != nullptr): the tile holding
@@ -308,14 +314,15 @@ __device__ __forceinline__ void gemm_dma_body(int M, int N, int K, const double*
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave % WM, wn = wave / WM;
     int bx = vb.x, by = vb.y;
-    if (vb.gx % 4 == 0 && vb.gy % 8 == 0 && (vb.gx * vb.gy) % 256 == 0) {
-        // dispatch id -> XCD (id % 8); 32 consecutive ids of one XCD form a 4 x 8
-        // patch (a bijection when every XCD receives whole patches)
+    constexpr int PM = GEMM_PATCH_M, PN = 32 / GEMM_PATCH_M;
+    if (vb.gx % PM == 0 && vb.gy % PN == 0 && (vb.gx * vb.gy) % 256 == 0) {
+        // dispatch id -> XCD (id % 8); 32 consecutive ids of one XCD form a
+        // PM x PN patch (a bijection when every XCD receives whole patches)
         const int id = vb.x + vb.gx * vb.y, x = id % 8, l = id / 8;
-        const int npm = vb.gx / 4;
+        const int npm = vb.gx / PM;
         const int patch = x + 8 * (l / 32), ll = l % 32;
-        bx = (patch % npm) * 4 + ll % 4;
-        by = (patch / npm) * 8 + ll / 4;
+        bx = (patch % npm) * PM + ll % PM;
+        by = (patch / npm) * PN + ll / PM;
     }
     const int i0 = bx * BM, n0 = by * BN;
     int kb, ke;
