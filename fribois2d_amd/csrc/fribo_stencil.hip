@@ -214,6 +214,21 @@ __device__ StepScalars step_scalars(const FluidDev& F, int n, int ns, double dt_
     return out;
 }
 
+// LDS slot of row r in sys_prepare: one pad slot per 32 rows, so that the
+// lanes of the chunk elimination, each reading row cs[p] + i of its own
+// chunk, fall on different banks (unpadded, 32-row chunks put all 64 lanes
+// on one bank)
+__device__ inline int sp_slot(int r) { return r + (r >> 5); }
+
+// Grid coefficients (am, ac, ap) of system K.
+__device__ inline void sys_coefs(const FluidDev& F, const KSys& K, const double*& am, const double*& ac,
+                                 const double*& ap)
+{
+    if (K.src == 0) { am = F.am3sk; ac = F.ac3sk; ap = F.ap3sk; }
+    else if (K.src == 1) { am = F.am2j; ac = F.ac2j; ap = F.ap2j; }
+    else { am = F.am3j; ac = F.ac3j; ap = F.ap3j; }
+}
+
 // Line-invariant factors of one partitioned Helmholtz system for this
 // substep (solve_fluid.F90:77-87,126-143,185-200,260-278: rows normalised by
 // 1/(1 - betadx*ac)).  Row coefficients go to LDS; lane p of the first wave
@@ -224,27 +239,28 @@ __device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, dou
 {
     const int t = threadIdx.x;
     const int n = K.n;
+    const int ns = sp_slot(n - 1) + 1;
     double* la = lds;
-    double* lc = lds + n;
-    double* lS = lds + 2 * n;
+    double* lc = lds + ns;
+    double* lS = lds + 2 * ns;
     const double *am, *ac, *ap;
-    if (K.src == 0) { am = F.am3sk; ac = F.ac3sk; ap = F.ap3sk; }
-    else if (K.src == 1) { am = F.am2j; ac = F.ac2j; ap = F.ap2j; }
-    else { am = F.am3j; ac = F.ac3j; ap = F.ap3j; }
+    sys_coefs(F, K, am, ac, ap);
     for (int r = t; r < n; r += blockDim.x) {
         const int idx = r + K.kofs;
         const double b = 1.0 / (1.0 - ac[idx] * betadx);
         const double amv = -am[idx] * betadx * b;
         const double apv = -ap[idx] * betadx * b;
-        la[r] = (r == 0) ? 0.0 : amv;
-        lc[r] = (r == n - 1) ? 0.0 : apv;
-        lS[r] = b;
+        const double av = (r == 0) ? 0.0 : amv, cv = (r == n - 1) ? 0.0 : apv;
+        const int q = sp_slot(r);
+        la[q] = av;
+        lc[q] = cv;
+        lS[q] = b;
         if (r == 0) K.bc[0] = amv;
         if (r == n - 1) K.bc[1] = apv;
         if (!K.inter) {
             K.S[r] = b;
-            K.a[r] = la[r];
-            K.c[r] = lc[r];
+            K.a[r] = av;
+            K.c[r] = cv;
         }
     }
     lds_barrier();
@@ -256,7 +272,7 @@ __device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, dou
         double cpp = 0.0, vpp = 1.0, gg = 1.0, wl = 0.0, glast = 1.0, vsum = 0.0;
 #pragma unroll 4
         for (int i = s; i <= e - 1; ++i) {
-            const double a = la[i], c = lc[i];
+            const double a = la[sp_slot(i)], c = lc[sp_slot(i)];
             const double iv = recip(1.0 - a * cpp);
             const double cpi = c * iv;
             const double vpi = (-a * vpp) * iv;
@@ -266,7 +282,7 @@ __device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, dou
             K.vp[idx] = vpi;
             K.g[idx] = gg;
             if (K.inter) {
-                K.S[idx] = lS[i];
+                K.S[idx] = lS[sp_slot(i)];
                 K.a[idx] = a;
                 K.c[idx] = c;
             }
@@ -279,17 +295,18 @@ __device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, dou
             cpp = cpi;
             vpp = vpi;
         }
+        const int se = sp_slot(e);
         if (K.inter) {
             const int idx = (e - s) * 64 + p;
-            K.S[idx] = lS[e];
-            K.a[idx] = la[e];
-            K.c[idx] = lc[e];
+            K.S[idx] = lS[se];
+            K.a[idx] = la[se];
+            K.c[idx] = lc[se];
         }
         K.wl[p] = wl;
         vs = vsum;
         ws = glast * wl;
-        const double ae = la[e];
-        ce = lc[e];
+        const double ae = la[se];
+        ce = lc[se];
         alpha = ae * vpp;
         beta = 1.0 + ae * wl;
     }
@@ -323,7 +340,7 @@ size_t sysprep_lds_bytes(const FluidDev& F)
         if (F.ks[i].n > n) n = F.ks[i].n;
         if (F.js[i].n > n) n = F.js[i].n;
     }
-    return (size_t)3 * n * sizeof(double);
+    return (size_t)3 * (n + (n >> 5) + 1) * sizeof(double);
 }
 
 // Pressure partials k_upd leaves: one per tile (the mirror form's tiles
@@ -347,9 +364,9 @@ __device__ inline void pr_load(const FluidDev& F, double* v)
 
 // Pressure mean of the previous substep (pr_submean, updvp.F90:125-141) from
 // the npr_part partials of k_upd: thread t adds entries t, t+1024, ... in
-// turn (the first PR_PRE from pr_load, then eight loads in flight at a
-// time), then block_sum.  Fixed order.
-__device__ inline double pr_mean(const FluidDev& F, double* sh, const double* v0)
+// turn (pr_partial: the first PR_PRE from pr_load, then eight loads in
+// flight at a time), then block_sum.  Fixed order.
+__device__ inline double pr_partial(const FluidDev& F, const double* v0)
 {
     const int t = threadIdx.x, n = npr_count(F);
     double s = 0.0;
@@ -364,7 +381,40 @@ __device__ inline double pr_mean(const FluidDev& F, double* sh, const double* v0
         for (int u = 0; u < 8; ++u)
             if (b + u * 1024 < n) s = s + v[u];
     }
-    return block_sum<1024>(sh, s) / (F.rext2 * F.alx3);
+    return s;
+}
+__device__ inline double pr_mean(const FluidDev& F, double* sh, const double* v0)
+{
+    return block_sum<1024>(sh, pr_partial(F, v0)) / (F.rext2 * F.alx3);
+}
+
+// Three block_sum<1024> reductions in one pass (each value summed in the
+// same order as block_sum, so the results are the same bits).
+__device__ inline void block_sum3(double* sh, double& a, double& b, double& c)
+{
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a = a + __shfl_xor(a, o, 64);
+        b = b + __shfl_xor(b, o, 64);
+        c = c + __shfl_xor(c, o, 64);
+    }
+    lds_barrier();
+    if ((t & 63) == 0) {
+        sh[t >> 6] = a;
+        sh[16 + (t >> 6)] = b;
+        sh[32 + (t >> 6)] = c;
+    }
+    lds_barrier();
+    a = sh[0];
+    b = sh[16];
+    c = sh[32];
+#pragma unroll
+    for (int w = 1; w < 16; ++w) {
+        a = a + sh[w];
+        b = b + sh[16 + w];
+        c = c + sh[32 + w];
+    }
 }
 
 // pr_submean and boucqt (iobcs.F90:81-108) by one workgroup of 1024
@@ -449,12 +499,15 @@ __device__ inline void bc_load(const FluidDev& F, int mean, int j0, BcRows& b)
 // step scalars are known, so that a single memory round trip serves the
 // common case n2m <= 2048; the outflow increments of those rows stay in
 // registers for the mass correction.
-__device__ void bc_rows(const FluidDev& F, const StepScalars& r, int mean, double* sh, double* sh2,
-                        const BcRows& pre, const double* prv)
+__device__ void bc_rows(const FluidDev& F, const StepScalars& r, int mean, double* sh, const BcRows& pre,
+                        const double* prv)
 {
     DevScalars* S = F.S;
     const int t = threadIdx.x, n2m = F.n2m;
-    const double pravg = mean ? pr_mean(F, sh, prv) : 0.0;
+    double psum = mean ? pr_partial(F, prv) : 0.0;
+    // the sum is formed here: sunk to its use after the pass, it would keep
+    // the preloaded partials live through the pass and spill
+    asm volatile("" : "+v"(psum));
     const double dt = r.dt, ga = r.ga, ro = r.ro;
     const double udx3c_n3 = F.udx3c[F.n3];
     double area = 0.0, qout = 0.0;
@@ -497,16 +550,18 @@ __device__ void bc_rows(const FluidDev& F, const StepScalars& r, int mean, doubl
         bc_load(F, mean, j0, b);
         pass(b, j0, false);
     }
+    // the pressure mean, the outflow area and the outflow flux in one reduction
+    block_sum3(sh, psum, area, qout);
+    const double tarea = area, tqout = qout;
     if (t == 0) {
         F.dqb2s[F.n2] = 0.0;
         F.dqb3s[F.n2] = 0.0;
         if (mean) {
+            const double pravg = psum / (F.rext2 * F.alx3);
             S->pend = pravg;
             S->goff = S->goff + pravg;
         }
     }
-    const double tarea = block_sum<1024>(sh, area);
-    const double tqout = block_sum<1024>(sh2, qout);
     const double qinf = 0.0 * tarea;   // dqb3s == 0 (steady inflow)
     const double cor = (qinf - tqout) / tarea;
     int i = 0;
@@ -570,7 +625,7 @@ __device__ __forceinline__ void bc_body(const FluidDev& F, const VBlock& vb, int
         S->ugkks = 1.0 / (F.g3rm[1] * F.g3rc[1]) * F.dx3q * 2.0 * r.betadx;
         S->ugkkn = 1.0 / (F.g3rm[F.n3m] * F.g3rc[F.n3]) * F.dx3q * 2.0 * r.betadx;
     }
-    bc_rows(F, r, mean, sh, sh2, pre, prv);
+    bc_rows(F, r, mean, sh, pre, prv);
     // q2 wall rows: q2(1,:) = q2(n2,:) = 0 on the ghost planes (helsolve.F90:32-33)
     if (t < 6) {
         int k = (t < 3) ? t - 2 : F.n3m + 1 + (t - 3);
