@@ -245,22 +245,39 @@ __device__ void sys_prepare(const FluidDev& F, const KSys& K, double betadx, dou
     double* lS = lds + 2 * ns;
     const double *am, *ac, *ap;
     sys_coefs(F, K, am, ac, ap);
-    for (int r = t; r < n; r += blockDim.x) {
-        const int idx = r + K.kofs;
-        const double b = 1.0 / (1.0 - ac[idx] * betadx);
-        const double amv = -am[idx] * betadx * b;
-        const double apv = -ap[idx] * betadx * b;
-        const double av = (r == 0) ? 0.0 : amv, cv = (r == n - 1) ? 0.0 : apv;
-        const int q = sp_slot(r);
-        la[q] = av;
-        lc[q] = cv;
-        lS[q] = b;
-        if (r == 0) K.bc[0] = amv;
-        if (r == n - 1) K.bc[1] = apv;
-        if (!K.inter) {
-            K.S[r] = b;
-            K.a[r] = av;
-            K.c[r] = cv;
+    // rows t and t + 1024 per pass (workgroups of 1024 threads), both rows'
+    // loads issued before the first row's stores, which the compiler would
+    // otherwise keep ahead of the second row's loads (the coefficient arrays
+    // and K's factor arrays may alias as far as it knows)
+    for (int r0 = t; r0 < n; r0 += 2048) {
+        double amr[2], acr[2], apr[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int idx = min(r0 + 1024 * u, n - 1) + K.kofs;
+            amr[u] = am[idx];
+            acr[u] = ac[idx];
+            apr[u] = ap[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int r = r0 + 1024 * u;
+            if (r < n) {
+                const double b = 1.0 / (1.0 - acr[u] * betadx);
+                const double amv = -amr[u] * betadx * b;
+                const double apv = -apr[u] * betadx * b;
+                const double av = (r == 0) ? 0.0 : amv, cv = (r == n - 1) ? 0.0 : apv;
+                const int q = sp_slot(r);
+                la[q] = av;
+                lc[q] = cv;
+                lS[q] = b;
+                if (r == 0) K.bc[0] = amv;
+                if (r == n - 1) K.bc[1] = apv;
+                if (!K.inter) {
+                    K.S[r] = b;
+                    K.a[r] = av;
+                    K.c[r] = cv;
+                }
+            }
         }
     }
     lds_barrier();
